@@ -1,0 +1,167 @@
+/*
+ * marqov_amd.h -- C ABI of the MI355X update path (Metropolis sweeps and
+ * Wolff single-cluster updates for Ising, +-J Edwards-Anderson, Potts,
+ * Heisenberg and Phi4 on regular lattices and CSR graphs).
+ *
+ * Plain C: opaque handles, POD arguments, int status (0 = ok).  A failing call
+ * returns a negative status and leaves a message in mq_last_error(ctx).
+ * One context owns one HIP stream on one device; every call on a context is
+ * ordered on that stream.  Handles are not thread-safe; distinct contexts may
+ * be used from distinct threads (the reference runs one Core per worker thread,
+ * src/libmarqov/marqovqueue.h:169-226).
+ *
+ * Sign convention follows the reference: E = J * sum_<ij> s_i.s_j, so J = -1 is
+ * ferromagnetic (src/hamiltonian/util/observables.h:209-216, config/Ising.ini).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef MARQOV_AMD_H
+#define MARQOV_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define MQ_API __attribute__((visibility("default")))
+#else
+#define MQ_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MQ_OK 0
+#define MQ_ERR_ARG (-1)
+#define MQ_ERR_HIP (-2)
+#define MQ_ERR_STATE (-3)
+#define MQ_ERR_UNSUPPORTED (-4)
+
+/* models (src/hamiltonian/{Ising,EdwardsAndersonIsing,Potts,Heisenberg,Phi4}.h) */
+#define MQ_ISING 1
+#define MQ_EA_ISING 2
+#define MQ_POTTS 3
+#define MQ_HEISENBERG 4
+#define MQ_PHI4 5
+
+/* O(3) Metropolis proposals: the reference's fresh uniform direction
+ * (src/hamiltonian/util/initializers.h:61-81) or s' = normalize(s + delta*r). */
+#define MQ_PROPOSE_FRESH 0
+#define MQ_PROPOSE_SMALL_ANGLE 1
+
+/* Heisenberg Wolff embeddings: the reference's random Cartesian component
+ * (src/hamiltonian/Heisenberg.h:191-243) or a random unit vector (Phi4.h:182-244). */
+#define MQ_EMBED_CARTESIAN 0
+#define MQ_EMBED_RANDOM_DIR 1
+
+/* observables per replica written by mq_measure */
+#define MQ_NOBS 4 /* e = E/N, |m|, m^2, m^4 (m = |sum s|/N; Potts: reference order parameter) */
+
+typedef struct mq_ctx mq_ctx;
+typedef struct mq_lattice mq_lattice;
+typedef struct mq_state mq_state;
+
+typedef struct mq_model_params {
+    int model;      /* MQ_ISING ... MQ_PHI4 */
+    double J;       /* coupling (Ising/EA/Heisenberg/Phi4) or Potts k */
+    int q;          /* Potts number of states (2..8) */
+    double lambda;  /* Phi4 quartic coefficient (Phi4.h:85-124) */
+    double mass;    /* Phi4 quadratic coefficient */
+    int proposal;   /* MQ_PROPOSE_* (Heisenberg) */
+    double delta;   /* small-angle step (Heisenberg) */
+    int embedding;  /* MQ_EMBED_* (Heisenberg Wolff) */
+} mq_model_params;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces the per-simulation setup of Core (src/libmarqov/core.h:468-494). */
+MQ_API int mq_ctx_create(int device, mq_ctx** out);
+MQ_API void mq_ctx_destroy(mq_ctx* ctx);
+MQ_API const char* mq_last_error(const mq_ctx* ctx);
+MQ_API int mq_sync(mq_ctx* ctx);
+/* device timing on the context's stream (hipEvent pair) */
+MQ_API int mq_timer_begin(mq_ctx* ctx);
+MQ_API int mq_timer_end(mq_ctx* ctx, double* ms);
+
+/* ---- lattices --------------------------------------------------------- */
+/* Periodic hypercube, dim 2 or 3, L even: FixedDimRegularHypercubic<dim>
+ * (src/lattice/regular_hypercubic.h:15-100); site index sum_j x_j L^j. */
+MQ_API int mq_lattice_regular(mq_ctx* ctx, int dim, int L, mq_lattice** out);
+/* General graph as CSR adjacency (symmetric, no self loops): replaces the
+ * nbrs() of GraphFromCSV / RandomGeometricGraph (src/lattice/graph_from_csv.h:32-57,
+ * random_geometric_graph.h:258-290).  The library colours it greedily. */
+MQ_API int mq_lattice_csr(mq_ctx* ctx, int64_t nsites, const int32_t* row_ptr, const int32_t* col_idx,
+                   mq_lattice** out);
+/* number of colour classes and their sizes (ncolors <= 64) */
+MQ_API int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes64);
+MQ_API int64_t mq_lattice_nsites(const mq_lattice* lat);
+MQ_API void mq_lattice_destroy(mq_lattice* lat);
+
+/* ---- states (replica batches) ------------------------------------------ */
+/* nrep replicas of one model; replica r lives on lattices[r].  Replaces the
+ * Space of Core (core.h:249-342) for a whole batch of simulations. */
+MQ_API int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int nrep, mq_lattice* const* lattices,
+                    mq_state** out);
+MQ_API void mq_state_destroy(mq_state* st);
+/* inverse temperature per replica (Core::beta, core.h:372-494) */
+MQ_API int mq_state_set_beta(mq_state* st, const double* beta);
+MQ_API int mq_state_get_beta(const mq_state* st, double* beta);
+/* EA couplings: rep_sample[r] names the disorder sample of replica r;
+ * bonds of sample k are drawn +1 with probability p_plus from (seed, k)
+ * (BimodalPDF, src/lattice/util/prob_density.h:25-35; symmetric by construction
+ * as RegularRandomBond makes them, src/lattice/regular_random_bond.h:38-54). */
+MQ_API int mq_state_random_bonds(mq_state* st, const int32_t* rep_sample, double p_plus, uint64_t seed);
+/* EA couplings from the host for one sample: regular lattices bonds[i*dim+d]
+ * couples i and i+e_d; CSR lattices one value per CSR slot (must be symmetric). */
+MQ_API int mq_state_set_bonds(mq_state* st, int sample, const int8_t* bonds);
+/* random start (Ising.h:99-107, Potts.h:149-156, core.h:774-782) on the device */
+MQ_API int mq_state_init(mq_state* st, uint64_t seed);
+/* host <-> device in the reference's site order (AoS Space, core.h:270-300):
+ * int8 per site for Ising/EA/Potts, float[3] per site for Heisenberg/Phi4. */
+MQ_API int mq_state_upload(mq_state* st, int rep, const void* host, int64_t bytes);
+MQ_API int mq_state_download(mq_state* st, int rep, void* host, int64_t bytes);
+
+/* ---- updates ---------------------------------------------------------- */
+/* nsweeps checkerboard / colour-class Metropolis sweeps of every replica;
+ * replaces EMCS's Metropolis loop (emcs.h:93-106) with Metropolis::move
+ * (metropolis.h:86-126).  Sweep t uses Philox counter phase 2*(sweep0+t)+colour. */
+MQ_API int mq_metropolis(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0);
+/* nclusters single-cluster Wolff moves of every replica; replaces EMCS's
+ * cluster loop (emcs.h:83-89) with Wolff::move (wolff.h:48-125, Ising.h:122-169,
+ * Heisenberg.h:191-243, Potts.h:262-322).  sizes (host, may be NULL) receives
+ * nrep*nclusters cluster sizes, replica-major. */
+MQ_API int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t cluster0, int64_t* sizes);
+/* one EMCS (emcs.h:69-109): ncluster Wolff moves then nmetro sweeps; writes
+ * the per-replica mean cluster size (NaN when ncluster = 0, as emcs.h:108). */
+MQ_API int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, uint64_t step, double* mean_cluster);
+/* observables, FP64 (observables.h:32-66,173-352; Potts.h:40-65): out[r*MQ_NOBS+k] */
+MQ_API int mq_measure(mq_state* st, double* out);
+/* total energy per replica (sum over bonds, FP64), as Core::calcAction/beta (core.h:1145-1151) */
+MQ_API int mq_energy(mq_state* st, double* out);
+/* FP64 renormalisation of O(3) spins to unit length */
+MQ_API int mq_renormalize(mq_state* st);
+
+/* ---- parallel tempering ---------------------------------------------- */
+/* Replica-exchange between neighbouring temperatures inside each group of
+ * ntemp replicas (replicas g*ntemp .. g*ntemp+ntemp-1 share a sample and hold
+ * the temperatures beta[perm]).  Pairs (k, k+1) with k = parity mod 2 are
+ * offered a swap accepted with min(1, exp((b_a-b_b)(E_a-E_b))) computed in FP64
+ * (calcMHratio / calcprob, src/libmarqov/marqovscheduler.h:44-57,445-465);
+ * an accepted swap exchanges the temperatures of the two replicas (the
+ * reference exchanges the state spaces, marqovscheduler.h:466-470; the Markov
+ * chain is the same).  accepted (host, may be NULL) gets the number of
+ * accepted swaps per group. */
+MQ_API int mq_pt_swap(mq_state* st, int ntemp, int parity, uint64_t seed, uint64_t step, int32_t* accepted);
+/* temperature index currently held by each replica (after swaps) */
+MQ_API int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep);
+
+/* ---- graphs ------------------------------------------------------------ */
+/* Capture `steps` repetitions of {nsweeps Metropolis sweeps; optional PT
+ * swap with ntemp temperatures} into one HIP graph and replay it `repeats`
+ * times; sweep and step counters advance per repetition. */
+MQ_API int mq_run_graph(mq_state* st, int nsweeps, int ntemp, uint64_t seed, uint64_t sweep0, uint64_t step0,
+                 int steps, int repeats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

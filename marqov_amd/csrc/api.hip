@@ -1,0 +1,671 @@
+// Host side of the C ABI: contexts, lattices, replica batches, acceptance
+// tables and the update / measurement entry points declared in
+// include/marqov_amd.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+#include "mq_internal.hpp"
+
+int mq_fail(mq_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+namespace {
+
+template <class T>
+hipError_t dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+Key2 key_of(uint64_t seed) { return Key2{(uint32_t)seed, (uint32_t)(seed >> 32)}; }
+
+bool is_vec(int model) { return model == MQ_HEISENBERG || model == MQ_PHI4; }
+bool is_int(int model) { return model == MQ_ISING || model == MQ_EA_ISING || model == MQ_POTTS; }
+
+void split_threshold(double p, uint32_t* hi, uint32_t* lo)
+{
+    if (!(p < 1.0)) {
+        *hi = 65536u;
+        *lo = 0u;
+        return;
+    }
+    double t = std::floor(std::ldexp(p, 32));
+    uint64_t T = (t <= 0.0) ? 0u : (uint64_t)t;
+    if (T > 0xFFFFFFFFull) T = 0xFFFFFFFFull;
+    *hi = (uint32_t)(T >> 16);
+    *lo = (uint32_t)(T & 0xFFFFu);
+}
+
+uint32_t prob_threshold(double p)
+{
+    if (!(p > 0.0)) return 0u;
+    double t = std::floor(std::ldexp(p, 32));
+    if (t >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)t;
+}
+
+int qmax_of(const mq_state* st)
+{
+    int q = 0;
+    for (auto* l : st->lat) q = std::max(q, l->kind == LAT_REGULAR ? 2 * l->dim : l->maxdeg);
+    return q;
+}
+
+// host natural order <-> device layout position of site i
+int64_t dev_pos(const mq_lattice* lat, int64_t i)
+{
+    if (lat->kind == LAT_REGULAR) return (int64_t)regular_colour(lat->dim, lat->L, i) * (lat->N / 2) + (i >> 1);
+    return lat->pos_of[i];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// clear n 64-bit words with a fill kernel (graph-capture safe)
+hipError_t zero64(void* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long*>(p), n);
+    return hipGetLastError();
+}
+
+int check_state(mq_state* st)
+{
+    if (!st) return MQ_ERR_ARG;
+    if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set (call mq_state_set_beta)");
+    if (st->p.model == MQ_EA_ISING && !st->d_bonds)
+        return mq_fail(st->ctx, MQ_ERR_STATE, "EA couplings not set (mq_state_random_bonds / mq_state_set_bonds)");
+    return MQ_OK;
+}
+
+int ensure_bond_storage(mq_state* st)
+{
+    if (st->d_bonds) return MQ_OK;
+    st->sample_bond_off.assign(st->nsamples, -1);
+    int64_t off = 0;
+    for (int r = 0; r < st->nrep; ++r) {
+        const int k = st->rep_sample[r];
+        if (st->sample_bond_off[k] >= 0) continue;
+        const mq_lattice* l = st->lat[r];
+        st->sample_bond_off[k] = off;
+        const int64_t n = (l->kind == LAT_REGULAR) ? l->N * l->dim : l->nnz;
+        off += (n + 255) / 256 * 256;
+    }
+    MQ_HIP(st->ctx, dalloc(&st->d_bonds, (size_t)std::max<int64_t>(off, 256)));
+    MQ_HIP(st->ctx, hipMemsetAsync(st->d_bonds, 0, (size_t)std::max<int64_t>(off, 256), st->ctx->stream));
+    std::vector<int64_t> rb(st->nrep);
+    for (int r = 0; r < st->nrep; ++r) rb[r] = st->sample_bond_off[st->rep_sample[r]];
+    MQ_HIP(st->ctx, hipMemcpyAsync(st->d_rep_bond_off, rb.data(), sizeof(int64_t) * st->nrep, hipMemcpyHostToDevice,
+                                   st->ctx->stream));
+    MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    return MQ_OK;
+}
+
+int ensure_wolff_scratch(mq_state* st, int nclusters)
+{
+    if (!st->d_queue) {
+        st->rep_site_off.resize(st->nrep);
+        int64_t off = 0;
+        for (int r = 0; r < st->nrep; ++r) {
+            st->rep_site_off[r] = off;
+            off += (st->lat[r]->N + 63) / 64 * 64;
+        }
+        st->total_sites = off;
+        MQ_HIP(st->ctx, dalloc(&st->d_queue, (size_t)off));
+        MQ_HIP(st->ctx, dalloc(&st->d_visit, (size_t)(off / 32)));
+        MQ_HIP(st->ctx, hipMemsetAsync(st->d_visit, 0, sizeof(uint32_t) * (size_t)(off / 32), st->ctx->stream));
+        MQ_HIP(st->ctx, dalloc(&st->d_rep_site_off, (size_t)st->nrep));
+        MQ_HIP(st->ctx, hipMemcpyAsync(st->d_rep_site_off, st->rep_site_off.data(), sizeof(int64_t) * st->nrep,
+                                       hipMemcpyHostToDevice, st->ctx->stream));
+        MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    }
+    const size_t need = (size_t)st->nrep * (size_t)std::max(nclusters, 1);
+    if (need > st->csize_cap) {
+        dfree(st->d_csize);
+        MQ_HIP(st->ctx, dalloc(&st->d_csize, need));
+        st->csize_cap = need;
+    }
+    return MQ_OK;
+}
+
+}  // namespace
+
+// ============================================================== context
+extern "C" int mq_ctx_create(int device, mq_ctx** out)
+{
+    if (!out) return MQ_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MQ_ERR_HIP;
+    if (device < 0 || device >= n) return MQ_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return MQ_ERR_HIP;
+    auto* c = new mq_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return MQ_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    *out = c;
+    return MQ_OK;
+}
+
+extern "C" void mq_ctx_destroy(mq_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* mq_last_error(const mq_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int mq_sync(mq_ctx* ctx)
+{
+    if (!ctx) return MQ_ERR_ARG;
+    MQ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return MQ_OK;
+}
+
+extern "C" int mq_timer_begin(mq_ctx* ctx)
+{
+    if (!ctx) return MQ_ERR_ARG;
+    MQ_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    return MQ_OK;
+}
+
+extern "C" int mq_timer_end(mq_ctx* ctx, double* ms)
+{
+    if (!ctx || !ms) return MQ_ERR_ARG;
+    MQ_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    MQ_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float f = 0.f;
+    MQ_HIP(ctx, hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+    *ms = f;
+    return MQ_OK;
+}
+
+// ============================================================== lattices
+extern "C" int mq_lattice_regular(mq_ctx* ctx, int dim, int L, mq_lattice** out)
+{
+    if (!ctx || !out) return MQ_ERR_ARG;
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "regular lattices: dim must be 2 or 3");
+    if (L < 2 || (L & 1)) return mq_fail(ctx, MQ_ERR_ARG, "regular lattices: L must be even and >= 2 (checkerboard)");
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    if (N > (int64_t)1 << 31) return mq_fail(ctx, MQ_ERR_ARG, "regular lattice too large (N > 2^31)");
+    auto* l = new mq_lattice();
+    l->ctx = ctx;
+    l->kind = LAT_REGULAR;
+    l->dim = dim;
+    l->L = L;
+    l->N = N;
+    l->ncolors = 2;
+    l->color_off = {0, N / 2, N};
+    l->maxdeg = 2 * dim;
+    *out = l;
+    return MQ_OK;
+}
+
+extern "C" int64_t mq_lattice_nsites(const mq_lattice* lat) { return lat ? lat->N : -1; }
+
+extern "C" int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes64)
+{
+    if (!lat || !ncolors) return MQ_ERR_ARG;
+    *ncolors = lat->ncolors;
+    if (sizes64)
+        for (int c = 0; c < lat->ncolors; ++c) sizes64[c] = lat->color_off[c + 1] - lat->color_off[c];
+    return MQ_OK;
+}
+
+extern "C" void mq_lattice_destroy(mq_lattice* lat)
+{
+    if (!lat) return;
+    dfree(lat->d_rp);
+    dfree(lat->d_ci);
+    dfree(lat->d_edge);
+    dfree(lat->d_slot_of_orig);
+    delete lat;
+}
+
+// =============================================================== states
+extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int nrep, mq_lattice* const* lattices,
+                               mq_state** out)
+{
+    if (!ctx || !params || !lattices || !out || nrep <= 0) return MQ_ERR_ARG;
+    *out = nullptr;
+    const int model = params->model;
+    if (model < MQ_ISING || model > MQ_PHI4) return mq_fail(ctx, MQ_ERR_ARG, "unknown model");
+    if (model == MQ_POTTS && (params->q < 2 || params->q > 7))
+        return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "Potts: q must be in [2,7]");
+    const int kind0 = lattices[0] ? lattices[0]->kind : -1;
+    const int dim0 = lattices[0] ? lattices[0]->dim : -1;
+    for (int r = 0; r < nrep; ++r) {
+        if (!lattices[r]) return mq_fail(ctx, MQ_ERR_ARG, "null lattice");
+        if (lattices[r]->kind != kind0 || (kind0 == LAT_REGULAR && lattices[r]->dim != dim0))
+            return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "all replicas of a state must use lattices of one kind and dim");
+    }
+    auto* st = new mq_state();
+    st->ctx = ctx;
+    st->p = *params;
+    st->nrep = nrep;
+    st->lat.assign(lattices, lattices + nrep);
+    st->site_bytes = is_vec(model) ? 16 : 1;
+    st->rep_off.resize(nrep);
+    int64_t off = 0;
+    for (int r = 0; r < nrep; ++r) {
+        st->rep_off[r] = off;
+        off += (st->lat[r]->N * st->site_bytes + 255) / 256 * 256;
+    }
+    st->spin_total = off;
+    for (int r = 0; r < nrep;) {
+        int e = r + 1;
+        while (e < nrep && st->lat[e] == st->lat[r]) ++e;
+        Group g;
+        g.lat = st->lat[r];
+        g.rep0 = r;
+        g.nrep = e - r;
+        g.off = st->rep_off[r];
+        g.stride = (g.lat->N * st->site_bytes + 255) / 256 * 256;
+        st->groups.push_back(g);
+        r = e;
+    }
+    st->tbl = 2 * qmax_of(st) + 1;
+    st->rep_sample.resize(nrep);
+    std::iota(st->rep_sample.begin(), st->rep_sample.end(), 0);
+    st->nsamples = nrep;
+    auto fail = [&](hipError_t e) {
+        ctx->err = std::string("allocation failed: ") + hipGetErrorString(e);
+        mq_state_destroy(st);
+        return MQ_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = dalloc(&st->d_spins, (size_t)off)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_thr, (size_t)nrep * st->tbl * 2)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_wthr, (size_t)nrep)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_betaf, (size_t)nrep)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_betad, (size_t)nrep)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_isum, (size_t)nrep * 8)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_fsum, (size_t)nrep * 8)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_rep_bond_off, (size_t)nrep)) != hipSuccess) return fail(e);
+    if ((e = dalloc(&st->d_energy, (size_t)nrep)) != hipSuccess) return fail(e);
+    if ((e = hipMemsetAsync(st->d_spins, 0, (size_t)off, ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = hipMemsetAsync(st->d_rep_bond_off, 0, sizeof(int64_t) * nrep, ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
+    *out = st;
+    return MQ_OK;
+}
+
+extern "C" void mq_state_destroy(mq_state* st)
+{
+    if (!st) return;
+    (void)hipStreamSynchronize(st->ctx->stream);
+    dfree(st->d_spins);
+    dfree(st->d_thr);
+    dfree(st->d_wthr);
+    dfree(st->d_betaf);
+    dfree(st->d_betad);
+    dfree(st->d_bonds);
+    dfree(st->d_rep_bond_off);
+    dfree(st->d_queue);
+    dfree(st->d_visit);
+    dfree(st->d_csize);
+    dfree(st->d_rep_site_off);
+    dfree(st->d_isum);
+    dfree(st->d_fsum);
+    dfree(st->d_temp_of_rep);
+    dfree(st->d_rep_of_temp);
+    dfree(st->d_pt_acc);
+    dfree(st->d_energy);
+    dfree(st->d_ladder);
+    delete st;
+}
+
+// Acceptance tables (host, FP64): Ising/EA entry e = q + qmax with
+// q = s_i sum_j b_ij s_j and dE = -2 J q; Potts entry e = dn + qmax with
+// dE = k dn.  The Wolff add probability is -expm1(2 beta J) for Ising
+// (Ising.h:139) and 1 - exp(beta k) for Potts (Potts.h:237-242, pref 2).
+static int upload_tables(mq_state* st)
+{
+    const int nrep = st->nrep, tbl = st->tbl, qmax = (tbl - 1) / 2;
+    std::vector<uint32_t> thr((size_t)nrep * tbl * 2);
+    std::vector<uint32_t> wthr(nrep);
+    std::vector<float> bf(nrep);
+    for (int r = 0; r < nrep; ++r) {
+        const double beta = st->beta[r];
+        bf[r] = (float)beta;
+        for (int e = 0; e < tbl; ++e) {
+            double dE;
+            if (st->p.model == MQ_POTTS) dE = st->p.J * (double)(e - qmax);
+            else dE = -2.0 * st->p.J * (double)(e - qmax);
+            const double p = (dE <= 0.0) ? 1.0 : std::exp(-beta * dE);
+            split_threshold(p, &thr[((size_t)r * tbl + e) * 2], &thr[((size_t)r * tbl + e) * 2 + 1]);
+        }
+        double pw = 0.0;
+        if (st->p.model == MQ_ISING || st->p.model == MQ_EA_ISING) pw = -std::expm1(2.0 * beta * st->p.J);
+        else if (st->p.model == MQ_POTTS) pw = -std::expm1(beta * st->p.J);
+        wthr[r] = prob_threshold(pw);
+    }
+    hipStream_t s = st->ctx->stream;
+    MQ_HIP(st->ctx, hipMemcpyAsync(st->d_thr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice, s));
+    MQ_HIP(st->ctx, hipMemcpyAsync(st->d_wthr, wthr.data(), wthr.size() * 4, hipMemcpyHostToDevice, s));
+    MQ_HIP(st->ctx, hipMemcpyAsync(st->d_betaf, bf.data(), bf.size() * 4, hipMemcpyHostToDevice, s));
+    MQ_HIP(st->ctx, hipMemcpyAsync(st->d_betad, st->beta.data(), st->beta.size() * 8, hipMemcpyHostToDevice, s));
+    MQ_HIP(st->ctx, hipStreamSynchronize(s));
+    return MQ_OK;
+}
+
+extern "C" int mq_state_set_beta(mq_state* st, const double* beta)
+{
+    if (!st || !beta) return MQ_ERR_ARG;
+    for (int r = 0; r < st->nrep; ++r)
+        if (!(beta[r] >= 0.0) || !std::isfinite(beta[r])) return mq_fail(st->ctx, MQ_ERR_ARG, "beta must be finite and >= 0");
+    st->beta.assign(beta, beta + st->nrep);
+    return upload_tables(st);
+}
+
+extern "C" int mq_state_get_beta(const mq_state* st, double* beta)
+{
+    if (!st || !beta || st->beta.empty()) return MQ_ERR_ARG;
+    std::copy(st->beta.begin(), st->beta.end(), beta);
+    return MQ_OK;
+}
+
+extern "C" int mq_state_random_bonds(mq_state* st, const int32_t* rep_sample, double p_plus, uint64_t seed)
+{
+    if (!st) return MQ_ERR_ARG;
+    if (!(p_plus >= 0.0 && p_plus <= 1.0)) return mq_fail(st->ctx, MQ_ERR_ARG, "p_plus must be in [0,1]");
+    if (rep_sample) {
+        int ns = 0;
+        for (int r = 0; r < st->nrep; ++r) {
+            if (rep_sample[r] < 0) return mq_fail(st->ctx, MQ_ERR_ARG, "negative sample id");
+            ns = std::max(ns, rep_sample[r] + 1);
+        }
+        std::vector<mq_lattice*> lat_of(ns, nullptr);
+        for (int r = 0; r < st->nrep; ++r) {
+            auto*& l = lat_of[rep_sample[r]];
+            if (l && l != st->lat[r]) return mq_fail(st->ctx, MQ_ERR_ARG, "replicas of one sample must share a lattice");
+            l = st->lat[r];
+        }
+        dfree(st->d_bonds);
+        st->rep_sample.assign(rep_sample, rep_sample + st->nrep);
+        st->nsamples = ns;
+    }
+    int rc = ensure_bond_storage(st);
+    if (rc) return rc;
+    std::vector<int> first(st->nsamples, -1);
+    for (int r = 0; r < st->nrep; ++r)
+        if (first[st->rep_sample[r]] < 0) first[st->rep_sample[r]] = r;
+    for (int k = 0; k < st->nsamples; ++k) {
+        if (first[k] < 0) continue;
+        const mq_lattice* l = st->lat[first[k]];
+        Group g{const_cast<mq_lattice*>(l), first[k], 1, 0, 0};
+        if (l->kind != LAT_REGULAR) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "random bonds on graphs: use mq_state_set_bonds");
+        MQ_HIP(st->ctx, launch_bonds_regular(st, g, k, p_plus, key_of(seed)));
+    }
+    MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    return MQ_OK;
+}
+
+extern "C" int mq_state_set_bonds(mq_state* st, int sample, const int8_t* bonds)
+{
+    if (!st || !bonds || sample < 0 || sample >= st->nsamples) return MQ_ERR_ARG;
+    int rc = ensure_bond_storage(st);
+    if (rc) return rc;
+    int r0 = -1;
+    for (int r = 0; r < st->nrep; ++r)
+        if (st->rep_sample[r] == sample) { r0 = r; break; }
+    if (r0 < 0) return mq_fail(st->ctx, MQ_ERR_ARG, "no replica uses this sample");
+    const mq_lattice* l = st->lat[r0];
+    std::vector<int8_t> dev;
+    if (l->kind == LAT_REGULAR) {
+        const int64_t N = l->N, half = N / 2;
+        const int dim = l->dim;
+        dev.resize((size_t)(N * dim));
+        for (int64_t i = 0; i < N; ++i) {
+            const int c = regular_colour(dim, l->L, i);
+            for (int d = 0; d < dim; ++d) {
+                const int8_t b = bonds[i * dim + d];
+                if (b != 1 && b != -1) return mq_fail(st->ctx, MQ_ERR_ARG, "bonds must be +1 or -1");
+                dev[(size_t)(c * dim * half + d * half + (i >> 1))] = b;
+            }
+        }
+    } else {
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "graph couplings not supported yet");
+    }
+    MQ_HIP(st->ctx, hipMemcpy(st->d_bonds + st->sample_bond_off[sample], dev.data(), dev.size(), hipMemcpyHostToDevice));
+    return MQ_OK;
+}
+
+extern "C" int mq_state_init(mq_state* st, uint64_t seed)
+{
+    if (!st) return MQ_ERR_ARG;
+    for (const auto& g : st->groups) {
+        if (g.lat->kind == LAT_REGULAR) MQ_HIP(st->ctx, launch_init_regular(st, g, key_of(seed)));
+        else MQ_HIP(st->ctx, launch_init_csr(st, g, key_of(seed)));
+    }
+    MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    return MQ_OK;
+}
+
+extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t bytes)
+{
+    if (!st || !host || rep < 0 || rep >= st->nrep) return MQ_ERR_ARG;
+    const mq_lattice* l = st->lat[rep];
+    const int64_t N = l->N;
+    if (is_vec(st->p.model)) {
+        if (bytes != N * 12) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected 3 floats per site");
+        const float* h = static_cast<const float*>(host);
+        std::vector<float> dev((size_t)N * 4, 0.f);
+        for (int64_t i = 0; i < N; ++i) {
+            const int64_t p = dev_pos(l, i);
+            dev[p * 4 + 0] = h[3 * i];
+            dev[p * 4 + 1] = h[3 * i + 1];
+            dev[p * 4 + 2] = h[3 * i + 2];
+        }
+        MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), dev.size() * 4, hipMemcpyHostToDevice));
+    } else {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected one int8 per site");
+        const int8_t* h = static_cast<const int8_t*>(host);
+        std::vector<int8_t> dev((size_t)N);
+        for (int64_t i = 0; i < N; ++i) {
+            const int8_t v = h[i];
+            if (st->p.model == MQ_POTTS ? (v < 0 || v >= st->p.q) : (v != 1 && v != -1))
+                return mq_fail(st->ctx, MQ_ERR_ARG, "upload: spin value out of range");
+            dev[dev_pos(l, i)] = v;
+        }
+        MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), dev.size(), hipMemcpyHostToDevice));
+    }
+    return MQ_OK;
+}
+
+extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t bytes)
+{
+    if (!st || !host || rep < 0 || rep >= st->nrep) return MQ_ERR_ARG;
+    const mq_lattice* l = st->lat[rep];
+    const int64_t N = l->N;
+    MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    if (is_vec(st->p.model)) {
+        if (bytes != N * 12) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected 3 floats per site");
+        std::vector<float> dev((size_t)N * 4);
+        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + st->rep_off[rep], dev.size() * 4, hipMemcpyDeviceToHost));
+        float* h = static_cast<float*>(host);
+        for (int64_t i = 0; i < N; ++i) {
+            const int64_t p = dev_pos(l, i);
+            h[3 * i] = dev[p * 4];
+            h[3 * i + 1] = dev[p * 4 + 1];
+            h[3 * i + 2] = dev[p * 4 + 2];
+        }
+    } else {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
+        std::vector<int8_t> dev((size_t)N);
+        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + st->rep_off[rep], dev.size(), hipMemcpyDeviceToHost));
+        int8_t* h = static_cast<int8_t*>(host);
+        for (int64_t i = 0; i < N; ++i) h[i] = dev[dev_pos(l, i)];
+    }
+    return MQ_OK;
+}
+
+// ============================================================== updates
+int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0)
+{
+    const Key2 key = key_of(seed);
+    for (int t = 0; t < nsweeps; ++t) {
+        const uint64_t sw = sweep0 + (uint64_t)t;
+        if (sw >= (1ull << 31)) return mq_fail(st->ctx, MQ_ERR_ARG, "sweep counter exceeds 2^31");
+        for (const auto& g : st->groups) {
+            const int nc = g.lat->ncolors;
+            for (int c = 0; c < nc; ++c) {
+                const uint32_t phase = (uint32_t)(sw * (uint64_t)(g.lat->kind == LAT_REGULAR ? 2 : 64) + c);
+                hipError_t e = (g.lat->kind == LAT_REGULAR) ? launch_sweep_regular(st, g, c, phase, key)
+                                                            : launch_sweep_csr(st, g, c, phase, key);
+                if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
+            }
+        }
+    }
+    return MQ_OK;
+}
+
+extern "C" int mq_metropolis(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0)
+{
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (nsweeps < 0) return mq_fail(st->ctx, MQ_ERR_ARG, "nsweeps < 0");
+    // colour-ordered sweeps per group; groups are independent replicas
+    return mq_metropolis_async(st, nsweeps, seed, sweep0);
+}
+
+extern "C" int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t cluster0, int64_t* sizes)
+{
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (nclusters < 0) return mq_fail(st->ctx, MQ_ERR_ARG, "nclusters < 0");
+    if (nclusters == 0) return MQ_OK;
+    if (st->p.model == MQ_EA_ISING)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff is not defined for the +-J spin glass (frustrated bonds)");
+    if (cluster0 + (uint64_t)nclusters > (1ull << 32)) return mq_fail(st->ctx, MQ_ERR_ARG, "cluster counter exceeds 2^32");
+    rc = ensure_wolff_scratch(st, nclusters);
+    if (rc) return rc;
+    for (const auto& g : st->groups) {
+        hipError_t e = (g.lat->kind == LAT_REGULAR) ? launch_wolff_regular(st, g, nclusters, (uint32_t)cluster0, key_of(seed))
+                                                    : launch_wolff_csr(st, g, nclusters, (uint32_t)cluster0, key_of(seed));
+        if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("wolff launch: ") + hipGetErrorString(e));
+    }
+    if (sizes) {
+        MQ_HIP(st->ctx, hipMemcpyAsync(sizes, st->d_csize, sizeof(int64_t) * (size_t)st->nrep * nclusters,
+                                       hipMemcpyDeviceToHost, st->ctx->stream));
+        MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    }
+    return MQ_OK;
+}
+
+extern "C" int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, uint64_t step, double* mean_cluster)
+{
+    int rc = check_state(st);
+    if (rc) return rc;
+    std::vector<int64_t> sizes;
+    if (ncluster > 0) {
+        sizes.resize((size_t)st->nrep * ncluster);
+        rc = mq_wolff(st, ncluster, seed, step * (uint64_t)ncluster, mean_cluster ? sizes.data() : nullptr);
+        if (rc) return rc;
+    }
+    rc = mq_metropolis(st, nmetro, seed, step * (uint64_t)nmetro);
+    if (rc) return rc;
+    if (mean_cluster) {
+        for (int r = 0; r < st->nrep; ++r) {
+            double a = 0;
+            for (int c = 0; c < ncluster; ++c) a += (double)sizes[(size_t)r * ncluster + c];
+            mean_cluster[r] = a / ncluster;  // NaN for ncluster = 0, as emcs.h:108
+        }
+    }
+    return MQ_OK;
+}
+
+static int measure_sums(mq_state* st, std::vector<long long>& is, std::vector<double>& fs)
+{
+    hipStream_t s = st->ctx->stream;
+    MQ_HIP(st->ctx, zero64(st->d_isum, (int64_t)st->nrep * 8, s));
+    MQ_HIP(st->ctx, zero64(st->d_fsum, (int64_t)st->nrep * 8, s));
+    for (const auto& g : st->groups) {
+        hipError_t e = (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
+        if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
+    }
+    is.resize((size_t)st->nrep * 8);
+    fs.resize((size_t)st->nrep * 8);
+    MQ_HIP(st->ctx, hipMemcpyAsync(is.data(), st->d_isum, is.size() * 8, hipMemcpyDeviceToHost, s));
+    MQ_HIP(st->ctx, hipMemcpyAsync(fs.data(), st->d_fsum, fs.size() * 8, hipMemcpyDeviceToHost, s));
+    MQ_HIP(st->ctx, hipStreamSynchronize(s));
+    return MQ_OK;
+}
+
+extern "C" int mq_measure(mq_state* st, double* out)
+{
+    if (!st || !out) return MQ_ERR_ARG;
+    if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
+    std::vector<long long> is;
+    std::vector<double> fs;
+    int rc = measure_sums(st, is, fs);
+    if (rc) return rc;
+    for (int r = 0; r < st->nrep; ++r) {
+        const double N = (double)st->lat[r]->N;
+        const long long* I = &is[(size_t)r * 8];
+        const double* F = &fs[(size_t)r * 8];
+        double e = 0, m = 0;
+        if (st->p.model == MQ_POTTS) {
+            e = st->p.J * (double)I[0] / N;
+            long long mx = 0;
+            for (int q = 0; q < st->p.q; ++q) mx = std::max(mx, I[1 + q]);
+            m = ((double)st->p.q * (double)mx - N) / (N * (st->p.q - 1));  // Potts.h:40-65
+        } else if (is_int(st->p.model)) {
+            e = st->p.J * (double)I[0] / N;
+            m = std::fabs((double)I[1]) / N;
+        } else {
+            e = st->p.J * F[0] / N;
+            if (st->p.model == MQ_PHI4) {
+                const double b = st->beta[r];
+                e += (st->p.mass / b) * F[4] / N + (st->p.lambda / b) * F[5] / N;
+            }
+            m = std::sqrt(F[1] * F[1] + F[2] * F[2] + F[3] * F[3]) / N;
+        }
+        out[(size_t)r * MQ_NOBS + 0] = e;
+        out[(size_t)r * MQ_NOBS + 1] = m;
+        out[(size_t)r * MQ_NOBS + 2] = m * m;
+        out[(size_t)r * MQ_NOBS + 3] = m * m * m * m;
+    }
+    return MQ_OK;
+}
+
+extern "C" int mq_energy(mq_state* st, double* out)
+{
+    if (!st || !out) return MQ_ERR_ARG;
+    std::vector<double> obs((size_t)st->nrep * MQ_NOBS);
+    int rc = mq_measure(st, obs.data());
+    if (rc) return rc;
+    for (int r = 0; r < st->nrep; ++r) out[r] = obs[(size_t)r * MQ_NOBS] * (double)st->lat[r]->N;
+    return MQ_OK;
+}

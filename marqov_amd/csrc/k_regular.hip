@@ -1,0 +1,549 @@
+// Regular-lattice kernels: random start, checkerboard Metropolis sweeps and
+// FP64 observable sums for Ising / +-J EA / Potts (int8 spins) and
+// Heisenberg / Phi4 (FP32 float4 spins).
+//
+// Layout in HBM (per replica): the two checkerboard colours are stored as two
+// contiguous half-lattices, colour c of site i = (x0+x1+x2)&1, half index
+// h = i >> 1 (x0 fastest, so h = (x0>>1) + (L/2)*(x1 + L*x2)).  A colour-c
+// site at (row, k) has its four in-plane neighbours at half index h +- L/2
+// (x1) and h +- L*L/2 (x2) in the other colour, and its two x0 neighbours at k
+// and k-1+2p (p = x0 & 1) of the same row.  One sweep therefore streams two
+// reads and one write of the lattice.
+//
+// EA couplings per disorder sample: [colour][d][h] int8, the +x_d bond of each
+// site stored with that site; the -x_d bond of a site is the +x_d bond of its
+// -x_d neighbour.
+#include "mq_internal.hpp"
+
+namespace {
+
+struct RegGeom {
+    int L, H;
+    int64_t plane, half;
+};
+
+__device__ __forceinline__ RegGeom reg_geom(int dim, int L)
+{
+    RegGeom g;
+    g.L = L;
+    g.H = L >> 1;
+    g.plane = (int64_t)g.H * L;
+    g.half = (dim == 3) ? g.plane * L : g.plane;
+    return g;
+}
+
+// neighbour half-indices (other colour) of colour-c site h, slot order
+// +x0,-x0,+x1,-x1,+x2,-x2 (regular_hypercubic.h:31-66)
+template <int DIM>
+__device__ __forceinline__ void reg_nbrs(const RegGeom& G, int colour, int64_t h, int64_t nb[2 * DIM])
+{
+    const int64_t row = h / G.H;
+    const int k = (int)(h - row * G.H);
+    const int x1 = (int)(row % G.L);
+    const int x2 = (DIM == 3) ? (int)(row / G.L) : 0;
+    const int p = (colour + x1 + x2) & 1;
+    const int64_t rb = h - k;
+    nb[0] = rb + (p ? ((k + 1 == G.H) ? 0 : k + 1) : k);
+    nb[1] = rb + (p ? k : ((k == 0) ? G.H - 1 : k - 1));
+    nb[2] = (x1 == G.L - 1) ? h - (int64_t)(G.L - 1) * G.H : h + G.H;
+    nb[3] = (x1 == 0) ? h + (int64_t)(G.L - 1) * G.H : h - G.H;
+    if (DIM == 3) {
+        nb[4] = (x2 == G.L - 1) ? h - (int64_t)(G.L - 1) * G.plane : h + G.plane;
+        nb[5] = (x2 == 0) ? h + (int64_t)(G.L - 1) * G.plane : h - G.plane;
+    }
+}
+
+// natural index -> (colour, half index)
+template <int DIM>
+__device__ __forceinline__ int reg_colour_of(int L, int64_t i)
+{
+    const int64_t x0 = i % L;
+    const int64_t r = i / L;
+    int s = (int)x0 + (int)(r % L);
+    if (DIM == 3) s += (int)(r / L);
+    return s & 1;
+}
+
+// ----------------------------------------------------------------- init
+template <int DIM>
+__global__ void __launch_bounds__(256) k_init_int8(int8_t* __restrict__ base, int64_t stride, int L, int model, int Q,
+                                                   int rep0, Key2 key)
+{
+    const int r = blockIdx.y;
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t N = 2 * G.half;
+    const int64_t q4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q4 * 4 >= N) return;
+    const uint4 w = philox4x32_10((uint32_t)q4, 0u, (uint32_t)(rep0 + r), ST_INIT, key);
+    int8_t* s = base + r * stride;
+    for (int j = 0; j < 4; ++j) {
+        const int64_t i = q4 * 4 + j;
+        if (i >= N) break;
+        const uint32_t x = u4_get(w, j);
+        int8_t v;
+        if (model == MQ_POTTS) v = (int8_t)(((uint64_t)x * (uint64_t)Q) >> 32);
+        else v = (x >> 31) ? (int8_t)1 : (int8_t)-1;
+        s[reg_colour_of<DIM>(L, i) * G.half + (i >> 1)] = v;
+    }
+}
+
+// fresh uniform direction in the reference's component order (randomdir.h:45-65)
+__device__ __forceinline__ float3 rnddir_f(uint32_t w0, uint32_t w1)
+{
+    const float a0 = 2.0f * u01f(w0);
+    const float a1 = 2.0f * u01f(w1);
+    const float rho = sqrtf(fmaxf(a0 * (2.0f - a0), 0.0f));
+    float sn, cs;
+    sincospif(a1, &sn, &cs);
+    return make_float3(1.0f - a0, rho * sn, rho * cs);
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_init_f4(float4* __restrict__ base, int64_t stride4, int L, int rep0, Key2 key)
+{
+    const int r = blockIdx.y;
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t N = 2 * G.half;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint4 w = philox4x32_10((uint32_t)i, 0u, (uint32_t)(rep0 + r), ST_INIT, key);
+    const float3 v = rnddir_f(w.x, w.y);
+    base[r * stride4 + reg_colour_of<DIM>(L, i) * G.half + (i >> 1)] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// ----------------------------------------------------------- Ising / EA sweep
+// One thread updates 8 consecutive half-sites of the active colour with one
+// Philox block: 8 x 16-bit uniforms compared against the high half of the
+// 32-bit acceptance threshold; an equal high half (probability 2^-16) draws the
+// low 16 bits from the tie stream, so the decision equals a full 32-bit compare.
+template <int DIM, bool BONDS>
+__global__ void __launch_bounds__(256) k_sweep_ising(int8_t* __restrict__ base, int64_t stride,
+                                                     const int8_t* __restrict__ bonds,
+                                                     const int64_t* __restrict__ rep_bond_off,
+                                                     const uint32_t* __restrict__ thr, int tbl, int L, int colour,
+                                                     uint32_t phase, int rep0, Key2 key)
+{
+    __shared__ uint32_t s_thr[2 * 16];
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    if (threadIdx.x < 2 * tbl) s_thr[threadIdx.x] = thr[(int64_t)grep * 2 * tbl + threadIdx.x];
+    __syncthreads();
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t h0 = g << 3;
+    if (h0 >= G.half) return;
+    int8_t* own = base + r * stride + colour * G.half;
+    const int8_t* oth = base + r * stride + (1 - colour) * G.half;
+    const int8_t* bown = nullptr;
+    const int8_t* both = nullptr;
+    if (BONDS) {
+        const int8_t* bs = bonds + rep_bond_off[grep];
+        bown = bs + (int64_t)colour * DIM * G.half;
+        both = bs + (int64_t)(1 - colour) * DIM * G.half;
+    }
+    const uint4 w = philox4x32_10((uint32_t)g, phase, (uint32_t)grep, ST_METRO, key);
+    uint4 tie = make_uint4(0, 0, 0, 0);
+    bool have_tie = false;
+    constexpr int QMAX = 2 * DIM;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int64_t h = h0 + m;
+        if (h >= G.half) break;
+        int64_t nb[2 * DIM];
+        reg_nbrs<DIM>(G, colour, h, nb);
+        int hs = 0;
+#pragma unroll
+        for (int sl = 0; sl < 2 * DIM; ++sl) {
+            int b = 1;
+            if (BONDS) {
+                const int d = sl >> 1;
+                b = (sl & 1) ? both[(int64_t)d * G.half + nb[sl]] : bown[(int64_t)d * G.half + h];
+            }
+            hs += b * oth[nb[sl]];
+        }
+        const int s = own[h];
+        const int e = s * hs + QMAX;
+        const uint32_t hi16 = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+        const uint32_t thi = s_thr[2 * e];
+        bool acc = hi16 < thi;
+        if (hi16 == thi) {
+            if (!have_tie) {
+                tie = philox4x32_10((uint32_t)g, phase, (uint32_t)grep, ST_TIE, key);
+                have_tie = true;
+            }
+            acc = ((u4_get(tie, m >> 1) >> (16 * (m & 1))) & 0xFFFFu) < s_thr[2 * e + 1];
+        }
+        if (acc) own[h] = (int8_t)(-s);
+    }
+}
+
+// ----------------------------------------------------------------- Potts sweep
+// One Philox block per site: word 0 -> proposal in [0,Q) (Potts.h:171-177),
+// word 1 -> 32-bit uniform against the threshold of dE = k*(#new - #old).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_sweep_potts(int8_t* __restrict__ base, int64_t stride,
+                                                     const uint32_t* __restrict__ thr, int tbl, int L, int Q,
+                                                     int colour, uint32_t phase, int rep0, Key2 key)
+{
+    __shared__ uint32_t s_thr[2 * 16];
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    if (threadIdx.x < 2 * tbl) s_thr[threadIdx.x] = thr[(int64_t)grep * 2 * tbl + threadIdx.x];
+    __syncthreads();
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= G.half) return;
+    int8_t* own = base + r * stride + colour * G.half;
+    const int8_t* oth = base + r * stride + (1 - colour) * G.half;
+    int64_t nb[2 * DIM];
+    reg_nbrs<DIM>(G, colour, h, nb);
+    const uint4 w = philox4x32_10((uint32_t)h, phase, (uint32_t)grep, ST_METRO, key);
+    const int snew = (int)(((uint64_t)w.x * (uint64_t)Q) >> 32);
+    const int sold = own[h];
+    int cn = 0, co = 0;
+#pragma unroll
+    for (int sl = 0; sl < 2 * DIM; ++sl) {
+        const int v = oth[nb[sl]];
+        cn += (v == snew);
+        co += (v == sold);
+    }
+    const int e = cn - co + 2 * DIM;
+    const uint64_t T = ((uint64_t)s_thr[2 * e] << 16) | s_thr[2 * e + 1];
+    if ((uint64_t)w.y < T) own[h] = (int8_t)snew;
+}
+
+// ------------------------------------------------------- Heisenberg / Phi4
+// One Philox block per site: words 0,1 -> direction, word 2 -> acceptance.
+// Local arithmetic in FP32; acceptance u < exp(-beta*dE).
+template <int DIM, bool BONDS>
+__global__ void __launch_bounds__(256) k_sweep_o3(float4* __restrict__ base, int64_t stride4,
+                                                  const int8_t* __restrict__ bonds,
+                                                  const int64_t* __restrict__ rep_bond_off,
+                                                  const float* __restrict__ betaf, int L, int model, int proposal,
+                                                  float J, float delta, float lambda, float mass, int colour,
+                                                  uint32_t phase, int rep0, Key2 key)
+{
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= G.half) return;
+    float4* own = base + r * stride4 + colour * G.half;
+    const float4* oth = base + r * stride4 + (1 - colour) * G.half;
+    int64_t nb[2 * DIM];
+    reg_nbrs<DIM>(G, colour, h, nb);
+    float nx = 0.f, ny = 0.f, nz = 0.f;
+#pragma unroll
+    for (int sl = 0; sl < 2 * DIM; ++sl) {
+        const float4 v = oth[nb[sl]];
+        float b = 1.0f;
+        if (BONDS) {
+            const int8_t* bs = bonds + rep_bond_off[grep];
+            const int d = sl >> 1;
+            b = (sl & 1) ? (float)bs[(int64_t)(1 - colour) * DIM * G.half + (int64_t)d * G.half + nb[sl]]
+                         : (float)bs[(int64_t)colour * DIM * G.half + (int64_t)d * G.half + h];
+        }
+        nx += b * v.x;
+        ny += b * v.y;
+        nz += b * v.z;
+    }
+    const float4 so = own[h];
+    const uint4 w = philox4x32_10((uint32_t)h, phase, (uint32_t)grep, ST_METRO, key);
+    const float3 rd = rnddir_f(w.x, w.y);
+    float3 sn;
+    if (model == MQ_PHI4) {
+        sn = make_float3(so.x + 0.5f * rd.x, so.y + 0.5f * rd.y, so.z + 0.5f * rd.z);
+    } else if (proposal == MQ_PROPOSE_SMALL_ANGLE) {
+        float3 t = make_float3(so.x + delta * rd.x, so.y + delta * rd.y, so.z + delta * rd.z);
+        const float inv = rsqrtf(t.x * t.x + t.y * t.y + t.z * t.z);
+        sn = make_float3(t.x * inv, t.y * inv, t.z * inv);
+    } else {
+        sn = rd;
+    }
+    const float beta = betaf[grep];
+    float dE = J * ((sn.x - so.x) * nx + (sn.y - so.y) * ny + (sn.z - so.z) * nz);
+    if (model == MQ_PHI4) {
+        const float n2 = sn.x * sn.x + sn.y * sn.y + sn.z * sn.z;
+        const float o2 = so.x * so.x + so.y * so.y + so.z * so.z;
+        dE += (mass / beta) * (n2 - o2) + (lambda / beta) * ((n2 - 1.0f) * (n2 - 1.0f) - (o2 - 1.0f) * (o2 - 1.0f));
+    }
+    bool acc = dE <= 0.0f;
+    if (!acc) acc = u01f(w.z) < __expf(-beta * dE);
+    if (acc) own[h] = make_float4(sn.x, sn.y, sn.z, 0.0f);
+}
+
+// --------------------------------------------------------------- measure
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ void atomic_add_any(double* p, double v) { atomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add_any(long long* p, long long v)
+{
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void block_sum_atomic(T (&v)[K], T* out)
+{
+    __shared__ T red[K][4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        T x = wave_sum(v[k]);
+        if (lane == 0) red[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        T s = 0;
+        for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) s += red[threadIdx.x][wv];
+        atomic_add_any(out + threadIdx.x, s);
+    }
+}
+
+// isum[r*8 + 0] = sum over bonds of b s_i s_j (Ising/EA) or #equal bonds (Potts)
+// isum[r*8 + 1] = sum s (Ising/EA); Potts: isum[r*8 + 1 + q] = count of colour q
+template <int DIM, bool BONDS>
+__global__ void __launch_bounds__(256) k_measure_int8(const int8_t* __restrict__ base, int64_t stride,
+                                                      const int8_t* __restrict__ bonds,
+                                                      const int64_t* __restrict__ rep_bond_off, int L, int model,
+                                                      int Q, int rep0, unsigned long long* __restrict__ isum)
+{
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (h < G.half) {
+        const int8_t* c0 = base + r * stride;
+        const int8_t* c1 = c0 + G.half;
+        int64_t nb[2 * DIM];
+        reg_nbrs<DIM>(G, 0, h, nb);
+        const int s = c0[h];
+        if (model == MQ_POTTS) {
+            int eq = 0;
+#pragma unroll
+            for (int sl = 0; sl < 2 * DIM; ++sl) eq += (c1[nb[sl]] == s);
+            acc[0] = eq;
+            for (int q = 0; q < Q && q < 7; ++q) acc[1 + q] = (s == q) + (c1[h] == q);
+        } else {
+            int hs = 0;
+#pragma unroll
+            for (int sl = 0; sl < 2 * DIM; ++sl) {
+                int b = 1;
+                if (BONDS) {
+                    const int8_t* bs = bonds + rep_bond_off[grep];
+                    const int d = sl >> 1;
+                    b = (sl & 1) ? bs[(int64_t)DIM * G.half + (int64_t)d * G.half + nb[sl]] : bs[(int64_t)d * G.half + h];
+                }
+                hs += b * c1[nb[sl]];
+            }
+            acc[0] = s * hs;
+            acc[1] = s + c1[h];
+        }
+    }
+    block_sum_atomic<long long, 8>(acc, reinterpret_cast<long long*>(isum) + (int64_t)grep * 8);
+}
+
+// fsum[r*8 + 0] = sum over bonds b s_i.s_j, [1..3] = M, [4] = sum |s|^2, [5] = sum (|s|^2-1)^2
+template <int DIM, bool BONDS>
+__global__ void __launch_bounds__(256) k_measure_f4(const float4* __restrict__ base, int64_t stride4,
+                                                    const int8_t* __restrict__ bonds,
+                                                    const int64_t* __restrict__ rep_bond_off, int L, int rep0,
+                                                    double* __restrict__ fsum)
+{
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (h < G.half) {
+        const float4* c0 = base + r * stride4;
+        const float4* c1 = c0 + G.half;
+        int64_t nb[2 * DIM];
+        reg_nbrs<DIM>(G, 0, h, nb);
+        const float4 s = c0[h];
+        const float4 t = c1[h];
+        double bs = 0;
+#pragma unroll
+        for (int sl = 0; sl < 2 * DIM; ++sl) {
+            const float4 v = c1[nb[sl]];
+            double b = 1.0;
+            if (BONDS) {
+                const int8_t* bb = bonds + rep_bond_off[grep];
+                const int d = sl >> 1;
+                b = (sl & 1) ? bb[(int64_t)DIM * G.half + (int64_t)d * G.half + nb[sl]] : bb[(int64_t)d * G.half + h];
+            }
+            bs += b * ((double)s.x * v.x + (double)s.y * v.y + (double)s.z * v.z);
+        }
+        acc[0] = bs;
+        acc[1] = (double)s.x + t.x;
+        acc[2] = (double)s.y + t.y;
+        acc[3] = (double)s.z + t.z;
+        const double n2s = (double)s.x * s.x + (double)s.y * s.y + (double)s.z * s.z;
+        const double n2t = (double)t.x * t.x + (double)t.y * t.y + (double)t.z * t.z;
+        acc[4] = n2s + n2t;
+        acc[5] = (n2s - 1) * (n2s - 1) + (n2t - 1) * (n2t - 1);
+    }
+    block_sum_atomic<double, 6>(acc, fsum + (int64_t)grep * 8);
+}
+
+// EA couplings for one sample: natural bond b = i*dim + d drawn from
+// (seed, sample, ST_DISORDER), +1 with probability T/2^32.
+template <int DIM>
+__global__ void __launch_bounds__(256) k_bonds(int8_t* __restrict__ out, int L, uint32_t T, int sample, Key2 key)
+{
+    const RegGeom G = reg_geom(DIM, L);
+    const int64_t N = 2 * G.half;
+    const int64_t q4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q4 * 4 >= N * DIM) return;
+    const uint4 w = philox4x32_10((uint32_t)q4, 0u, (uint32_t)sample, ST_DISORDER, key);
+    for (int j = 0; j < 4; ++j) {
+        const int64_t b = q4 * 4 + j;
+        if (b >= N * DIM) break;
+        const int64_t i = b / DIM;
+        const int d = (int)(b - i * DIM);
+        out[(int64_t)reg_colour_of<DIM>(L, i) * DIM * G.half + (int64_t)d * G.half + (i >> 1)] =
+            (u4_get(w, j) < T) ? (int8_t)1 : (int8_t)-1;
+    }
+}
+
+inline dim3 grid_for(int64_t work, int block, int nrep)
+{
+    return dim3((unsigned)((work + block - 1) / block), (unsigned)nrep, 1);
+}
+
+inline bool is_vec(int model) { return model == MQ_HEISENBERG || model == MQ_PHI4; }
+
+}  // namespace
+
+hipError_t launch_init_regular(mq_state* st, const Group& g, Key2 key)
+{
+    const int L = g.lat->L, dim = g.lat->dim;
+    const int64_t N = g.lat->N;
+    hipStream_t s = st->ctx->stream;
+    if (is_vec(st->p.model)) {
+        float4* b = reinterpret_cast<float4*>(st->d_spins + g.off);
+        const int64_t s4 = g.stride / 16;
+        if (dim == 3) hipLaunchKernelGGL(k_init_f4<3>, grid_for(N, 256, g.nrep), dim3(256), 0, s, b, s4, L, g.rep0, key);
+        else hipLaunchKernelGGL(k_init_f4<2>, grid_for(N, 256, g.nrep), dim3(256), 0, s, b, s4, L, g.rep0, key);
+    } else {
+        int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
+        if (dim == 3)
+            hipLaunchKernelGGL(k_init_int8<3>, grid_for((N + 3) / 4, 256, g.nrep), dim3(256), 0, s, b, g.stride, L,
+                               st->p.model, st->p.q, g.rep0, key);
+        else
+            hipLaunchKernelGGL(k_init_int8<2>, grid_for((N + 3) / 4, 256, g.nrep), dim3(256), 0, s, b, g.stride, L,
+                               st->p.model, st->p.q, g.rep0, key);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key)
+{
+    const int L = g.lat->L, dim = g.lat->dim;
+    const int64_t half = g.lat->N / 2;
+    hipStream_t s = st->ctx->stream;
+    const int model = st->p.model;
+    const bool bonds = (model == MQ_EA_ISING) || (st->d_bonds != nullptr);
+    if (model == MQ_ISING || model == MQ_EA_ISING) {
+        int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
+        const dim3 grid = grid_for((half + 7) / 8, 256, g.nrep);
+        if (dim == 3) {
+            if (bonds)
+                hipLaunchKernelGGL((k_sweep_ising<3, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+            else
+                hipLaunchKernelGGL((k_sweep_ising<3, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+        } else {
+            if (bonds)
+                hipLaunchKernelGGL((k_sweep_ising<2, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+            else
+                hipLaunchKernelGGL((k_sweep_ising<2, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+        }
+    } else if (model == MQ_POTTS) {
+        int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
+        const dim3 grid = grid_for(half, 256, g.nrep);
+        if (dim == 3)
+            hipLaunchKernelGGL(k_sweep_potts<3>, grid, dim3(256), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
+                               colour, phase, g.rep0, key);
+        else
+            hipLaunchKernelGGL(k_sweep_potts<2>, grid, dim3(256), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
+                               colour, phase, g.rep0, key);
+    } else {
+        float4* b = reinterpret_cast<float4*>(st->d_spins + g.off);
+        const int64_t s4 = g.stride / 16;
+        const dim3 grid = grid_for(half, 256, g.nrep);
+        const float J = (float)st->p.J, delta = (float)st->p.delta, lam = (float)st->p.lambda, mass = (float)st->p.mass;
+        if (dim == 3) {
+            if (bonds)
+                hipLaunchKernelGGL((k_sweep_o3<3, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
+                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, g.rep0, key);
+            else
+                hipLaunchKernelGGL((k_sweep_o3<3, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds,
+                                   st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
+                                   colour, phase, g.rep0, key);
+        } else {
+            if (bonds)
+                hipLaunchKernelGGL((k_sweep_o3<2, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
+                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, g.rep0, key);
+            else
+                hipLaunchKernelGGL((k_sweep_o3<2, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds,
+                                   st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
+                                   colour, phase, g.rep0, key);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_measure_regular(mq_state* st, const Group& g)
+{
+    const int L = g.lat->L, dim = g.lat->dim;
+    const int64_t half = g.lat->N / 2;
+    hipStream_t s = st->ctx->stream;
+    const int model = st->p.model;
+    const bool bonds = (model == MQ_EA_ISING) || (st->d_bonds != nullptr);
+    const dim3 grid = grid_for(half, 256, g.nrep);
+    if (is_vec(model)) {
+        const float4* b = reinterpret_cast<const float4*>(st->d_spins + g.off);
+        const int64_t s4 = g.stride / 16;
+        if (dim == 3) {
+            if (bonds) hipLaunchKernelGGL((k_measure_f4<3, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
+            else hipLaunchKernelGGL((k_measure_f4<3, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
+        } else {
+            if (bonds) hipLaunchKernelGGL((k_measure_f4<2, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
+            else hipLaunchKernelGGL((k_measure_f4<2, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
+        }
+    } else {
+        const int8_t* b = reinterpret_cast<const int8_t*>(st->d_spins + g.off);
+        unsigned long long* is = reinterpret_cast<unsigned long long*>(st->d_isum);
+        if (dim == 3) {
+            if (bonds) hipLaunchKernelGGL((k_measure_int8<3, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds, st->d_rep_bond_off, L, model, st->p.q, g.rep0, is);
+            else hipLaunchKernelGGL((k_measure_int8<3, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds, st->d_rep_bond_off, L, model, st->p.q, g.rep0, is);
+        } else {
+            if (bonds) hipLaunchKernelGGL((k_measure_int8<2, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds, st->d_rep_bond_off, L, model, st->p.q, g.rep0, is);
+            else hipLaunchKernelGGL((k_measure_int8<2, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds, st->d_rep_bond_off, L, model, st->p.q, g.rep0, is);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key)
+{
+    const int L = g.lat->L, dim = g.lat->dim;
+    const int64_t N = g.lat->N;
+    double t = std::floor(std::ldexp(p_plus, 32));
+    uint32_t T = (p_plus <= 0) ? 0u : (t >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)t);
+    int8_t* out = st->d_bonds + st->sample_bond_off[sample];
+    const dim3 grid = grid_for((N * dim + 3) / 4, 256, 1);
+    if (dim == 3) hipLaunchKernelGGL(k_bonds<3>, grid, dim3(256), 0, st->ctx->stream, out, L, T, sample, key);
+    else hipLaunchKernelGGL(k_bonds<2>, grid, dim3(256), 0, st->ctx->stream, out, L, T, sample, key);
+    return hipGetLastError();
+}

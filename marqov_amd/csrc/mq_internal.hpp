@@ -1,0 +1,179 @@
+// Internal declarations shared by the HIP translation units of libmarqov_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/marqov_amd.h"
+
+// Philox stream ids (counter word 3).  The CPU replay in oracle/replay.c uses
+// the same layout.
+enum : uint32_t {
+    ST_INIT = 0,
+    ST_METRO = 1,
+    ST_TIE = 2,
+    ST_WSEED = 3,
+    ST_WBOND = 4,
+    ST_PT = 5,
+    ST_DISORDER = 6,
+};
+
+enum { LAT_REGULAR = 0, LAT_CSR = 1 };
+
+#define MQ_MAX_COLORS 64
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Key bumps are uniform per launch, so
+// the compiler keeps them in SGPRs; each round is two 32x32->64 multiplies and
+// two three-way xors per lane.
+struct Key2 {
+    uint32_t k0, k1;
+};
+
+__host__ __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, Key2 key)
+{
+    uint32_t k0 = key.k0, k1 = key.k1;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+__host__ __device__ __forceinline__ uint32_t u4_get(const uint4& v, int i)
+{
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+// [0,1) float from the top 24 bits
+__device__ __forceinline__ float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }
+
+// ---------------------------------------------------------------------------
+struct mq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    int num_cus = 256;
+};
+
+struct mq_lattice {
+    mq_ctx* ctx = nullptr;
+    int kind = LAT_REGULAR;
+    int dim = 0, L = 0;
+    int64_t N = 0;
+    // CSR graphs: vertices renumbered so that each colour class is contiguous
+    int ncolors = 2;
+    std::vector<int64_t> color_off;  // ncolors+1 (positions in the new numbering)
+    std::vector<int32_t> pos_of;     // original vertex -> new position
+    std::vector<int32_t> vert_of;    // new position -> original vertex
+    std::vector<int32_t> h_rp, h_ci; // original numbering (host copy)
+    int32_t* d_rp = nullptr;         // new numbering
+    int32_t* d_ci = nullptr;
+    int32_t* d_edge = nullptr;       // canonical edge id of each slot
+    int32_t* d_slot_of_orig = nullptr;
+    int maxdeg = 0;
+    int64_t nnz = 0;
+};
+
+// A run of consecutive replicas on the same lattice (uniform stride).
+struct Group {
+    mq_lattice* lat;
+    int rep0, nrep;
+    int64_t off;     // byte offset of replica rep0 in d_spins
+    int64_t stride;  // bytes per replica
+};
+
+struct mq_state {
+    mq_ctx* ctx = nullptr;
+    mq_model_params p{};
+    int nrep = 0;
+    std::vector<mq_lattice*> lat;
+    std::vector<Group> groups;
+    int site_bytes = 1;  // 1 (int8) or 16 (float4)
+    std::vector<int64_t> rep_off;
+    char* d_spins = nullptr;
+    int64_t spin_total = 0;
+    // per-replica temperatures and acceptance tables
+    std::vector<double> beta;
+    int tbl = 0;                 // entries per replica
+    uint32_t* d_thr = nullptr;   // [nrep][tbl] pairs (hi, lo)
+    uint32_t* d_wthr = nullptr;  // [nrep] Wolff add thresholds (integer models)
+    float* d_betaf = nullptr;    // [nrep]
+    double* d_betad = nullptr;   // [nrep]
+    // disorder (EA)
+    std::vector<int32_t> rep_sample;
+    int nsamples = 0;
+    int8_t* d_bonds = nullptr;
+    std::vector<int64_t> sample_bond_off;
+    int64_t* d_rep_bond_off = nullptr;  // [nrep] byte offset of the replica's sample
+    // Wolff scratch
+    int32_t* d_queue = nullptr;
+    uint32_t* d_visit = nullptr;
+    int64_t* d_csize = nullptr;
+    size_t csize_cap = 0;
+    std::vector<int64_t> rep_site_off;  // per replica site offset into queue/visit scratch
+    int64_t* d_rep_site_off = nullptr;
+    int64_t total_sites = 0;
+    // measurement
+    long long* d_isum = nullptr;  // [nrep][8]
+    double* d_fsum = nullptr;     // [nrep][8]
+    // parallel tempering
+    int32_t* d_temp_of_rep = nullptr;  // [nrep]
+    int32_t* d_rep_of_temp = nullptr;  // [nrep] (grouped by ntemp)
+    int32_t* d_pt_acc = nullptr;
+    double* d_energy = nullptr;        // [nrep]
+    std::vector<double> beta_ladder;   // per replica index at creation (temperature k)
+    double* d_ladder = nullptr;
+    int pt_ntemp = 0;
+};
+
+// --------------------------------------------------------------------------
+// error helpers
+int mq_fail(mq_ctx* ctx, int code, const std::string& msg);
+#define MQ_HIP(ctx, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return mq_fail((ctx), MQ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// --------------------------------------------------------------------------
+// kernel launchers (each returns hipError_t of the launch)
+hipError_t launch_init_regular(mq_state* st, const Group& g, Key2 key);
+hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
+hipError_t launch_measure_regular(mq_state* st, const Group& g);
+hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
+hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key);
+
+hipError_t launch_init_csr(mq_state* st, const Group& g, Key2 key);
+hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
+hipError_t launch_measure_csr(mq_state* st, const Group& g);
+hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
+
+hipError_t launch_energy_all(mq_state* st);
+hipError_t launch_pt_swap(mq_state* st, int ntemp, int parity, uint32_t step, Key2 key);
+hipError_t launch_renorm(mq_state* st);
+
+// host-side layout helpers
+inline int regular_colour(int dim, int L, int64_t i)
+{
+    int64_t s = 0;
+    for (int d = 0; d < dim; ++d) {
+        s += i % L;
+        i /= L;
+    }
+    return (int)(s & 1);
+}

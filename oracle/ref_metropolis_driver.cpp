@@ -1,0 +1,81 @@
+/*
+ * oracle/ref_metropolis_driver.cpp -- drives the reference's own Metropolis
+ * move (test infrastructure; built only into oracle/_ref/).
+ *
+ * This program is ours; it includes the reference headers where they lie
+ * (src/libmarqov/metropolis.h, src/libmarqov/rngcache.h,
+ * src/hamiltonian/util/termcollection.h, src/lattice/util/regular_lattice.h)
+ * and instantiates MARQOV::Metropolis<H, RegularLattice>::move exactly as
+ * EMCS does (src/libmarqov/emcs.h:93-106), with the RNGCache<std::mt19937_64>
+ * the reference Core owns (core.h:372).  The Hamiltonians below are declared
+ * the way a MARQOV user declares one (an `interactions` array of
+ * Standard_Interaction terms, as src/hamiltonian/Ising.h:62-80 does); the
+ * initial states follow Ising::initstatespace (Ising.h:99-107) and
+ * Core::init_hot (core.h:774-782).
+ *
+ * The reference's Wolff/EMCS/Core headers need MARQOV::Space from the
+ * HDF5-bound core.h and are not built here.
+ *
+ * usage: ref_metropolis ising|heis DIM L BETA J NSWEEPS SEED
+ * prints the final state, one site per line, in site-index order.
+ */
+#include <array>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "util/svmath.h"
+#include "metropolis.h"
+#include "hamiltonian/util/termcollection.h"
+#include "lattice/util/regular_lattice.h"
+
+template <class SV>
+struct UserHam {
+    double J;
+    static constexpr int SymD = std::tuple_size<SV>::value;
+    typedef SV StateVector;
+    std::array<Standard_Interaction<SV>*, 1> interactions;
+    explicit UserHam(double j) : J(j), interactions{{new Standard_Interaction<SV>(j)}} {}
+    ~UserHam() { delete interactions[0]; }
+};
+
+template <class SV>
+static void run(int dim, int L, double beta, double J, int nsweeps, unsigned long long seed, bool ising)
+{
+    RegularLattice lat(L, dim);
+    UserHam<SV> ham(J);
+    RNGCache<std::mt19937_64> rng(seed);
+    std::vector<SV> s(lat.size());
+    for (std::size_t i = 0; i < lat.size(); ++i) {
+        if (ising) s[i][0] = (rng.real() > 0.5) ? 1 : -1;
+        else set_to_rnddir(rng, s[i]);
+    }
+    for (int t = 0; t < nsweeps; ++t)
+        for (std::size_t i = 0; i < lat.size(); ++i) {
+            const int rsite = rng.integer(lat.size());
+            MARQOV::Metropolis<UserHam<SV>, RegularLattice>::move(ham, lat, s, rng, beta, rsite);
+        }
+    for (std::size_t i = 0; i < lat.size(); ++i) {
+        for (int k = 0; k < UserHam<SV>::SymD; ++k) std::printf(k ? " %.17g" : "%.17g", double(s[i][k]));
+        std::printf("\n");
+    }
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 8) {
+        std::fprintf(stderr, "usage: %s ising|heis DIM L BETA J NSWEEPS SEED\n", argv[0]);
+        return 2;
+    }
+    const bool ising = std::strcmp(argv[1], "ising") == 0;
+    const int dim = std::atoi(argv[2]);
+    const int L = std::atoi(argv[3]);
+    const double beta = std::atof(argv[4]);
+    const double J = std::atof(argv[5]);
+    const int nsweeps = std::atoi(argv[6]);
+    const unsigned long long seed = std::strtoull(argv[7], nullptr, 10);
+    if (ising) run<std::array<int, 1>>(dim, L, beta, J, nsweeps, seed, true);
+    else run<std::array<double, 3>>(dim, L, beta, J, nsweeps, seed, false);
+    return 0;
+}

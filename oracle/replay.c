@@ -1,0 +1,388 @@
+/*
+ * oracle/replay.c -- CPU replay of the GPU update schedule (test infrastructure).
+ *
+ * Only tests/, the smoke check and the bench's cpu_baseline leg may load this
+ * library, and only as the checker; the product path never calls it.
+ *
+ * What it restates, per function:
+ *  - orc_thresholds_*: the integer acceptance tables the library builds on the
+ *    host.  Acceptance follows the reference rule "accept if dE <= 0, else with
+ *    probability exp(-beta*dE)" (src/libmarqov/metropolishelpers.h:109-142,
+ *    metropolis.h:60-64), with the probability quantised to 32 bits.
+ *  - orc_ising_sweeps: a checkerboard Metropolis sweep of Ising / +-J
+ *    Edwards-Anderson spins on a periodic hypercubic lattice.  dE follows
+ *    compute_interactionenergydiff (metropolishelpers.h:598-623) with the
+ *    bond-weighted neighbourhood of nbrhoodloop (metropolishelpers.h:456-487):
+ *    dE = J * (s_new - s_old) * sum_j b_ij s_j.  Site order differs from the
+ *    reference's random-site order (emcs.h:93-106) by design: all sites of one
+ *    colour, then all of the other.
+ *  - orc_wolff_ising: single-cluster Wolff for Ising (Ising.h:122-169); a site
+ *    joins through an active bond with p = -expm1(2*beta*J) (Ising.h:139).
+ *    Bond activations come from per-bond counters, so the cluster is the
+ *    connected component of active bonds and does not depend on visit order.
+ *  - orc_potts_*: Potts Metropolis (Potts.h:87-107,165-178) and Wolff
+ *    (Potts.h:189-322) on the same schedule.
+ *  - orc_energy_* / orc_mag_*: observables (hamiltonian/util/observables.h:32-66,
+ *    173-231; Potts.h:40-65).
+ * Lattice indexing follows FixedDimRegularHypercubic / RegularLattice
+ * (src/lattice/regular_hypercubic.h:31-66, src/lattice/util/regular_lattice.h:54-89):
+ * site index sum_j x_j L^j with x_0 fastest, neighbour order +x0,-x0,+x1,-x1,...
+ *
+ * Random-number layout (identical on the GPU):
+ *   key = 64-bit seed; counter = (c0, c1, replica, stream)
+ *   STREAM_INIT  : c0 = site>>2, c1 = 0, word site&3
+ *   STREAM_METRO : c0 = (site>>1)>>3, c1 = 2*sweep+colour, 16-bit half (site>>1)&7
+ *   STREAM_TIE   : same counter as METRO, low 16 bits of the 32-bit uniform
+ *   STREAM_WSEED : c0 = 0, c1 = cluster number
+ *   STREAM_WBOND : c0 = bond>>2 with bond = site*dim+d, c1 = cluster number, word bond&3
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "philox.h"
+
+enum { ST_INIT = 0, ST_METRO = 1, ST_TIE = 2, ST_WSEED = 3, ST_WBOND = 4, ST_PT = 5, ST_DISORDER = 6 };
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* ---------------------------------------------------------------- lattice */
+
+static void coords(int dim, int L, int64_t i, int x[3])
+{
+    for (int d = 0; d < dim; ++d) { x[d] = (int)(i % L); i /= L; }
+}
+
+static int64_t nbr(int dim, int L, int64_t i, int d, int sign)
+{
+    int64_t pw = 1;
+    for (int k = 0; k < d; ++k) pw *= L;
+    int x[3];
+    coords(dim, L, i, x);
+    int xn = x[d] + sign;
+    if (xn == L) xn = 0;
+    if (xn < 0) xn = L - 1;
+    return i + (int64_t)(xn - x[d]) * pw;
+}
+
+EXPORT int64_t orc_neighbour(int dim, int L, int64_t i, int slot)
+{
+    /* slot order +x0,-x0,+x1,-x1,... as in regular_hypercubic.h:31-66 */
+    return nbr(dim, L, i, slot >> 1, (slot & 1) ? -1 : +1);
+}
+
+static int colour_of(int dim, int L, int64_t i)
+{
+    int x[3];
+    coords(dim, L, i, x);
+    int s = 0;
+    for (int d = 0; d < dim; ++d) s += x[d];
+    return s & 1;
+}
+
+/* ------------------------------------------------------------- thresholds */
+
+static void split_threshold(double p, uint32_t* hi, uint32_t* lo)
+{
+    if (!(p < 1.0)) { *hi = 65536u; *lo = 0u; return; }
+    double t = floor(ldexp(p, 32));
+    uint64_t T = (t <= 0.0) ? 0u : (uint64_t)t;
+    if (T > 0xFFFFFFFFull) T = 0xFFFFFFFFull;
+    *hi = (uint32_t)(T >> 16);
+    *lo = (uint32_t)(T & 0xFFFFu);
+}
+
+/* Ising / EA: entry e = q + qmax, q = s_i * sum_j b_ij s_j, dE = -2 J q. */
+EXPORT void orc_thresholds_ising(double beta, double J, int qmax, uint32_t* thi, uint32_t* tlo)
+{
+    for (int e = 0; e <= 2 * qmax; ++e) {
+        int q = e - qmax;
+        double dE = -2.0 * J * (double)q;
+        double p = (dE <= 0.0) ? 1.0 : exp(-beta * dE);
+        split_threshold(p, &thi[e], &tlo[e]);
+    }
+}
+
+/* Potts: entry e = (n_new - n_old) + zmax, dE = k * (n_new - n_old). */
+EXPORT void orc_thresholds_potts(double beta, double k, int zmax, uint32_t* thi, uint32_t* tlo)
+{
+    for (int e = 0; e <= 2 * zmax; ++e) {
+        double dE = k * (double)(e - zmax);
+        double p = (dE <= 0.0) ? 1.0 : exp(-beta * dE);
+        split_threshold(p, &thi[e], &tlo[e]);
+    }
+}
+
+/* Wolff add probability as a 32-bit threshold: add iff u32 < T. */
+EXPORT uint32_t orc_wolff_threshold(double p)
+{
+    if (!(p > 0.0)) return 0u;
+    double t = floor(ldexp(p, 32));
+    if (t >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)t;
+}
+
+/* ------------------------------------------------------------------- init */
+
+EXPORT void orc_init_ising(int64_t N, int8_t* s, uint64_t seed, uint32_t rep)
+{
+    for (int64_t i = 0; i < N; ++i) {
+        uint32_t w[4];
+        orc_philox_u64key((uint32_t)(i >> 2), 0u, rep, ST_INIT, seed, w);
+        s[i] = (w[i & 3] >> 31) ? (int8_t)1 : (int8_t)-1;
+    }
+}
+
+EXPORT void orc_init_potts(int64_t N, int8_t* s, int Q, uint64_t seed, uint32_t rep)
+{
+    for (int64_t i = 0; i < N; ++i) {
+        uint32_t w[4];
+        orc_philox_u64key((uint32_t)(i >> 2), 0u, rep, ST_INIT, seed, w);
+        s[i] = (int8_t)(((uint64_t)w[i & 3] * (uint64_t)Q) >> 32);
+    }
+}
+
+/* EA +-J couplings with P(+1) = p_plus: bonds[i*dim+d] couples i and i+e_d. */
+EXPORT void orc_init_bonds(int dim, int64_t N, int8_t* bonds, double p_plus, uint64_t seed, uint32_t sample)
+{
+    uint32_t T = orc_wolff_threshold(p_plus);
+    for (int64_t b = 0; b < N * dim; ++b) {
+        uint32_t w[4];
+        orc_philox_u64key((uint32_t)(b >> 2), 0u, sample, ST_DISORDER, seed, w);
+        bonds[b] = (w[b & 3] < T) ? (int8_t)1 : (int8_t)-1;
+    }
+}
+
+/* ------------------------------------------------------------ Metropolis */
+
+static int8_t bond_of(const int8_t* bonds, int dim, int L, int64_t i, int slot)
+{
+    if (!bonds) return 1;
+    int d = slot >> 1;
+    if ((slot & 1) == 0) return bonds[i * dim + d];
+    int64_t j = nbr(dim, L, i, d, -1);
+    return bonds[j * dim + d];
+}
+
+static int accept16(uint32_t hi16, uint32_t thi, uint32_t tlo, uint32_t g, uint32_t phase,
+                    uint32_t rep, uint64_t seed, int m)
+{
+    if (hi16 < thi) return 1;
+    if (hi16 > thi) return 0;
+    uint32_t t[4];
+    orc_philox_u64key(g, phase, rep, ST_TIE, seed, t);
+    uint32_t lo16 = (t[m >> 1] >> (16 * (m & 1))) & 0xFFFFu;
+    return lo16 < tlo;
+}
+
+EXPORT void orc_ising_sweeps(int dim, int L, int8_t* s, const int8_t* bonds,
+                             const uint32_t* thi, const uint32_t* tlo,
+                             int nsweeps, uint64_t seed, uint32_t sweep0, uint32_t rep)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    const int qmax = 2 * dim;
+    for (int t = 0; t < nsweeps; ++t) {
+        for (int c = 0; c < 2; ++c) {
+            uint32_t phase = 2u * (sweep0 + (uint32_t)t) + (uint32_t)c;
+            for (int64_t i = 0; i < N; ++i) {
+                if (colour_of(dim, L, i) != c) continue;
+                int64_t h = i >> 1;
+                uint32_t g = (uint32_t)(h >> 3);
+                int m = (int)(h & 7);
+                int hsum = 0;
+                for (int slot = 0; slot < 2 * dim; ++slot) {
+                    int64_t j = orc_neighbour(dim, L, i, slot);
+                    hsum += bond_of(bonds, dim, L, i, slot) * s[j];
+                }
+                int q = s[i] * hsum;
+                int e = q + qmax;
+                uint32_t w[4];
+                orc_philox_u64key(g, phase, rep, ST_METRO, seed, w);
+                uint32_t hi16 = (w[m >> 1] >> (16 * (m & 1))) & 0xFFFFu;
+                if (accept16(hi16, thi[e], tlo[e], g, phase, rep, seed, m)) s[i] = (int8_t)-s[i];
+            }
+        }
+    }
+}
+
+/* Potts Metropolis: proposal uniform in [0,Q) (may equal the old value, as in
+ * Potts.h:171-177); dE = k*(#nbr==new - #nbr==old) (Potts.h:87-107).
+ * Per site one Philox block: word 0 -> proposal, word 1 -> 32-bit uniform.
+ * counter: c0 = site>>1 (colour-half index), c1 = 2*sweep+colour. */
+EXPORT void orc_potts_sweeps(int dim, int L, int8_t* s, int Q, const uint32_t* thi, const uint32_t* tlo,
+                             int nsweeps, uint64_t seed, uint32_t sweep0, uint32_t rep)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    const int zmax = 2 * dim;
+    for (int t = 0; t < nsweeps; ++t) {
+        for (int c = 0; c < 2; ++c) {
+            uint32_t phase = 2u * (sweep0 + (uint32_t)t) + (uint32_t)c;
+            for (int64_t i = 0; i < N; ++i) {
+                if (colour_of(dim, L, i) != c) continue;
+                uint32_t w[4];
+                orc_philox_u64key((uint32_t)(i >> 1), phase, rep, ST_METRO, seed, w);
+                int snew = (int)(((uint64_t)w[0] * (uint64_t)Q) >> 32);
+                int sold = s[i];
+                int nn = 0, no = 0;
+                for (int slot = 0; slot < 2 * dim; ++slot) {
+                    int sj = s[orc_neighbour(dim, L, i, slot)];
+                    nn += (sj == snew);
+                    no += (sj == sold);
+                }
+                int e = nn - no + zmax;
+                uint64_t T = ((uint64_t)thi[e] << 16) | tlo[e];
+                if ((uint64_t)w[1] < T) s[i] = (int8_t)snew;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ Wolff */
+
+static int32_t* g_stack = NULL;
+static int64_t g_stack_n = 0;
+static uint8_t* g_mark = NULL;
+static int64_t g_mark_n = 0;
+
+static int ensure_work(int64_t N)
+{
+    if (g_stack_n < N) {
+        free(g_stack);
+        g_stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
+        g_stack_n = N;
+    }
+    if (g_mark_n < N) {
+        free(g_mark);
+        g_mark = (uint8_t*)calloc((size_t)N, 1);
+        g_mark_n = N;
+    }
+    return g_stack && g_mark;
+}
+
+static uint32_t bond_uniform(int dim, int L, int64_t i, int slot, uint32_t ncl, uint32_t rep, uint64_t seed)
+{
+    int d = slot >> 1;
+    int64_t owner = (slot & 1) ? nbr(dim, L, i, d, -1) : i;
+    int64_t b = owner * dim + d;
+    uint32_t w[4];
+    orc_philox_u64key((uint32_t)(b >> 2), ncl, rep, ST_WBOND, seed, w);
+    return w[b & 3];
+}
+
+/* Returns the cluster sizes in sizes[nclusters]. */
+EXPORT int orc_wolff_ising(int dim, int L, int8_t* s, uint32_t tw, int nclusters,
+                           uint64_t seed, uint32_t cl0, uint32_t rep, int64_t* sizes)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    if (!ensure_work(N)) return -1;
+    for (int c = 0; c < nclusters; ++c) {
+        uint32_t ncl = cl0 + (uint32_t)c;
+        uint32_t w[4];
+        orc_philox_u64key(0u, ncl, rep, ST_WSEED, seed, w);
+        int64_t seedsite = (int64_t)(((uint64_t)w[0] * (uint64_t)N) >> 32);
+        int8_t val = s[seedsite];
+        int64_t top = 0, size = 1;
+        g_stack[top++] = (int32_t)seedsite;
+        g_mark[seedsite] = 1;
+        while (top > 0) {
+            int64_t cur = g_stack[--top];
+            for (int slot = 0; slot < 2 * dim; ++slot) {
+                int64_t j = orc_neighbour(dim, L, cur, slot);
+                if (g_mark[j] || s[j] != val) continue;
+                if (bond_uniform(dim, L, cur, slot, ncl, rep, seed) < tw) {
+                    g_mark[j] = 1;
+                    g_stack[top++] = (int32_t)j;
+                    ++size;
+                }
+            }
+        }
+        /* flip the cluster and clear marks */
+        for (int64_t i = 0; i < N; ++i)
+            if (g_mark[i]) { s[i] = (int8_t)-s[i]; g_mark[i] = 0; }
+        if (sizes) sizes[c] = size;
+    }
+    return 0;
+}
+
+/* Potts Wolff: new colour uniform among the Q-1 others (Potts.h:221-229),
+ * drawn from word 1 of the seed block; neighbours equal to the seed's old
+ * colour join with p = 1 - exp(-beta*|k|) (Potts.h:237-242 with cpl = k*(-1/2)
+ * fed to the pref=2 rule of wolff.h:35-39). */
+EXPORT int orc_wolff_potts(int dim, int L, int8_t* s, int Q, uint32_t tw, int nclusters,
+                           uint64_t seed, uint32_t cl0, uint32_t rep, int64_t* sizes)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    if (!ensure_work(N)) return -1;
+    for (int c = 0; c < nclusters; ++c) {
+        uint32_t ncl = cl0 + (uint32_t)c;
+        uint32_t w[4];
+        orc_philox_u64key(0u, ncl, rep, ST_WSEED, seed, w);
+        int64_t seedsite = (int64_t)(((uint64_t)w[0] * (uint64_t)N) >> 32);
+        int oldq = s[seedsite];
+        int newq = (oldq + 1 + (int)(((uint64_t)w[1] * (uint64_t)(Q - 1)) >> 32)) % Q;
+        int64_t top = 0, size = 1;
+        g_stack[top++] = (int32_t)seedsite;
+        g_mark[seedsite] = 1;
+        while (top > 0) {
+            int64_t cur = g_stack[--top];
+            for (int slot = 0; slot < 2 * dim; ++slot) {
+                int64_t j = orc_neighbour(dim, L, cur, slot);
+                if (g_mark[j] || s[j] != oldq) continue;
+                if (bond_uniform(dim, L, cur, slot, ncl, rep, seed) < tw) {
+                    g_mark[j] = 1;
+                    g_stack[top++] = (int32_t)j;
+                    ++size;
+                }
+            }
+        }
+        for (int64_t i = 0; i < N; ++i)
+            if (g_mark[i]) { s[i] = (int8_t)newq; g_mark[i] = 0; }
+        if (sizes) sizes[c] = size;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------ observables */
+
+/* sum over bonds (each once) of b_ij s_i s_j */
+EXPORT int64_t orc_bond_sum_ising(int dim, int L, const int8_t* s, const int8_t* bonds)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    int64_t acc = 0;
+    for (int64_t i = 0; i < N; ++i)
+        for (int d = 0; d < dim; ++d) {
+            int64_t j = nbr(dim, L, i, d, +1);
+            int b = bonds ? bonds[i * dim + d] : 1;
+            acc += (int64_t)b * s[i] * s[j];
+        }
+    return acc;
+}
+
+EXPORT int64_t orc_mag_sum_ising(int64_t N, const int8_t* s)
+{
+    int64_t m = 0;
+    for (int64_t i = 0; i < N; ++i) m += s[i];
+    return m;
+}
+
+/* number of equal-colour bonds (each once) */
+EXPORT int64_t orc_bond_sum_potts(int dim, int L, const int8_t* s)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    int64_t acc = 0;
+    for (int64_t i = 0; i < N; ++i)
+        for (int d = 0; d < dim; ++d) acc += (s[i] == s[nbr(dim, L, i, d, +1)]);
+    return acc;
+}
+
+/* Philox4x32-10 exported for the known-answer tests. */
+EXPORT void orc_philox4x32_10_export(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
+{
+    orc_philox4x32_10(ctr, key, out);
+}

@@ -1,0 +1,168 @@
+"""ctypes binding of the oracle (oracle/liboracle.so): the checker, never the product."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_METRO = os.path.join(ORACLE_DIR, "_ref", "ref_metropolis")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+_D = ctypes.c_double
+
+ISING, EA, POTTS, HEIS, PHI4 = 1, 2, 3, 4, 5
+
+_SIG = {
+    "orc_neighbour": (_I64, [_I, _I, _I64, _I]),
+    "orc_thresholds_ising": (None, [_D, _D, _I, _P, _P]),
+    "orc_thresholds_potts": (None, [_D, _D, _I, _P, _P]),
+    "orc_wolff_threshold": (_U32, [_D]),
+    "orc_init_ising": (None, [_I64, _P, _U64, _U32]),
+    "orc_init_potts": (None, [_I64, _P, _I, _U64, _U32]),
+    "orc_init_bonds": (None, [_I, _I64, _P, _D, _U64, _U32]),
+    "orc_ising_sweeps": (None, [_I, _I, _P, _P, _P, _P, _I, _U64, _U32, _U32]),
+    "orc_potts_sweeps": (None, [_I, _I, _P, _I, _P, _P, _I, _U64, _U32, _U32]),
+    "orc_wolff_ising": (_I, [_I, _I, _P, _U32, _I, _U64, _U32, _U32, _P]),
+    "orc_wolff_potts": (_I, [_I, _I, _P, _I, _U32, _I, _U64, _U32, _U32, _P]),
+    "orc_bond_sum_ising": (_I64, [_I, _I, _P, _P]),
+    "orc_mag_sum_ising": (_I64, [_I64, _P]),
+    "orc_bond_sum_potts": (_I64, [_I, _I, _P]),
+    "ref_run_regular": (_I, [_I, _I, _I, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
+    "ref_run_csr": (_I, [_I, _I64, _P, _P, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
+    "ref_mt19937_64_nth": (_U64, [_U64, _I64]),
+    "ref_rngcache_draws": (None, [_U64, _I, _U64, _I64, _P]),
+    "ref_bench_regular": (_I, [_I, _I, _I, _D, _I, _D, _I, _I, _I, _U64, _I, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True, capture_output=True)
+        l = ctypes.CDLL(LIB)
+        for k, (r, a) in _SIG.items():
+            f = getattr(l, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = l
+    return _lib
+
+
+def p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def philox(ctr, key):
+    """Philox4x32-10 through the oracle's C code (via a tiny numpy-free helper)."""
+    from ctypes import c_uint32
+    c = (c_uint32 * 4)(*ctr)
+    k = (c_uint32 * 2)(*key)
+    o = (c_uint32 * 4)()
+    _philox_fn()(c, k, o)
+    return list(o)
+
+
+_ph = None
+
+
+def _philox_fn():
+    global _ph
+    if _ph is None:
+        f = lib().orc_philox4x32_10_export
+        f.restype = None
+        f.argtypes = [_P, _P, _P]
+        _ph = f
+    return _ph
+
+
+def thresholds_ising(beta, J, qmax):
+    hi = np.zeros(2 * qmax + 1, np.uint32)
+    lo = np.zeros(2 * qmax + 1, np.uint32)
+    lib().orc_thresholds_ising(beta, J, qmax, p(hi), p(lo))
+    return hi, lo
+
+
+def thresholds_potts(beta, k, zmax):
+    hi = np.zeros(2 * zmax + 1, np.uint32)
+    lo = np.zeros(2 * zmax + 1, np.uint32)
+    lib().orc_thresholds_potts(beta, k, zmax, p(hi), p(lo))
+    return hi, lo
+
+
+def init_ising(N, seed, rep):
+    s = np.zeros(N, np.int8)
+    lib().orc_init_ising(N, p(s), seed, rep)
+    return s
+
+
+def init_potts(N, Q, seed, rep):
+    s = np.zeros(N, np.int8)
+    lib().orc_init_potts(N, p(s), Q, seed, rep)
+    return s
+
+
+def init_bonds(dim, N, p_plus, seed, sample):
+    b = np.zeros(N * dim, np.int8)
+    lib().orc_init_bonds(dim, N, p(b), p_plus, seed, sample)
+    return b
+
+
+def ising_sweeps(dim, L, s, beta, J, nsweeps, seed, sweep0, rep, bonds=None):
+    hi, lo = thresholds_ising(beta, J, 2 * dim)
+    s = np.ascontiguousarray(s, np.int8).copy()
+    lib().orc_ising_sweeps(dim, L, p(s), None if bonds is None else p(np.ascontiguousarray(bonds, np.int8)),
+                           p(hi), p(lo), nsweeps, seed, sweep0, rep)
+    return s
+
+
+def potts_sweeps(dim, L, s, Q, beta, k, nsweeps, seed, sweep0, rep):
+    hi, lo = thresholds_potts(beta, k, 2 * dim)
+    s = np.ascontiguousarray(s, np.int8).copy()
+    lib().orc_potts_sweeps(dim, L, p(s), Q, p(hi), p(lo), nsweeps, seed, sweep0, rep)
+    return s
+
+
+def wolff_ising(dim, L, s, beta, J, nclusters, seed, cl0, rep):
+    s = np.ascontiguousarray(s, np.int8).copy()
+    tw = lib().orc_wolff_threshold(-np.expm1(2.0 * beta * J))
+    sizes = np.zeros(nclusters, np.int64)
+    lib().orc_wolff_ising(dim, L, p(s), tw, nclusters, seed, cl0, rep, p(sizes))
+    return s, sizes
+
+
+def wolff_potts(dim, L, s, Q, beta, k, nclusters, seed, cl0, rep):
+    s = np.ascontiguousarray(s, np.int8).copy()
+    tw = lib().orc_wolff_threshold(-np.expm1(beta * k))
+    sizes = np.zeros(nclusters, np.int64)
+    lib().orc_wolff_potts(dim, L, p(s), Q, tw, nclusters, seed, cl0, rep, p(sizes))
+    return s, sizes
+
+
+def ref_run(model, dim, L, beta, J=-1.0, Q=3, lam=4.5, mass=1.0, ncluster=0, nmetro=1, nwarm=0, nmeas=0,
+            seed=1, init=True, spins=None, bonds=None):
+    """Reference-faithful CPU restatement: returns (spins, obs[nmeas,2], mean_cluster[nmeas])."""
+    N = L ** dim
+    vec = model in (HEIS, PHI4)
+    is_ = np.zeros(N, np.int32)
+    ds = np.zeros(3 * N, np.float64)
+    if spins is not None:
+        if vec:
+            ds[:] = np.asarray(spins, np.float64).reshape(-1)
+        else:
+            is_[:] = np.asarray(spins).reshape(-1)
+    obs = np.zeros((max(nmeas, 1), 2))
+    cl = np.zeros(max(nmeas, 1))
+    b = None if bonds is None else np.ascontiguousarray(bonds, np.int8)
+    lib().ref_run_regular(model, dim, L, J, Q, lam, mass, None if b is None else p(b), beta, ncluster, nmetro, nwarm,
+                          nmeas, seed, int(init), p(is_), p(ds), p(obs), p(cl), None)
+    return (ds.reshape(N, 3) if vec else is_), obs[:nmeas], cl[:nmeas]

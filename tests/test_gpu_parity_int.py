@@ -1,0 +1,148 @@
+"""Bit-exact parity of the HIP integer-spin path against the CPU replay oracle.
+
+Same seed, same Philox counters, same colour schedule and the same integer
+acceptance tables: every spin must match after every call.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(ctx, model, dim, L, nrep=1, **kw):
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, dim, L)
+    st = gpu.State(ctx, model, [lat] * nrep, **kw)
+    st._lat = lat
+    return st
+
+
+@pytest.mark.parametrize("dim,L", [(2, 4), (2, 6), (2, 16), (3, 4), (3, 8), (3, 10)])
+def test_init_matches_oracle(ctx, dim, L):
+    st = _state(ctx, "ising", dim, L, nrep=3)
+    st.set_beta(0.3)
+    st.init(seed=11)
+    for r in range(3):
+        assert np.array_equal(st.download(r), ob.init_ising(L ** dim, 11, r)), (dim, L, r)
+
+
+@pytest.mark.parametrize("dim,L,beta", [(2, 8, 0.4406868), (2, 16, 0.3), (2, 6, 1.0), (3, 8, 0.2216546),
+                                        (3, 6, 0.5), (3, 16, 0.1)])
+def test_ising_sweeps_bit_exact(ctx, dim, L, beta):
+    nrep = 2
+    st = _state(ctx, "ising", dim, L, nrep=nrep, J=-1.0)
+    st.set_beta([beta, beta * 0.5])
+    st.init(seed=5)
+    st.metropolis(7, seed=123, sweep0=3)
+    for r in range(nrep):
+        s0 = ob.init_ising(L ** dim, 5, r)
+        exp = ob.ising_sweeps(dim, L, s0, [beta, beta * 0.5][r], -1.0, 7, 123, 3, r)
+        got = st.download(r)
+        assert np.array_equal(got, exp), (dim, L, r, int((got != exp).sum()))
+
+
+def test_ising_upload_roundtrip_and_antiferro(ctx):
+    dim, L = 3, 6
+    st = _state(ctx, "ising", dim, L, J=+1.0)
+    st.set_beta(0.7)
+    rng = np.random.default_rng(0)
+    s = np.where(rng.random(L ** 3) < 0.5, 1, -1).astype(np.int8)
+    st.upload(0, s)
+    assert np.array_equal(st.download(0), s)
+    st.metropolis(5, seed=9, sweep0=0)
+    exp = ob.ising_sweeps(dim, L, s, 0.7, 1.0, 5, 9, 0, 0)
+    assert np.array_equal(st.download(0), exp)
+
+
+@pytest.mark.parametrize("dim,L", [(2, 8), (3, 6)])
+def test_ea_sweeps_bit_exact(ctx, dim, L):
+    N = L ** dim
+    nrep = 4
+    st = _state(ctx, "ea", dim, L, nrep=nrep, J=-1.0)
+    betas = [0.5, 1.0, 2.0, 3.3]
+    st.set_beta(betas)
+    st.random_bonds(rep_sample=[0, 0, 1, 1], p_plus=0.5, seed=44)
+    st.init(seed=7)
+    st.metropolis(6, seed=77, sweep0=0)
+    for r in range(nrep):
+        b = ob.init_bonds(dim, N, 0.5, 44, r // 2)
+        exp = ob.ising_sweeps(dim, L, ob.init_ising(N, 7, r), betas[r], -1.0, 6, 77, 0, r, bonds=b)
+        assert np.array_equal(st.download(r), exp), r
+
+
+def test_ea_host_bonds(ctx):
+    dim, L = 3, 4
+    N = L ** dim
+    st = _state(ctx, "ea", dim, L, J=-1.0)
+    st.set_beta(1.5)
+    rng = np.random.default_rng(3)
+    b = np.where(rng.random(N * dim) < 0.5, 1, -1).astype(np.int8)
+    st.set_bonds(0, b)
+    st.init(seed=2)
+    st.metropolis(4, seed=8, sweep0=10)
+    exp = ob.ising_sweeps(dim, L, ob.init_ising(N, 2, 0), 1.5, -1.0, 4, 8, 10, 0, bonds=b)
+    assert np.array_equal(st.download(0), exp)
+
+
+@pytest.mark.parametrize("dim,L,Q", [(2, 8, 3), (2, 6, 4), (3, 4, 3)])
+def test_potts_sweeps_bit_exact(ctx, dim, L, Q):
+    N = L ** dim
+    st = _state(ctx, "potts", dim, L, nrep=2, J=-1.0, q=Q)
+    st.set_beta([1.0, 0.6])
+    st.init(seed=3)
+    for r in range(2):
+        assert np.array_equal(st.download(r), ob.init_potts(N, Q, 3, r))
+    st.metropolis(5, seed=31, sweep0=2)
+    for r in range(2):
+        exp = ob.potts_sweeps(dim, L, ob.init_potts(N, Q, 3, r), Q, [1.0, 0.6][r], -1.0, 5, 31, 2, r)
+        assert np.array_equal(st.download(r), exp), r
+
+
+@pytest.mark.parametrize("dim,L,beta", [(2, 16, 0.4406868), (2, 8, 0.6), (3, 8, 0.2216546), (3, 6, 0.4)])
+def test_wolff_ising_bit_exact(ctx, dim, L, beta):
+    N = L ** dim
+    nrep = 3
+    st = _state(ctx, "ising", dim, L, nrep=nrep, J=-1.0)
+    st.set_beta(beta)
+    st.init(seed=21)
+    sizes = st.wolff(6, seed=99, cluster0=5)
+    for r in range(nrep):
+        exp, esz = ob.wolff_ising(dim, L, ob.init_ising(N, 21, r), beta, -1.0, 6, 99, 5, r)
+        assert np.array_equal(sizes[r], esz), (r, sizes[r], esz)
+        assert np.array_equal(st.download(r), exp), r
+
+
+@pytest.mark.parametrize("dim,L,Q", [(2, 8, 3), (3, 4, 4)])
+def test_wolff_potts_bit_exact(ctx, dim, L, Q):
+    N = L ** dim
+    st = _state(ctx, "potts", dim, L, nrep=2, J=-1.0, q=Q)
+    st.set_beta(1.1)
+    st.init(seed=4)
+    sizes = st.wolff(5, seed=55, cluster0=0)
+    for r in range(2):
+        exp, esz = ob.wolff_potts(dim, L, ob.init_potts(N, Q, 4, r), Q, 1.1, -1.0, 5, 55, 0, r)
+        assert np.array_equal(sizes[r], esz)
+        assert np.array_equal(st.download(r), exp)
+
+
+@pytest.mark.parametrize("model,dim,L", [("ising", 2, 8), ("ising", 3, 6), ("potts", 2, 6)])
+def test_measure_matches_oracle(ctx, model, dim, L):
+    N = L ** dim
+    st = _state(ctx, model, dim, L, nrep=2, J=-1.0, q=3)
+    st.set_beta(0.5)
+    st.init(seed=6)
+    st.metropolis(3, seed=1)
+    obs = st.measure()
+    for r in range(2):
+        s = st.download(r)
+        if model == "ising":
+            e = -1.0 * ob.lib().orc_bond_sum_ising(dim, L, ob.p(s), None) / N
+            m = abs(int(s.astype(np.int64).sum())) / N
+        else:
+            e = -1.0 * ob.lib().orc_bond_sum_potts(dim, L, ob.p(s)) / N
+            cnt = np.bincount(s.astype(np.int64), minlength=3)
+            m = (3 * cnt.max() - N) / (N * 2)
+        assert obs[r, 0] == pytest.approx(e, abs=1e-12)
+        assert obs[r, 1] == pytest.approx(m, abs=1e-12)
