@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--L", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip plain and Wolff legs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (counter calibration for HBM traffic)")
     return ap.parse_args()
 
 
@@ -57,6 +59,15 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": src.numel() * 4, "copies": 5}))
+        return
     from marqov_amd import gpu
 
     ctx = gpu.Context(0)

@@ -177,6 +177,141 @@ __global__ void __launch_bounds__(256) k_sweep_ising(int8_t* __restrict__ base, 
     }
 }
 
+
+// ------------------------------------------- 3D Ising, row-vectorized sweep
+// Specialisation for 3D Ising without couplings and L % 32 == 0.  A thread
+// owns 16 consecutive half-sites of one row (one 16-byte load/store); its six
+// neighbour chunks are the other colour's chunks at the same position in rows
+// y+-1 and z+-1, the chunk itself, and the chunk shifted by one byte toward
+// -x0 (p = 0) or +x0 (p = 1), completed by one dword from the adjacent chunk.
+// Spins are +-1 bytes (0x01 / 0xFF); bit 1 of a byte is set iff the spin is
+// -1, so the doubled count of down neighbours is a byte-wise sum of
+// (word & 0x02020202), and the table index e = q + 6 (q = s_i h_i) is
+// 2*aligned = own_down ? 2n : 12 - 2n per byte.  Random numbers and
+// acceptance are those of k_sweep_ising (group g = h >> 3).
+// The block -> tile mapping gives each XCD a contiguous slab of rows so the
+// y+-1 / z+-1 rows are re-read from the same L2.
+__global__ void __launch_bounds__(256) k_sweep_ising3_row16(int8_t* __restrict__ base, int64_t stride,
+                                                            const uint32_t* __restrict__ thr, int lgL, int colour,
+                                                            uint32_t phase, int rep0, int lgbpr, Key2 key)
+{
+    __shared__ uint32_t s_thi[13];
+    __shared__ uint32_t s_tlo[13];
+    const uint32_t B = gridDim.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t lb = ((B & 7u) == 0u) ? (b & 7u) * (B >> 3) + (b >> 3) : b;
+    const uint32_t r = lb >> lgbpr;
+    const uint32_t grep = (uint32_t)rep0 + r;
+    if (threadIdx.x < 13) {
+        const uint32_t hi = thr[grep * 26u + 2u * threadIdx.x];
+        const uint32_t lo = thr[grep * 26u + 2u * threadIdx.x + 1u];
+        // "always" (hi = 65536) becomes hi = 0xFFFF, lo = 0x10000: same decision
+        s_thi[threadIdx.x] = hi > 0xFFFFu ? 0xFFFFu : hi;
+        s_tlo[threadIdx.x] = hi > 0xFFFFu ? 0x10000u : lo;
+    }
+    __syncthreads();
+    const uint32_t L = 1u << lgL;
+    const uint32_t H = L >> 1;
+    const uint32_t lgcpr = (uint32_t)lgL - 5u;        // chunks of 16 per row: H/16
+    const uint32_t lgplane = 2u * (uint32_t)lgL - 1u;  // H*L
+    const uint32_t plane = 1u << lgplane;
+    const uint32_t half = plane << lgL;
+    const uint32_t chunk = ((lb - (r << lgbpr)) << 8) + threadIdx.x;
+    const uint32_t row = chunk >> lgcpr;
+    const uint32_t c = chunk & ((1u << lgcpr) - 1u);
+    const uint32_t y = row & (L - 1u);
+    const uint32_t z = row >> lgL;
+    const uint32_t p = (colour + y + z) & 1u;
+    const uint32_t rb = row * H;
+    const uint32_t h0 = rb + (c << 4);
+    int8_t* own = base + (int64_t)r * stride + (int64_t)colour * half;
+    const int8_t* oth = base + (int64_t)r * stride + (int64_t)(1 - colour) * half;
+
+    const uint4 S = *reinterpret_cast<const uint4*>(own + h0);
+    const uint4 O = *reinterpret_cast<const uint4*>(oth + h0);
+    const uint32_t cmax = (1u << lgcpr) - 1u;
+    const uint32_t xo = p ? rb + ((c == cmax) ? 0u : ((c + 1u) << 4)) : rb + ((c == 0u) ? H : (c << 4)) - 4u;
+    const uint32_t X = *reinterpret_cast<const uint32_t*>(oth + xo);
+    const uint32_t hym = (y == 0u) ? h0 + (L - 1u) * H : h0 - H;
+    const uint32_t hyp = (y == L - 1u) ? h0 - (L - 1u) * H : h0 + H;
+    const uint32_t hzm = (z == 0u) ? h0 + (L - 1u) * plane : h0 - plane;
+    const uint32_t hzp = (z == L - 1u) ? h0 - (L - 1u) * plane : h0 + plane;
+    const uint4 Ym = *reinterpret_cast<const uint4*>(oth + hym);
+    const uint4 Yp = *reinterpret_cast<const uint4*>(oth + hyp);
+    const uint4 Zm = *reinterpret_cast<const uint4*>(oth + hzm);
+    const uint4 Zp = *reinterpret_cast<const uint4*>(oth + hzp);
+
+    // x-shifted neighbour words
+    uint32_t xs0, xs1, xs2, xs3;
+    if (p) {
+        xs0 = __builtin_amdgcn_alignbyte(O.y, O.x, 1);
+        xs1 = __builtin_amdgcn_alignbyte(O.z, O.y, 1);
+        xs2 = __builtin_amdgcn_alignbyte(O.w, O.z, 1);
+        xs3 = __builtin_amdgcn_alignbyte(X, O.w, 1);
+    } else {
+        xs0 = __builtin_amdgcn_alignbyte(O.x, X, 3);
+        xs1 = __builtin_amdgcn_alignbyte(O.y, O.x, 3);
+        xs2 = __builtin_amdgcn_alignbyte(O.z, O.y, 3);
+        xs3 = __builtin_amdgcn_alignbyte(O.w, O.z, 3);
+    }
+    const uint32_t M2 = 0x02020202u;
+    // per byte: 4*e with e = q + 6 (LDS byte address of the threshold entry)
+    auto eword = [&](uint32_t s, uint32_t o, uint32_t xs, uint32_t ym, uint32_t yp, uint32_t zm, uint32_t zp) {
+        const uint32_t n2 = (o & M2) + (xs & M2) + (ym & M2) + (yp & M2) + (zm & M2) + (zp & M2);
+        const uint32_t mask = ((s & M2) >> 1) * 0xFFu;
+        return ((n2 & mask) | ((0x0C0C0C0Cu - n2) & ~mask)) << 2;
+    };
+    uint32_t E[4];
+    E[0] = eword(S.x, O.x, xs0, Ym.x, Yp.x, Zm.x, Zp.x);
+    E[1] = eword(S.y, O.y, xs1, Ym.y, Yp.y, Zm.y, Zp.y);
+    E[2] = eword(S.z, O.z, xs2, Ym.z, Yp.z, Zm.z, Zp.z);
+    E[3] = eword(S.w, O.w, xs3, Ym.w, Yp.w, Zm.w, Zp.w);
+
+    const uint32_t g0 = h0 >> 3;
+    const char* thi_b = reinterpret_cast<const char*>(s_thi);
+    uint32_t acc = 0u;  // bit 15-site: site accepted
+    bool any_tie = false;
+#pragma unroll
+    for (int half8 = 0; half8 < 2; ++half8) {
+        const uint4 w = philox4x32_10(g0 + half8, phase, grep, ST_METRO, key);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int site = half8 * 8 + m;
+            const uint32_t ea = (E[site >> 2] >> (8 * (site & 3))) & 0xFFu;
+            const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+            const uint32_t t = *reinterpret_cast<const uint32_t*>(thi_b + ea);
+            acc = acc + acc + (u < t ? 1u : 0u);
+            any_tie |= (u == t);
+        }
+    }
+    if (__builtin_expect(any_tie, 0)) {
+        // an equal high half: redo this thread's 16 decisions with the low halves
+        acc = 0u;
+#pragma unroll
+        for (int half8 = 0; half8 < 2; ++half8) {
+            const uint4 w = philox4x32_10(g0 + half8, phase, grep, ST_METRO, key);
+            const uint4 tw = philox4x32_10(g0 + half8, phase, grep, ST_TIE, key);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int site = half8 * 8 + m;
+                const uint32_t ea = (E[site >> 2] >> (8 * (site & 3))) & 0xFFu;
+                const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+                const uint32_t lo = (u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+                const uint32_t t = *reinterpret_cast<const uint32_t*>(thi_b + ea);
+                const uint32_t tl = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(s_tlo) + ea);
+                const bool a = (u < t) || (u == t && lo < tl);
+                acc = acc + acc + (a ? 1u : 0u);
+            }
+        }
+    }
+    // acc holds site 0 in bit 15 ... site 15 in bit 0: expand 4 bits -> 4 bytes
+    const uint32_t rev = __builtin_bitreverse32(acc) >> 16;  // site k in bit k
+    auto expand = [](uint32_t nib) { return ((nib * 0x00204081u) & 0x01010101u) * 0xFEu; };
+    const uint4 Snew = make_uint4(S.x ^ expand(rev & 15u), S.y ^ expand((rev >> 4) & 15u),
+                                  S.z ^ expand((rev >> 8) & 15u), S.w ^ expand((rev >> 12) & 15u));
+    *reinterpret_cast<uint4*>(own + h0) = Snew;
+}
+
 // ----------------------------------------------------------------- Potts sweep
 // One Philox block per site: word 0 -> proposal in [0,Q) (Potts.h:171-177),
 // word 1 -> 32-bit uniform against the threshold of dE = k*(#new - #old).
@@ -453,6 +588,18 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
     if (model == MQ_ISING || model == MQ_EA_ISING) {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
         const dim3 grid = grid_for((half + 7) / 8, 256, g.nrep);
+        const bool pow2 = (L & (L - 1)) == 0;
+        if (dim == 3 && !bonds && pow2 && L >= 32 && L <= 1024 && st->tbl == 13 && !st->generic_only) {
+            int lgL = 0;
+            while ((1 << lgL) < L) ++lgL;
+            const int lgbpr = 3 * lgL - 13;  // (N/2)/16/256 blocks per replica
+            const int64_t nblk = ((int64_t)1 << lgbpr) * g.nrep;
+            if (nblk < (1ll << 31)) {
+                hipLaunchKernelGGL(k_sweep_ising3_row16, dim3((unsigned)nblk), dim3(256), 0, s, b, g.stride,
+                                   st->d_thr, lgL, colour, phase, g.rep0, lgbpr, key);
+                return hipGetLastError();
+            }
+        }
         if (dim == 3) {
             if (bonds)
                 hipLaunchKernelGGL((k_sweep_ising<3, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,

@@ -138,6 +138,7 @@ struct mq_state {
     std::vector<double> beta_ladder;   // per replica index at creation (temperature k)
     double* d_ladder = nullptr;
     int pt_ntemp = 0;
+    bool generic_only = false;  // force the generic kernels (parity of the specialised ones)
 };
 
 // --------------------------------------------------------------------------

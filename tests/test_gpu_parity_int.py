@@ -146,3 +146,31 @@ def test_measure_matches_oracle(ctx, model, dim, L):
             m = (3 * cnt.max() - N) / (N * 2)
         assert obs[r, 0] == pytest.approx(e, abs=1e-12)
         assert obs[r, 1] == pytest.approx(m, abs=1e-12)
+
+
+@pytest.mark.parametrize("L,nsweeps", [(32, 6), (64, 2)])
+def test_ising3_row16_kernel_bit_exact(ctx, L, nsweeps):
+    """L % 32 == 0 selects the row-vectorised 3D kernel; it must replay bit-exactly."""
+    dim = 3
+    nrep = 2
+    st = _state(ctx, "ising", dim, L, nrep=nrep, J=-1.0)
+    betas = [0.2216546, 0.35]
+    st.set_beta(betas)
+    st.init(seed=9)
+    st.metropolis(nsweeps, seed=1234, sweep0=17)
+    for r in range(nrep):
+        exp = ob.ising_sweeps(dim, L, ob.init_ising(L ** 3, 9, r), betas[r], -1.0, nsweeps, 1234, 17, r)
+        got = st.download(r)
+        assert np.array_equal(got, exp), (L, r, int((got != exp).sum()))
+
+
+def test_ising3_row16_antiferro_and_hot(ctx):
+    """Antiferromagnetic J and beta = 0 (every move accepted) through the fast kernel."""
+    dim, L = 3, 32
+    for J, beta in ((+1.0, 0.4), (-1.0, 0.0), (-1.0, 5.0)):
+        st = _state(ctx, "ising", dim, L, J=J)
+        st.set_beta(beta)
+        st.init(seed=3)
+        st.metropolis(3, seed=5, sweep0=0)
+        exp = ob.ising_sweeps(dim, L, ob.init_ising(L ** 3, 3, 0), beta, J, 3, 5, 0, 0)
+        assert np.array_equal(st.download(0), exp), (J, beta)

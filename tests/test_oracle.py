@@ -1,0 +1,183 @@
+"""Pins the oracle itself (CPU only).
+
+- Philox4x32-10 against the Random123 known-answer vectors and against torch's
+  independent implementation (ATen/core/PhiloxRNGEngine.h) via a committed fixture.
+- std::mt19937_64 against the value the C++ standard mandates.
+- The reference-faithful CPU restatement against the reference's own compiled
+  Metropolis move (oracle/_ref), bit for bit.
+- The CPU replay of the GPU schedule and the restatement against exact
+  enumeration and Kaufman's exact torus energy, with planted mistakes showing
+  that the checks have the power to catch a wrong sign or a dropped term.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import exact  # noqa: E402
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+# Random123 1.09 kat_vectors, philox4x32 10 rounds
+R123_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", R123_KAT)
+def test_philox_random123_kat(ctr, key, out):
+    assert tuple(ob.philox(ctr, key)) == out
+
+
+def test_philox_matches_torch_engine_fixture():
+    with open(os.path.join(GOLDEN, "philox_torch.json")) as f:
+        fx = json.load(f)
+    assert len(fx["cases"]) >= 50
+    for c in fx["cases"]:
+        assert ob.philox(c["ctr"], c["key"]) == c["out"], c
+
+
+def test_mt19937_64_standard_value():
+    # C++ [rand.predef]: the 10000th invocation of a default-constructed
+    # mt19937_64 (seed 5489) produces 9981545732273789042.
+    assert ob.lib().ref_mt19937_64_nth(5489, 10000) == 9981545732273789042
+
+
+def test_rngcache_mapping():
+    """real() = (x >> 11) * 2^-53 and integer(r) = x % r (rngcache.h:179,204)."""
+    n = 3000
+    raw = np.zeros(n, np.uint64)
+    ob.lib().ref_rngcache_draws(0, 0, 0, n, ob.p(raw))
+    reals = np.zeros(n, np.uint64)
+    ob.lib().ref_rngcache_draws(0, 1, 0, n, ob.p(reals))
+    ints = np.zeros(n, np.uint64)
+    ob.lib().ref_rngcache_draws(0, 2, 7, n, ob.p(ints))
+    r = reals.view(np.float64)
+    assert np.array_equal(r, (raw >> np.uint64(11)).astype(np.float64) * 2.0 ** -53)
+    assert np.array_equal(ints, raw % np.uint64(7))
+    # the cache refills every 1024 draws from the same engine stream
+    assert ob.lib().ref_mt19937_64_nth(0, 1025) == int(raw[1024])
+
+
+def _ref_metropolis(model, dim, L, beta, J, nsweeps, seed):
+    out = subprocess.run([ob.REF_METRO, model, str(dim), str(L), str(beta), str(J), str(nsweeps), str(seed)],
+                         capture_output=True, text=True, check=True).stdout
+    return np.array([[float(x) for x in ln.split()] for ln in out.strip().split("\n")])
+
+
+@pytest.mark.skipif(not os.path.exists(ob.REF_METRO), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("model,dim,L,beta,J,ns", [
+    ("ising", 2, 16, 0.4406868, -1.0, 40), ("ising", 3, 8, 0.2216546, -1.0, 40),
+    ("ising", 2, 10, 0.3, 1.0, 20), ("heis", 3, 6, 0.693, -1.0, 20), ("heis", 2, 8, 1.5, -1.0, 20)])
+def test_restatement_bit_exact_vs_reference_build(model, dim, L, beta, J, ns):
+    ref = _ref_metropolis(model, dim, L, beta, J, ns, 12345)
+    m = ob.ISING if model == "ising" else ob.HEIS
+    mine, _, _ = ob.ref_run(m, dim, L, beta, J=J, nmetro=1, nwarm=ns, seed=12345)
+    if model == "ising":
+        assert np.array_equal(ref[:, 0].astype(int), mine)
+    else:
+        assert np.array_equal(ref, mine)
+
+
+def test_kaufman_matches_enumeration():
+    for L in (2, 4):
+        for b in (0.1, 0.4406868, 0.9):
+            assert exact.kaufman_energy(b, L, L) == pytest.approx(exact.enumerate_ising(2, L, b)["e"], abs=1e-12)
+
+
+def _mean_err(x, nblocks=20):
+    x = np.asarray(x, np.float64)
+    n = len(x) // nblocks * nblocks
+    b = x[:n].reshape(nblocks, -1).mean(1)
+    return b.mean(), b.std(ddof=1) / np.sqrt(nblocks)
+
+
+def _replay_chain(dim, L, beta, J, nsteps, seed, bonds=None, wolff=0):
+    """CPU replay of the GPU schedule: returns per-step e and m."""
+    N = L ** dim
+    s = ob.init_ising(N, seed, 0)
+    e, m = [], []
+    nb = exact.neighbour_table(dim, L)
+    B = exact.bond_table(dim, L, bonds)
+    for t in range(nsteps):
+        if wolff:
+            s, _ = ob.wolff_ising(dim, L, s, beta, J, wolff, seed, t * wolff, 0)
+        s = ob.ising_sweeps(dim, L, s, beta, J, 1, seed, t, 0, bonds=bonds)
+        si = s.astype(np.int64)
+        e.append(J * (si[:, None] * si[nb] * B).sum() / 2.0 / N)
+        m.append(si.sum() / N)
+    return np.array(e), np.array(m)
+
+
+@pytest.mark.parametrize("dim,L,beta,wolff", [(2, 4, 0.4406868, 0), (2, 4, 0.3, 2), (3, 2, 0.5, 0)])
+def test_replay_metropolis_matches_enumeration(dim, L, beta, wolff):
+    ex = exact.enumerate_ising(dim, L, beta)
+    e, m = _replay_chain(dim, L, beta, -1.0, 20000, seed=3, wolff=wolff)
+    me, se = _mean_err(e[500:])
+    mm, sm = _mean_err(m[500:] ** 2)
+    assert abs(me - ex["e"]) < 4 * se + 1e-9, (me, se, ex["e"])
+    assert abs(mm - ex["m2"]) < 4 * sm + 1e-9, (mm, sm, ex["m2"])
+
+
+def test_replay_ea_matches_enumeration():
+    dim, L, beta = 3, 2, 0.8
+    bonds = ob.init_bonds(dim, L ** dim, 0.5, 44, 0)
+    ex = exact.enumerate_ising(dim, L, beta, bonds=bonds)
+    e, m = _replay_chain(dim, L, beta, -1.0, 20000, seed=5, bonds=bonds)
+    me, se = _mean_err(e[500:])
+    assert abs(me - ex["e"]) < 4 * se + 1e-9, (me, se, ex["e"])
+
+
+def test_planted_sign_error_is_caught():
+    """A chain run with the coupling sign flipped must fail the same check."""
+    dim, L, beta = 2, 4, 0.4406868
+    ex = exact.enumerate_ising(dim, L, beta, J=-1.0)
+    e, _ = _replay_chain(dim, L, beta, +1.0, 4000, seed=3)
+    e = -e  # energies measured with the ferromagnetic sign
+    me, se = _mean_err(e[200:])
+    assert abs(me - ex["e"]) > 10 * se
+
+
+def test_replay_potts_matches_enumeration():
+    dim, L, Q, beta = 3, 2, 3, 0.7
+    N = L ** dim
+    ex = exact.enumerate_potts(dim, L, Q, beta)
+    nb = exact.neighbour_table(dim, L)
+    s = ob.init_potts(N, Q, 2, 0)
+    es = []
+    for t in range(20000):
+        if t % 3 == 0:
+            s, _ = ob.wolff_potts(dim, L, s, Q, beta, -1.0, 1, 2, t, 0)
+        s = ob.potts_sweeps(dim, L, s, Q, beta, -1.0, 1, 2, t, 0)
+        es.append(-1.0 * (s[:, None] == s[nb]).sum() / 2.0 / N)
+    me, se = _mean_err(np.array(es[500:]))
+    assert abs(me - ex["e"]) < 4 * se, (me, se, ex["e"])
+
+
+@pytest.mark.parametrize("model", [ob.ISING, ob.POTTS])
+def test_restatement_matches_enumeration(model):
+    dim, L, beta = (2, 4, 0.4406868) if model == ob.ISING else (3, 2, 0.7)
+    ex = exact.enumerate_ising(dim, L, beta) if model == ob.ISING else exact.enumerate_potts(dim, L, 3, beta)
+    _, obs, _ = ob.ref_run(model, dim, L, beta, Q=3, ncluster=1, nmetro=1, nwarm=200, nmeas=40000, seed=9)
+    me, se = _mean_err(obs[:, 0])
+    assert abs(me - ex["e"]) < 4 * se, (me, se, ex["e"])
+
+
+def test_restatement_c1_energy_vs_kaufman():
+    """2D L=64 at beta_c, 10 Wolff + 1 sweep per EMCS, vs the exact torus energy."""
+    L, beta = 64, 0.4406868
+    _, obs, cl = ob.ref_run(ob.ISING, 2, L, beta, ncluster=10, nmetro=1, nwarm=300, nmeas=3000, seed=1)
+    me, se = _mean_err(obs[:, 0])
+    ka = exact.kaufman_energy(beta, L, L)
+    assert abs(me - ka) < 3 * se, (me, se, ka)
+    assert 1000 < cl.mean() < 2200
