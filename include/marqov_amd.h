@@ -94,6 +94,9 @@ MQ_API int mq_lattice_csr(mq_ctx* ctx, int64_t nsites, const int32_t* row_ptr, c
 /* number of colour classes and their sizes (ncolors <= 64) */
 MQ_API int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes64);
 MQ_API int64_t mq_lattice_nsites(const mq_lattice* lat);
+/* colour class of every site in the reference's site order (regular lattices:
+ * (x0+x1+x2)&1; graphs: greedy colouring in vertex order) */
+MQ_API int mq_lattice_color_of(const mq_lattice* lat, int32_t* colour);
 MQ_API void mq_lattice_destroy(mq_lattice* lat);
 
 /* ---- states (replica batches) ------------------------------------------ */

@@ -47,6 +47,7 @@ SIGNATURES = {
     "mq_lattice_csr": (_I, [_P, _I64, _P, _P, ctypes.POINTER(_P)]),
     "mq_lattice_colors": (_I, [_P, ctypes.POINTER(_I), _P]),
     "mq_lattice_nsites": (_I64, [_P]),
+    "mq_lattice_color_of": (_I, [_P, _P]),
     "mq_lattice_destroy": (None, [_P]),
     "mq_state_create": (_I, [_P, ctypes.POINTER(ModelParams), _I, _P, ctypes.POINTER(_P)]),
     "mq_state_destroy": (None, [_P]),

@@ -2,6 +2,7 @@
 // tables and the update / measurement entry points declared in
 // include/marqov_amd.h.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -249,7 +250,8 @@ extern "C" void mq_lattice_destroy(mq_lattice* lat)
     dfree(lat->d_rp);
     dfree(lat->d_ci);
     dfree(lat->d_edge);
-    dfree(lat->d_slot_of_orig);
+    dfree(lat->d_pos_of);
+    dfree(lat->d_vert_of);
     delete lat;
 }
 
@@ -296,6 +298,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         r = e;
     }
     st->tbl = 2 * qmax_of(st) + 1;
+    if (const char* kc = std::getenv("MARQOV_AMD_KERNELS")) st->kernel_choice = std::atoi(kc);
     st->rep_sample.resize(nrep);
     std::iota(st->rep_sample.begin(), st->rep_sample.end(), 0);
     st->nsamples = nrep;
@@ -343,6 +346,8 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_pt_acc);
     dfree(st->d_energy);
     dfree(st->d_ladder);
+    dfree(st->d_dyn);
+    if (st->d_lds_groups) (void)hipFree(st->d_lds_groups);
     delete st;
 }
 
@@ -386,13 +391,16 @@ extern "C" int mq_state_set_beta(mq_state* st, const double* beta)
     for (int r = 0; r < st->nrep; ++r)
         if (!(beta[r] >= 0.0) || !std::isfinite(beta[r])) return mq_fail(st->ctx, MQ_ERR_ARG, "beta must be finite and >= 0");
     st->beta.assign(beta, beta + st->nrep);
+    st->pt_ntemp = 0;  // temperatures are re-assigned: restart the PT bookkeeping
     return upload_tables(st);
 }
 
 extern "C" int mq_state_get_beta(const mq_state* st, double* beta)
 {
     if (!st || !beta || st->beta.empty()) return MQ_ERR_ARG;
-    std::copy(st->beta.begin(), st->beta.end(), beta);
+    // current temperature of every replica (parallel tempering moves them)
+    MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    MQ_HIP(st->ctx, hipMemcpy(beta, st->d_betad, sizeof(double) * st->nrep, hipMemcpyDeviceToHost));
     return MQ_OK;
 }
 
@@ -456,7 +464,15 @@ extern "C" int mq_state_set_bonds(mq_state* st, int sample, const int8_t* bonds)
             }
         }
     } else {
-        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "graph couplings not supported yet");
+        const int64_t nnz = l->nnz;
+        dev.resize((size_t)std::max<int64_t>(nnz, 1));
+        for (int64_t k = 0; k < nnz; ++k) {
+            const int8_t b = bonds[k];
+            if (b != 1 && b != -1) return mq_fail(st->ctx, MQ_ERR_ARG, "bonds must be +1 or -1");
+            if (bonds[l->h_mirror[k]] != b) return mq_fail(st->ctx, MQ_ERR_ARG, "graph couplings must be symmetric");
+            dev[(size_t)l->h_slot_of_orig[k]] = b;
+        }
+        dev.resize((size_t)nnz);
     }
     MQ_HIP(st->ctx, hipMemcpy(st->d_bonds + st->sample_bond_off[sample], dev.data(), dev.size(), hipMemcpyHostToDevice));
     return MQ_OK;
@@ -535,6 +551,11 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
 int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0)
 {
     const Key2 key = key_of(seed);
+    if (nsweeps > 0 && st->kernel_choice == 0 && sweep0 + (uint64_t)nsweeps < (1ull << 31)) {
+        hipError_t e = launch_sweeps_lds(st, nsweeps, (uint32_t)sweep0, key);
+        if (e == hipSuccess) return MQ_OK;
+        if (e != hipErrorNotSupported) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("lds sweep: ") + hipGetErrorString(e));
+    }
     for (int t = 0; t < nsweeps; ++t) {
         const uint64_t sw = sweep0 + (uint64_t)t;
         if (sw >= (1ull << 31)) return mq_fail(st->ctx, MQ_ERR_ARG, "sweep counter exceeds 2^31");
@@ -631,6 +652,9 @@ extern "C" int mq_measure(mq_state* st, double* out)
     std::vector<double> fs;
     int rc = measure_sums(st, is, fs);
     if (rc) return rc;
+    std::vector<double> bcur(st->nrep);
+    rc = mq_state_get_beta(st, bcur.data());
+    if (rc) return rc;
     for (int r = 0; r < st->nrep; ++r) {
         const double N = (double)st->lat[r]->N;
         const long long* I = &is[(size_t)r * 8];
@@ -647,7 +671,7 @@ extern "C" int mq_measure(mq_state* st, double* out)
         } else {
             e = st->p.J * F[0] / N;
             if (st->p.model == MQ_PHI4) {
-                const double b = st->beta[r];
+                const double b = bcur[r];
                 e += (st->p.mass / b) * F[4] / N + (st->p.lambda / b) * F[5] / N;
             }
             m = std::sqrt(F[1] * F[1] + F[2] * F[2] + F[3] * F[3]) / N;

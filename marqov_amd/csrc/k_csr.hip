@@ -1,0 +1,543 @@
+// General graphs in CSR form: greedy colouring on the host, colour-class
+// Metropolis sweeps, frontier-BFS Wolff and FP64 observables on the device.
+//
+// Layout: vertices are renumbered so that every colour class is contiguous
+// (position = class offset + rank by original id); the device CSR holds
+// positions.  Spins per replica are stored by position.  Random numbers are
+// keyed by original vertex ids and original CSR slots, so a CPU replay needs
+// only the colour of every vertex, not the renumbering.
+//
+//   Metropolis: counter (vertex, 64*sweep + colour, replica, METRO); Ising/EA
+//     compare word 0 with the 32-bit threshold of dE = -2 J q, q = s_i sum b_ij s_j;
+//     Potts proposes (w0 * Q) >> 32 and compares word 1; O(3)/Phi4 use words 0,1
+//     for the direction and word 2 for acceptance.
+//   Wolff bonds: the edge {u, v} is keyed by the original CSR slot of v in row u
+//     with u < v (its canonical slot), counter (slot >> 2, cluster, replica, WBOND).
+#include <algorithm>
+#include <numeric>
+
+#include "mq_internal.hpp"
+
+namespace {
+
+__device__ __forceinline__ float3 rnddir_c(uint32_t w0, uint32_t w1)
+{
+    const float a0 = 2.0f * u01f(w0);
+    const float a1 = 2.0f * u01f(w1);
+    const float rho = sqrtf(fmaxf(a0 * (2.0f - a0), 0.0f));
+    float sn, cs;
+    sincospif(a1, &sn, &cs);
+    return make_float3(1.0f - a0, rho * sn, rho * cs);
+}
+
+__global__ void __launch_bounds__(256) k_csr_init(char* __restrict__ base, int64_t stride, const int32_t* __restrict__ vert_of,
+                                                  int64_t N, int model, int Q, int rep0, Key2 key)
+{
+    const int r = blockIdx.y;
+    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= N) return;
+    const uint32_t v = (uint32_t)vert_of[pos];
+    if (model == MQ_HEISENBERG || model == MQ_PHI4) {
+        const uint4 w = philox4x32_10(v, 0u, (uint32_t)(rep0 + r), ST_INIT, key);
+        const float3 d = rnddir_c(w.x, w.y);
+        reinterpret_cast<float4*>(base + r * stride)[pos] = make_float4(d.x, d.y, d.z, 0.f);
+    } else {
+        const uint4 w = philox4x32_10(v >> 2, 0u, (uint32_t)(rep0 + r), ST_INIT, key);
+        const uint32_t x = u4_get(w, (int)(v & 3u));
+        int8_t s;
+        if (model == MQ_POTTS) s = (int8_t)(((uint64_t)x * (uint64_t)Q) >> 32);
+        else s = (x >> 31) ? (int8_t)1 : (int8_t)-1;
+        reinterpret_cast<int8_t*>(base + r * stride)[pos] = s;
+    }
+}
+
+template <bool BONDS>
+__global__ void __launch_bounds__(256) k_csr_sweep_int8(int8_t* __restrict__ base, int64_t stride,
+                                                        const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                        const int32_t* __restrict__ vert_of,
+                                                        const int8_t* __restrict__ bonds,
+                                                        const int64_t* __restrict__ rep_bond_off,
+                                                        const uint32_t* __restrict__ thr, int tbl, int64_t c0,
+                                                        int64_t nc, int model, int Q, uint32_t phase,
+                                                        const uint32_t* __restrict__ dyn, int rep0, Key2 key)
+{
+    if (dyn) phase += dyn[0];
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    const int64_t pos = c0 + t;
+    int8_t* s = base + r * stride;
+    const int qmax = (tbl - 1) >> 1;
+    const int a = rp[pos], b = rp[pos + 1];
+    const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
+    const uint32_t* T = thr + (int64_t)grep * 2 * tbl;
+    const int si = s[pos];
+    if (model == MQ_POTTS) {
+        const int snew = (int)(((uint64_t)w.x * (uint64_t)Q) >> 32);
+        int dn = 0;
+        for (int k = a; k < b; ++k) {
+            const int v = s[ci[k]];
+            dn += (v == snew) - (v == si);
+        }
+        const int e = dn + qmax;
+        const uint64_t th = ((uint64_t)T[2 * e] << 16) | T[2 * e + 1];
+        if ((uint64_t)w.y < th) s[pos] = (int8_t)snew;
+    } else {
+        int h = 0;
+        const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
+        for (int k = a; k < b; ++k) h += (BONDS ? (int)bb[k] : 1) * s[ci[k]];
+        const int e = si * h + qmax;
+        const uint64_t th = ((uint64_t)T[2 * e] << 16) | T[2 * e + 1];
+        if ((uint64_t)w.x < th) s[pos] = (int8_t)(-si);
+    }
+}
+
+template <bool BONDS>
+__global__ void __launch_bounds__(256) k_csr_sweep_o3(float4* __restrict__ base, int64_t stride4,
+                                                      const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                      const int32_t* __restrict__ vert_of,
+                                                      const int8_t* __restrict__ bonds,
+                                                      const int64_t* __restrict__ rep_bond_off,
+                                                      const float* __restrict__ betaf, int64_t c0, int64_t nc,
+                                                      int model, int proposal, float J, float delta, float lambda,
+                                                      float mass, uint32_t phase, const uint32_t* __restrict__ dyn,
+                                                      int rep0, Key2 key)
+{
+    if (dyn) phase += dyn[0];
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    const int64_t pos = c0 + t;
+    float4* s = base + r * stride4;
+    const int a = rp[pos], b = rp[pos + 1];
+    float nx = 0.f, ny = 0.f, nz = 0.f;
+    const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
+    for (int k = a; k < b; ++k) {
+        const float4 v = s[ci[k]];
+        const float f = BONDS ? (float)bb[k] : 1.0f;
+        nx += f * v.x;
+        ny += f * v.y;
+        nz += f * v.z;
+    }
+    const float4 so = s[pos];
+    const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
+    const float3 rd = rnddir_c(w.x, w.y);
+    float3 sn;
+    if (model == MQ_PHI4) {
+        sn = make_float3(so.x + 0.5f * rd.x, so.y + 0.5f * rd.y, so.z + 0.5f * rd.z);
+    } else if (proposal == MQ_PROPOSE_SMALL_ANGLE) {
+        const float3 q = make_float3(so.x + delta * rd.x, so.y + delta * rd.y, so.z + delta * rd.z);
+        const float inv = rsqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+        sn = make_float3(q.x * inv, q.y * inv, q.z * inv);
+    } else {
+        sn = rd;
+    }
+    const float beta = betaf[grep];
+    float dE = J * ((sn.x - so.x) * nx + (sn.y - so.y) * ny + (sn.z - so.z) * nz);
+    if (model == MQ_PHI4) {
+        const float n2 = sn.x * sn.x + sn.y * sn.y + sn.z * sn.z;
+        const float o2 = so.x * so.x + so.y * so.y + so.z * so.z;
+        dE += (mass / beta) * (n2 - o2) + (lambda / beta) * ((n2 - 1.f) * (n2 - 1.f) - (o2 - 1.f) * (o2 - 1.f));
+    }
+    bool acc = dE <= 0.f;
+    if (!acc) acc = u01f(w.z) < __expf(-beta * dE);
+    if (acc) s[pos] = make_float4(sn.x, sn.y, sn.z, 0.f);
+}
+
+template <typename T>
+__device__ __forceinline__ T wsum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// isum: [0] sum over edges b s_i s_j (Ising/EA) or #equal edges (Potts), [1] sum s,
+// Potts [1+q] colour counts; fsum (O(3)): [0] sum over edges b s_i.s_j, [1..3] M,
+// [4] sum |s|^2, [5] sum (|s|^2-1)^2.  Each edge counted once (from its lower end).
+template <bool BONDS, bool VEC>
+__global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ base, int64_t stride,
+                                                     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                     const int8_t* __restrict__ bonds,
+                                                     const int64_t* __restrict__ rep_bond_off, int64_t N, int model,
+                                                     int Q, int rep0, long long* __restrict__ isum,
+                                                     double* __restrict__ fsum)
+{
+    __shared__ double red[8][4];
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (pos < N) {
+        const int a = rp[pos], b = rp[pos + 1];
+        const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
+        if (VEC) {
+            const float4* s = reinterpret_cast<const float4*>(base + r * stride);
+            const float4 v = s[pos];
+            double e = 0;
+            for (int k = a; k < b; ++k) {
+                if (ci[k] <= pos) continue;
+                const float4 u = s[ci[k]];
+                e += (BONDS ? (double)bb[k] : 1.0) * ((double)v.x * u.x + (double)v.y * u.y + (double)v.z * u.z);
+            }
+            const double n2 = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
+            acc[0] = e;
+            acc[1] = v.x;
+            acc[2] = v.y;
+            acc[3] = v.z;
+            acc[4] = n2;
+            acc[5] = (n2 - 1) * (n2 - 1);
+        } else {
+            const int8_t* s = reinterpret_cast<const int8_t*>(base + r * stride);
+            const int si = s[pos];
+            long long e = 0;
+            for (int k = a; k < b; ++k) {
+                if (ci[k] <= pos) continue;
+                const int sj = s[ci[k]];
+                if (model == MQ_POTTS) e += (si == sj);
+                else e += (long long)(BONDS ? bb[k] : 1) * si * sj;
+            }
+            acc[0] = (double)e;
+            if (model == MQ_POTTS) {
+                for (int q = 0; q < Q && q < 7; ++q) acc[1 + q] = (si == q);
+            } else {
+                acc[1] = si;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const double x = wsum(acc[k]);
+        if (lane == 0) red[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        double t = 0;
+        for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) t += red[threadIdx.x][wv];
+        if (VEC) atomicAdd(fsum + (int64_t)grep * 8 + threadIdx.x, t);
+        else atomicAdd(reinterpret_cast<unsigned long long*>(isum + (int64_t)grep * 8 + threadIdx.x),
+                       (unsigned long long)(long long)llrint(t));
+    }
+}
+
+enum { CK_ISING = 0, CK_POTTS = 1, CK_CART = 2, CK_RDIR = 3 };
+
+template <int KIND>
+__global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int64_t stride, int64_t N,
+                                                   const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                   const int32_t* __restrict__ edge, const int32_t* __restrict__ pos_of,
+                                                   int nclusters, uint32_t cl0, int rep0, int Q,
+                                                   const uint32_t* __restrict__ wthr, const float* __restrict__ betaf,
+                                                   float J, int32_t* __restrict__ queue, uint32_t* __restrict__ visit,
+                                                   const int64_t* __restrict__ rep_site_off, int64_t* __restrict__ csize,
+                                                   Key2 key)
+{
+    constexpr bool VEC = (KIND == CK_CART || KIND == CK_RDIR);
+    __shared__ int s_tail;
+    const int r = blockIdx.x;
+    const int grep = rep0 + r;
+    int8_t* sp8 = reinterpret_cast<int8_t*>(base + r * stride);
+    float4* sp4 = reinterpret_cast<float4*>(base + r * stride);
+    int32_t* q = queue + rep_site_off[grep];
+    uint32_t* vis = visit + (rep_site_off[grep] >> 5);
+    const uint32_t tw = VEC ? 0u : wthr[grep];
+    const float beta = VEC ? betaf[grep] : 0.f;
+    for (int c = 0; c < nclusters; ++c) {
+        const uint32_t ncl = cl0 + (uint32_t)c;
+        const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
+        const int64_t seed = pos_of[(int64_t)(((uint64_t)sw.x * (uint64_t)N) >> 32)];
+        int val = 0, newq = 0, axis = 0;
+        float3 rd = make_float3(0.f, 0.f, 0.f);
+        if (!VEC) {
+            val = sp8[seed];
+            if (KIND == CK_POTTS) newq = (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q;
+        } else if (KIND == CK_CART) {
+            axis = (int)(((uint64_t)sw.y * 3ull) >> 32);
+        } else {
+            rd = rnddir_c(sw.y, sw.z);
+        }
+        __syncthreads();
+        auto flip = [&](int64_t p) {
+            if (KIND == CK_ISING) sp8[p] = (int8_t)(-val);
+            else if (KIND == CK_POTTS) sp8[p] = (int8_t)newq;
+            else {
+                float4 v = sp4[p];
+                if (KIND == CK_CART) {
+                    if (axis == 0) v.x = -v.x;
+                    else if (axis == 1) v.y = -v.y;
+                    else v.z = -v.z;
+                } else {
+                    const float d = v.x * rd.x + v.y * rd.y + v.z * rd.z;
+                    v.x -= 2.f * d * rd.x;
+                    v.y -= 2.f * d * rd.y;
+                    v.z -= 2.f * d * rd.z;
+                }
+                sp4[p] = v;
+            }
+        };
+        if (threadIdx.x == 0) {
+            flip(seed);
+            q[0] = (int32_t)seed;
+            atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+            s_tail = 1;
+        }
+        __syncthreads();
+        int head = 0;
+        while (true) {
+            const int tail = s_tail;
+            __syncthreads();
+            if (head >= tail) break;
+            for (int idx = head + (int)threadIdx.x; idx < tail; idx += blockDim.x) {
+                const int64_t cur = q[idx];
+                float pc = 0.f;
+                if (VEC) {
+                    const float4 vc = sp4[cur];
+                    pc = (KIND == CK_CART) ? (axis == 0 ? vc.x : (axis == 1 ? vc.y : vc.z))
+                                           : (vc.x * rd.x + vc.y * rd.y + vc.z * rd.z);
+                }
+                for (int k = rp[cur]; k < rp[cur + 1]; ++k) {
+                    const int64_t j = ci[k];
+                    const uint32_t eid = (uint32_t)edge[k];
+                    bool add;
+                    if (!VEC) {
+                        if (sp8[j] != val) continue;
+                        const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                        add = u4_get(bw, (int)(eid & 3u)) < tw;
+                    } else {
+                        const float4 vj = sp4[j];
+                        const float pj = (KIND == CK_CART) ? (axis == 0 ? vj.x : (axis == 1 ? vj.y : vj.z))
+                                                           : (vj.x * rd.x + vj.y * rd.y + vj.z * rd.z);
+                        const float cpl = J * pc * pj;
+                        if (!(cpl > 0.f)) continue;
+                        const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                        add = u01f(u4_get(bw, (int)(eid & 3u))) < -expm1f(-2.0f * beta * cpl);
+                    }
+                    if (!add) continue;
+                    const uint32_t bit = 1u << (j & 31);
+                    if (atomicOr(&vis[j >> 5], bit) & bit) continue;
+                    flip(j);
+                    q[atomicAdd(&s_tail, 1)] = (int32_t)j;
+                }
+            }
+            __syncthreads();
+            head = tail;
+        }
+        const int size = s_tail;
+        for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+        __syncthreads();
+        if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
+    }
+}
+
+inline dim3 grid_for(int64_t work, int nrep) { return dim3((unsigned)((work + 255) / 256), (unsigned)nrep, 1); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, const int32_t* col_idx, mq_lattice** out)
+{
+    if (!ctx || !out || !row_ptr || !col_idx || N <= 0) return MQ_ERR_ARG;
+    *out = nullptr;
+    if (N >= ((int64_t)1 << 31)) return mq_fail(ctx, MQ_ERR_ARG, "graph too large");
+    if (row_ptr[0] != 0) return mq_fail(ctx, MQ_ERR_ARG, "row_ptr[0] must be 0");
+    const int64_t nnz = row_ptr[N];
+    for (int64_t i = 0; i < N; ++i) {
+        if (row_ptr[i + 1] < row_ptr[i]) return mq_fail(ctx, MQ_ERR_ARG, "row_ptr not monotone");
+        for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+            const int j = col_idx[k];
+            if (j < 0 || j >= N || j == i) return mq_fail(ctx, MQ_ERR_ARG, "bad neighbour index or self loop");
+        }
+    }
+    // symmetry and canonical slots: slot of (u -> v) mirrored to (v -> u)
+    std::vector<int32_t> mirror(nnz, -1);
+    {
+        std::vector<int32_t> fill(row_ptr, row_ptr + N);
+        std::vector<std::pair<int64_t, int32_t>> keyed;
+        keyed.reserve(nnz);
+        for (int64_t u = 0; u < N; ++u)
+            for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) keyed.push_back({u * N + col_idx[k], k});
+        std::vector<std::pair<int64_t, int32_t>> sorted = keyed;
+        std::sort(sorted.begin(), sorted.end());
+        for (size_t t = 1; t < sorted.size(); ++t)
+            if (sorted[t].first == sorted[t - 1].first) return mq_fail(ctx, MQ_ERR_ARG, "duplicate edge");
+        for (int64_t u = 0; u < N; ++u)
+            for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) {
+                const int64_t rev = (int64_t)col_idx[k] * N + u;
+                auto it = std::lower_bound(sorted.begin(), sorted.end(), std::make_pair(rev, (int32_t)-1));
+                if (it == sorted.end() || it->first != rev) return mq_fail(ctx, MQ_ERR_ARG, "adjacency is not symmetric");
+                mirror[k] = it->second;
+            }
+        (void)fill;
+    }
+    // greedy colouring in vertex order
+    std::vector<int32_t> colour(N, -1);
+    int ncol = 0;
+    for (int64_t u = 0; u < N; ++u) {
+        uint64_t used = 0;
+        for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) {
+            const int c = colour[col_idx[k]];
+            if (c >= 0) used |= 1ull << c;
+        }
+        int c = 0;
+        while (c < 64 && (used >> c) & 1ull) ++c;
+        if (c >= MQ_MAX_COLORS) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "greedy colouring needs more than 64 colours");
+        colour[u] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    auto* l = new mq_lattice();
+    l->ctx = ctx;
+    l->kind = LAT_CSR;
+    l->N = N;
+    l->nnz = nnz;
+    l->ncolors = ncol;
+    l->color_off.assign(ncol + 1, 0);
+    for (int64_t u = 0; u < N; ++u) l->color_off[colour[u] + 1]++;
+    for (int c = 0; c < ncol; ++c) l->color_off[c + 1] += l->color_off[c];
+    l->pos_of.resize(N);
+    l->vert_of.resize(N);
+    l->colour.assign(colour.begin(), colour.end());
+    {
+        std::vector<int64_t> next(l->color_off.begin(), l->color_off.end() - 1);
+        for (int64_t u = 0; u < N; ++u) {
+            const int64_t p = next[colour[u]]++;
+            l->pos_of[u] = (int32_t)p;
+            l->vert_of[p] = (int32_t)u;
+        }
+    }
+    l->h_rp.assign(row_ptr, row_ptr + N + 1);
+    l->h_ci.assign(col_idx, col_idx + nnz);
+    std::vector<int32_t> rp(N + 1), ci(nnz), edge(nnz), slot_of_orig(nnz);
+    rp[0] = 0;
+    int maxdeg = 0;
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t u = l->vert_of[p];
+        const int deg = row_ptr[u + 1] - row_ptr[u];
+        maxdeg = std::max(maxdeg, deg);
+        rp[p + 1] = rp[p] + deg;
+        for (int t = 0; t < deg; ++t) {
+            const int k = row_ptr[u] + t;
+            const int64_t v = col_idx[k];
+            ci[rp[p] + t] = l->pos_of[v];
+            edge[rp[p] + t] = (u < v) ? k : mirror[k];
+            slot_of_orig[k] = rp[p] + t;
+        }
+    }
+    l->maxdeg = maxdeg;
+    l->h_slot_of_orig = slot_of_orig;
+    l->h_mirror = mirror;
+    auto cleanup = [&](hipError_t e) {
+        ctx->err = std::string("csr upload: ") + hipGetErrorString(e);
+        mq_lattice_destroy(l);
+        return MQ_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&l->d_rp, sizeof(int32_t) * (N + 1))) != hipSuccess) return cleanup(e);
+    if ((e = hipMalloc(&l->d_ci, sizeof(int32_t) * std::max<int64_t>(nnz, 1))) != hipSuccess) return cleanup(e);
+    if ((e = hipMalloc(&l->d_edge, sizeof(int32_t) * std::max<int64_t>(nnz, 1))) != hipSuccess) return cleanup(e);
+    if ((e = hipMalloc(&l->d_pos_of, sizeof(int32_t) * N)) != hipSuccess) return cleanup(e);
+    if ((e = hipMalloc(&l->d_vert_of, sizeof(int32_t) * N)) != hipSuccess) return cleanup(e);
+    if ((e = hipMemcpy(l->d_rp, rp.data(), sizeof(int32_t) * (N + 1), hipMemcpyHostToDevice)) != hipSuccess) return cleanup(e);
+    if (nnz) {
+        if ((e = hipMemcpy(l->d_ci, ci.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice)) != hipSuccess) return cleanup(e);
+        if ((e = hipMemcpy(l->d_edge, edge.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice)) != hipSuccess) return cleanup(e);
+    }
+    if ((e = hipMemcpy(l->d_pos_of, l->pos_of.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice)) != hipSuccess) return cleanup(e);
+    if ((e = hipMemcpy(l->d_vert_of, l->vert_of.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice)) != hipSuccess) return cleanup(e);
+    *out = l;
+    return MQ_OK;
+}
+
+extern "C" int mq_lattice_color_of(const mq_lattice* lat, int32_t* colour)
+{
+    if (!lat || !colour) return MQ_ERR_ARG;
+    if (lat->kind == LAT_CSR) {
+        std::copy(lat->colour.begin(), lat->colour.end(), colour);
+    } else {
+        for (int64_t i = 0; i < lat->N; ++i) colour[i] = regular_colour(lat->dim, lat->L, i);
+    }
+    return MQ_OK;
+}
+
+hipError_t launch_init_csr(mq_state* st, const Group& g, Key2 key)
+{
+    hipLaunchKernelGGL(k_csr_init, grid_for(g.lat->N, g.nrep), dim3(256), 0, st->ctx->stream, st->d_spins + g.off,
+                       g.stride, g.lat->d_vert_of, g.lat->N, st->p.model, st->p.q, g.rep0, key);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key)
+{
+    const mq_lattice* l = g.lat;
+    const int64_t c0 = l->color_off[colour], nc = l->color_off[colour + 1] - c0;
+    if (nc == 0) return hipSuccess;
+    const int model = st->p.model;
+    const bool bonds = st->d_bonds != nullptr;
+    hipStream_t s = st->ctx->stream;
+    if (model == MQ_HEISENBERG || model == MQ_PHI4) {
+        float4* b = reinterpret_cast<float4*>(st->d_spins + g.off);
+        const float J = (float)st->p.J, d = (float)st->p.delta, lam = (float)st->p.lambda, m = (float)st->p.mass;
+        if (bonds)
+            hipLaunchKernelGGL(k_csr_sweep_o3<true>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride / 16, l->d_rp,
+                               l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_betaf, c0, nc, model,
+                               st->p.proposal, J, d, lam, m, phase, st->cur_dyn, g.rep0, key);
+        else
+            hipLaunchKernelGGL(k_csr_sweep_o3<false>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride / 16, l->d_rp,
+                               l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_betaf, c0, nc, model,
+                               st->p.proposal, J, d, lam, m, phase, st->cur_dyn, g.rep0, key);
+    } else {
+        int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
+        if (bonds)
+            hipLaunchKernelGGL(k_csr_sweep_int8<true>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride, l->d_rp,
+                               l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc,
+                               model, st->p.q, phase, st->cur_dyn, g.rep0, key);
+        else
+            hipLaunchKernelGGL(k_csr_sweep_int8<false>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride, l->d_rp,
+                               l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc,
+                               model, st->p.q, phase, st->cur_dyn, g.rep0, key);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_measure_csr(mq_state* st, const Group& g)
+{
+    const mq_lattice* l = g.lat;
+    const bool vec = st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4;
+    const bool bonds = st->d_bonds != nullptr;
+    hipStream_t s = st->ctx->stream;
+    const dim3 grid = grid_for(l->N, g.nrep);
+#define MQ_M(B, V)                                                                                                  \
+    hipLaunchKernelGGL((k_csr_measure<B, V>), grid, dim3(256), 0, s, st->d_spins + g.off, g.stride, l->d_rp,       \
+                       l->d_ci, st->d_bonds, st->d_rep_bond_off, l->N, st->p.model, st->p.q, g.rep0, st->d_isum, \
+                       st->d_fsum)
+    if (vec) {
+        if (bonds) MQ_M(true, true);
+        else MQ_M(false, true);
+    } else {
+        if (bonds) MQ_M(true, false);
+        else MQ_M(false, false);
+    }
+#undef MQ_M
+    return hipGetLastError();
+}
+
+hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key)
+{
+    const mq_lattice* l = g.lat;
+    const int model = st->p.model;
+    hipStream_t s = st->ctx->stream;
+    const float J = (float)st->p.J;
+#define MQ_W(K)                                                                                                   \
+    hipLaunchKernelGGL((k_csr_wolff<K>), dim3(g.nrep), dim3(256), 0, s, st->d_spins + g.off, g.stride, l->N,      \
+                       l->d_rp, l->d_ci, l->d_edge, l->d_pos_of, nclusters, cl0, g.rep0, st->p.q, st->d_wthr,    \
+                       st->d_betaf, J, st->d_queue, st->d_visit, st->d_rep_site_off, st->d_csize, key)
+    if (model == MQ_ISING) MQ_W(CK_ISING);
+    else if (model == MQ_POTTS) MQ_W(CK_POTTS);
+    else if (model == MQ_HEISENBERG && st->p.embedding == MQ_EMBED_CARTESIAN) MQ_W(CK_CART);
+    else if (model == MQ_HEISENBERG || model == MQ_PHI4) MQ_W(CK_RDIR);
+    else return hipErrorInvalidValue;
+#undef MQ_W
+    return hipGetLastError();
+}

@@ -1,0 +1,286 @@
+// Parallel tempering, O(3) renormalisation and HIP-graph replay.
+//
+// PT follows the reference's acceptance (src/libmarqov/marqovscheduler.h:44-57,
+// 389-397, 445-465): endiff = S_aa - S_ab + S_bb - S_ba with S_xy = beta_x E(y),
+// i.e. (beta_a - beta_b)(E_a - E_b); ratio = calcMHratio(endiff) (exp with the
+// double range clamped); accept iff u < min(1, ratio).  The reference swaps the
+// state spaces of the two simulations; here the two replicas exchange their
+// temperatures (acceptance tables, beta) instead, which moves 2 table rows
+// rather than two lattices and leaves the Markov chain unchanged.  Pairs are
+// neighbouring temperatures (k, k+1) with k = parity mod 2 inside each group of
+// ntemp replicas (deterministic even/odd plan instead of the reference's random
+// pair plan, createPTplan marqovscheduler.h:202-211).  For models whose
+// Hamiltonian depends on beta only through on-site terms scaled by 1/beta
+// (Phi4, Phi4.h:110-124), beta_x E_x(y) = beta_x J I(y) + O(y), so only the
+// interaction sum I enters endiff.
+#include <cfloat>
+
+#include "mq_internal.hpp"
+
+int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0);
+
+namespace {
+
+__global__ void k_zero64(unsigned long long* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// interaction energy J*I per replica from the measurement sums
+__global__ void k_energy(const long long* __restrict__ isum, const double* __restrict__ fsum, int nrep, int vec,
+                         int potts, double J, double* __restrict__ energy)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrep) return;
+    energy[r] = vec ? J * fsum[(int64_t)r * 8] : J * (double)isum[(int64_t)r * 8];
+    (void)potts;
+}
+
+__device__ __forceinline__ double mh_ratio(double en)
+{
+    // calcMHratio (marqovscheduler.h:44-57)
+    if (en > log(DBL_MIN)) {
+        if (en > log(DBL_MAX)) return DBL_MAX;
+        return exp(en);
+    }
+    return 0.0;
+}
+
+__global__ void k_pt_swap(int ngroups, int ntemp, int parity, uint32_t step, const uint32_t* __restrict__ dyn, Key2 key,
+                          int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
+                          const double* __restrict__ energy, const double* __restrict__ ladder,
+                          uint32_t* __restrict__ thr, int tbl, uint32_t* __restrict__ wthr, float* __restrict__ betaf,
+                          double* __restrict__ betad, int32_t* __restrict__ acc)
+{
+    if (dyn) {
+        step += dyn[1];
+        if (parity < 0) parity = (int)(step & 1u);
+    }
+    if (parity < 0) parity = (int)(step & 1u);
+    const int npairs = (ntemp - parity) / 2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ngroups * npairs) return;
+    const int g = (int)(t / npairs);
+    const int k = parity + 2 * (int)(t - (int64_t)g * npairs);
+    if (k + 1 >= ntemp) return;
+    const int64_t sa = (int64_t)g * ntemp + k, sb = sa + 1;
+    const int a = rep_of_temp[sa], b = rep_of_temp[sb];
+    const double en = (ladder[sa] - ladder[sb]) * (energy[a] - energy[b]);
+    const double ratio = mh_ratio(en);
+    const uint4 w = philox4x32_10((uint32_t)k, step, (uint32_t)g, ST_PT, key);
+    const double u = (double)((((uint64_t)w.x << 32) | w.y) >> 11) * 0x1.0p-53;
+    if (!(u < fmin(1.0, ratio))) return;
+    rep_of_temp[sa] = b;
+    rep_of_temp[sb] = a;
+    temp_of_rep[a] = k + 1;
+    temp_of_rep[b] = k;
+    for (int e = 0; e < 2 * tbl; ++e) {
+        const uint32_t x = thr[(int64_t)a * 2 * tbl + e];
+        thr[(int64_t)a * 2 * tbl + e] = thr[(int64_t)b * 2 * tbl + e];
+        thr[(int64_t)b * 2 * tbl + e] = x;
+    }
+    const uint32_t ww = wthr[a];
+    wthr[a] = wthr[b];
+    wthr[b] = ww;
+    const float bf = betaf[a];
+    betaf[a] = betaf[b];
+    betaf[b] = bf;
+    const double bd = betad[a];
+    betad[a] = betad[b];
+    betad[b] = bd;
+    atomicAdd(acc + g, 1);
+}
+
+__global__ void k_advance(uint32_t* dyn, uint32_t dphase, uint32_t dstep)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        dyn[0] += dphase;
+        dyn[1] += dstep;
+    }
+}
+
+// FP64 renormalisation of float4 spins: s <- s / |s|
+__global__ void k_renorm(float4* __restrict__ s, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 v = s[i];
+    const double nn = sqrt((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+    if (nn > 0.0) {
+        v.x = (float)(v.x / nn);
+        v.y = (float)(v.y / nn);
+        v.z = (float)(v.z / nn);
+    }
+    s[i] = v;
+}
+
+hipError_t zero_fill(void* p, int64_t n64, hipStream_t s)
+{
+    if (n64 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero64, dim3((unsigned)((n64 + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long*>(p), n64);
+    return hipGetLastError();
+}
+
+// device energies (no host synchronisation: usable inside a graph)
+hipError_t energy_async(mq_state* st)
+{
+    hipStream_t s = st->ctx->stream;
+    hipError_t e;
+    if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
+    if ((e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
+    for (const auto& g : st->groups) {
+        e = (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
+        if (e != hipSuccess) return e;
+    }
+    const int vec = (st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4);
+    hipLaunchKernelGGL(k_energy, dim3((unsigned)((st->nrep + 255) / 256)), dim3(256), 0, s, st->d_isum, st->d_fsum,
+                       st->nrep, vec, st->p.model == MQ_POTTS, st->p.J, st->d_energy);
+    return hipGetLastError();
+}
+
+int ensure_pt(mq_state* st, int ntemp)
+{
+    if (ntemp < 2 || st->nrep % ntemp != 0)
+        return mq_fail(st->ctx, MQ_ERR_ARG, "ntemp must be >= 2 and divide the number of replicas");
+    for (int g = 0; g < st->nrep / ntemp; ++g)
+        for (int k = 1; k < ntemp; ++k)
+            if (st->lat[g * ntemp + k] != st->lat[g * ntemp])
+                return mq_fail(st->ctx, MQ_ERR_ARG, "replicas of one PT group must share a lattice");
+    if (st->pt_ntemp == ntemp && st->d_rep_of_temp) return MQ_OK;
+    if (!st->d_rep_of_temp) {
+        MQ_HIP(st->ctx, hipMalloc(&st->d_rep_of_temp, sizeof(int32_t) * st->nrep));
+        MQ_HIP(st->ctx, hipMalloc(&st->d_temp_of_rep, sizeof(int32_t) * st->nrep));
+        MQ_HIP(st->ctx, hipMalloc(&st->d_ladder, sizeof(double) * st->nrep));
+        MQ_HIP(st->ctx, hipMalloc(&st->d_pt_acc, sizeof(int32_t) * (st->nrep + 2)));
+    }
+    std::vector<int32_t> rot(st->nrep), tor(st->nrep);
+    for (int r = 0; r < st->nrep; ++r) {
+        rot[r] = r;
+        tor[r] = r % ntemp;
+    }
+    MQ_HIP(st->ctx, hipMemcpy(st->d_rep_of_temp, rot.data(), sizeof(int32_t) * st->nrep, hipMemcpyHostToDevice));
+    MQ_HIP(st->ctx, hipMemcpy(st->d_temp_of_rep, tor.data(), sizeof(int32_t) * st->nrep, hipMemcpyHostToDevice));
+    MQ_HIP(st->ctx, hipMemcpy(st->d_ladder, st->beta.data(), sizeof(double) * st->nrep, hipMemcpyHostToDevice));
+    st->pt_ntemp = ntemp;
+    return MQ_OK;
+}
+
+hipError_t pt_swap_async(mq_state* st, int ntemp, int parity, uint32_t step, Key2 key)
+{
+    const int ngroups = st->nrep / ntemp;
+    const int64_t work = (int64_t)ngroups * ((ntemp + 1) / 2);
+    hipLaunchKernelGGL(k_pt_swap, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st->ctx->stream, ngroups, ntemp,
+                       parity, step, st->cur_dyn, key, st->d_rep_of_temp, st->d_temp_of_rep, st->d_energy, st->d_ladder,
+                       st->d_thr, st->tbl, st->d_wthr, st->d_betaf, st->d_betad, st->d_pt_acc);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int mq_pt_swap(mq_state* st, int ntemp, int parity, uint64_t seed, uint64_t step, int32_t* accepted)
+{
+    if (!st) return MQ_ERR_ARG;
+    if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
+    if (parity != 0 && parity != 1) return mq_fail(st->ctx, MQ_ERR_ARG, "parity must be 0 or 1");
+    int rc = ensure_pt(st, ntemp);
+    if (rc) return rc;
+    const Key2 key{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    hipStream_t s = st->ctx->stream;
+    MQ_HIP(st->ctx, zero_fill(st->d_pt_acc, (st->nrep + 1) / 2, s));
+    MQ_HIP(st->ctx, energy_async(st));
+    MQ_HIP(st->ctx, pt_swap_async(st, ntemp, parity, (uint32_t)step, key));
+    if (accepted) {
+        MQ_HIP(st->ctx, hipMemcpyAsync(accepted, st->d_pt_acc, sizeof(int32_t) * (st->nrep / ntemp),
+                                       hipMemcpyDeviceToHost, s));
+    }
+    MQ_HIP(st->ctx, hipStreamSynchronize(s));
+    return MQ_OK;
+}
+
+extern "C" int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep)
+{
+    if (!st || !temp_of_rep) return MQ_ERR_ARG;
+    if (!st->d_temp_of_rep) {
+        for (int r = 0; r < st->nrep; ++r) temp_of_rep[r] = -1;
+        return MQ_OK;
+    }
+    MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+    MQ_HIP(st->ctx, hipMemcpy(temp_of_rep, st->d_temp_of_rep, sizeof(int32_t) * st->nrep, hipMemcpyDeviceToHost));
+    return MQ_OK;
+}
+
+extern "C" int mq_renormalize(mq_state* st)
+{
+    if (!st) return MQ_ERR_ARG;
+    if (st->site_bytes != 16) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "renormalisation applies to O(3) spins");
+    const int64_t n = st->spin_total / 16;
+    hipLaunchKernelGGL(k_renorm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st->ctx->stream,
+                       reinterpret_cast<float4*>(st->d_spins), n);
+    MQ_HIP(st->ctx, hipGetLastError());
+    return MQ_OK;
+}
+
+extern "C" int mq_run_graph(mq_state* st, int nsweeps, int ntemp, uint64_t seed, uint64_t sweep0, uint64_t step0,
+                            int steps, int repeats)
+{
+    if (!st) return MQ_ERR_ARG;
+    if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
+    if (st->p.model == MQ_EA_ISING && !st->d_bonds) return mq_fail(st->ctx, MQ_ERR_STATE, "EA couplings not set");
+    if (nsweeps < 0 || steps <= 0 || repeats <= 0) return mq_fail(st->ctx, MQ_ERR_ARG, "bad graph sizes");
+    const bool regular = st->lat[0]->kind == LAT_REGULAR;
+    const uint64_t per_sweep = regular ? 2 : 64;
+    const uint64_t total = sweep0 + (uint64_t)nsweeps * steps * repeats;
+    if (total >= (1ull << 31)) return mq_fail(st->ctx, MQ_ERR_ARG, "sweep counter exceeds 2^31");
+    if (ntemp > 1) {
+        int rc = ensure_pt(st, ntemp);
+        if (rc) return rc;
+    }
+    hipStream_t s = st->ctx->stream;
+    if (!st->d_dyn) MQ_HIP(st->ctx, hipMalloc(&st->d_dyn, 2 * sizeof(uint32_t)));
+    MQ_HIP(st->ctx, zero_fill(st->d_dyn, 1, s));
+    if (ntemp > 1) MQ_HIP(st->ctx, zero_fill(st->d_pt_acc, (st->nrep + 1) / 2, s));
+    MQ_HIP(st->ctx, hipStreamSynchronize(s));
+    const Key2 key{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    MQ_HIP(st->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    st->cur_dyn = st->d_dyn;
+    int rc = MQ_OK;
+    hipError_t e = hipSuccess;
+    for (int t = 0; t < steps && rc == MQ_OK && e == hipSuccess; ++t) {
+        rc = mq_metropolis_async(st, nsweeps, seed, sweep0 + (uint64_t)t * nsweeps);
+        if (rc == MQ_OK && ntemp > 1) {
+            e = energy_async(st);
+            if (e == hipSuccess) e = pt_swap_async(st, ntemp, -1, (uint32_t)(step0 + t), key);
+        }
+    }
+    if (rc == MQ_OK && e == hipSuccess) {
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, st->d_dyn, (uint32_t)(per_sweep * nsweeps * steps),
+                           (uint32_t)steps);
+        e = hipGetLastError();
+    }
+    st->cur_dyn = nullptr;
+    hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc != MQ_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return mq_fail(st->ctx, MQ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return mq_fail(st->ctx, MQ_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    for (int i = 0; i < repeats && e == hipSuccess; ++i) e = hipGraphLaunch(exec, s);
+    hipError_t es = hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess || es != hipSuccess)
+        return mq_fail(st->ctx, MQ_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(e != hipSuccess ? e : es));
+    return MQ_OK;
+}

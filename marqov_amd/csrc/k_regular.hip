@@ -121,12 +121,13 @@ __global__ void __launch_bounds__(256) k_sweep_ising(int8_t* __restrict__ base, 
                                                      const int8_t* __restrict__ bonds,
                                                      const int64_t* __restrict__ rep_bond_off,
                                                      const uint32_t* __restrict__ thr, int tbl, int L, int colour,
-                                                     uint32_t phase, int rep0, Key2 key)
+                                                     uint32_t phase, const uint32_t* __restrict__ dyn, int rep0, Key2 key)
 {
     __shared__ uint32_t s_thr[2 * 16];
     const int r = blockIdx.y;
     const int grep = rep0 + r;
     if (threadIdx.x < 2 * tbl) s_thr[threadIdx.x] = thr[(int64_t)grep * 2 * tbl + threadIdx.x];
+    if (dyn) phase += dyn[0];
     __syncthreads();
     const RegGeom G = reg_geom(DIM, L);
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,8 +194,10 @@ __global__ void __launch_bounds__(256) k_sweep_ising(int8_t* __restrict__ base, 
 // y+-1 / z+-1 rows are re-read from the same L2.
 __global__ void __launch_bounds__(256) k_sweep_ising3_row16(int8_t* __restrict__ base, int64_t stride,
                                                             const uint32_t* __restrict__ thr, int lgL, int colour,
-                                                            uint32_t phase, int rep0, int lgbpr, Key2 key)
+                                                            uint32_t phase, const uint32_t* __restrict__ dyn, int rep0,
+                                                            int lgbpr, Key2 key)
 {
+    if (dyn) phase += dyn[0];
     __shared__ uint32_t s_thi[13];
     __shared__ uint32_t s_tlo[13];
     const uint32_t B = gridDim.x;
@@ -312,15 +315,134 @@ __global__ void __launch_bounds__(256) k_sweep_ising3_row16(int8_t* __restrict__
     *reinterpret_cast<uint4*>(own + h0) = Snew;
 }
 
+
+// ------------------------------------------- LDS-resident multi-sweep kernel
+// Small lattices (whole replica plus its couplings fit in LDS): one workgroup
+// per replica loads the spins (and EA couplings) into LDS once, runs nsweeps
+// checkerboard sweeps there with a barrier between colours, and writes the
+// spins back.  One launch covers the replicas of every lattice size through a
+// small descriptor table.  Random numbers and acceptance are those of
+// k_sweep_ising (group g = h >> 3, phase 2*sweep + colour), so the result is
+// bit-identical to nsweeps launches of the streaming kernel.
+struct LdsGroup {
+    int64_t off;     // byte offset of the group's first replica
+    int64_t stride;  // bytes per replica
+    int L;
+    uint32_t magic_h, magic_l;  // ceil(2^32 / (L/2)), ceil(2^32 / L): exact division for indices < 2^16
+    int rep0, nrep, block0;
+};
+
+template <int DIM>
+__device__ __forceinline__ void lds_nbrs(int L, int H, uint32_t mh, uint32_t ml, int colour, int h, int nb[2 * DIM])
+{
+    const int row = (int)__umulhi((uint32_t)h, mh);
+    const int k = h - row * H;
+    int x1 = row, x2 = 0;
+    if (DIM == 3) {
+        x2 = (int)__umulhi((uint32_t)row, ml);
+        x1 = row - x2 * L;
+    }
+    const int plane = H * L;
+    const int p = (colour + x1 + x2) & 1;
+    const int rb = h - k;
+    nb[0] = rb + (p ? ((k + 1 == H) ? 0 : k + 1) : k);
+    nb[1] = rb + (p ? k : ((k == 0) ? H - 1 : k - 1));
+    nb[2] = (x1 == L - 1) ? h - (L - 1) * H : h + H;
+    nb[3] = (x1 == 0) ? h + (L - 1) * H : h - H;
+    if (DIM == 3) {
+        nb[4] = (x2 == L - 1) ? h - (L - 1) * plane : h + plane;
+        nb[5] = (x2 == 0) ? h + (L - 1) * plane : h - plane;
+    }
+}
+
+template <int DIM, bool BONDS>
+__global__ void __launch_bounds__(256) k_sweep_lds(char* __restrict__ spins, const LdsGroup* __restrict__ groups,
+                                                   int ngroups, const int8_t* __restrict__ bonds,
+                                                   const int64_t* __restrict__ rep_bond_off,
+                                                   const uint32_t* __restrict__ thr, int tbl, int nsweeps,
+                                                   uint32_t sweep0, const uint32_t* __restrict__ dyn, Key2 key)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t s_thr[2 * 16];
+    int gi = 0;
+    while (gi + 1 < ngroups && (int)blockIdx.x >= groups[gi + 1].block0) ++gi;
+    const LdsGroup G = groups[gi];
+    const int r = (int)blockIdx.x - G.block0;
+    const int grep = G.rep0 + r;
+    const int L = G.L, H = L >> 1;
+    int N = L * L;
+    if (DIM == 3) N *= L;
+    const int half = N >> 1;
+    int8_t* S = reinterpret_cast<int8_t*>(smem);
+    int8_t* Bd = S + ((N + 15) & ~15);
+    int8_t* gs = reinterpret_cast<int8_t*>(spins + G.off + (int64_t)r * G.stride);
+    if (threadIdx.x < 2 * tbl) s_thr[threadIdx.x] = thr[(int64_t)grep * 2 * tbl + threadIdx.x];
+    for (int i = threadIdx.x; i < N; i += blockDim.x) S[i] = gs[i];
+    if (BONDS) {
+        const int8_t* gb = bonds + rep_bond_off[grep];
+        for (int i = threadIdx.x; i < DIM * N; i += blockDim.x) Bd[i] = gb[i];
+    }
+    uint32_t phase0 = 2u * sweep0;
+    if (dyn) phase0 += dyn[0];
+    __syncthreads();
+    constexpr int QMAX = 2 * DIM;
+    const int ngr = (half + 7) >> 3;
+    for (int t = 0; t < nsweeps; ++t) {
+        for (int colour = 0; colour < 2; ++colour) {
+            const uint32_t phase = phase0 + 2u * (uint32_t)t + (uint32_t)colour;
+            int8_t* own = S + colour * half;
+            const int8_t* oth = S + (1 - colour) * half;
+            const int8_t* bown = Bd + colour * DIM * half;
+            const int8_t* both = Bd + (1 - colour) * DIM * half;
+            for (int g = threadIdx.x; g < ngr; g += blockDim.x) {
+                const uint4 w = philox4x32_10((uint32_t)g, phase, (uint32_t)grep, ST_METRO, key);
+                bool have_tie = false;
+                uint4 tie = make_uint4(0, 0, 0, 0);
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int h = 8 * g + m;
+                    if (h >= half) break;
+                    int nb[2 * DIM];
+                    lds_nbrs<DIM>(L, H, G.magic_h, G.magic_l, colour, h, nb);
+                    int hs = 0;
+#pragma unroll
+                    for (int sl = 0; sl < 2 * DIM; ++sl) {
+                        int b = 1;
+                        if (BONDS) b = (sl & 1) ? both[(sl >> 1) * half + nb[sl]] : bown[(sl >> 1) * half + h];
+                        hs += b * oth[nb[sl]];
+                    }
+                    const int s = own[h];
+                    const int e = s * hs + QMAX;
+                    const uint32_t hi16 = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+                    const uint32_t thi = s_thr[2 * e];
+                    bool acc = hi16 < thi;
+                    if (hi16 == thi) {
+                        if (!have_tie) {
+                            tie = philox4x32_10((uint32_t)g, phase, (uint32_t)grep, ST_TIE, key);
+                            have_tie = true;
+                        }
+                        acc = ((u4_get(tie, m >> 1) >> (16 * (m & 1))) & 0xFFFFu) < s_thr[2 * e + 1];
+                    }
+                    if (acc) own[h] = (int8_t)(-s);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < N; i += blockDim.x) gs[i] = S[i];
+}
+
 // ----------------------------------------------------------------- Potts sweep
 // One Philox block per site: word 0 -> proposal in [0,Q) (Potts.h:171-177),
 // word 1 -> 32-bit uniform against the threshold of dE = k*(#new - #old).
 template <int DIM>
 __global__ void __launch_bounds__(256) k_sweep_potts(int8_t* __restrict__ base, int64_t stride,
                                                      const uint32_t* __restrict__ thr, int tbl, int L, int Q,
-                                                     int colour, uint32_t phase, int rep0, Key2 key)
+                                                     int colour, uint32_t phase, const uint32_t* __restrict__ dyn,
+                                                     int rep0, Key2 key)
 {
     __shared__ uint32_t s_thr[2 * 16];
+    if (dyn) phase += dyn[0];
     const int r = blockIdx.y;
     const int grep = rep0 + r;
     if (threadIdx.x < 2 * tbl) s_thr[threadIdx.x] = thr[(int64_t)grep * 2 * tbl + threadIdx.x];
@@ -356,8 +478,9 @@ __global__ void __launch_bounds__(256) k_sweep_o3(float4* __restrict__ base, int
                                                   const int64_t* __restrict__ rep_bond_off,
                                                   const float* __restrict__ betaf, int L, int model, int proposal,
                                                   float J, float delta, float lambda, float mass, int colour,
-                                                  uint32_t phase, int rep0, Key2 key)
+                                                  uint32_t phase, const uint32_t* __restrict__ dyn, int rep0, Key2 key)
 {
+    if (dyn) phase += dyn[0];
     const int r = blockIdx.y;
     const int grep = rep0 + r;
     const RegGeom G = reg_geom(DIM, L);
@@ -589,41 +712,41 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
         const dim3 grid = grid_for((half + 7) / 8, 256, g.nrep);
         const bool pow2 = (L & (L - 1)) == 0;
-        if (dim == 3 && !bonds && pow2 && L >= 32 && L <= 1024 && st->tbl == 13 && !st->generic_only) {
+        if (dim == 3 && !bonds && pow2 && L >= 32 && L <= 1024 && st->tbl == 13 && st->kernel_choice != 2) {
             int lgL = 0;
             while ((1 << lgL) < L) ++lgL;
             const int lgbpr = 3 * lgL - 13;  // (N/2)/16/256 blocks per replica
             const int64_t nblk = ((int64_t)1 << lgbpr) * g.nrep;
             if (nblk < (1ll << 31)) {
                 hipLaunchKernelGGL(k_sweep_ising3_row16, dim3((unsigned)nblk), dim3(256), 0, s, b, g.stride,
-                                   st->d_thr, lgL, colour, phase, g.rep0, lgbpr, key);
+                                   st->d_thr, lgL, colour, phase, st->cur_dyn, g.rep0, lgbpr, key);
                 return hipGetLastError();
             }
         }
         if (dim == 3) {
             if (bonds)
                 hipLaunchKernelGGL((k_sweep_ising<3, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
-                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
             else
                 hipLaunchKernelGGL((k_sweep_ising<3, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
-                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
         } else {
             if (bonds)
                 hipLaunchKernelGGL((k_sweep_ising<2, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
-                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
             else
                 hipLaunchKernelGGL((k_sweep_ising<2, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
-                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, g.rep0, key);
+                                   st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
         }
     } else if (model == MQ_POTTS) {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
         const dim3 grid = grid_for(half, 256, g.nrep);
         if (dim == 3)
             hipLaunchKernelGGL(k_sweep_potts<3>, grid, dim3(256), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
-                               colour, phase, g.rep0, key);
+                               colour, phase, st->cur_dyn, g.rep0, key);
         else
             hipLaunchKernelGGL(k_sweep_potts<2>, grid, dim3(256), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
-                               colour, phase, g.rep0, key);
+                               colour, phase, st->cur_dyn, g.rep0, key);
     } else {
         float4* b = reinterpret_cast<float4*>(st->d_spins + g.off);
         const int64_t s4 = g.stride / 16;
@@ -632,19 +755,19 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
         if (dim == 3) {
             if (bonds)
                 hipLaunchKernelGGL((k_sweep_o3<3, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
-                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, g.rep0, key);
+                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0, key);
             else
                 hipLaunchKernelGGL((k_sweep_o3<3, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds,
                                    st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
-                                   colour, phase, g.rep0, key);
+                                   colour, phase, st->cur_dyn, g.rep0, key);
         } else {
             if (bonds)
                 hipLaunchKernelGGL((k_sweep_o3<2, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
-                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, g.rep0, key);
+                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0, key);
             else
                 hipLaunchKernelGGL((k_sweep_o3<2, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds,
                                    st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
-                                   colour, phase, g.rep0, key);
+                                   colour, phase, st->cur_dyn, g.rep0, key);
         }
     }
     return hipGetLastError();
@@ -692,5 +815,59 @@ hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double
     const dim3 grid = grid_for((N * dim + 3) / 4, 256, 1);
     if (dim == 3) hipLaunchKernelGGL(k_bonds<3>, grid, dim3(256), 0, st->ctx->stream, out, L, T, sample, key);
     else hipLaunchKernelGGL(k_bonds<2>, grid, dim3(256), 0, st->ctx->stream, out, L, T, sample, key);
+    return hipGetLastError();
+}
+
+// Multi-sweep launch over all groups when every replica (and its couplings) fits
+// in 64 KiB of LDS.  Returns hipErrorNotSupported when the state does not qualify.
+hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key)
+{
+    const int model = st->p.model;
+    if (model != MQ_ISING && model != MQ_EA_ISING) return hipErrorNotSupported;
+    const bool bonds = (model == MQ_EA_ISING) || (st->d_bonds != nullptr);
+    const int dim = st->lat[0]->dim;
+    if (st->lat[0]->kind != LAT_REGULAR) return hipErrorNotSupported;
+    std::vector<LdsGroup> gs;
+    int blocks = 0;
+    size_t lds = 0;
+    for (const auto& g : st->groups) {
+        const int64_t N = g.lat->N;
+        const size_t need = (size_t)((N + 15) & ~15ll) + (bonds ? (size_t)(N * dim) : 0);
+        if (N >= 65536 || need > 65536 - 256) return hipErrorNotSupported;
+        lds = std::max(lds, need);
+        LdsGroup d;
+        d.off = g.off;
+        d.stride = g.stride;
+        d.L = g.lat->L;
+        d.magic_h = (uint32_t)((((uint64_t)1 << 32) + (uint64_t)(g.lat->L / 2) - 1) / (uint64_t)(g.lat->L / 2));
+        d.magic_l = (uint32_t)((((uint64_t)1 << 32) + (uint64_t)g.lat->L - 1) / (uint64_t)g.lat->L);
+        d.rep0 = g.rep0;
+        d.nrep = g.nrep;
+        d.block0 = blocks;
+        blocks += g.nrep;
+        gs.push_back(d);
+    }
+    if (gs.size() > 64) return hipErrorNotSupported;
+    if (!st->d_lds_groups) {
+        if (hipMalloc(&st->d_lds_groups, 64 * sizeof(LdsGroup)) != hipSuccess) return hipErrorOutOfMemory;
+    }
+    if (st->lds_groups_uploaded != (int)gs.size()) {
+        hipError_t e = hipMemcpy(st->d_lds_groups, gs.data(), gs.size() * sizeof(LdsGroup), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
+        st->lds_groups_uploaded = (int)gs.size();
+    }
+    const LdsGroup* dg = reinterpret_cast<const LdsGroup*>(st->d_lds_groups);
+    hipStream_t s = st->ctx->stream;
+#define MQ_L(D, B)                                                                                               \
+    hipLaunchKernelGGL((k_sweep_lds<D, B>), dim3(blocks), dim3(256), lds, s, st->d_spins, dg, (int)gs.size(),     \
+                       st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, nsweeps, sweep0, st->cur_dyn, key)
+    if (dim == 3) {
+        if (bonds) MQ_L(3, true);
+        else MQ_L(3, false);
+    } else {
+        if (bonds) MQ_L(2, true);
+        else MQ_L(2, false);
+    }
+#undef MQ_L
     return hipGetLastError();
 }

@@ -82,8 +82,12 @@ struct mq_lattice {
     std::vector<int32_t> h_rp, h_ci; // original numbering (host copy)
     int32_t* d_rp = nullptr;         // new numbering
     int32_t* d_ci = nullptr;
-    int32_t* d_edge = nullptr;       // canonical edge id of each slot
-    int32_t* d_slot_of_orig = nullptr;
+    int32_t* d_edge = nullptr;       // canonical (original) slot id of each slot
+    int32_t* d_pos_of = nullptr;     // original vertex -> position
+    int32_t* d_vert_of = nullptr;    // position -> original vertex
+    std::vector<int32_t> colour;     // colour of each original vertex
+    std::vector<int32_t> h_slot_of_orig;  // original CSR slot -> device slot
+    std::vector<int32_t> h_mirror;        // original slot (u->v) -> original slot (v->u)
     int maxdeg = 0;
     int64_t nnz = 0;
 };
@@ -138,7 +142,12 @@ struct mq_state {
     std::vector<double> beta_ladder;   // per replica index at creation (temperature k)
     double* d_ladder = nullptr;
     int pt_ntemp = 0;
-    bool generic_only = false;  // force the generic kernels (parity of the specialised ones)
+    bool generic_only = false;
+    const uint32_t* cur_dyn = nullptr;  // device phase/step offsets while a graph is captured
+    uint32_t* d_dyn = nullptr;          // [0] sweep-phase offset, [1] PT-step offset
+    void* d_lds_groups = nullptr;       // descriptor table of the LDS multi-sweep kernel
+    int lds_groups_uploaded = 0;
+    int kernel_choice = 0;              // 0 auto, 1 streaming kernels only, 2 generic streaming kernel only  // force the generic kernels (parity of the specialised ones)
 };
 
 // --------------------------------------------------------------------------
@@ -158,6 +167,7 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
 hipError_t launch_measure_regular(mq_state* st, const Group& g);
 hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
 hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key);
+hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key);
 
 hipError_t launch_init_csr(mq_state* st, const Group& g, Key2 key);
 hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);

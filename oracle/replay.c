@@ -386,3 +386,84 @@ EXPORT void orc_philox4x32_10_export(const uint32_t* ctr, const uint32_t* key, u
 {
     orc_philox4x32_10(ctr, key, out);
 }
+
+/* ------------------------------------------------------------ CSR graphs */
+/* Colour-class sweeps on a general graph (original vertex numbering); the
+ * colour of every vertex is an input (the library's greedy colouring).
+ * counter (vertex, 64*sweep + colour, replica, METRO): Ising/EA accept iff
+ * word 0 < T; Potts proposes (word0*Q)>>32 and accepts iff word 1 < T. */
+EXPORT void orc_csr_sweeps(int64_t N, const int32_t* rp, const int32_t* ci, const int32_t* colour, int ncol,
+                           int8_t* s, const int8_t* bonds, const uint32_t* thi, const uint32_t* tlo, int qmax,
+                           int potts, int Q, int nsweeps, uint64_t seed, uint32_t sweep0, uint32_t rep)
+{
+    for (int t = 0; t < nsweeps; ++t)
+        for (int c = 0; c < ncol; ++c) {
+            uint32_t phase = 64u * (sweep0 + (uint32_t)t) + (uint32_t)c;
+            for (int64_t v = 0; v < N; ++v) {
+                if (colour[v] != c) continue;
+                uint32_t w[4];
+                orc_philox_u64key((uint32_t)v, phase, rep, ST_METRO, seed, w);
+                int si = s[v];
+                if (potts) {
+                    int snew = (int)(((uint64_t)w[0] * (uint64_t)Q) >> 32);
+                    int dn = 0;
+                    for (int k = rp[v]; k < rp[v + 1]; ++k) dn += (s[ci[k]] == snew) - (s[ci[k]] == si);
+                    int e = dn + qmax;
+                    uint64_t T = ((uint64_t)thi[e] << 16) | tlo[e];
+                    if ((uint64_t)w[1] < T) s[v] = (int8_t)snew;
+                } else {
+                    int h = 0;
+                    for (int k = rp[v]; k < rp[v + 1]; ++k) h += (bonds ? bonds[k] : 1) * s[ci[k]];
+                    int e = si * h + qmax;
+                    uint64_t T = ((uint64_t)thi[e] << 16) | tlo[e];
+                    if ((uint64_t)w[0] < T) s[v] = (int8_t)(-si);
+                }
+            }
+        }
+}
+
+static int64_t csr_mirror(const int32_t* rp, const int32_t* ci, int64_t u, int64_t k)
+{
+    int64_t v = ci[k];
+    for (int64_t t = rp[v]; t < rp[v + 1]; ++t)
+        if (ci[t] == u) return t;
+    return -1;
+}
+
+/* Wolff on a graph: edge {u,v} keyed by its canonical slot (slot of v in row
+ * u for u < v).  potts = 0: Ising; 1: Potts with Q states. */
+EXPORT int orc_csr_wolff(int64_t N, const int32_t* rp, const int32_t* ci, int8_t* s, int potts, int Q, uint32_t tw,
+                         int nclusters, uint64_t seed, uint32_t cl0, uint32_t rep, int64_t* sizes)
+{
+    if (!ensure_work(N)) return -1;
+    for (int c = 0; c < nclusters; ++c) {
+        uint32_t ncl = cl0 + (uint32_t)c;
+        uint32_t w[4];
+        orc_philox_u64key(0u, ncl, rep, ST_WSEED, seed, w);
+        int64_t seedv = (int64_t)(((uint64_t)w[0] * (uint64_t)N) >> 32);
+        int val = s[seedv];
+        int newq = potts ? (val + 1 + (int)(((uint64_t)w[1] * (uint64_t)(Q - 1)) >> 32)) % Q : -val;
+        int64_t top = 0, size = 1;
+        g_stack[top++] = (int32_t)seedv;
+        g_mark[seedv] = 1;
+        while (top > 0) {
+            int64_t u = g_stack[--top];
+            for (int64_t k = rp[u]; k < rp[u + 1]; ++k) {
+                int64_t v = ci[k];
+                if (g_mark[v] || s[v] != val) continue;
+                int64_t id = (u < v) ? k : csr_mirror(rp, ci, u, k);
+                uint32_t b[4];
+                orc_philox_u64key((uint32_t)(id >> 2), ncl, rep, ST_WBOND, seed, b);
+                if (b[id & 3] < tw) {
+                    g_mark[v] = 1;
+                    g_stack[top++] = (int32_t)v;
+                    ++size;
+                }
+            }
+        }
+        for (int64_t i = 0; i < N; ++i)
+            if (g_mark[i]) { s[i] = (int8_t)newq; g_mark[i] = 0; }
+        if (sizes) sizes[c] = size;
+    }
+    return 0;
+}

@@ -34,6 +34,8 @@ _SIG = {
     "orc_bond_sum_ising": (_I64, [_I, _I, _P, _P]),
     "orc_mag_sum_ising": (_I64, [_I64, _P]),
     "orc_bond_sum_potts": (_I64, [_I, _I, _P]),
+    "orc_csr_sweeps": (None, [_I64, _P, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I, _I, _U64, _U32, _U32]),
+    "orc_csr_wolff": (_I, [_I64, _P, _P, _P, _I, _I, _U32, _I, _U64, _U32, _U32, _P]),
     "ref_run_regular": (_I, [_I, _I, _I, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_run_csr": (_I, [_I, _I64, _P, _P, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_mt19937_64_nth": (_U64, [_U64, _I64]),
@@ -166,3 +168,29 @@ def ref_run(model, dim, L, beta, J=-1.0, Q=3, lam=4.5, mass=1.0, ncluster=0, nme
     lib().ref_run_regular(model, dim, L, J, Q, lam, mass, None if b is None else p(b), beta, ncluster, nmetro, nwarm,
                           nmeas, seed, int(init), p(is_), p(ds), p(obs), p(cl), None)
     return (ds.reshape(N, 3) if vec else is_), obs[:nmeas], cl[:nmeas]
+
+
+def csr_sweeps(rp, ci, colour, s, beta, J, nsweeps, seed, sweep0, rep, potts=False, Q=3, bonds=None):
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    colour = np.ascontiguousarray(colour, np.int32)
+    N = len(rp) - 1
+    qmax = int(np.diff(rp).max())
+    hi, lo = thresholds_potts(beta, J, qmax) if potts else thresholds_ising(beta, J, qmax)
+    s = np.ascontiguousarray(s, np.int8).copy()
+    b = None if bonds is None else np.ascontiguousarray(bonds, np.int8)
+    lib().orc_csr_sweeps(N, p(rp), p(ci), p(colour), int(colour.max()) + 1, p(s), None if b is None else p(b),
+                         p(hi), p(lo), qmax, int(potts), Q, nsweeps, seed, sweep0, rep)
+    return s
+
+
+def csr_wolff(rp, ci, s, beta, J, nclusters, seed, cl0, rep, potts=False, Q=3):
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    N = len(rp) - 1
+    pr = -np.expm1(beta * J) if potts else -np.expm1(2.0 * beta * J)
+    tw = lib().orc_wolff_threshold(pr)
+    s = np.ascontiguousarray(s, np.int8).copy()
+    sizes = np.zeros(nclusters, np.int64)
+    lib().orc_csr_wolff(N, p(rp), p(ci), p(s), int(potts), Q, tw, nclusters, seed, cl0, rep, p(sizes))
+    return s, sizes

@@ -1,0 +1,105 @@
+"""Graph (CSR) path: greedy colouring is proper, and colour-class sweeps and Wolff
+clusters replay bit-exactly on the CPU given the library's colouring."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from marqov_amd import graphs
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(n, seed):
+    rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n), seed)
+    return rp, ci
+
+
+def _lat(ctx, rp, ci):
+    from marqov_amd import gpu
+    return gpu.CsrLattice(ctx, rp, ci)
+
+
+def _colours(ctx, lat):
+    c = np.zeros(lat.N, np.int32)
+    ctx.check(ctx.lib.mq_lattice_color_of(lat.h, c.ctypes.data), "mq_lattice_color_of")
+    return c
+
+
+def test_colouring_is_proper(ctx):
+    rp, ci = _graph(3000, 2)
+    lat = _lat(ctx, rp, ci)
+    col = _colours(ctx, lat)
+    src = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    assert (col[src] != col[ci]).all()
+    sizes = lat.colors()
+    assert sizes.sum() == len(rp) - 1 and len(sizes) == col.max() + 1
+
+
+@pytest.mark.parametrize("model,beta", [("ising", 0.6), ("ising", 2.0), ("potts", 1.2)])
+def test_csr_sweeps_bit_exact(ctx, model, beta):
+    from marqov_amd import gpu
+    rp, ci = _graph(2500, 3)
+    lat = _lat(ctx, rp, ci)
+    col = _colours(ctx, lat)
+    st = gpu.State(ctx, model, [lat, lat], J=-1.0, q=3)
+    st.set_beta([beta, beta / 2])
+    st.init(seed=8)
+    st.metropolis(5, seed=21, sweep0=4)
+    N = len(rp) - 1
+    for r in range(2):
+        s0 = ob.init_potts(N, 3, 8, r) if model == "potts" else ob.init_ising(N, 8, r)
+        exp = ob.csr_sweeps(rp, ci, col, s0, [beta, beta / 2][r], -1.0, 5, 21, 4, r, potts=(model == "potts"))
+        assert np.array_equal(st.download(r), exp), r
+
+
+def test_csr_ea_bonds_bit_exact(ctx):
+    from marqov_amd import gpu
+    rp, ci = _graph(1500, 4)
+    N = len(rp) - 1
+    lat = _lat(ctx, rp, ci)
+    col = _colours(ctx, lat)
+    # symmetric random couplings per edge
+    src = np.repeat(np.arange(N), np.diff(rp))
+    lo, hi = np.minimum(src, ci), np.maximum(src, ci)
+    h = (lo.astype(np.int64) * 2654435761 + hi * 40503) % 1000003
+    bonds = np.where(h % 2 == 0, 1, -1).astype(np.int8)
+    st = gpu.State(ctx, "ea", [lat], J=-1.0)
+    st.set_beta(1.3)
+    st.set_bonds(0, bonds)
+    st.init(seed=5)
+    st.metropolis(4, seed=6, sweep0=0)
+    exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(N, 5, 0), 1.3, -1.0, 4, 6, 0, 0, bonds=bonds)
+    assert np.array_equal(st.download(0), exp)
+
+
+@pytest.mark.parametrize("model,beta", [("ising", 0.9), ("potts", 1.6)])
+def test_csr_wolff_bit_exact(ctx, model, beta):
+    from marqov_amd import gpu
+    rp, ci = _graph(2000, 7)
+    N = len(rp) - 1
+    lat = _lat(ctx, rp, ci)
+    st = gpu.State(ctx, model, [lat, lat], J=-1.0, q=3)
+    st.set_beta(beta)
+    st.init(seed=2)
+    sizes = st.wolff(6, seed=13, cluster0=2)
+    for r in range(2):
+        s0 = ob.init_potts(N, 3, 2, r) if model == "potts" else ob.init_ising(N, 2, r)
+        exp, esz = ob.csr_wolff(rp, ci, s0, beta, -1.0, 6, 13, 2, r, potts=(model == "potts"))
+        assert np.array_equal(sizes[r], esz), (sizes[r], esz)
+        assert np.array_equal(st.download(r), exp)
+
+
+def test_csr_regular_graph_observables(ctx):
+    """The periodic square lattice given as a graph measures like the regular path."""
+    from marqov_amd import gpu
+    rp, ci = graphs.regular_csr(2, 8)
+    lat = _lat(ctx, rp, ci)
+    st = gpu.State(ctx, "ising", [lat], J=-1.0)
+    st.set_beta(0.4)
+    st.init(seed=1)
+    st.metropolis(3, seed=2)
+    s = st.download(0)
+    e = -1.0 * ob.lib().orc_bond_sum_ising(2, 8, ob.p(s), None) / 64
+    obs = st.measure()
+    assert obs[0, 0] == pytest.approx(e, abs=1e-12)
+    assert obs[0, 1] == pytest.approx(abs(int(s.astype(int).sum())) / 64, abs=1e-12)

@@ -174,3 +174,41 @@ def test_ising3_row16_antiferro_and_hot(ctx):
         st.metropolis(3, seed=5, sweep0=0)
         exp = ob.ising_sweeps(dim, L, ob.init_ising(L ** 3, 3, 0), beta, J, 3, 5, 0, 0)
         assert np.array_equal(st.download(0), exp), (J, beta)
+
+
+@pytest.mark.parametrize("kernels", ["0", "1"])
+def test_mixed_L_ea_batch_bit_exact(ctx, kernels, monkeypatch):
+    """Mixed lattice sizes in one state (LDS multi-sweep kernel when kernels=0,
+    streaming kernels when kernels=1) against the CPU replay."""
+    from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
+    dim = 3
+    Ls = [4, 6, 8]
+    lats = {L: gpu.RegularLattice(ctx, dim, L) for L in Ls}
+    reps = [lats[L] for L in Ls for _ in range(4)]
+    st = gpu.State(ctx, "ea", reps, J=-1.0)
+    betas = np.tile([0.5, 1.1, 2.0, 3.3], 3)
+    st.set_beta(betas)
+    sample = np.repeat(np.arange(3), 4)
+    st.random_bonds(rep_sample=sample, p_plus=0.5, seed=4)
+    st.init(seed=12)
+    st.metropolis(5, seed=3, sweep0=1)
+    for r, lat in enumerate(reps):
+        N = lat.N
+        b = ob.init_bonds(dim, N, 0.5, 4, int(sample[r]))
+        exp = ob.ising_sweeps(dim, lat.L, ob.init_ising(N, 12, r), betas[r], -1.0, 5, 3, 1, r, bonds=b)
+        assert np.array_equal(st.download(r), exp), (kernels, r)
+
+
+@pytest.mark.parametrize("kernels", ["0", "1"])
+def test_2d_ising_batch_bit_exact(ctx, kernels, monkeypatch):
+    from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
+    lat = gpu.RegularLattice(ctx, 2, 64)
+    st = gpu.State(ctx, "ising", [lat] * 3, J=-1.0)
+    st.set_beta([0.4406868, 0.3, 0.6])
+    st.init(seed=2)
+    st.metropolis(4, seed=9, sweep0=0)
+    for r in range(3):
+        exp = ob.ising_sweeps(2, 64, ob.init_ising(4096, 2, r), [0.4406868, 0.3, 0.6][r], -1.0, 4, 9, 0, r)
+        assert np.array_equal(st.download(r), exp), (kernels, r)

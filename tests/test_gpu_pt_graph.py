@@ -1,0 +1,115 @@
+"""Parallel tempering and HIP-graph replay, bit-exact against a CPU replay."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import exact_path  # noqa: F401  (puts oracle/ on sys.path)
+import exact
+
+pytestmark = pytest.mark.gpu
+
+DIM, L = 3, 4
+N = L ** DIM
+NT = 8          # temperatures per sample
+NS = 3          # samples
+BETAS = 0.5 * (3.3 / 0.5) ** (np.arange(NT) / (NT - 1))
+
+
+def _ea_state(ctx):
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, DIM, L)
+    st = gpu.State(ctx, "ea", [lat] * (NS * NT), J=-1.0)
+    st._lat = lat
+    st.set_beta(np.tile(BETAS, NS))
+    st.random_bonds(rep_sample=np.repeat(np.arange(NS), NT), p_plus=0.5, seed=4)
+    st.init(seed=1)
+    return st
+
+
+class Replay:
+    """CPU replay of sweeps + even/odd PT swaps (marqovscheduler.h:44-57,389-397)."""
+
+    def __init__(self):
+        self.s = [ob.init_ising(N, 1, r) for r in range(NS * NT)]
+        self.bonds = [ob.init_bonds(DIM, N, 0.5, 4, k) for k in range(NS)]
+        self.rot = np.arange(NS * NT)          # slot -> replica
+        self.tor = np.tile(np.arange(NT), NS)  # replica -> temperature index
+        self.nb = exact.neighbour_table(DIM, L)
+        self.B = [exact.bond_table(DIM, L, b) for b in self.bonds]
+
+    def sweeps(self, n, seed, sweep0):
+        for r in range(NS * NT):
+            self.s[r] = ob.ising_sweeps(DIM, L, self.s[r], BETAS[self.tor[r]], -1.0, n, seed, sweep0, r,
+                                        bonds=self.bonds[r // NT])
+
+    def energy(self, r):
+        si = self.s[r].astype(np.int64)
+        return -1.0 * (si[:, None] * si[self.nb] * self.B[r // NT]).sum() / 2.0
+
+    def swap(self, parity, seed, step):
+        acc = np.zeros(NS, np.int32)
+        for g in range(NS):
+            for k in range(parity, NT - 1, 2):
+                sa, sb = g * NT + k, g * NT + k + 1
+                a, b = self.rot[sa], self.rot[sb]
+                en = (BETAS[k] - BETAS[k + 1]) * (self.energy(a) - self.energy(b))
+                ratio = 0.0 if en <= np.log(np.finfo(float).tiny) else (
+                    np.finfo(float).max if en > np.log(np.finfo(float).max) else np.exp(en))
+                w = ob.philox([k, step & 0xffffffff, g, 5], [seed & 0xffffffff, seed >> 32])
+                u = (((w[0] << 32) | w[1]) >> 11) * 2.0 ** -53
+                if u < min(1.0, ratio):
+                    self.rot[sa], self.rot[sb] = b, a
+                    self.tor[a], self.tor[b] = k + 1, k
+                    acc[g] += 1
+        return acc
+
+
+def test_pt_swaps_bit_exact(ctx):
+    st = _ea_state(ctx)
+    rp = Replay()
+    for step in range(6):
+        st.metropolis(2, seed=7, sweep0=2 * step)
+        rp.sweeps(2, 7, 2 * step)
+        acc = st.pt_swap(NT, step & 1, seed=9, step=step)
+        eacc = rp.swap(step & 1, 9, step)
+        assert np.array_equal(acc, eacc), (step, acc, eacc)
+        assert np.array_equal(st.pt_temps(), rp.tor), step
+    for r in range(NS * NT):
+        assert np.array_equal(st.download(r), rp.s[r]), r
+    b = np.zeros(NS * NT)
+    ctx.check(ctx.lib.mq_state_get_beta(st.h, b.ctypes.data), "get_beta")
+    assert np.array_equal(b, BETAS[rp.tor])
+
+
+def test_graph_replays_match_eager(ctx):
+    """Each graph replay (with eager work and synchronisation in between) equals the eager run."""
+    g = _ea_state(ctx)
+    e = _ea_state(ctx)
+    nsw, steps = 2, 3
+    done = 0
+    for call, repeats in enumerate((1, 1, 2)):
+        g.run_graph(nsw, NT, seed=11, sweep0=done * nsw, step0=done, steps=steps, repeats=repeats)
+        for t in range(done, done + steps * repeats):
+            e.metropolis(nsw, seed=11, sweep0=t * nsw)
+            e.pt_swap(NT, t & 1, seed=11, step=t)
+        done += steps * repeats
+        ctx.sync()
+        assert np.array_equal(g.pt_temps(), e.pt_temps()), call
+        for r in range(NS * NT):
+            assert np.array_equal(g.download(r), e.download(r)), (call, r)
+        # eager work on the graph's state between replays
+        g.measure()
+
+
+def test_graph_sweeps_only_regular_ising(ctx):
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, 3, 32)
+    a = gpu.State(ctx, "ising", [lat, lat], J=-1.0)
+    b = gpu.State(ctx, "ising", [lat, lat], J=-1.0)
+    for st in (a, b):
+        st.set_beta([0.22, 0.3])
+        st.init(seed=3)
+    a.run_graph(3, 0, seed=5, sweep0=0, step0=0, steps=2, repeats=3)
+    b.metropolis(18, seed=5, sweep0=0)
+    for r in range(2):
+        assert np.array_equal(a.download(r), b.download(r))

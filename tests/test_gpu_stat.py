@@ -1,0 +1,121 @@
+"""Statistical parity of the HIP path (FP32 local arithmetic, FP64 observables)
+against the reference-faithful FP64 CPU restatement and exact results.
+
+Independent replicas give clean error bars: per-replica time averages, then
+mean +- standard error over replicas (GPU) or seeds (CPU).  The tolerance is
+4 combined standard errors."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import exact_path  # noqa: F401
+import exact
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_series(ctx, model, dim, L, beta, nrep, nwarm, nmeas, ncluster=1, nmetro=1, seed=3, **kw):
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, dim, L)
+    st = gpu.State(ctx, model, [lat] * nrep, **kw)
+    st.set_beta(beta)
+    st.init(seed=seed)
+    for t in range(nwarm):
+        st.emcs(ncluster, nmetro, seed, t)
+    obs = np.zeros((nmeas, nrep, 4))
+    for t in range(nmeas):
+        st.emcs(ncluster, nmetro, seed, nwarm + t)
+        obs[t] = st.measure()
+    return obs, st
+
+
+def cpu_series(model, dim, L, beta, nseeds, nwarm, nmeas, ncluster=1, nmetro=1, **kw):
+    res = []
+    for s in range(nseeds):
+        _, o, _ = ob.ref_run(model, dim, L, beta, ncluster=ncluster, nmetro=nmetro, nwarm=nwarm, nmeas=nmeas,
+                             seed=100 + s, **kw)
+        res.append(o)
+    return np.array(res)  # seeds x nmeas x 2
+
+
+def agree(gpu_per_rep, cpu_per_seed, k=4.0):
+    g, c = np.asarray(gpu_per_rep), np.asarray(cpu_per_seed)
+    sg = g.std(ddof=1) / np.sqrt(len(g))
+    sc = c.std(ddof=1) / np.sqrt(len(c))
+    d = abs(g.mean() - c.mean())
+    return d < k * np.hypot(sg, sc), (g.mean(), sg, c.mean(), sc)
+
+
+@pytest.mark.parametrize("proposal,embedding", [("fresh", "cartesian"), ("small", "random")])
+def test_heisenberg_vs_fp64_restatement(ctx, proposal, embedding):
+    beta = 0.693
+    obs, _ = gpu_series(ctx, "heisenberg", 3, 6, beta, 64, 100, 400, J=-1.0, proposal=proposal, delta=0.6,
+                        embedding=embedding)
+    cpu = cpu_series(ob.HEIS, 3, 6, beta, 12, 100, 1500, J=-1.0)
+    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1))
+    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1))
+    assert ok_e, ie
+    assert ok_m, im
+
+
+def test_phi4_vs_fp64_restatement(ctx):
+    beta, lam, mass = 0.69, 4.5, 1.0  # config/Phi4.ini
+    obs, _ = gpu_series(ctx, "phi4", 3, 6, beta, 64, 100, 400, J=-1.0, lam=lam, mass=mass)
+    cpu = cpu_series(ob.PHI4, 3, 6, beta, 12, 100, 1500, J=-1.0, lam=lam, mass=mass)
+    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1))
+    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1))
+    assert ok_e, ie
+    assert ok_m, im
+
+
+def test_potts3_2d_vs_restatement(ctx):
+    beta = 0.9
+    obs, _ = gpu_series(ctx, "potts", 2, 8, beta, 64, 100, 400, J=-1.0, q=3)
+    cpu = cpu_series(ob.POTTS, 2, 8, beta, 12, 100, 1500, J=-1.0, Q=3)
+    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1))
+    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1))
+    assert ok_e, ie
+    assert ok_m, im
+
+
+def test_ising_c1_energy_vs_kaufman(ctx):
+    """configs[0] schedule (10 Wolff + 1 sweep per EMCS) at L=64, beta_c, many replicas."""
+    L, beta = 64, 0.4406868
+    obs, _ = gpu_series(ctx, "ising", 2, L, beta, 256, 200, 300, ncluster=10, nmetro=1, J=-1.0)
+    e = obs[:, :, 0].mean(0)
+    ka = exact.kaufman_energy(beta, L, L)
+    se = e.std(ddof=1) / np.sqrt(len(e))
+    assert abs(e.mean() - ka) < 4 * se, (e.mean(), se, ka)
+    m2, m4 = obs[:, :, 2].mean(), obs[:, :, 3].mean()
+    binder = 1 - m4 / (3 * m2 * m2)
+    assert 0.55 < binder < 0.66, binder  # U* = 0.6107 (Salas & Sokal 2000) at the critical point
+
+
+def test_heisenberg_renormalize_keeps_unit_length(ctx):
+    _, st = gpu_series(ctx, "heisenberg", 3, 8, 0.693, 2, 0, 50, J=-1.0, proposal="small", delta=0.5)
+    s = st.download(0).astype(np.float64)
+    drift = np.abs(np.linalg.norm(s, axis=1) - 1).max()
+    st.renormalize()
+    s2 = st.download(0).astype(np.float64)
+    assert np.abs(np.linalg.norm(s2, axis=1) - 1).max() <= 1e-6
+    assert drift < 1e-4
+
+
+def test_full_size_c2_properties(ctx):
+    """configs[1] size (3D L=256): measured energy equals the energy of the downloaded
+    state; a sweep at beta = 0 accepts every move, so it flips every spin exactly once."""
+    from marqov_amd import gpu
+    L = 256
+    lat = gpu.RegularLattice(ctx, 3, L)
+    st = gpu.State(ctx, "ising", [lat], J=-1.0)
+    st.set_beta(0.2216546)
+    st.init(seed=2)
+    st.metropolis(20, seed=2)
+    s = st.download(0)
+    e_host = -1.0 * ob.lib().orc_bond_sum_ising(3, L, ob.p(s), None) / L ** 3
+    obs = st.measure()
+    assert obs[0, 0] == pytest.approx(e_host, abs=1e-12)
+    assert -1.2 < obs[0, 0] < -0.9
+    st.set_beta(0.0)
+    st.metropolis(1, seed=3, sweep0=100)
+    assert np.array_equal(st.download(0), -s)  # every site is offered one flip per sweep
