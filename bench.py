@@ -1,26 +1,30 @@
 #!/usr/bin/env python3
 """Throughput of the MARQOV update path on one MI355X.
 
-Headline workload: 3D simple-cubic Ising, L=256 (16.8M sites), periodic, J=-1,
+Headline (value): 3D simple-cubic Ising, L=256 (16.8M sites), periodic, J=-1,
 beta=0.2216546, int8 spins in the colour-split checkerboard layout, Philox4x32-10,
-R replicas of that lattice swept together (batched; R=32 puts the working set
-above the 256 MB Infinity Cache).  A step is one full checkerboard Metropolis
-sweep of every replica.  value = spin-flip attempts/ns over the timed steps.
+R=32 replicas of that lattice swept together (working set 512 MiB, above the
+256 MB Infinity Cache).  A step is one full checkerboard Metropolis sweep of
+every replica; value = spin-flip attempts/ns over the timed steps.
 
-Beside it on the same JSON line:
-  plain     - the same sweep, one replica per launch (attempts/ns)
-  wolff     - Wolff sites/ns: 2D Ising L=64 at beta_c, 10 clusters per replica
-              per step, batched (many replicas, one workgroup each) vs plain (1)
-  roofline  - the sweep kernel's algorithmic HBM rate (3 B per attempt: two reads
-              and one write of the int8 lattice per sweep) vs 8 TB/s
-  cpu_baseline - the reference-faithful CPU restatement (random-site Metropolis,
-              mt19937_64) timed on this host's cores, one replica per thread.
+Beside it on the same JSON line (skip with --quick):
+  plain        the same sweep, one replica per launch
+  c1_2d_L64    configs[0] schedule (10 Wolff + 1 sweep per EMCS), batched vs plain
+  c3_heis_L128 Heisenberg FP32 sweeps + Wolff, batched vs plain
+  c4_ea_batch  512 +-J samples x 32 temperatures, L in {8,12,16,24}, HIP-graph
+               captured {sweep, PT swap}, batched vs plain
+  c5_rgg       2D random geometric graph N=2^20, Ising and Potts-3, 64 replicas vs 1
+  roofline     the headline sweep kernel's algorithmic HBM rate (3 B/attempt)
+  cpu_baseline the reference-faithful CPU restatement (random-site Metropolis,
+               mt19937_64), one replica per thread on this host's cores
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -35,22 +39,134 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--replicas", type=int, default=32)
     ap.add_argument("--L", type=int, default=256)
+    ap.add_argument("--quick", action="store_true", help="headline only (no configs[0],[2..4] legs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extra", action="store_true", help="skip plain and Wolff legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM traffic)")
     return ap.parse_args()
 
 
-def time_sweeps(ctx, st, steps, warmup, seed):
-    st.metropolis(warmup, seed=seed, sweep0=0)
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(ctx, fn):
     ctx.sync()
     t0 = time.perf_counter()
     ctx.timer_begin()
-    st.metropolis(steps, seed=seed, sweep0=warmup)
+    fn()
     ms = ctx.timer_end()
-    wall = time.perf_counter() - t0
-    return ms, wall
+    return ms, time.perf_counter() - t0
+
+
+def leg_c1(gpu, ctx):
+    """2D Ising L=64 beta_c, EMCS = 10 Wolff + 1 sweep."""
+    lat = gpu.RegularLattice(ctx, 2, 64)
+    res = {}
+    for name, nrep, nstep in (("batched", 1024, 20), ("plain", 1, 20)):
+        st = gpu.State(ctx, "ising", [lat] * nrep, J=-1.0)
+        st.set_beta(0.4406868)
+        st.init(seed=1)
+        for t in range(10):
+            st.emcs(10, 1, 1, t)
+        sizes = st.wolff(10, 1, cluster0=10 ** 6, sizes=True)
+        mean_cl = float(sizes.mean())
+        ms_w, _ = timed(ctx, lambda: st.wolff(10 * nstep, 1, cluster0=2 * 10 ** 6, sizes=False))
+        ms_m, _ = timed(ctx, lambda: st.metropolis(nstep, 1, sweep0=10 ** 5))
+        res[name] = {"replicas": nrep, "wolff_sites_per_ns": round(nrep * 10 * nstep * mean_cl / (ms_w * 1e6), 4),
+                     "attempts_per_ns": round(nrep * 4096 * nstep / (ms_m * 1e6), 3),
+                     "mean_cluster": round(mean_cl, 1),
+                     "emcs_per_s": round(nrep * nstep / ((ms_w + ms_m) * 1e-3), 1)}
+        del st
+    res["batched_over_plain_attempts"] = round(res["batched"]["attempts_per_ns"] / res["plain"]["attempts_per_ns"], 2)
+    res["batched_over_plain_wolff"] = round(res["batched"]["wolff_sites_per_ns"] / res["plain"]["wolff_sites_per_ns"], 2)
+    return res
+
+
+def leg_c3(gpu, ctx):
+    """3D Heisenberg L=128 beta=0.693, FP32 float4 spins, small-angle proposal."""
+    L, beta = 128, 0.693
+    lat = gpu.RegularLattice(ctx, 3, L)
+    N = L ** 3
+    res = {}
+    for name, nrep, nstep in (("batched", 8, 20), ("plain", 1, 20)):
+        st = gpu.State(ctx, "heisenberg", [lat] * nrep, J=-1.0, proposal="small", delta=0.6, embedding="cartesian")
+        st.set_beta(beta)
+        st.init(seed=3)
+        st.metropolis(5, 3, 0)
+        ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
+        sizes = st.wolff(4, 3, cluster0=0, sizes=True)
+        ms_w, _ = timed(ctx, lambda: st.wolff(8, 3, cluster0=4, sizes=False))
+        res[name] = {"replicas": nrep, "attempts_per_ns": round(nrep * N * nstep / (ms * 1e6), 3),
+                     "wolff_sites_per_ns": round(nrep * 8 * float(sizes.mean()) / (ms_w * 1e6), 4),
+                     "mean_cluster": round(float(sizes.mean()), 1),
+                     "roofline_frac": round(48.0 * nrep * N * nstep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if name == "batched":
+            obs = st.measure()
+            res[name]["e"] = float(obs[:, 0].mean())
+        del st
+    res["batched_over_plain"] = round(res["batched"]["attempts_per_ns"] / res["plain"]["attempts_per_ns"], 2)
+    return res
+
+
+def leg_c4(gpu, ctx):
+    """512 EA samples (128 each of L=8,12,16,24) x 32 temperatures."""
+    Ls = [8, 12, 16, 24]
+    nt, per = 32, 128
+    betas = 0.5 * (3.3 / 0.5) ** (np.arange(nt) / (nt - 1))
+    lats = {L: gpu.RegularLattice(ctx, 3, L) for L in Ls}
+    reps = [lats[L] for L in Ls for _ in range(per) for _ in range(nt)]
+    nrep = len(reps)
+    sites = sum(lat.N for lat in reps)
+    st = gpu.State(ctx, "ea", reps, J=-1.0)
+    st.set_beta(np.tile(betas, nrep // nt))
+    st.random_bonds(rep_sample=np.repeat(np.arange(nrep // nt), nt), p_plus=0.5, seed=4)
+    st.init(seed=4)
+    nsw, steps = 10, 5
+    st.run_graph(nsw, nt, 4, 0, 0, steps, 1)
+    ms, _ = timed(ctx, lambda: st.run_graph(nsw, nt, 4, nsw * steps, steps, steps, 4))
+    acc = st.pt_swap(nt, 0, 4, 10 ** 6)
+    batched = sites * nsw * steps * 4 / (ms * 1e6)
+    del st
+    # plain: one L=24 replica per launch, streaming kernels
+    os.environ["MARQOV_AMD_KERNELS"] = "1"
+    sq = gpu.State(ctx, "ea", [lats[24]], J=-1.0)
+    del os.environ["MARQOV_AMD_KERNELS"]
+    sq.set_beta(1.0)
+    sq.random_bonds(p_plus=0.5, seed=4)
+    sq.init(seed=4)
+    ms_p, _ = timed(ctx, lambda: sq.metropolis(200, 4, 0))
+    plain = 24 ** 3 * 200 / (ms_p * 1e6)
+    del sq
+    return {"replicas": nrep, "spin_sites": sites, "attempts_per_ns": round(batched, 3),
+            "graph": f"{steps} x ({nsw} sweeps + energy + PT swap) per replay, 4 replays",
+            "pt_accept_fraction_even_pairs": round(float(acc.sum()) / (len(acc) * 16), 3),
+            "plain_attempts_per_ns": round(plain, 4), "batched_over_plain": round(batched / plain, 1)}
+
+
+def leg_c5(gpu, ctx):
+    """2D RGG N=2^20, mean degree 6, Ising and Potts-3, 64 replicas vs 1."""
+    from marqov_amd import graphs
+    n = 1 << 20
+    rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n), 5)
+    lat = gpu.CsrLattice(ctx, rp, ci)
+    res = {"N": n, "edges": int(rp[-1] // 2), "colours": int(len(lat.colors()))}
+    for model, beta in (("ising", 0.55), ("potts", 1.2)):
+        for name, nrep, nstep in (("batched", 64, 10), ("plain", 1, 10)):
+            st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, q=3)
+            st.set_beta(beta)
+            st.init(seed=5)
+            st.metropolis(2, 5, 0)
+            ms, _ = timed(ctx, lambda: st.metropolis(nstep, 5, sweep0=2))
+            res[f"{model}_{name}_attempts_per_ns"] = round(nrep * n * nstep / (ms * 1e6), 3)
+            if name == "batched":
+                sizes = st.wolff(2, 5, cluster0=0, sizes=True)
+                ms_w, _ = timed(ctx, lambda: st.wolff(4, 5, cluster0=2, sizes=False))
+                res[f"{model}_wolff_sites_per_ns"] = round(nrep * 4 * float(sizes.mean()) / (ms_w * 1e6), 4)
+            del st
+        res[f"{model}_batched_over_plain"] = round(res[f"{model}_batched_attempts_per_ns"] /
+                                                   res[f"{model}_plain_attempts_per_ns"], 2)
+    return res
 
 
 def main():
@@ -75,15 +191,15 @@ def main():
     N = L ** 3
     lat = gpu.RegularLattice(ctx, 3, L)
 
-    # ---- batched Metropolis (headline)
+    # ---- headline: batched Metropolis on 3D Ising L=256
+    log("headline: 3D Ising batched sweeps")
     st = gpu.State(ctx, "ising", [lat] * R, J=-1.0)
     st.set_beta(beta)
     st.init(seed=2)
-    ms, wall = time_sweeps(ctx, st, a.steps, a.warmup, seed=2)
-    attempts = a.steps * R * N
-    value = attempts / (ms * 1e6)
-    launches = 2 * a.steps
-    launch_ms = ms / launches
+    st.metropolis(a.warmup, seed=2, sweep0=0)
+    ms, wall = timed(ctx, lambda: st.metropolis(a.steps, seed=2, sweep0=a.warmup))
+    value = a.steps * R * N / (ms * 1e6)
+    launch_ms = ms / (2 * a.steps)
     bytes_per_launch = 3.0 * R * (N // 2)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     obs = st.measure()
@@ -107,45 +223,37 @@ def main():
                                f"{R} replicas per launch", "replicas": R, "L": L, "sites": N},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_sweep_ising<3,false>", "launch_ms": launch_ms,
+                     "kernel": "k_sweep_ising3_row16", "launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "observables": {"e_mean": float(obs[:, 0].mean()), "m_abs_mean": float(obs[:, 1].mean())},
     }
 
-    if not a.no_extra:
-        # ---- plain Metropolis: one replica per launch
+    if not a.quick:
+        log("plain 3D Ising")
         sp = gpu.State(ctx, "ising", [lat], J=-1.0)
         sp.set_beta(beta)
         sp.init(seed=2)
+        sp.metropolis(4, seed=2, sweep0=0)
         k = max(a.steps // 4, 10)
-        ms_p, _ = time_sweeps(ctx, sp, k, max(a.warmup // 4, 2), seed=2)
+        ms_p, _ = timed(ctx, lambda: sp.metropolis(k, seed=2, sweep0=4))
         line["plain"] = {"attempts_per_ns": round(k * N / (ms_p * 1e6), 3), "replicas": 1, "steps": k}
         line["batched_over_plain"] = round(value / line["plain"]["attempts_per_ns"], 3)
+        sizes = sp.wolff(4, 2, cluster0=0, sizes=True)
+        ms_w, _ = timed(ctx, lambda: sp.wolff(16, 2, cluster0=4, sizes=False))
+        line["plain"]["wolff_sites_per_ns"] = round(16 * float(sizes.mean()) / (ms_w * 1e6), 4)
+        line["plain"]["wolff_mean_cluster"] = round(float(sizes.mean()), 1)
         del sp
-        # ---- Wolff sites/ns on 2D L=64 at beta_c
-        l2 = gpu.RegularLattice(ctx, 2, 64)
-        wres = {}
-        for name, nrep in (("batched", 2048), ("plain", 1)):
-            sw = gpu.State(ctx, "ising", [l2] * nrep, J=-1.0)
-            sw.set_beta(0.4406868)
-            sw.init(seed=1)
-            sw.metropolis(20, seed=1)
-            sw.wolff(20, seed=1, cluster0=0, sizes=False)
-            ctx.sync()
-            ctx.timer_begin()
-            sizes = sw.wolff(10 * 5, seed=1, cluster0=20, sizes=False)
-            wms = ctx.timer_end()
-            sizes = sw.wolff(10, seed=1, cluster0=70, sizes=True)
-            mean = float(sizes.mean())
-            wres[name] = {"sites_per_ns": round(nrep * 50 * mean / (wms * 1e6), 4), "replicas": nrep,
-                          "mean_cluster": round(mean, 1)}
-            del sw
-        line["wolff"] = {"workload": "2D Ising L=64 beta=0.4406868, 50 clusters per replica", **wres}
+        for name, fn in (("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3), ("c4_ea_batch", leg_c4),
+                         ("c5_rgg", leg_c5)):
+            log(name)
+            try:
+                line[name] = fn(gpu, ctx)
+            except Exception as ex:  # a failing leg is reported, the headline still prints
+                line[name] = {"error": repr(ex)}
 
     if not a.no_cpu:
+        log("cpu baseline")
         sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import numpy as np
-
         import oracle_bind as ob
         ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         nth = max(1, min(16, ncpu))

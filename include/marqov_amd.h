@@ -70,6 +70,8 @@ typedef struct mq_model_params {
     int proposal;   /* MQ_PROPOSE_* (Heisenberg) */
     double delta;   /* small-angle step (Heisenberg) */
     int embedding;  /* MQ_EMBED_* (Heisenberg Wolff) */
+    int spin_bits;  /* Ising storage: 0 or 8 = int8 per spin, 1 = multi-spin coded
+                       (1 bit per spin; 3D, L a power of two >= 64; no Wolff) */
 } mq_model_params;
 
 /* ---- context ---------------------------------------------------------- */

@@ -1,0 +1,202 @@
+// marqov_amd.hpp -- C++ host API over the C ABI of marqov_amd.h.
+//
+// RAII owners for contexts, lattices and replica batches, errors as
+// std::runtime_error (the reference's convention, src/libmarqov/core.h:338-339),
+// and `GpuEMCS`, the per-batch counterpart of MARQOV's elementary Monte Carlo
+// step EMCS<H,L>::move (src/libmarqov/emcs.h:62-109): ncluster Wolff moves,
+// then nmetro sweeps, returning the mean cluster size (NaN for ncluster = 0).
+// The model tags mirror the reference's Hamiltonians
+// (src/hamiltonian/{Ising,EdwardsAndersonIsing,Potts,Heisenberg,Phi4}.h) with
+// their parameters and sign convention (J = -1 is ferromagnetic).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "marqov_amd.h"
+
+namespace marqov_amd {
+
+class Error : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+class Context {
+public:
+    explicit Context(int device = 0)
+    {
+        if (mq_ctx_create(device, &h_) != MQ_OK) throw Error("marqov_amd: no usable HIP device");
+    }
+    ~Context() { mq_ctx_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    mq_ctx* get() const { return h_; }
+    void check(int rc, const char* what) const
+    {
+        if (rc != MQ_OK) throw Error(std::string(what) + ": " + mq_last_error(h_));
+    }
+    void sync() const { check(mq_sync(h_), "mq_sync"); }
+
+private:
+    mq_ctx* h_ = nullptr;
+};
+
+// Periodic hypercube: FixedDimRegularHypercubic<dim> (src/lattice/regular_hypercubic.h:15-100)
+class RegularLattice {
+public:
+    RegularLattice(Context& ctx, int dim, int L) : ctx_(ctx), dim_(dim), L_(L)
+    {
+        ctx.check(mq_lattice_regular(ctx.get(), dim, L, &h_), "mq_lattice_regular");
+    }
+    ~RegularLattice() { mq_lattice_destroy(h_); }
+    RegularLattice(const RegularLattice&) = delete;
+    RegularLattice& operator=(const RegularLattice&) = delete;
+    mq_lattice* get() const { return h_; }
+    std::size_t size() const { return (std::size_t)mq_lattice_nsites(h_); }
+    int dim() const { return dim_; }
+    int length() const { return L_; }
+
+private:
+    Context& ctx_;
+    int dim_, L_;
+    mq_lattice* h_ = nullptr;
+};
+
+// General graph from CSR adjacency: GraphFromCSV (src/lattice/graph_from_csv.h:32-57)
+class GraphLattice {
+public:
+    GraphLattice(Context& ctx, const std::vector<int32_t>& row_ptr, const std::vector<int32_t>& col_idx)
+    {
+        ctx.check(mq_lattice_csr(ctx.get(), (int64_t)row_ptr.size() - 1, row_ptr.data(), col_idx.data(), &h_),
+                  "mq_lattice_csr");
+    }
+    ~GraphLattice() { mq_lattice_destroy(h_); }
+    GraphLattice(const GraphLattice&) = delete;
+    GraphLattice& operator=(const GraphLattice&) = delete;
+    mq_lattice* get() const { return h_; }
+    std::size_t size() const { return (std::size_t)mq_lattice_nsites(h_); }
+
+private:
+    mq_lattice* h_ = nullptr;
+};
+
+// Model tags with the reference's parameters.
+struct Ising { double J = -1.0; int spin_bits = 8; };
+struct EdwardsAndersonIsing { double J = -1.0; };
+template <int Q> struct Potts { double J = -1.0; };
+struct Heisenberg { double J = -1.0; int proposal = MQ_PROPOSE_FRESH; double delta = 0.5; int embedding = MQ_EMBED_CARTESIAN; };
+struct Phi4 { double lambda = 4.5, mass = 1.0; };  // J = -1 fixed as Phi4.h:93
+
+inline mq_model_params params_of(const Ising& m) { mq_model_params p{}; p.model = MQ_ISING; p.J = m.J; p.spin_bits = m.spin_bits; return p; }
+inline mq_model_params params_of(const EdwardsAndersonIsing& m) { mq_model_params p{}; p.model = MQ_EA_ISING; p.J = m.J; return p; }
+template <int Q> inline mq_model_params params_of(const Potts<Q>& m) { mq_model_params p{}; p.model = MQ_POTTS; p.J = m.J; p.q = Q; return p; }
+inline mq_model_params params_of(const Heisenberg& m)
+{
+    mq_model_params p{};
+    p.model = MQ_HEISENBERG;
+    p.J = m.J;
+    p.proposal = m.proposal;
+    p.delta = m.delta;
+    p.embedding = m.embedding;
+    return p;
+}
+inline mq_model_params params_of(const Phi4& m)
+{
+    mq_model_params p{};
+    p.model = MQ_PHI4;
+    p.J = -1.0;
+    p.lambda = m.lambda;
+    p.mass = m.mass;
+    return p;
+}
+
+struct Observables {
+    double e, m, m2, m4;  // E/N, |m|, m^2, m^4
+};
+
+// A batch of replicas of one model: the device counterpart of nrep Core
+// statespaces (src/libmarqov/core.h:249-342).
+class Batch {
+public:
+    template <class Model, class Lattice>
+    Batch(Context& ctx, const Model& model, const std::vector<const Lattice*>& lattices) : ctx_(ctx)
+    {
+        std::vector<mq_lattice*> ls;
+        for (auto* l : lattices) ls.push_back(l->get());
+        nrep_ = (int)ls.size();
+        const mq_model_params p = params_of(model);
+        ctx.check(mq_state_create(ctx.get(), &p, nrep_, ls.data(), &h_), "mq_state_create");
+    }
+    ~Batch() { mq_state_destroy(h_); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    mq_state* get() const { return h_; }
+    int nrep() const { return nrep_; }
+
+    void set_beta(const std::vector<double>& beta) { ctx_.check(mq_state_set_beta(h_, beta.data()), "mq_state_set_beta"); }
+    void init(uint64_t seed) { ctx_.check(mq_state_init(h_, seed), "mq_state_init"); }
+    void random_bonds(const std::vector<int32_t>& rep_sample, double p_plus, uint64_t seed)
+    {
+        ctx_.check(mq_state_random_bonds(h_, rep_sample.data(), p_plus, seed), "mq_state_random_bonds");
+    }
+    void metropolis(int nsweeps, uint64_t seed, uint64_t sweep0) { ctx_.check(mq_metropolis(h_, nsweeps, seed, sweep0), "mq_metropolis"); }
+    std::vector<int64_t> wolff(int nclusters, uint64_t seed, uint64_t cluster0)
+    {
+        std::vector<int64_t> sizes((size_t)nrep_ * nclusters);
+        ctx_.check(mq_wolff(h_, nclusters, seed, cluster0, sizes.data()), "mq_wolff");
+        return sizes;
+    }
+    std::vector<Observables> measure()
+    {
+        std::vector<double> raw((size_t)nrep_ * MQ_NOBS);
+        ctx_.check(mq_measure(h_, raw.data()), "mq_measure");
+        std::vector<Observables> o(nrep_);
+        for (int r = 0; r < nrep_; ++r) o[r] = {raw[4 * r], raw[4 * r + 1], raw[4 * r + 2], raw[4 * r + 3]};
+        return o;
+    }
+    std::vector<int32_t> pt_swap(int ntemp, int parity, uint64_t seed, uint64_t step)
+    {
+        std::vector<int32_t> acc(nrep_ / ntemp);
+        ctx_.check(mq_pt_swap(h_, ntemp, parity, seed, step, acc.data()), "mq_pt_swap");
+        return acc;
+    }
+    template <class T>
+    std::vector<T> download(int rep, std::size_t nsites, int per_site)
+    {
+        std::vector<T> v(nsites * per_site);
+        ctx_.check(mq_state_download(h_, rep, v.data(), (int64_t)(v.size() * sizeof(T))), "mq_state_download");
+        return v;
+    }
+
+private:
+    Context& ctx_;
+    mq_state* h_ = nullptr;
+    int nrep_ = 0;
+};
+
+// One elementary Monte Carlo step of every replica (emcs.h:69-109).
+// `step` numbers the EMCS so that every step draws fresh Philox counters.
+class GpuEMCS {
+public:
+    GpuEMCS(Batch& batch, uint64_t seed) : b_(batch), seed_(seed) {}
+    std::vector<double> move(int ncluster, int nmetro)
+    {
+        std::vector<double> mean(b_.nrep());
+        int rc = mq_emcs(b_.get(), ncluster, nmetro, seed_, step_++, mean.data());
+        if (rc != MQ_OK) throw Error("mq_emcs failed");
+        return mean;
+    }
+    uint64_t steps() const { return step_; }
+
+private:
+    Batch& b_;
+    uint64_t seed_;
+    uint64_t step_ = 0;
+};
+
+}  // namespace marqov_amd

@@ -26,6 +26,7 @@ class ModelParams(ctypes.Structure):
         ("proposal", ctypes.c_int),
         ("delta", ctypes.c_double),
         ("embedding", ctypes.c_int),
+        ("spin_bits", ctypes.c_int),
     ]
 
 

@@ -272,17 +272,32 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         if (lattices[r]->kind != kind0 || (kind0 == LAT_REGULAR && lattices[r]->dim != dim0))
             return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "all replicas of a state must use lattices of one kind and dim");
     }
+    const bool msc = params->spin_bits == 1;
+    if (params->spin_bits != 0 && params->spin_bits != 1 && params->spin_bits != 8)
+        return mq_fail(ctx, MQ_ERR_ARG, "spin_bits must be 0, 1 or 8");
+    if (msc) {
+        if (model != MQ_ISING) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "multi-spin coding is for the Ising model");
+        for (int r = 0; r < nrep; ++r)
+            if (!msc_supported(lattices[r]))
+                return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "multi-spin coding needs 3D lattices with L a power of two >= 64");
+        if (params->J == 0.0) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "multi-spin coding needs J != 0");
+    }
     auto* st = new mq_state();
     st->ctx = ctx;
     st->p = *params;
     st->nrep = nrep;
+    st->msc = msc;
     st->lat.assign(lattices, lattices + nrep);
     st->site_bytes = is_vec(model) ? 16 : 1;
+    auto rep_bytes = [&](const mq_lattice* l) {
+        const int64_t b = msc ? l->N / 8 : l->N * st->site_bytes;
+        return (b + 255) / 256 * 256;
+    };
     st->rep_off.resize(nrep);
     int64_t off = 0;
     for (int r = 0; r < nrep; ++r) {
         st->rep_off[r] = off;
-        off += (st->lat[r]->N * st->site_bytes + 255) / 256 * 256;
+        off += rep_bytes(st->lat[r]);
     }
     st->spin_total = off;
     for (int r = 0; r < nrep;) {
@@ -293,7 +308,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         g.rep0 = r;
         g.nrep = e - r;
         g.off = st->rep_off[r];
-        g.stride = (g.lat->N * st->site_bytes + 255) / 256 * 256;
+        g.stride = rep_bytes(g.lat);
         st->groups.push_back(g);
         r = e;
     }
@@ -347,6 +362,7 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_energy);
     dfree(st->d_ladder);
     dfree(st->d_dyn);
+    if (st->d_msc_cls) (void)hipFree(st->d_msc_cls);
     if (st->d_lds_groups) (void)hipFree(st->d_lds_groups);
     delete st;
 }
@@ -378,6 +394,7 @@ static int upload_tables(mq_state* st)
     }
     hipStream_t s = st->ctx->stream;
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_thr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice, s));
+    if (st->msc) MQ_HIP(st->ctx, msc_upload_classes(st, thr));
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_wthr, wthr.data(), wthr.size() * 4, hipMemcpyHostToDevice, s));
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_betaf, bf.data(), bf.size() * 4, hipMemcpyHostToDevice, s));
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_betad, st->beta.data(), st->beta.size() * 8, hipMemcpyHostToDevice, s));
@@ -482,7 +499,8 @@ extern "C" int mq_state_init(mq_state* st, uint64_t seed)
 {
     if (!st) return MQ_ERR_ARG;
     for (const auto& g : st->groups) {
-        if (g.lat->kind == LAT_REGULAR) MQ_HIP(st->ctx, launch_init_regular(st, g, key_of(seed)));
+        if (st->msc) MQ_HIP(st->ctx, launch_init_msc(st, g, key_of(seed)));
+        else if (g.lat->kind == LAT_REGULAR) MQ_HIP(st->ctx, launch_init_regular(st, g, key_of(seed)));
         else MQ_HIP(st->ctx, launch_init_csr(st, g, key_of(seed)));
     }
     MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
@@ -503,6 +521,16 @@ extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t 
             dev[p * 4 + 0] = h[3 * i];
             dev[p * 4 + 1] = h[3 * i + 1];
             dev[p * 4 + 2] = h[3 * i + 2];
+        }
+        MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), dev.size() * 4, hipMemcpyHostToDevice));
+    } else if (st->msc) {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected one int8 per site");
+        const int8_t* h = static_cast<const int8_t*>(host);
+        std::vector<uint32_t> dev((size_t)(N / 32), 0u);
+        for (int64_t i = 0; i < N; ++i) {
+            if (h[i] != 1 && h[i] != -1) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: spin value out of range");
+            const int64_t p = dev_pos(l, i);
+            if (h[i] < 0) dev[(size_t)(p >> 5)] |= 1u << (p & 31);
         }
         MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), dev.size() * 4, hipMemcpyHostToDevice));
     } else {
@@ -537,6 +565,15 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
             h[3 * i + 1] = dev[p * 4 + 1];
             h[3 * i + 2] = dev[p * 4 + 2];
         }
+    } else if (st->msc) {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
+        std::vector<uint32_t> dev((size_t)(N / 32));
+        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + st->rep_off[rep], dev.size() * 4, hipMemcpyDeviceToHost));
+        int8_t* h = static_cast<int8_t*>(host);
+        for (int64_t i = 0; i < N; ++i) {
+            const int64_t p = dev_pos(l, i);
+            h[i] = ((dev[(size_t)(p >> 5)] >> (p & 31)) & 1u) ? (int8_t)-1 : (int8_t)1;
+        }
     } else {
         if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
         std::vector<int8_t> dev((size_t)N);
@@ -551,7 +588,7 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
 int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0)
 {
     const Key2 key = key_of(seed);
-    if (nsweeps > 0 && st->kernel_choice == 0 && sweep0 + (uint64_t)nsweeps < (1ull << 31)) {
+    if (nsweeps > 0 && st->kernel_choice == 0 && !st->msc && sweep0 + (uint64_t)nsweeps < (1ull << 31)) {
         hipError_t e = launch_sweeps_lds(st, nsweeps, (uint32_t)sweep0, key);
         if (e == hipSuccess) return MQ_OK;
         if (e != hipErrorNotSupported) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("lds sweep: ") + hipGetErrorString(e));
@@ -563,8 +600,9 @@ int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep
             const int nc = g.lat->ncolors;
             for (int c = 0; c < nc; ++c) {
                 const uint32_t phase = (uint32_t)(sw * (uint64_t)(g.lat->kind == LAT_REGULAR ? 2 : 64) + c);
-                hipError_t e = (g.lat->kind == LAT_REGULAR) ? launch_sweep_regular(st, g, c, phase, key)
-                                                            : launch_sweep_csr(st, g, c, phase, key);
+                hipError_t e = st->msc ? launch_sweep_msc(st, g, c, phase, key)
+                               : (g.lat->kind == LAT_REGULAR) ? launch_sweep_regular(st, g, c, phase, key)
+                                                              : launch_sweep_csr(st, g, c, phase, key);
                 if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
             }
         }
@@ -589,6 +627,7 @@ extern "C" int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t clu
     if (nclusters == 0) return MQ_OK;
     if (st->p.model == MQ_EA_ISING)
         return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff is not defined for the +-J spin glass (frustrated bonds)");
+    if (st->msc) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff runs on int8 storage (spin_bits = 8)");
     if (cluster0 + (uint64_t)nclusters > (1ull << 32)) return mq_fail(st->ctx, MQ_ERR_ARG, "cluster counter exceeds 2^32");
     rc = ensure_wolff_scratch(st, nclusters);
     if (rc) return rc;
@@ -633,7 +672,8 @@ static int measure_sums(mq_state* st, std::vector<long long>& is, std::vector<do
     MQ_HIP(st->ctx, zero64(st->d_isum, (int64_t)st->nrep * 8, s));
     MQ_HIP(st->ctx, zero64(st->d_fsum, (int64_t)st->nrep * 8, s));
     for (const auto& g : st->groups) {
-        hipError_t e = (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
+        hipError_t e = st->msc ? launch_measure_msc(st, g)
+                       : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
         if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
     }
     is.resize((size_t)st->nrep * 8);

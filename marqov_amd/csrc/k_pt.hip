@@ -131,7 +131,8 @@ hipError_t energy_async(mq_state* st)
     if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     if ((e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     for (const auto& g : st->groups) {
-        e = (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
+        e = st->msc ? launch_measure_msc(st, g)
+            : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
         if (e != hipSuccess) return e;
     }
     const int vec = (st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4);

@@ -109,7 +109,7 @@ class State:
     """A batch of replicas of one model (one Core::statespace per replica)."""
 
     def __init__(self, ctx, model, lattices, J=-1.0, q=3, lam=4.5, mass=1.0, proposal="fresh", delta=0.5,
-                 embedding="cartesian"):
+                 embedding="cartesian", spin_bits=8):
         self.ctx = ctx
         self.model = model
         self.lattices = list(lattices)
@@ -125,6 +125,7 @@ class State:
         p.proposal = {"fresh": L.MQ_PROPOSE_FRESH, "small": L.MQ_PROPOSE_SMALL_ANGLE}[proposal]
         p.delta = delta
         p.embedding = {"cartesian": L.MQ_EMBED_CARTESIAN, "random": L.MQ_EMBED_RANDOM_DIR}[embedding]
+        p.spin_bits = spin_bits
         self.params = p
         arr = (ctypes.c_void_p * self.nrep)(*[lat.h.value for lat in self.lattices])
         self._arr = arr

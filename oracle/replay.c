@@ -467,3 +467,41 @@ EXPORT int orc_csr_wolff(int64_t N, const int32_t* rp, const int32_t* ci, int8_t
     }
     return 0;
 }
+
+/* ---------------------------------------------- multi-spin-coded 3D Ising */
+/* Replays the 1-bit sweep: a site of anti-aligned count c is flipped if dE <= 0,
+ * else iff U < T (T = thi<<16 | tlo) where U's bit 31-k is bit (h & 31) of the
+ * level-k word Philox(h >> 5, 2*sweep + colour, rep, 8 + k/4)[k % 4]; an
+ * "always" entry (thi = 65536) flips without a draw. */
+EXPORT void orc_ising_msc_sweeps(int L, int8_t* s, const uint32_t* thi, const uint32_t* tlo, int nsweeps,
+                                 uint64_t seed, uint32_t sweep0, uint32_t rep)
+{
+    const int dim = 3;
+    int64_t N = (int64_t)L * L * L;
+    for (int t = 0; t < nsweeps; ++t)
+        for (int c = 0; c < 2; ++c) {
+            uint32_t phase = 2u * (sweep0 + (uint32_t)t) + (uint32_t)c;
+            for (int64_t i = 0; i < N; ++i) {
+                if (colour_of(dim, L, i) != c) continue;
+                int hs = 0;
+                for (int slot = 0; slot < 6; ++slot) hs += s[orc_neighbour(dim, L, i, slot)];
+                int e = s[i] * hs + 6;
+                int flip;
+                if (thi[e] > 0xFFFFu) {
+                    flip = 1;
+                } else {
+                    uint32_t T = (thi[e] << 16) | tlo[e];
+                    int64_t h = i >> 1;
+                    uint32_t widx = (uint32_t)(h >> 5), j = (uint32_t)(h & 31);
+                    flip = 0;
+                    for (int k = 0; k < 32; ++k) {
+                        uint32_t w[4];
+                        orc_philox_u64key(widx, phase, rep, 8u + (uint32_t)(k >> 2), seed, w);
+                        uint32_t rb = (w[k & 3] >> j) & 1u, tb = (T >> (31 - k)) & 1u;
+                        if (rb != tb) { flip = (rb == 0u); break; }
+                    }
+                }
+                if (flip) s[i] = (int8_t)-s[i];
+            }
+        }
+}

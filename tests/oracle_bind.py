@@ -36,6 +36,7 @@ _SIG = {
     "orc_bond_sum_potts": (_I64, [_I, _I, _P]),
     "orc_csr_sweeps": (None, [_I64, _P, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I, _I, _U64, _U32, _U32]),
     "orc_csr_wolff": (_I, [_I64, _P, _P, _P, _I, _I, _U32, _I, _U64, _U32, _U32, _P]),
+    "orc_ising_msc_sweeps": (None, [_I, _P, _P, _P, _I, _U64, _U32, _U32]),
     "ref_run_regular": (_I, [_I, _I, _I, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_run_csr": (_I, [_I, _I64, _P, _P, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_mt19937_64_nth": (_U64, [_U64, _I64]),
@@ -194,3 +195,10 @@ def csr_wolff(rp, ci, s, beta, J, nclusters, seed, cl0, rep, potts=False, Q=3):
     sizes = np.zeros(nclusters, np.int64)
     lib().orc_csr_wolff(N, p(rp), p(ci), p(s), int(potts), Q, tw, nclusters, seed, cl0, rep, p(sizes))
     return s, sizes
+
+
+def ising_msc_sweeps(L, s, beta, J, nsweeps, seed, sweep0, rep):
+    hi, lo = thresholds_ising(beta, J, 6)
+    s = np.ascontiguousarray(s, np.int8).copy()
+    lib().orc_ising_msc_sweeps(L, p(s), p(hi), p(lo), nsweeps, seed, sweep0, rep)
+    return s

@@ -212,3 +212,39 @@ def test_2d_ising_batch_bit_exact(ctx, kernels, monkeypatch):
     for r in range(3):
         exp = ob.ising_sweeps(2, 64, ob.init_ising(4096, 2, r), [0.4406868, 0.3, 0.6][r], -1.0, 4, 9, 0, r)
         assert np.array_equal(st.download(r), exp), (kernels, r)
+
+
+@pytest.mark.parametrize("J,betas", [(-1.0, [0.2216546, 0.4]), (+1.0, [0.3, 0.0])])
+def test_msc_sweeps_bit_exact(ctx, J, betas):
+    """1-bit multi-spin-coded 3D Ising (L=64) against its CPU replay; the random start
+    equals the int8 layout's."""
+    from marqov_amd import gpu
+    L = 64
+    lat = gpu.RegularLattice(ctx, 3, L)
+    st = gpu.State(ctx, "ising", [lat, lat], J=J, spin_bits=1)
+    st.set_beta(betas)
+    st.init(seed=6)
+    for r in range(2):
+        assert np.array_equal(st.download(r), ob.init_ising(L ** 3, 6, r))
+    st.metropolis(2, seed=41, sweep0=5)
+    for r in range(2):
+        exp = ob.ising_msc_sweeps(L, ob.init_ising(L ** 3, 6, r), betas[r], J, 2, 41, 5, r)
+        got = st.download(r)
+        assert np.array_equal(got, exp), (r, int((got != exp).sum()))
+    obs = st.measure()
+    for r in range(2):
+        s = st.download(r)
+        e = J * ob.lib().orc_bond_sum_ising(3, L, ob.p(s), None) / L ** 3
+        assert obs[r, 0] == pytest.approx(e, abs=1e-12)
+        assert obs[r, 1] == pytest.approx(abs(int(s.astype(np.int64).sum())) / L ** 3, abs=1e-12)
+
+
+def test_msc_upload_roundtrip(ctx):
+    from marqov_amd import gpu
+    L = 64
+    lat = gpu.RegularLattice(ctx, 3, L)
+    st = gpu.State(ctx, "ising", [lat], J=-1.0, spin_bits=1)
+    st.set_beta(0.2)
+    s = np.where(np.random.default_rng(1).random(L ** 3) < 0.3, 1, -1).astype(np.int8)
+    st.upload(0, s)
+    assert np.array_equal(st.download(0), s)

@@ -181,3 +181,33 @@ def test_restatement_c1_energy_vs_kaufman():
     ka = exact.kaufman_energy(beta, L, L)
     assert abs(me - ka) < 3 * se, (me, se, ka)
     assert 1000 < cl.mean() < 2200
+
+
+@pytest.mark.parametrize("kind", ["int8", "msc"])
+@pytest.mark.parametrize("J,beta", [(-1.0, 0.3), (1.0, 0.25)])
+def test_replay_acceptance_per_class(kind, J, beta):
+    """One sweep from a random 3D configuration: colour-0 sites see their initial
+    neighbours, so each one's flip is one Bernoulli draw with p = min(1, exp(-beta dE)),
+    dE = -2 J s_i h_i.  Per class, the flip fraction must match p (binomial, 4 sigma):
+    this pins the 16-bit/tie rule and the bit-serial rule, bit order included."""
+    L = 64
+    N = L ** 3
+    s0 = ob.init_ising(N, 17, 0)
+    if kind == "int8":
+        s1 = ob.ising_sweeps(3, L, s0, beta, J, 1, 99, 0, 0)
+    else:
+        s1 = ob.ising_msc_sweeps(L, s0, beta, J, 1, 99, 0, 0)
+    nb = exact.neighbour_table(3, L)
+    idx = np.arange(N)
+    x = np.stack([(idx // L ** d) % L for d in range(3)], 1)
+    c0 = (x.sum(1) % 2) == 0
+    q = s0.astype(np.int64) * s0[nb].astype(np.int64).sum(1)
+    flipped = s1 != s0
+    for qv in range(-6, 7, 2):
+        sel = c0 & (q == qv)
+        n = int(sel.sum())
+        if n < 500:
+            continue
+        p = min(1.0, np.exp(-beta * (-2.0 * J * qv)))
+        f = flipped[sel].mean()
+        assert abs(f - p) <= 4 * np.sqrt(max(p * (1 - p), 1e-12) / n) + 1e-12, (kind, qv, f, p, n)
