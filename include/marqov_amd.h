@@ -159,6 +159,22 @@ MQ_API int mq_pt_swap(mq_state* st, int ntemp, int parity, uint64_t seed, uint64
 /* temperature index currently held by each replica (after swaps) */
 MQ_API int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep);
 
+/* ---- launch shapes (set by the autotuner, marqov_amd/autotune.py) ------- */
+/* No counterpart in the reference.  kernels: 0 = automatic (LDS multi-sweep
+ * kernel for lattices that fit in LDS, row-vectorised 3D kernel where it
+ * applies), 1 = streaming kernels only, 2 = generic streaming kernel only.
+ * block: threads per block of the streaming sweep kernels (64..512, power of 2);
+ * xcd_remap: give each XCD a contiguous slab of rows; lds_block: threads per
+ * block of the LDS multi-sweep kernel (64..512).  Results do not depend on these. */
+typedef struct mq_tuning {
+    int kernels;
+    int block;
+    int xcd_remap;
+    int lds_block;
+} mq_tuning;
+MQ_API int mq_state_set_tuning(mq_state* st, const mq_tuning* t);
+MQ_API int mq_state_get_tuning(const mq_state* st, mq_tuning* t);
+
 /* ---- graphs ------------------------------------------------------------ */
 /* Capture `steps` repetitions of {nsweeps Metropolis sweeps; optional PT
  * swap with ntemp temperatures} into one HIP graph and replay it `repeats`

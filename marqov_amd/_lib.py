@@ -16,6 +16,11 @@ MQ_EMBED_CARTESIAN, MQ_EMBED_RANDOM_DIR = 0, 1
 MQ_NOBS = 4
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("kernels", ctypes.c_int), ("block", ctypes.c_int), ("xcd_remap", ctypes.c_int),
+                ("lds_block", ctypes.c_int)]
+
+
 class ModelParams(ctypes.Structure):
     _fields_ = [
         ("model", ctypes.c_int),
@@ -67,6 +72,8 @@ SIGNATURES = {
     "mq_renormalize": (_I, [_P]),
     "mq_pt_swap": (_I, [_P, _I, _I, _U64, _U64, _P]),
     "mq_pt_get_temps": (_I, [_P, _P]),
+    "mq_state_set_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
+    "mq_state_get_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
     "mq_run_graph": (_I, [_P, _I, _I, _U64, _U64, _U64, _I, _I]),
 }
 

@@ -335,6 +335,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     if ((e = hipMemsetAsync(st->d_spins, 0, (size_t)off, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemsetAsync(st->d_rep_bond_off, 0, sizeof(int64_t) * nrep, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
+    if ((e = prepare_lds(st)) != hipSuccess) return fail(e);
     *out = st;
     return MQ_OK;
 }
@@ -731,5 +732,28 @@ extern "C" int mq_energy(mq_state* st, double* out)
     int rc = mq_measure(st, obs.data());
     if (rc) return rc;
     for (int r = 0; r < st->nrep; ++r) out[r] = obs[(size_t)r * MQ_NOBS] * (double)st->lat[r]->N;
+    return MQ_OK;
+}
+
+extern "C" int mq_state_set_tuning(mq_state* st, const mq_tuning* t)
+{
+    if (!st || !t) return MQ_ERR_ARG;
+    auto pow2 = [](int v) { return v >= 64 && v <= 512 && (v & (v - 1)) == 0; };
+    if (t->kernels < 0 || t->kernels > 2 || !pow2(t->block) || !pow2(t->lds_block) || (t->xcd_remap != 0 && t->xcd_remap != 1))
+        return mq_fail(st->ctx, MQ_ERR_ARG, "tuning: kernels in 0..2, block and lds_block powers of two in 64..512");
+    st->kernel_choice = t->kernels;
+    st->tune_block = t->block;
+    st->tune_xcd = t->xcd_remap;
+    st->lds_block = t->lds_block;
+    return MQ_OK;
+}
+
+extern "C" int mq_state_get_tuning(const mq_state* st, mq_tuning* t)
+{
+    if (!st || !t) return MQ_ERR_ARG;
+    t->kernels = st->kernel_choice;
+    t->block = st->tune_block;
+    t->xcd_remap = st->tune_xcd;
+    t->lds_block = st->lds_block;
     return MQ_OK;
 }

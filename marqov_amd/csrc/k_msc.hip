@@ -41,10 +41,10 @@ struct MscClasses {
 };
 
 template <bool FERRO>
-__global__ void __launch_bounds__(256) k_sweep_msc(uint32_t* __restrict__ base, int64_t stride_words,
+__global__ void __launch_bounds__(512) k_sweep_msc(uint32_t* __restrict__ base, int64_t stride_words,
                                                    const MscClasses* __restrict__ cls, int lgL, int colour,
                                                    uint32_t phase, const uint32_t* __restrict__ dyn, int rep0,
-                                                   int lgbpr, Key2 key)
+                                                   int lgbpr, int lgblk, Key2 key)
 {
     if (dyn) phase += dyn[0];
     const uint32_t lb = blockIdx.x;
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) k_sweep_msc(uint32_t* __restrict__ base, 
     const uint32_t wpr = 1u << lgwpr;
     const uint32_t plane_w = (1u << (2 * lgL - 1)) >> 5;  // words per z-plane
     const uint32_t half_w = plane_w << lgL;
-    const uint32_t widx = ((lb - (r << lgbpr)) << 8) + threadIdx.x;  // word index in the colour array
+    const uint32_t widx = ((lb - (r << lgbpr)) << lgblk) + threadIdx.x;  // word index in the colour array
     const uint32_t row = widx >> lgwpr;
     const uint32_t wi = widx & (wpr - 1u);
     const uint32_t y = row & (L - 1u);
@@ -173,9 +173,9 @@ __global__ void __launch_bounds__(256) k_measure_msc(const uint32_t* __restrict_
                                c1[(z == 0u) ? widx + (L - 1u) * plane_w : widx - plane_w],
                                c1[(z == L - 1u) ? widx - (L - 1u) * plane_w : widx + plane_w]};
         int anti = 0;
-        for (int k = 0; k < 6; ++k) anti += __popc(S ^ n[k]);
+        for (int k = 0; k < 6; ++k) anti += (int)__popc(S ^ n[k]);  // __popc returns unsigned
         bs = 6 * 32 - 2 * anti;
-        ms = 64 - 2 * (__popc(S) + __popc(c1[widx]));
+        ms = 64 - 2 * ((int)__popc(S) + (int)__popc(c1[widx]));
     }
     for (int o = 32; o > 0; o >>= 1) {
         bs += __shfl_xor(bs, o, 64);
@@ -232,16 +232,19 @@ hipError_t msc_upload_classes(mq_state* st, const std::vector<uint32_t>& thr)
 hipError_t launch_sweep_msc(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key)
 {
     const int lgL = lg2(g.lat->L);
-    const int lgbpr = 3 * lgL - 1 - 5 - 8;  // words per colour / 256
+    int lgblk = 0;
+    while ((1 << lgblk) < st->tune_block) ++lgblk;
+    const int lgbpr = 3 * lgL - 1 - 5 - lgblk;  // words per colour / block
+    if (lgbpr < 0) return hipErrorInvalidValue;
     const int64_t blocks = ((int64_t)1 << lgbpr) * g.nrep;
     uint32_t* b = reinterpret_cast<uint32_t*>(st->d_spins + g.off);
     const MscClasses* cls = reinterpret_cast<const MscClasses*>(st->d_msc_cls);
     if (st->p.J < 0.0)
-        hipLaunchKernelGGL(k_sweep_msc<true>, dim3((unsigned)blocks), dim3(256), 0, st->ctx->stream, b, g.stride / 4,
-                           cls, lgL, colour, phase, st->cur_dyn, g.rep0, lgbpr, key);
+        hipLaunchKernelGGL(k_sweep_msc<true>, dim3((unsigned)blocks), dim3(1 << lgblk), 0, st->ctx->stream, b,
+                           g.stride / 4, cls, lgL, colour, phase, st->cur_dyn, g.rep0, lgbpr, lgblk, key);
     else
-        hipLaunchKernelGGL(k_sweep_msc<false>, dim3((unsigned)blocks), dim3(256), 0, st->ctx->stream, b, g.stride / 4,
-                           cls, lgL, colour, phase, st->cur_dyn, g.rep0, lgbpr, key);
+        hipLaunchKernelGGL(k_sweep_msc<false>, dim3((unsigned)blocks), dim3(1 << lgblk), 0, st->ctx->stream, b,
+                           g.stride / 4, cls, lgL, colour, phase, st->cur_dyn, g.rep0, lgbpr, lgblk, key);
     return hipGetLastError();
 }
 

@@ -266,10 +266,12 @@ extern "C" int mq_run_graph(mq_state* st, int nsweeps, int ntemp, uint64_t seed,
     hipError_t ec = hipStreamEndCapture(s, &graph);
     if (rc != MQ_OK) {
         if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
         return rc;
     }
     if (e != hipSuccess || ec != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
         return mq_fail(st->ctx, MQ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
     }
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_init_f4(float4* __restrict__ base, int6
 // 32-bit acceptance threshold; an equal high half (probability 2^-16) draws the
 // low 16 bits from the tie stream, so the decision equals a full 32-bit compare.
 template <int DIM, bool BONDS>
-__global__ void __launch_bounds__(256) k_sweep_ising(int8_t* __restrict__ base, int64_t stride,
+__global__ void __launch_bounds__(512) k_sweep_ising(int8_t* __restrict__ base, int64_t stride,
                                                      const int8_t* __restrict__ bonds,
                                                      const int64_t* __restrict__ rep_bond_off,
                                                      const uint32_t* __restrict__ thr, int tbl, int L, int colour,
@@ -192,17 +192,17 @@ __global__ void __launch_bounds__(256) k_sweep_ising(int8_t* __restrict__ base, 
 // acceptance are those of k_sweep_ising (group g = h >> 3).
 // The block -> tile mapping gives each XCD a contiguous slab of rows so the
 // y+-1 / z+-1 rows are re-read from the same L2.
-__global__ void __launch_bounds__(256) k_sweep_ising3_row16(int8_t* __restrict__ base, int64_t stride,
+__global__ void __launch_bounds__(512) k_sweep_ising3_row16(int8_t* __restrict__ base, int64_t stride,
                                                             const uint32_t* __restrict__ thr, int lgL, int colour,
                                                             uint32_t phase, const uint32_t* __restrict__ dyn, int rep0,
-                                                            int lgbpr, Key2 key)
+                                                            int lgbpr, int lgblk, int xcd, Key2 key)
 {
     if (dyn) phase += dyn[0];
     __shared__ uint32_t s_thi[13];
     __shared__ uint32_t s_tlo[13];
     const uint32_t B = gridDim.x;
     const uint32_t b = blockIdx.x;
-    const uint32_t lb = ((B & 7u) == 0u) ? (b & 7u) * (B >> 3) + (b >> 3) : b;
+    const uint32_t lb = (xcd && (B & 7u) == 0u) ? (b & 7u) * (B >> 3) + (b >> 3) : b;
     const uint32_t r = lb >> lgbpr;
     const uint32_t grep = (uint32_t)rep0 + r;
     if (threadIdx.x < 13) {
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(256) k_sweep_ising3_row16(int8_t* __restrict__
     const uint32_t lgplane = 2u * (uint32_t)lgL - 1u;  // H*L
     const uint32_t plane = 1u << lgplane;
     const uint32_t half = plane << lgL;
-    const uint32_t chunk = ((lb - (r << lgbpr)) << 8) + threadIdx.x;
+    const uint32_t chunk = ((lb - (r << lgbpr)) << lgblk) + threadIdx.x;
     const uint32_t row = chunk >> lgcpr;
     const uint32_t c = chunk & ((1u << lgcpr) - 1u);
     const uint32_t y = row & (L - 1u);
@@ -356,7 +356,7 @@ __device__ __forceinline__ void lds_nbrs(int L, int H, uint32_t mh, uint32_t ml,
 }
 
 template <int DIM, bool BONDS>
-__global__ void __launch_bounds__(256) k_sweep_lds(char* __restrict__ spins, const LdsGroup* __restrict__ groups,
+__global__ void __launch_bounds__(512) k_sweep_lds(char* __restrict__ spins, const LdsGroup* __restrict__ groups,
                                                    int ngroups, const int8_t* __restrict__ bonds,
                                                    const int64_t* __restrict__ rep_bond_off,
                                                    const uint32_t* __restrict__ thr, int tbl, int nsweeps,
@@ -436,7 +436,7 @@ __global__ void __launch_bounds__(256) k_sweep_lds(char* __restrict__ spins, con
 // One Philox block per site: word 0 -> proposal in [0,Q) (Potts.h:171-177),
 // word 1 -> 32-bit uniform against the threshold of dE = k*(#new - #old).
 template <int DIM>
-__global__ void __launch_bounds__(256) k_sweep_potts(int8_t* __restrict__ base, int64_t stride,
+__global__ void __launch_bounds__(512) k_sweep_potts(int8_t* __restrict__ base, int64_t stride,
                                                      const uint32_t* __restrict__ thr, int tbl, int L, int Q,
                                                      int colour, uint32_t phase, const uint32_t* __restrict__ dyn,
                                                      int rep0, Key2 key)
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(256) k_sweep_potts(int8_t* __restrict__ base, 
 // One Philox block per site: words 0,1 -> direction, word 2 -> acceptance.
 // Local arithmetic in FP32; acceptance u < exp(-beta*dE).
 template <int DIM, bool BONDS>
-__global__ void __launch_bounds__(256) k_sweep_o3(float4* __restrict__ base, int64_t stride4,
+__global__ void __launch_bounds__(512) k_sweep_o3(float4* __restrict__ base, int64_t stride4,
                                                   const int8_t* __restrict__ bonds,
                                                   const int64_t* __restrict__ rep_bond_off,
                                                   const float* __restrict__ betaf, int L, int model, int proposal,
@@ -710,62 +710,67 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
     const bool bonds = (model == MQ_EA_ISING) || (st->d_bonds != nullptr);
     if (model == MQ_ISING || model == MQ_EA_ISING) {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
-        const dim3 grid = grid_for((half + 7) / 8, 256, g.nrep);
+        const int blk = st->tune_block;
+        const dim3 grid = grid_for((half + 7) / 8, blk, g.nrep);
         const bool pow2 = (L & (L - 1)) == 0;
         if (dim == 3 && !bonds && pow2 && L >= 32 && L <= 1024 && st->tbl == 13 && st->kernel_choice != 2) {
             int lgL = 0;
             while ((1 << lgL) < L) ++lgL;
-            const int lgbpr = 3 * lgL - 13;  // (N/2)/16/256 blocks per replica
-            const int64_t nblk = ((int64_t)1 << lgbpr) * g.nrep;
-            if (nblk < (1ll << 31)) {
-                hipLaunchKernelGGL(k_sweep_ising3_row16, dim3((unsigned)nblk), dim3(256), 0, s, b, g.stride,
-                                   st->d_thr, lgL, colour, phase, st->cur_dyn, g.rep0, lgbpr, key);
+            int lgblk = 0;
+            while ((1 << lgblk) < st->tune_block) ++lgblk;
+            const int lgbpr = 3 * lgL - 5 - lgblk;  // (N/2)/16 chunks per replica / block
+            const int64_t nblk = ((int64_t)1 << std::max(lgbpr, 0)) * g.nrep;
+            if (lgbpr >= 0 && nblk < (1ll << 31)) {
+                hipLaunchKernelGGL(k_sweep_ising3_row16, dim3((unsigned)nblk), dim3(1 << lgblk), 0, s, b, g.stride,
+                                   st->d_thr, lgL, colour, phase, st->cur_dyn, g.rep0, lgbpr, lgblk, st->tune_xcd, key);
                 return hipGetLastError();
             }
         }
         if (dim == 3) {
             if (bonds)
-                hipLaunchKernelGGL((k_sweep_ising<3, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                hipLaunchKernelGGL((k_sweep_ising<3, true>), grid, dim3(blk), 0, s, b, g.stride, st->d_bonds,
                                    st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
             else
-                hipLaunchKernelGGL((k_sweep_ising<3, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                hipLaunchKernelGGL((k_sweep_ising<3, false>), grid, dim3(blk), 0, s, b, g.stride, st->d_bonds,
                                    st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
         } else {
             if (bonds)
-                hipLaunchKernelGGL((k_sweep_ising<2, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                hipLaunchKernelGGL((k_sweep_ising<2, true>), grid, dim3(blk), 0, s, b, g.stride, st->d_bonds,
                                    st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
             else
-                hipLaunchKernelGGL((k_sweep_ising<2, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds,
+                hipLaunchKernelGGL((k_sweep_ising<2, false>), grid, dim3(blk), 0, s, b, g.stride, st->d_bonds,
                                    st->d_rep_bond_off, st->d_thr, st->tbl, L, colour, phase, st->cur_dyn, g.rep0, key);
         }
     } else if (model == MQ_POTTS) {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
-        const dim3 grid = grid_for(half, 256, g.nrep);
+        const int blk = st->tune_block;
+        const dim3 grid = grid_for(half, blk, g.nrep);
         if (dim == 3)
-            hipLaunchKernelGGL(k_sweep_potts<3>, grid, dim3(256), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
+            hipLaunchKernelGGL(k_sweep_potts<3>, grid, dim3(blk), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
                                colour, phase, st->cur_dyn, g.rep0, key);
         else
-            hipLaunchKernelGGL(k_sweep_potts<2>, grid, dim3(256), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
+            hipLaunchKernelGGL(k_sweep_potts<2>, grid, dim3(blk), 0, s, b, g.stride, st->d_thr, st->tbl, L, st->p.q,
                                colour, phase, st->cur_dyn, g.rep0, key);
     } else {
         float4* b = reinterpret_cast<float4*>(st->d_spins + g.off);
         const int64_t s4 = g.stride / 16;
-        const dim3 grid = grid_for(half, 256, g.nrep);
+        const int blk = st->tune_block;
+        const dim3 grid = grid_for(half, blk, g.nrep);
         const float J = (float)st->p.J, delta = (float)st->p.delta, lam = (float)st->p.lambda, mass = (float)st->p.mass;
         if (dim == 3) {
             if (bonds)
-                hipLaunchKernelGGL((k_sweep_o3<3, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
+                hipLaunchKernelGGL((k_sweep_o3<3, true>), grid, dim3(blk), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
                                    st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0, key);
             else
-                hipLaunchKernelGGL((k_sweep_o3<3, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds,
+                hipLaunchKernelGGL((k_sweep_o3<3, false>), grid, dim3(blk), 0, s, b, s4, st->d_bonds,
                                    st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
                                    colour, phase, st->cur_dyn, g.rep0, key);
         } else {
             if (bonds)
-                hipLaunchKernelGGL((k_sweep_o3<2, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
+                hipLaunchKernelGGL((k_sweep_o3<2, true>), grid, dim3(blk), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
                                    st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0, key);
             else
-                hipLaunchKernelGGL((k_sweep_o3<2, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds,
+                hipLaunchKernelGGL((k_sweep_o3<2, false>), grid, dim3(blk), 0, s, b, s4, st->d_bonds,
                                    st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
                                    colour, phase, st->cur_dyn, g.rep0, key);
         }
@@ -818,23 +823,23 @@ hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double
     return hipGetLastError();
 }
 
-// Multi-sweep launch over all groups when every replica (and its couplings) fits
-// in 64 KiB of LDS.  Returns hipErrorNotSupported when the state does not qualify.
-hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key)
+// Descriptor table of the LDS multi-sweep kernel, built once when the state is
+// created (no allocation or copy may happen while a graph is being captured).
+hipError_t prepare_lds(mq_state* st)
 {
+    st->lds_ok = false;
     const int model = st->p.model;
-    if (model != MQ_ISING && model != MQ_EA_ISING) return hipErrorNotSupported;
-    const bool bonds = (model == MQ_EA_ISING) || (st->d_bonds != nullptr);
+    if (model != MQ_ISING && model != MQ_EA_ISING) return hipSuccess;
+    if (st->msc || st->lat[0]->kind != LAT_REGULAR) return hipSuccess;
     const int dim = st->lat[0]->dim;
-    if (st->lat[0]->kind != LAT_REGULAR) return hipErrorNotSupported;
     std::vector<LdsGroup> gs;
     int blocks = 0;
-    size_t lds = 0;
+    size_t need0 = 0, need1 = 0;
     for (const auto& g : st->groups) {
         const int64_t N = g.lat->N;
-        const size_t need = (size_t)((N + 15) & ~15ll) + (bonds ? (size_t)(N * dim) : 0);
-        if (N >= 65536 || need > 65536 - 256) return hipErrorNotSupported;
-        lds = std::max(lds, need);
+        if (N >= 65536) return hipSuccess;
+        need0 = std::max(need0, (size_t)((N + 15) & ~15ll));
+        need1 = std::max(need1, (size_t)((N + 15) & ~15ll) + (size_t)(N * dim));
         LdsGroup d;
         d.off = g.off;
         d.stride = g.stride;
@@ -847,19 +852,34 @@ hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 ke
         blocks += g.nrep;
         gs.push_back(d);
     }
-    if (gs.size() > 64) return hipErrorNotSupported;
-    if (!st->d_lds_groups) {
-        if (hipMalloc(&st->d_lds_groups, 64 * sizeof(LdsGroup)) != hipSuccess) return hipErrorOutOfMemory;
-    }
-    if (st->lds_groups_uploaded != (int)gs.size()) {
-        hipError_t e = hipMemcpy(st->d_lds_groups, gs.data(), gs.size() * sizeof(LdsGroup), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return e;
-        st->lds_groups_uploaded = (int)gs.size();
-    }
+    if (gs.size() > 64 || need0 > 65536 - 256) return hipSuccess;
+    hipError_t e = hipMalloc(&st->d_lds_groups, gs.size() * sizeof(LdsGroup));
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(st->d_lds_groups, gs.data(), gs.size() * sizeof(LdsGroup), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    st->lds_ok = true;
+    st->lds_ngroups = (int)gs.size();
+    st->lds_blocks = blocks;
+    st->lds_need[0] = need0;
+    st->lds_need[1] = need1;
+    return hipSuccess;
+}
+
+// Multi-sweep launch over all groups when every replica (and its couplings) fits
+// in 64 KiB of LDS.  Returns hipErrorNotSupported when the state does not qualify.
+hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key)
+{
+    if (!st->lds_ok) return hipErrorNotSupported;
+    const bool bonds = (st->p.model == MQ_EA_ISING) || (st->d_bonds != nullptr);
+    const size_t lds = st->lds_need[bonds ? 1 : 0];
+    if (lds > 65536 - 256) return hipErrorNotSupported;
+    const int dim = st->lat[0]->dim;
+    const int blocks = st->lds_blocks;
     const LdsGroup* dg = reinterpret_cast<const LdsGroup*>(st->d_lds_groups);
+    const int ng = st->lds_ngroups;
     hipStream_t s = st->ctx->stream;
 #define MQ_L(D, B)                                                                                               \
-    hipLaunchKernelGGL((k_sweep_lds<D, B>), dim3(blocks), dim3(256), lds, s, st->d_spins, dg, (int)gs.size(),     \
+    hipLaunchKernelGGL((k_sweep_lds<D, B>), dim3(blocks), dim3(st->lds_block), lds, s, st->d_spins, dg, ng,     \
                        st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, nsweeps, sweep0, st->cur_dyn, key)
     if (dim == 3) {
         if (bonds) MQ_L(3, true);

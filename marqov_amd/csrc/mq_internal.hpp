@@ -146,9 +146,14 @@ struct mq_state {
     const uint32_t* cur_dyn = nullptr;  // device phase/step offsets while a graph is captured
     uint32_t* d_dyn = nullptr;          // [0] sweep-phase offset, [1] PT-step offset
     void* d_lds_groups = nullptr;       // descriptor table of the LDS multi-sweep kernel
-    int lds_groups_uploaded = 0;
+    bool lds_ok = false;
+    int lds_ngroups = 0, lds_blocks = 0;
+    size_t lds_need[2] = {0, 0};        // LDS bytes per block without / with couplings
     bool msc = false;                   // 1-bit multi-spin-coded Ising storage
     void* d_msc_cls = nullptr;          // per-replica Bernoulli classes of the MSC sweep
+    int tune_block = 256;               // threads per block of the streaming sweep kernels
+    int tune_xcd = 1;                   // XCD-aware block remap of the row-vectorised kernel
+    int lds_block = 256;                // threads per block of the LDS multi-sweep kernel
     int kernel_choice = 0;              // 0 auto, 1 streaming kernels only, 2 generic streaming kernel only  // force the generic kernels (parity of the specialised ones)
 };
 
@@ -170,6 +175,7 @@ hipError_t launch_measure_regular(mq_state* st, const Group& g);
 hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
 hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key);
 hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key);
+hipError_t prepare_lds(mq_state* st);
 bool msc_supported(const mq_lattice* lat);
 hipError_t msc_upload_classes(mq_state* st, const std::vector<uint32_t>& thr);
 hipError_t launch_sweep_msc(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);

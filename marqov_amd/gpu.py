@@ -200,6 +200,15 @@ class State:
         self.ctx.check(self.ctx.lib.mq_pt_get_temps(self.h, _ptr(t)), "mq_pt_get_temps")
         return t
 
+    def set_tuning(self, kernels=0, block=256, xcd_remap=1, lds_block=256):
+        t = L.Tuning(kernels, block, xcd_remap, lds_block)
+        self.ctx.check(self.ctx.lib.mq_state_set_tuning(self.h, ctypes.byref(t)), "mq_state_set_tuning")
+
+    def get_tuning(self):
+        t = L.Tuning()
+        self.ctx.check(self.ctx.lib.mq_state_get_tuning(self.h, ctypes.byref(t)), "mq_state_get_tuning")
+        return {"kernels": t.kernels, "block": t.block, "xcd_remap": t.xcd_remap, "lds_block": t.lds_block}
+
     def run_graph(self, nsweeps, ntemp, seed, sweep0, step0, steps, repeats):
         self.ctx.check(self.ctx.lib.mq_run_graph(self.h, nsweeps, ntemp, seed, sweep0, step0, steps, repeats),
                        "mq_run_graph")
