@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--L", type=int, default=256)
     ap.add_argument("--quick", action="store_true", help="headline only (no configs[0],[2..4] legs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="", help="comma-separated subset of the extra legs to run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM traffic)")
     return ap.parse_args()
@@ -57,6 +58,27 @@ def timed(ctx, fn):
     fn()
     ms = ctx.timer_end()
     return ms, time.perf_counter() - t0
+
+
+def leg_c2_msc(gpu, ctx, L=256, R=32, steps=100):
+    """configs[1] multi-spin-coded variant: 1 bit per spin, R replicas, exact bit-serial acceptance."""
+    lat = gpu.RegularLattice(ctx, 3, L)
+    N = L ** 3
+    res = {}
+    for name, nrep in (("batched", R), ("plain", 1)):
+        st = gpu.State(ctx, "ising", [lat] * nrep, J=-1.0, spin_bits=1)
+        st.set_beta(0.2216546)
+        st.init(seed=2)
+        st.metropolis(10, seed=2, sweep0=0)
+        ms, _ = timed(ctx, lambda: st.metropolis(steps, seed=2, sweep0=10))
+        rate = nrep * N * steps / (ms * 1e6)
+        res[name] = {"replicas": nrep, "attempts_per_ns": round(rate, 3),
+                     "roofline_frac": round(0.375 * rate / HBM_PEAK_GBS, 4)}
+        if name == "batched":
+            res[name]["e"] = float(st.measure()[:, 0].mean())
+        del st
+    res["batched_over_plain"] = round(res["batched"]["attempts_per_ns"] / res["plain"]["attempts_per_ns"], 2)
+    return res
 
 
 def leg_c1(gpu, ctx):
@@ -222,7 +244,11 @@ def main():
         "config": {"workload": f"3D Ising L={L} periodic J=-1 beta={beta} int8 checkerboard Metropolis, "
                                f"{R} replicas per launch", "replicas": R, "L": L, "sites": N},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     # HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE of this configuration,
+                     # scaled by a 1 GiB copy (factors 2.0 / 1.0): 539.1 MB read + 268.4 MB written
+                     "traffic": 807.5e6 if (L, R) == (256, 32) else None,
+                     "traffic_source": "profiles/k_sweep_ising3_row16_3d_L256_R32.md",
                      "kernel": "k_sweep_ising3_row16", "launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "observables": {"e_mean": float(obs[:, 0].mean()), "m_abs_mean": float(obs[:, 1].mean())},
@@ -243,8 +269,10 @@ def main():
         line["plain"]["wolff_sites_per_ns"] = round(16 * float(sizes.mean()) / (ms_w * 1e6), 4)
         line["plain"]["wolff_mean_cluster"] = round(float(sizes.mean()), 1)
         del sp
-        for name, fn in (("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3), ("c4_ea_batch", leg_c4),
-                         ("c5_rgg", leg_c5)):
+        for name, fn in (("c2_msc", leg_c2_msc), ("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3),
+                         ("c4_ea_batch", leg_c4), ("c5_rgg", leg_c5)):
+            if a.legs and name not in a.legs.split(","):
+                continue
             log(name)
             try:
                 line[name] = fn(gpu, ctx)

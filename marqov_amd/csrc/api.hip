@@ -293,13 +293,11 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         const int64_t b = msc ? l->N / 8 : l->N * st->site_bytes;
         return (b + 255) / 256 * 256;
     };
+    // groups of consecutive replicas on one lattice; regular lattices store each
+    // replica contiguously, graphs interleave the replicas of a group site by site
     st->rep_off.resize(nrep);
+    st->rep_pos_stride.resize(nrep);
     int64_t off = 0;
-    for (int r = 0; r < nrep; ++r) {
-        st->rep_off[r] = off;
-        off += rep_bytes(st->lat[r]);
-    }
-    st->spin_total = off;
     for (int r = 0; r < nrep;) {
         int e = r + 1;
         while (e < nrep && st->lat[e] == st->lat[r]) ++e;
@@ -307,11 +305,24 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         g.lat = st->lat[r];
         g.rep0 = r;
         g.nrep = e - r;
-        g.off = st->rep_off[r];
-        g.stride = rep_bytes(g.lat);
+        g.off = off;
+        if (g.lat->kind == LAT_CSR) {
+            g.stride = st->site_bytes;
+            g.pos_stride = (int64_t)st->site_bytes * g.nrep;
+            off += (g.lat->N * g.pos_stride + 255) / 256 * 256;
+        } else {
+            g.stride = rep_bytes(g.lat);
+            g.pos_stride = st->site_bytes;
+            off += g.stride * g.nrep;
+        }
+        for (int k = r; k < e; ++k) {
+            st->rep_off[k] = g.off + (k - r) * g.stride;
+            st->rep_pos_stride[k] = g.pos_stride;
+        }
         st->groups.push_back(g);
         r = e;
     }
+    st->spin_total = off;
     st->tbl = 2 * qmax_of(st) + 1;
     if (const char* kc = std::getenv("MARQOV_AMD_KERNELS")) st->kernel_choice = std::atoi(kc);
     st->rep_sample.resize(nrep);
@@ -508,6 +519,39 @@ extern "C" int mq_state_init(mq_state* st, uint64_t seed)
     return MQ_OK;
 }
 
+
+// Copy one replica's sites between a dense host array (elem bytes per site, in
+// device position order) and the device: one contiguous copy when the replica
+// is stored contiguously, else through a host image of the whole group block.
+static int copy_replica(mq_state* st, int rep, void* host, size_t elem, bool to_device)
+{
+    const mq_lattice* l = st->lat[rep];
+    const int64_t N = l->N;
+    const int64_t ps = st->rep_pos_stride[rep];
+    char* dev = st->d_spins + st->rep_off[rep];
+    if (ps == (int64_t)elem) {
+        if (to_device) MQ_HIP(st->ctx, hipMemcpy(dev, host, (size_t)N * elem, hipMemcpyHostToDevice));
+        else MQ_HIP(st->ctx, hipMemcpy(host, dev, (size_t)N * elem, hipMemcpyDeviceToHost));
+        return MQ_OK;
+    }
+    const Group* g = nullptr;
+    for (const auto& gg : st->groups)
+        if (rep >= gg.rep0 && rep < gg.rep0 + gg.nrep) g = &gg;
+    const size_t bytes = (size_t)(N * ps);
+    std::vector<char> img(bytes);
+    char* gbase = st->d_spins + g->off;
+    const int64_t within = (int64_t)(rep - g->rep0) * g->stride;
+    MQ_HIP(st->ctx, hipMemcpy(img.data(), gbase, bytes, hipMemcpyDeviceToHost));
+    char* h = static_cast<char*>(host);
+    for (int64_t p = 0; p < N; ++p) {
+        char* slot = img.data() + p * ps + within;
+        if (to_device) std::memcpy(slot, h + p * elem, elem);
+        else std::memcpy(h + p * elem, slot, elem);
+    }
+    if (to_device) MQ_HIP(st->ctx, hipMemcpy(gbase, img.data(), bytes, hipMemcpyHostToDevice));
+    return MQ_OK;
+}
+
 extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t bytes)
 {
     if (!st || !host || rep < 0 || rep >= st->nrep) return MQ_ERR_ARG;
@@ -523,7 +567,8 @@ extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t 
             dev[p * 4 + 1] = h[3 * i + 1];
             dev[p * 4 + 2] = h[3 * i + 2];
         }
-        MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), dev.size() * 4, hipMemcpyHostToDevice));
+        int rc = copy_replica(st, rep, dev.data(), 16, true);
+        if (rc) return rc;
     } else if (st->msc) {
         if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected one int8 per site");
         const int8_t* h = static_cast<const int8_t*>(host);
@@ -544,7 +589,8 @@ extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t 
                 return mq_fail(st->ctx, MQ_ERR_ARG, "upload: spin value out of range");
             dev[dev_pos(l, i)] = v;
         }
-        MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), dev.size(), hipMemcpyHostToDevice));
+        int rc = copy_replica(st, rep, dev.data(), 1, true);
+        if (rc) return rc;
     }
     return MQ_OK;
 }
@@ -558,7 +604,8 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
     if (is_vec(st->p.model)) {
         if (bytes != N * 12) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected 3 floats per site");
         std::vector<float> dev((size_t)N * 4);
-        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + st->rep_off[rep], dev.size() * 4, hipMemcpyDeviceToHost));
+        int rc = copy_replica(st, rep, dev.data(), 16, false);
+        if (rc) return rc;
         float* h = static_cast<float*>(host);
         for (int64_t i = 0; i < N; ++i) {
             const int64_t p = dev_pos(l, i);
@@ -578,7 +625,8 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
     } else {
         if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
         std::vector<int8_t> dev((size_t)N);
-        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + st->rep_off[rep], dev.size(), hipMemcpyDeviceToHost));
+        int rc = copy_replica(st, rep, dev.data(), 1, false);
+        if (rc) return rc;
         int8_t* h = static_cast<int8_t*>(host);
         for (int64_t i = 0; i < N; ++i) h[i] = dev[dev_pos(l, i)];
     }

@@ -3,7 +3,11 @@
 //
 // Layout: vertices are renumbered so that every colour class is contiguous
 // (position = class offset + rank by original id); the device CSR holds
-// positions.  Spins per replica are stored by position.  Random numbers are
+// positions.  The replicas of one graph are interleaved: the spin of replica r
+// at position p sits at p * R + r (R replicas), so a wavefront sweeping one
+// vertex for 64 replicas makes 64-byte coalesced neighbour reads while the CSR
+// row and column indices are wave-uniform loads shared by all replicas.
+// Kernels take (replica stride, position stride) in elements.  Random numbers are
 // keyed by original vertex ids and original CSR slots, so a CPU replay needs
 // only the colour of every vertex, not the renumbering.
 //
@@ -30,8 +34,9 @@ __device__ __forceinline__ float3 rnddir_c(uint32_t w0, uint32_t w1)
     return make_float3(1.0f - a0, rho * sn, rho * cs);
 }
 
-__global__ void __launch_bounds__(256) k_csr_init(char* __restrict__ base, int64_t stride, const int32_t* __restrict__ vert_of,
-                                                  int64_t N, int model, int Q, int rep0, Key2 key)
+__global__ void __launch_bounds__(256) k_csr_init(char* __restrict__ base, int64_t rs, int64_t ps,
+                                                  const int32_t* __restrict__ vert_of, int64_t N, int model, int Q,
+                                                  int rep0, Key2 key)
 {
     const int r = blockIdx.y;
     const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,19 +45,20 @@ __global__ void __launch_bounds__(256) k_csr_init(char* __restrict__ base, int64
     if (model == MQ_HEISENBERG || model == MQ_PHI4) {
         const uint4 w = philox4x32_10(v, 0u, (uint32_t)(rep0 + r), ST_INIT, key);
         const float3 d = rnddir_c(w.x, w.y);
-        reinterpret_cast<float4*>(base + r * stride)[pos] = make_float4(d.x, d.y, d.z, 0.f);
+        reinterpret_cast<float4*>(base)[r * rs + pos * ps] = make_float4(d.x, d.y, d.z, 0.f);
     } else {
         const uint4 w = philox4x32_10(v >> 2, 0u, (uint32_t)(rep0 + r), ST_INIT, key);
         const uint32_t x = u4_get(w, (int)(v & 3u));
         int8_t s;
         if (model == MQ_POTTS) s = (int8_t)(((uint64_t)x * (uint64_t)Q) >> 32);
         else s = (x >> 31) ? (int8_t)1 : (int8_t)-1;
-        reinterpret_cast<int8_t*>(base + r * stride)[pos] = s;
+        reinterpret_cast<int8_t*>(base)[r * rs + pos * ps] = s;
     }
 }
 
+// one thread per (vertex of the colour class, replica), replica fastest
 template <bool BONDS>
-__global__ void __launch_bounds__(256) k_csr_sweep_int8(int8_t* __restrict__ base, int64_t stride,
+__global__ void __launch_bounds__(512) k_csr_sweep_int8(int8_t* __restrict__ base, int64_t rs, int64_t ps, int R,
                                                         const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                         const int32_t* __restrict__ vert_of,
                                                         const int8_t* __restrict__ bonds,
@@ -62,39 +68,40 @@ __global__ void __launch_bounds__(256) k_csr_sweep_int8(int8_t* __restrict__ bas
                                                         const uint32_t* __restrict__ dyn, int rep0, Key2 key)
 {
     if (dyn) phase += dyn[0];
-    const int r = blockIdx.y;
-    const int grep = rep0 + r;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nc) return;
-    const int64_t pos = c0 + t;
-    int8_t* s = base + r * stride;
+    if (t >= nc * R) return;
+    const int64_t vl = t / R;
+    const int r = (int)(t - vl * R);
+    const int grep = rep0 + r;
+    const int64_t pos = c0 + vl;
+    int8_t* s = base + r * rs;
     const int qmax = (tbl - 1) >> 1;
     const int a = rp[pos], b = rp[pos + 1];
     const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
     const uint32_t* T = thr + (int64_t)grep * 2 * tbl;
-    const int si = s[pos];
+    const int si = s[pos * ps];
     if (model == MQ_POTTS) {
         const int snew = (int)(((uint64_t)w.x * (uint64_t)Q) >> 32);
         int dn = 0;
         for (int k = a; k < b; ++k) {
-            const int v = s[ci[k]];
+            const int v = s[(int64_t)ci[k] * ps];
             dn += (v == snew) - (v == si);
         }
         const int e = dn + qmax;
         const uint64_t th = ((uint64_t)T[2 * e] << 16) | T[2 * e + 1];
-        if ((uint64_t)w.y < th) s[pos] = (int8_t)snew;
+        if ((uint64_t)w.y < th) s[pos * ps] = (int8_t)snew;
     } else {
         int h = 0;
         const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
-        for (int k = a; k < b; ++k) h += (BONDS ? (int)bb[k] : 1) * s[ci[k]];
+        for (int k = a; k < b; ++k) h += (BONDS ? (int)bb[k] : 1) * s[(int64_t)ci[k] * ps];
         const int e = si * h + qmax;
         const uint64_t th = ((uint64_t)T[2 * e] << 16) | T[2 * e + 1];
-        if ((uint64_t)w.x < th) s[pos] = (int8_t)(-si);
+        if ((uint64_t)w.x < th) s[pos * ps] = (int8_t)(-si);
     }
 }
 
 template <bool BONDS>
-__global__ void __launch_bounds__(256) k_csr_sweep_o3(float4* __restrict__ base, int64_t stride4,
+__global__ void __launch_bounds__(512) k_csr_sweep_o3(float4* __restrict__ base, int64_t rs, int64_t ps, int R,
                                                       const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                       const int32_t* __restrict__ vert_of,
                                                       const int8_t* __restrict__ bonds,
@@ -105,23 +112,24 @@ __global__ void __launch_bounds__(256) k_csr_sweep_o3(float4* __restrict__ base,
                                                       int rep0, Key2 key)
 {
     if (dyn) phase += dyn[0];
-    const int r = blockIdx.y;
-    const int grep = rep0 + r;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nc) return;
-    const int64_t pos = c0 + t;
-    float4* s = base + r * stride4;
+    if (t >= nc * R) return;
+    const int64_t vl = t / R;
+    const int r = (int)(t - vl * R);
+    const int grep = rep0 + r;
+    const int64_t pos = c0 + vl;
+    float4* s = base + r * rs;
     const int a = rp[pos], b = rp[pos + 1];
     float nx = 0.f, ny = 0.f, nz = 0.f;
     const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
     for (int k = a; k < b; ++k) {
-        const float4 v = s[ci[k]];
+        const float4 v = s[(int64_t)ci[k] * ps];
         const float f = BONDS ? (float)bb[k] : 1.0f;
         nx += f * v.x;
         ny += f * v.y;
         nz += f * v.z;
     }
-    const float4 so = s[pos];
+    const float4 so = s[pos * ps];
     const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
     const float3 rd = rnddir_c(w.x, w.y);
     float3 sn;
@@ -143,7 +151,7 @@ __global__ void __launch_bounds__(256) k_csr_sweep_o3(float4* __restrict__ base,
     }
     bool acc = dE <= 0.f;
     if (!acc) acc = u01f(w.z) < __expf(-beta * dE);
-    if (acc) s[pos] = make_float4(sn.x, sn.y, sn.z, 0.f);
+    if (acc) s[pos * ps] = make_float4(sn.x, sn.y, sn.z, 0.f);
 }
 
 template <typename T>
@@ -158,7 +166,7 @@ __device__ __forceinline__ T wsum(T v)
 // Potts [1+q] colour counts; fsum (O(3)): [0] sum over edges b s_i.s_j, [1..3] M,
 // [4] sum |s|^2, [5] sum (|s|^2-1)^2.  Each edge counted once (from its lower end).
 template <bool BONDS, bool VEC>
-__global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ base, int64_t stride,
+__global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ base, int64_t rs, int64_t ps,
                                                      const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                      const int8_t* __restrict__ bonds,
                                                      const int64_t* __restrict__ rep_bond_off, int64_t N, int model,
@@ -174,12 +182,12 @@ __global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ ba
         const int a = rp[pos], b = rp[pos + 1];
         const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
         if (VEC) {
-            const float4* s = reinterpret_cast<const float4*>(base + r * stride);
-            const float4 v = s[pos];
+            const float4* s = reinterpret_cast<const float4*>(base) + r * rs;
+            const float4 v = s[pos * ps];
             double e = 0;
             for (int k = a; k < b; ++k) {
                 if (ci[k] <= pos) continue;
-                const float4 u = s[ci[k]];
+                const float4 u = s[(int64_t)ci[k] * ps];
                 e += (BONDS ? (double)bb[k] : 1.0) * ((double)v.x * u.x + (double)v.y * u.y + (double)v.z * u.z);
             }
             const double n2 = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
@@ -190,12 +198,12 @@ __global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ ba
             acc[4] = n2;
             acc[5] = (n2 - 1) * (n2 - 1);
         } else {
-            const int8_t* s = reinterpret_cast<const int8_t*>(base + r * stride);
-            const int si = s[pos];
+            const int8_t* s = reinterpret_cast<const int8_t*>(base) + r * rs;
+            const int si = s[pos * ps];
             long long e = 0;
             for (int k = a; k < b; ++k) {
                 if (ci[k] <= pos) continue;
-                const int sj = s[ci[k]];
+                const int sj = s[(int64_t)ci[k] * ps];
                 if (model == MQ_POTTS) e += (si == sj);
                 else e += (long long)(BONDS ? bb[k] : 1) * si * sj;
             }
@@ -226,7 +234,7 @@ __global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ ba
 enum { CK_ISING = 0, CK_POTTS = 1, CK_CART = 2, CK_RDIR = 3 };
 
 template <int KIND>
-__global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int64_t stride, int64_t N,
+__global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int64_t rs, int64_t ps, int64_t N,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                    const int32_t* __restrict__ edge, const int32_t* __restrict__ pos_of,
                                                    int nclusters, uint32_t cl0, int rep0, int Q,
@@ -239,8 +247,9 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
     __shared__ int s_tail;
     const int r = blockIdx.x;
     const int grep = rep0 + r;
-    int8_t* sp8 = reinterpret_cast<int8_t*>(base + r * stride);
-    float4* sp4 = reinterpret_cast<float4*>(base + r * stride);
+    // element (position p) of this replica: sp8[p * ps] / sp4[p * ps]
+    int8_t* sp8 = reinterpret_cast<int8_t*>(base) + r * rs;
+    float4* sp4 = reinterpret_cast<float4*>(base) + r * rs;
     int32_t* q = queue + rep_site_off[grep];
     uint32_t* vis = visit + (rep_site_off[grep] >> 5);
     const uint32_t tw = VEC ? 0u : wthr[grep];
@@ -252,7 +261,7 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
         int val = 0, newq = 0, axis = 0;
         float3 rd = make_float3(0.f, 0.f, 0.f);
         if (!VEC) {
-            val = sp8[seed];
+            val = sp8[seed * ps];
             if (KIND == CK_POTTS) newq = (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q;
         } else if (KIND == CK_CART) {
             axis = (int)(((uint64_t)sw.y * 3ull) >> 32);
@@ -261,10 +270,10 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
         }
         __syncthreads();
         auto flip = [&](int64_t p) {
-            if (KIND == CK_ISING) sp8[p] = (int8_t)(-val);
-            else if (KIND == CK_POTTS) sp8[p] = (int8_t)newq;
+            if (KIND == CK_ISING) sp8[p * ps] = (int8_t)(-val);
+            else if (KIND == CK_POTTS) sp8[p * ps] = (int8_t)newq;
             else {
-                float4 v = sp4[p];
+                float4 v = sp4[p * ps];
                 if (KIND == CK_CART) {
                     if (axis == 0) v.x = -v.x;
                     else if (axis == 1) v.y = -v.y;
@@ -275,7 +284,7 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
                     v.y -= 2.f * d * rd.y;
                     v.z -= 2.f * d * rd.z;
                 }
-                sp4[p] = v;
+                sp4[p * ps] = v;
             }
         };
         if (threadIdx.x == 0) {
@@ -294,7 +303,7 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
                 const int64_t cur = q[idx];
                 float pc = 0.f;
                 if (VEC) {
-                    const float4 vc = sp4[cur];
+                    const float4 vc = sp4[cur * ps];
                     pc = (KIND == CK_CART) ? (axis == 0 ? vc.x : (axis == 1 ? vc.y : vc.z))
                                            : (vc.x * rd.x + vc.y * rd.y + vc.z * rd.z);
                 }
@@ -303,11 +312,11 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
                     const uint32_t eid = (uint32_t)edge[k];
                     bool add;
                     if (!VEC) {
-                        if (sp8[j] != val) continue;
+                        if (sp8[j * ps] != val) continue;
                         const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
                         add = u4_get(bw, (int)(eid & 3u)) < tw;
                     } else {
-                        const float4 vj = sp4[j];
+                        const float4 vj = sp4[j * ps];
                         const float pj = (KIND == CK_CART) ? (axis == 0 ? vj.x : (axis == 1 ? vj.y : vj.z))
                                                            : (vj.x * rd.x + vj.y * rd.y + vj.z * rd.z);
                         const float cpl = J * pc * pj;
@@ -464,7 +473,7 @@ extern "C" int mq_lattice_color_of(const mq_lattice* lat, int32_t* colour)
 hipError_t launch_init_csr(mq_state* st, const Group& g, Key2 key)
 {
     hipLaunchKernelGGL(k_csr_init, grid_for(g.lat->N, g.nrep), dim3(256), 0, st->ctx->stream, st->d_spins + g.off,
-                       g.stride, g.lat->d_vert_of, g.lat->N, st->p.model, st->p.q, g.rep0, key);
+                       (int64_t)1, (int64_t)g.nrep, g.lat->d_vert_of, g.lat->N, st->p.model, st->p.q, g.rep0, key);
     return hipGetLastError();
 }
 
@@ -476,25 +485,28 @@ hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t p
     const int model = st->p.model;
     const bool bonds = st->d_bonds != nullptr;
     hipStream_t s = st->ctx->stream;
+    const int64_t work = nc * g.nrep;
+    const int blk = st->tune_block;
+    const dim3 grid((unsigned)((work + blk - 1) / blk));
     if (model == MQ_HEISENBERG || model == MQ_PHI4) {
         float4* b = reinterpret_cast<float4*>(st->d_spins + g.off);
         const float J = (float)st->p.J, d = (float)st->p.delta, lam = (float)st->p.lambda, m = (float)st->p.mass;
         if (bonds)
-            hipLaunchKernelGGL(k_csr_sweep_o3<true>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride / 16, l->d_rp,
+            hipLaunchKernelGGL(k_csr_sweep_o3<true>, grid, dim3(blk), 0, s, b, (int64_t)1, (int64_t)g.nrep, g.nrep, l->d_rp,
                                l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_betaf, c0, nc, model,
                                st->p.proposal, J, d, lam, m, phase, st->cur_dyn, g.rep0, key);
         else
-            hipLaunchKernelGGL(k_csr_sweep_o3<false>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride / 16, l->d_rp,
+            hipLaunchKernelGGL(k_csr_sweep_o3<false>, grid, dim3(blk), 0, s, b, (int64_t)1, (int64_t)g.nrep, g.nrep, l->d_rp,
                                l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_betaf, c0, nc, model,
                                st->p.proposal, J, d, lam, m, phase, st->cur_dyn, g.rep0, key);
     } else {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
         if (bonds)
-            hipLaunchKernelGGL(k_csr_sweep_int8<true>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride, l->d_rp,
+            hipLaunchKernelGGL(k_csr_sweep_int8<true>, grid, dim3(blk), 0, s, b, (int64_t)1, (int64_t)g.nrep, g.nrep, l->d_rp,
                                l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc,
                                model, st->p.q, phase, st->cur_dyn, g.rep0, key);
         else
-            hipLaunchKernelGGL(k_csr_sweep_int8<false>, grid_for(nc, g.nrep), dim3(256), 0, s, b, g.stride, l->d_rp,
+            hipLaunchKernelGGL(k_csr_sweep_int8<false>, grid, dim3(blk), 0, s, b, (int64_t)1, (int64_t)g.nrep, g.nrep, l->d_rp,
                                l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc,
                                model, st->p.q, phase, st->cur_dyn, g.rep0, key);
     }
@@ -509,7 +521,8 @@ hipError_t launch_measure_csr(mq_state* st, const Group& g)
     hipStream_t s = st->ctx->stream;
     const dim3 grid = grid_for(l->N, g.nrep);
 #define MQ_M(B, V)                                                                                                  \
-    hipLaunchKernelGGL((k_csr_measure<B, V>), grid, dim3(256), 0, s, st->d_spins + g.off, g.stride, l->d_rp,       \
+    hipLaunchKernelGGL((k_csr_measure<B, V>), grid, dim3(256), 0, s, st->d_spins + g.off, (int64_t)1,             \
+                       (int64_t)g.nrep, l->d_rp,                                                                 \
                        l->d_ci, st->d_bonds, st->d_rep_bond_off, l->N, st->p.model, st->p.q, g.rep0, st->d_isum, \
                        st->d_fsum)
     if (vec) {
@@ -530,7 +543,8 @@ hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_
     hipStream_t s = st->ctx->stream;
     const float J = (float)st->p.J;
 #define MQ_W(K)                                                                                                   \
-    hipLaunchKernelGGL((k_csr_wolff<K>), dim3(g.nrep), dim3(256), 0, s, st->d_spins + g.off, g.stride, l->N,      \
+    hipLaunchKernelGGL((k_csr_wolff<K>), dim3(g.nrep), dim3(256), 0, s, st->d_spins + g.off, (int64_t)1,          \
+                       (int64_t)g.nrep, l->N,                                                                    \
                        l->d_rp, l->d_ci, l->d_edge, l->d_pos_of, nclusters, cl0, g.rep0, st->p.q, st->d_wthr,    \
                        st->d_betaf, J, st->d_queue, st->d_visit, st->d_rep_site_off, st->d_csize, key)
     if (model == MQ_ISING) MQ_W(CK_ISING);

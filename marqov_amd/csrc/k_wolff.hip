@@ -180,6 +180,80 @@ __global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, 
     }
 }
 
+
+// LDS-resident variant for small int8 lattices (N <= 16384): the replica's
+// spins, its visited bitmap and a 16-bit cluster queue live in LDS for all
+// nclusters clusters; spins are written back once.  Same bond counters and
+// membership rule as k_wolff_regular, so the clusters are identical.
+template <int DIM, int KIND>
+__global__ void __launch_bounds__(256) k_wolff_lds(char* __restrict__ base, int64_t stride, int L, int nclusters,
+                                                   uint32_t cl0, int rep0, int Q, const uint32_t* __restrict__ wthr,
+                                                   int64_t* __restrict__ csize, Key2 key)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_tail;
+    const int r = blockIdx.x;
+    const int grep = rep0 + r;
+    int N = L * L;
+    if (DIM == 3) N *= L;
+    const int half = N >> 1;
+    int8_t* S = reinterpret_cast<int8_t*>(smem);
+    uint32_t* vis = reinterpret_cast<uint32_t*>(smem + ((N + 15) & ~15));
+    uint16_t* q = reinterpret_cast<uint16_t*>(smem + ((N + 15) & ~15) + (((N + 31) / 32) * 4 + 15) / 16 * 16);
+    int8_t* gs = reinterpret_cast<int8_t*>(base + r * stride);
+    for (int i = threadIdx.x; i < N; i += blockDim.x) S[i] = gs[i];
+    for (int i = threadIdx.x; i < (N + 31) / 32; i += blockDim.x) vis[i] = 0u;
+    const uint32_t tw = wthr[grep];
+    __syncthreads();
+    for (int c = 0; c < nclusters; ++c) {
+        const uint32_t ncl = cl0 + (uint32_t)c;
+        const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
+        const int seed = (int)(((uint64_t)sw.x * (uint64_t)N) >> 32);
+        const int seed_a = (int)split_addr<DIM>(L, half, seed);
+        const int val = S[seed_a];
+        int newq = 0;
+        if (KIND == WK_POTTS) newq = (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            S[seed_a] = (KIND == WK_ISING) ? (int8_t)(-val) : (int8_t)newq;
+            q[0] = (uint16_t)seed;
+            vis[seed >> 5] |= 1u << (seed & 31);
+            s_tail = 1;
+        }
+        __syncthreads();
+        int head = 0;
+        while (true) {
+            const int tail = s_tail;
+            __syncthreads();
+            if (head >= tail) break;
+            for (int idx = head + (int)threadIdx.x; idx < tail; idx += blockDim.x) {
+                const int cur = q[idx];
+#pragma unroll
+                for (int sl = 0; sl < 2 * DIM; ++sl) {
+                    const int j = (int)nat_nbr<DIM>(L, cur, sl);
+                    const int ja = (int)split_addr<DIM>(L, half, j);
+                    if (S[ja] != val) continue;
+                    const int owner = (sl & 1) ? j : cur;
+                    const uint32_t b = (uint32_t)owner * DIM + (uint32_t)(sl >> 1);
+                    const uint4 bw = philox4x32_10(b >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                    if (!(u4_get(bw, (int)(b & 3u)) < tw)) continue;
+                    const uint32_t bit = 1u << (j & 31);
+                    if (atomicOr(&vis[j >> 5], bit) & bit) continue;
+                    S[ja] = (KIND == WK_ISING) ? (int8_t)(-val) : (int8_t)newq;
+                    q[atomicAdd(&s_tail, 1)] = (uint16_t)j;
+                }
+            }
+            __syncthreads();
+            head = tail;
+        }
+        const int size = s_tail;
+        for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+        __syncthreads();
+        if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
+    }
+    for (int i = threadIdx.x; i < N; i += blockDim.x) gs[i] = S[i];
+}
+
 }  // namespace
 
 hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key)
@@ -196,6 +270,22 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
     hipStream_t s = st->ctx->stream;
     char* base = st->d_spins + g.off;
     const float J = (float)st->p.J;
+    const int64_t N = g.lat->N;
+    if ((kind == WK_ISING || kind == WK_POTTS) && N <= 16384 && st->kernel_choice == 0) {
+        const size_t lds = (size_t)((N + 15) & ~15ll) + (size_t)(((N + 31) / 32) * 4 + 15) / 16 * 16 + (size_t)N * 2;
+#define MQ_WL(D, K)                                                                                                   \
+    hipLaunchKernelGGL((k_wolff_lds<D, K>), grid, block, lds, s, base, g.stride, L, nclusters, cl0, g.rep0, st->p.q, \
+                       st->d_wthr, st->d_csize, key)
+        if (dim == 3) {
+            if (kind == WK_ISING) MQ_WL(3, WK_ISING);
+            else MQ_WL(3, WK_POTTS);
+        } else {
+            if (kind == WK_ISING) MQ_WL(2, WK_ISING);
+            else MQ_WL(2, WK_POTTS);
+        }
+#undef MQ_WL
+        return hipGetLastError();
+    }
 #define MQ_W(D, K)                                                                                               \
     hipLaunchKernelGGL((k_wolff_regular<D, K>), grid, block, 0, s, base, g.stride, L, nclusters, cl0, g.rep0,  \
                        st->p.q, st->d_wthr, st->d_betaf, J, st->d_queue, st->d_visit, st->d_rep_site_off,       \

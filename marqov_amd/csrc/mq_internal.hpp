@@ -96,8 +96,9 @@ struct mq_lattice {
 struct Group {
     mq_lattice* lat;
     int rep0, nrep;
-    int64_t off;     // byte offset of replica rep0 in d_spins
-    int64_t stride;  // bytes per replica
+    int64_t off;         // byte offset of replica rep0 in d_spins
+    int64_t stride;      // bytes between consecutive replicas
+    int64_t pos_stride;  // bytes between consecutive sites of one replica
 };
 
 struct mq_state {
@@ -107,7 +108,8 @@ struct mq_state {
     std::vector<mq_lattice*> lat;
     std::vector<Group> groups;
     int site_bytes = 1;  // 1 (int8) or 16 (float4)
-    std::vector<int64_t> rep_off;
+    std::vector<int64_t> rep_off;        // byte offset of site 0 of each replica
+    std::vector<int64_t> rep_pos_stride; // bytes between consecutive sites of each replica
     char* d_spins = nullptr;
     int64_t spin_total = 0;
     // per-replica temperatures and acceptance tables

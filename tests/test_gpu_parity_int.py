@@ -248,3 +248,26 @@ def test_msc_upload_roundtrip(ctx):
     s = np.where(np.random.default_rng(1).random(L ** 3) < 0.3, 1, -1).astype(np.int8)
     st.upload(0, s)
     assert np.array_equal(st.download(0), s)
+
+
+@pytest.mark.parametrize("kernels", ["0", "1"])
+@pytest.mark.parametrize("model,dim,L", [("ising", 2, 64), ("ising", 3, 16), ("potts", 2, 32)])
+def test_wolff_lds_and_streaming_bit_exact(ctx, kernels, model, dim, L, monkeypatch):
+    """Small lattices grow clusters in LDS (kernels=0) or in global memory (kernels=1):
+    both must equal the CPU replay."""
+    from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
+    N = L ** dim
+    lat = gpu.RegularLattice(ctx, dim, L)
+    st = gpu.State(ctx, model, [lat] * 3, J=-1.0, q=3)
+    beta = {2: 0.4406868, 3: 0.2216546}[dim] if model == "ising" else 1.0
+    st.set_beta(beta)
+    st.init(seed=30)
+    sizes = st.wolff(7, seed=31, cluster0=3)
+    for r in range(3):
+        if model == "ising":
+            exp, esz = ob.wolff_ising(dim, L, ob.init_ising(N, 30, r), beta, -1.0, 7, 31, 3, r)
+        else:
+            exp, esz = ob.wolff_potts(dim, L, ob.init_potts(N, 3, 30, r), 3, beta, -1.0, 7, 31, 3, r)
+        assert np.array_equal(sizes[r], esz), (r, sizes[r], esz)
+        assert np.array_equal(st.download(r), exp), r

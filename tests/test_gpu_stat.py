@@ -38,11 +38,13 @@ def cpu_series(model, dim, L, beta, nseeds, nwarm, nmeas, ncluster=1, nmetro=1, 
     return np.array(res)  # seeds x nmeas x 2
 
 
-def agree(gpu_per_rep, cpu_per_seed, k=4.0):
+def agree(gpu_per_rep, cpu_per_seed, k=4.0, label=""):
     g, c = np.asarray(gpu_per_rep), np.asarray(cpu_per_seed)
     sg = g.std(ddof=1) / np.sqrt(len(g))
     sc = c.std(ddof=1) / np.sqrt(len(c))
     d = abs(g.mean() - c.mean())
+    print(f"[fp32-vs-fp64] {label}: gpu {g.mean():.6f} +- {sg:.6f}  cpu {c.mean():.6f} +- {sc:.6f}  "
+          f"diff {d:.6f} = {d / np.hypot(sg, sc):.2f} sigma", flush=True)
     return d < k * np.hypot(sg, sc), (g.mean(), sg, c.mean(), sc)
 
 
@@ -52,8 +54,8 @@ def test_heisenberg_vs_fp64_restatement(ctx, proposal, embedding):
     obs, _ = gpu_series(ctx, "heisenberg", 3, 6, beta, 64, 100, 400, J=-1.0, proposal=proposal, delta=0.6,
                         embedding=embedding)
     cpu = cpu_series(ob.HEIS, 3, 6, beta, 12, 100, 1500, J=-1.0)
-    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1))
-    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1))
+    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1), label=f"heisenberg {proposal}/{embedding} E/N")
+    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1), label=f"heisenberg {proposal}/{embedding} |m|")
     assert ok_e, ie
     assert ok_m, im
 
@@ -62,8 +64,8 @@ def test_phi4_vs_fp64_restatement(ctx):
     beta, lam, mass = 0.69, 4.5, 1.0  # config/Phi4.ini
     obs, _ = gpu_series(ctx, "phi4", 3, 6, beta, 64, 100, 400, J=-1.0, lam=lam, mass=mass)
     cpu = cpu_series(ob.PHI4, 3, 6, beta, 12, 100, 1500, J=-1.0, lam=lam, mass=mass)
-    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1))
-    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1))
+    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1), label="phi4 E/N")
+    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1), label="phi4 |m|")
     assert ok_e, ie
     assert ok_m, im
 
@@ -72,8 +74,8 @@ def test_potts3_2d_vs_restatement(ctx):
     beta = 0.9
     obs, _ = gpu_series(ctx, "potts", 2, 8, beta, 64, 100, 400, J=-1.0, q=3)
     cpu = cpu_series(ob.POTTS, 2, 8, beta, 12, 100, 1500, J=-1.0, Q=3)
-    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1))
-    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1))
+    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1), label="potts3 E/N")
+    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1), label="potts3 m")
     assert ok_e, ie
     assert ok_m, im
 
@@ -119,3 +121,28 @@ def test_full_size_c2_properties(ctx):
     st.set_beta(0.0)
     st.metropolis(1, seed=3, sweep0=100)
     assert np.array_equal(st.download(0), -s)  # every site is offered one flip per sweep
+
+
+@pytest.mark.parametrize("model", ["heisenberg", "phi4"])
+def test_fp64_observable_accumulation(ctx, model):
+    """GPU FP64 sums over FP32 spins equal an FP64 CPU recomputation from the
+    downloaded spins to round-off (relative error stated: < 1e-12)."""
+    from marqov_amd import gpu
+    L, beta = 16, 0.693
+    lat = gpu.RegularLattice(ctx, 3, L)
+    st = gpu.State(ctx, model, [lat, lat], J=-1.0, lam=4.5, mass=1.0)
+    st.set_beta(beta)
+    st.init(seed=5)
+    st.metropolis(10, seed=5)
+    obs = st.measure()
+    nb = exact.neighbour_table(3, L)
+    for r in range(2):
+        s = st.download(r).astype(np.float64)
+        N = L ** 3
+        e = -1.0 * (s[:, None, :] * s[nb]).sum() / (2 * N)
+        if model == "phi4":
+            n2 = (s * s).sum(1)
+            e += (1.0 / beta) * n2.sum() / N + (4.5 / beta) * ((n2 - 1) ** 2).sum() / N
+        m = np.linalg.norm(s.sum(0)) / N
+        assert abs(obs[r, 0] - e) <= 1e-12 * max(1.0, abs(e)), (obs[r, 0], e)
+        assert abs(obs[r, 1] - m) <= 1e-12, (obs[r, 1], m)
