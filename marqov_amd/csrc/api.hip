@@ -272,9 +272,23 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         if (lattices[r]->kind != kind0 || (kind0 == LAT_REGULAR && lattices[r]->dim != dim0))
             return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "all replicas of a state must use lattices of one kind and dim");
     }
-    const bool msc = params->spin_bits == 1;
+    const bool eabits = params->spin_bits == 1 && model == MQ_EA_ISING;
+    const bool msc = params->spin_bits == 1 && !eabits;
     if (params->spin_bits != 0 && params->spin_bits != 1 && params->spin_bits != 8)
         return mq_fail(ctx, MQ_ERR_ARG, "spin_bits must be 0, 1 or 8");
+    if (eabits) {
+        for (int r = 0; r < nrep; ++r)
+            if (lattices[r]->kind != LAT_REGULAR)
+                return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "bit-sliced EA batches need regular lattices");
+        for (int r = 0; r < nrep;) {
+            int e = r + 1;
+            while (e < nrep && lattices[e] == lattices[r]) ++e;
+            if ((e - r) % 32 != 0)
+                return mq_fail(ctx, MQ_ERR_ARG, "bit-sliced EA: each run of replicas on one lattice must be a multiple of 32");
+            r = e;
+        }
+        if (params->J == 0.0) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "bit-sliced EA needs J != 0");
+    }
     if (msc) {
         if (model != MQ_ISING) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "multi-spin coding is for the Ising model");
         for (int r = 0; r < nrep; ++r)
@@ -287,6 +301,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     st->p = *params;
     st->nrep = nrep;
     st->msc = msc;
+    st->eabits = eabits;
     st->lat.assign(lattices, lattices + nrep);
     st->site_bytes = is_vec(model) ? 16 : 1;
     auto rep_bytes = [&](const mq_lattice* l) {
@@ -306,7 +321,12 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         g.rep0 = r;
         g.nrep = e - r;
         g.off = off;
-        if (g.lat->kind == LAT_CSR) {
+        if (eabits) {
+            // one uint32 word per site for each group of 32 replicas
+            g.stride = (g.lat->N * 4 + 255) / 256 * 256;
+            g.pos_stride = 4;
+            off += g.stride * (g.nrep / 32);
+        } else if (g.lat->kind == LAT_CSR) {
             g.stride = st->site_bytes;
             g.pos_stride = (int64_t)st->site_bytes * g.nrep;
             off += (g.lat->N * g.pos_stride + 255) / 256 * 256;
@@ -316,7 +336,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
             off += g.stride * g.nrep;
         }
         for (int k = r; k < e; ++k) {
-            st->rep_off[k] = g.off + (k - r) * g.stride;
+            st->rep_off[k] = g.off + (eabits ? (int64_t)((k - r) / 32) : (int64_t)(k - r)) * g.stride;
             st->rep_pos_stride[k] = g.pos_stride;
         }
         st->groups.push_back(g);
@@ -328,6 +348,10 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     st->rep_sample.resize(nrep);
     std::iota(st->rep_sample.begin(), st->rep_sample.end(), 0);
     st->nsamples = nrep;
+    if (eabits) {  // one disorder sample per word of 32 replicas by default
+        for (int r = 0; r < nrep; ++r) st->rep_sample[r] = r / 32;
+        st->nsamples = nrep / 32;
+    }
     auto fail = [&](hipError_t e) {
         ctx->err = std::string("allocation failed: ") + hipGetErrorString(e);
         mq_state_destroy(st);
@@ -347,6 +371,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     if ((e = hipMemsetAsync(st->d_rep_bond_off, 0, sizeof(int64_t) * nrep, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
     if ((e = prepare_lds(st)) != hipSuccess) return fail(e);
+    if (eabits && (e = hipMalloc(&st->d_wtab, wtab_bytes() * (size_t)(nrep / 32))) != hipSuccess) return fail(e);
     *out = st;
     return MQ_OK;
 }
@@ -375,6 +400,7 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_ladder);
     dfree(st->d_dyn);
     if (st->d_msc_cls) (void)hipFree(st->d_msc_cls);
+    if (st->d_wtab) (void)hipFree(st->d_wtab);
     if (st->d_lds_groups) (void)hipFree(st->d_lds_groups);
     delete st;
 }
@@ -407,6 +433,7 @@ static int upload_tables(mq_state* st)
     hipStream_t s = st->ctx->stream;
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_thr, thr.data(), thr.size() * 4, hipMemcpyHostToDevice, s));
     if (st->msc) MQ_HIP(st->ctx, msc_upload_classes(st, thr));
+    if (st->eabits) MQ_HIP(st->ctx, launch_wtab_eabits(st));
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_wthr, wthr.data(), wthr.size() * 4, hipMemcpyHostToDevice, s));
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_betaf, bf.data(), bf.size() * 4, hipMemcpyHostToDevice, s));
     MQ_HIP(st->ctx, hipMemcpyAsync(st->d_betad, st->beta.data(), st->beta.size() * 8, hipMemcpyHostToDevice, s));
@@ -449,6 +476,10 @@ extern "C" int mq_state_random_bonds(mq_state* st, const int32_t* rep_sample, do
             if (l && l != st->lat[r]) return mq_fail(st->ctx, MQ_ERR_ARG, "replicas of one sample must share a lattice");
             l = st->lat[r];
         }
+        if (st->eabits)
+            for (int r = 0; r < st->nrep; ++r)
+                if (rep_sample[r] != rep_sample[r - r % 32])
+                    return mq_fail(st->ctx, MQ_ERR_ARG, "bit-sliced EA: the 32 replicas of a word must share a sample");
         dfree(st->d_bonds);
         st->rep_sample.assign(rep_sample, rep_sample + st->nrep);
         st->nsamples = ns;
@@ -511,7 +542,8 @@ extern "C" int mq_state_init(mq_state* st, uint64_t seed)
 {
     if (!st) return MQ_ERR_ARG;
     for (const auto& g : st->groups) {
-        if (st->msc) MQ_HIP(st->ctx, launch_init_msc(st, g, key_of(seed)));
+        if (st->eabits) MQ_HIP(st->ctx, launch_init_eabits(st, g, key_of(seed)));
+        else if (st->msc) MQ_HIP(st->ctx, launch_init_msc(st, g, key_of(seed)));
         else if (g.lat->kind == LAT_REGULAR) MQ_HIP(st->ctx, launch_init_regular(st, g, key_of(seed)));
         else MQ_HIP(st->ctx, launch_init_csr(st, g, key_of(seed)));
     }
@@ -569,6 +601,19 @@ extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t 
         }
         int rc = copy_replica(st, rep, dev.data(), 16, true);
         if (rc) return rc;
+    } else if (st->eabits) {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected one int8 per site");
+        const int8_t* h = static_cast<const int8_t*>(host);
+        std::vector<uint32_t> dev((size_t)N);
+        MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + st->rep_off[rep], (size_t)N * 4, hipMemcpyDeviceToHost));
+        const uint32_t bit = 1u << (rep % 32);
+        for (int64_t i = 0; i < N; ++i) {
+            if (h[i] != 1 && h[i] != -1) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: spin value out of range");
+            uint32_t& w = dev[(size_t)dev_pos(l, i)];
+            w = (h[i] < 0) ? (w | bit) : (w & ~bit);
+        }
+        MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), (size_t)N * 4, hipMemcpyHostToDevice));
     } else if (st->msc) {
         if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected one int8 per site");
         const int8_t* h = static_cast<const int8_t*>(host);
@@ -613,6 +658,13 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
             h[3 * i + 1] = dev[p * 4 + 1];
             h[3 * i + 2] = dev[p * 4 + 2];
         }
+    } else if (st->eabits) {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
+        std::vector<uint32_t> dev((size_t)N);
+        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + st->rep_off[rep], (size_t)N * 4, hipMemcpyDeviceToHost));
+        int8_t* h = static_cast<int8_t*>(host);
+        const int b = rep % 32;
+        for (int64_t i = 0; i < N; ++i) h[i] = ((dev[(size_t)dev_pos(l, i)] >> b) & 1u) ? (int8_t)-1 : (int8_t)1;
     } else if (st->msc) {
         if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
         std::vector<uint32_t> dev((size_t)(N / 32));
@@ -637,7 +689,7 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
 int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0)
 {
     const Key2 key = key_of(seed);
-    if (nsweeps > 0 && st->kernel_choice == 0 && !st->msc && sweep0 + (uint64_t)nsweeps < (1ull << 31)) {
+    if (nsweeps > 0 && st->kernel_choice == 0 && !st->msc && !st->eabits && sweep0 + (uint64_t)nsweeps < (1ull << 31)) {
         hipError_t e = launch_sweeps_lds(st, nsweeps, (uint32_t)sweep0, key);
         if (e == hipSuccess) return MQ_OK;
         if (e != hipErrorNotSupported) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("lds sweep: ") + hipGetErrorString(e));
@@ -649,7 +701,8 @@ int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep
             const int nc = g.lat->ncolors;
             for (int c = 0; c < nc; ++c) {
                 const uint32_t phase = (uint32_t)(sw * (uint64_t)(g.lat->kind == LAT_REGULAR ? 2 : 64) + c);
-                hipError_t e = st->msc ? launch_sweep_msc(st, g, c, phase, key)
+                hipError_t e = st->eabits ? launch_sweep_eabits(st, g, c, phase, key)
+                               : st->msc ? launch_sweep_msc(st, g, c, phase, key)
                                : (g.lat->kind == LAT_REGULAR) ? launch_sweep_regular(st, g, c, phase, key)
                                                               : launch_sweep_csr(st, g, c, phase, key);
                 if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
@@ -721,7 +774,8 @@ static int measure_sums(mq_state* st, std::vector<long long>& is, std::vector<do
     MQ_HIP(st->ctx, zero64(st->d_isum, (int64_t)st->nrep * 8, s));
     MQ_HIP(st->ctx, zero64(st->d_fsum, (int64_t)st->nrep * 8, s));
     for (const auto& g : st->groups) {
-        hipError_t e = st->msc ? launch_measure_msc(st, g)
+        hipError_t e = st->eabits ? launch_measure_eabits(st, g)
+                       : st->msc ? launch_measure_msc(st, g)
                        : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
         if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
     }

@@ -1,0 +1,326 @@
+// Bit-sliced +-J Edwards-Anderson batches: 32 replicas of one disorder sample
+// (typically 32 temperatures) share one uint32 per site.
+//
+// Layout: replicas come in word groups of 32 consecutive replicas on one lattice
+// and one sample; word group w stores, per colour and half index h (the int8
+// layout's order), one uint32 whose bit j is the spin of replica 32w + j (set =
+// down).  The sample's couplings ([colour][d][h] int8) are read once per 32
+// replicas.
+//
+// Sweep (one lane per site word): neighbour words n_k are folded with their bond
+// sign (n_k ^ ~0 for b = -1), the anti-aligned count c = sum_k (s ^ n_k) is summed
+// bit-sliced into three planes, and each bit j is accepted if its class is
+// "always" (dE <= 0) or it wins the bit-serial draw U_j < T_j[c] (MSB first):
+// level k uses one random word (bit j = bit 31-k of U_j) and the per-bit
+// threshold word W[k][c] whose bit j is bit 31-k of replica (32w+j)'s 32-bit
+// threshold for class c.  W is rebuilt from the per-replica tables whenever
+// temperatures move (set_beta, parallel-tempering swaps).
+//   level k word = Philox(h, 2*sweep + colour, w, 8 + k/4)[k % 4]
+#include "mq_internal.hpp"
+
+namespace {
+
+constexpr uint32_t ST_BITS = 8;
+
+struct WTab {
+    uint32_t W[32][3];  // per level, per Bernoulli class: bit j = threshold bit of replica j
+    uint32_t A[3];      // per class: bit j set when replica j accepts without a draw
+    uint32_t pad;
+};
+
+// planes (b2 b1 b0) -> mask of words whose count equals V
+template <int V>
+__device__ __forceinline__ uint32_t cls_mask(uint32_t b0, uint32_t b1, uint32_t b2)
+{
+    return ((V & 4) ? b2 : ~b2) & ((V & 2) ? b1 : ~b1) & ((V & 1) ? b0 : ~b0);
+}
+
+template <int DIM>
+__device__ __forceinline__ void nbrs_h(int L, int colour, int64_t h, int64_t nb[2 * DIM])
+{
+    const int H = L >> 1;
+    const int64_t plane = (int64_t)H * L;
+    const int64_t row = h / H;
+    const int k = (int)(h - row * H);
+    const int x1 = (int)(row % L);
+    const int x2 = (DIM == 3) ? (int)(row / L) : 0;
+    const int p = (colour + x1 + x2) & 1;
+    const int64_t rb = h - k;
+    nb[0] = rb + (p ? ((k + 1 == H) ? 0 : k + 1) : k);
+    nb[1] = rb + (p ? k : ((k == 0) ? H - 1 : k - 1));
+    nb[2] = (x1 == L - 1) ? h - (int64_t)(L - 1) * H : h + H;
+    nb[3] = (x1 == 0) ? h + (int64_t)(L - 1) * H : h - H;
+    if (DIM == 3) {
+        nb[4] = (x2 == L - 1) ? h - (int64_t)(L - 1) * plane : h + plane;
+        nb[5] = (x2 == 0) ? h + (int64_t)(L - 1) * plane : h - plane;
+    }
+}
+
+// bit-sliced sum of 2*DIM one-bit inputs a[k] = s ^ n[k] -> planes (b0, b1, b2)
+template <int DIM>
+__device__ __forceinline__ void anti_count(uint32_t s, const uint32_t (&n)[2 * DIM], uint32_t& b0, uint32_t& b1,
+                                           uint32_t& b2)
+{
+    // full adders on the raw neighbour words, then fold s (maj(a^s,b^s,c^s) = maj(a,b,c)^s)
+    const uint32_t s1 = n[0] ^ n[1] ^ n[2] ^ s;
+    const uint32_t c1 = ((n[0] & n[1]) | (n[0] & n[2]) | (n[1] & n[2])) ^ s;
+    if (DIM == 3) {
+        const uint32_t s2 = n[3] ^ n[4] ^ n[5] ^ s;
+        const uint32_t c2 = ((n[3] & n[4]) | (n[3] & n[5]) | (n[4] & n[5])) ^ s;
+        b0 = s1 ^ s2;
+        const uint32_t k0 = s1 & s2;
+        b1 = c1 ^ c2 ^ k0;
+        b2 = (c1 & c2) | (c1 & k0) | (c2 & k0);
+    } else {
+        const uint32_t a3 = n[3] ^ s;  // 4 inputs: (s1 + 2 c1) + a3
+        b0 = s1 ^ a3;
+        const uint32_t k0 = s1 & a3;
+        b1 = c1 ^ k0;
+        b2 = c1 & k0;
+    }
+}
+
+template <int DIM, bool FERRO>
+__global__ void __launch_bounds__(256) k_sweep_eabits(uint32_t* __restrict__ base, int64_t wstride, int L,
+                                                      const int8_t* __restrict__ bonds,
+                                                      const int64_t* __restrict__ rep_bond_off,
+                                                      const WTab* __restrict__ wtab, int gw0, int bpw, int colour,
+                                                      uint32_t phase, const uint32_t* __restrict__ dyn, Key2 key)
+{
+    __shared__ WTab T;
+    if (dyn) phase += dyn[0];
+    const int wl = blockIdx.x / bpw;  // word group within this lattice group
+    const int gw = gw0 + wl;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(wtab + gw);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&T);
+        for (int i = threadIdx.x; i < (int)(sizeof(WTab) / 4); i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    int64_t N = (int64_t)L * L;
+    if (DIM == 3) N *= L;
+    const int64_t half = N >> 1;
+    const int64_t h = (int64_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
+    if (h >= half) return;
+    uint32_t* own = base + (int64_t)wl * wstride + colour * half;
+    const uint32_t* oth = base + (int64_t)wl * wstride + (1 - colour) * half;
+    const int8_t* bs = bonds + rep_bond_off[32 * gw];
+    const int8_t* bown = bs + (int64_t)colour * DIM * half;
+    const int8_t* both = bs + (int64_t)(1 - colour) * DIM * half;
+    int64_t nb[2 * DIM];
+    nbrs_h<DIM>(L, colour, h, nb);
+    uint32_t n[2 * DIM];
+#pragma unroll
+    for (int sl = 0; sl < 2 * DIM; ++sl) {
+        const int d = sl >> 1;
+        const int b = (sl & 1) ? both[(int64_t)d * half + nb[sl]] : bown[(int64_t)d * half + h];
+        n[sl] = oth[nb[sl]] ^ (b < 0 ? 0xFFFFFFFFu : 0u);
+    }
+    const uint32_t S = own[h];
+    uint32_t b0, b1, b2;
+    anti_count<DIM>(S, n, b0, b1, b2);
+    // Bernoulli classes: ferro c in [0, DIM), antiferro c in (DIM, 2 DIM]
+    constexpr int C0 = FERRO ? 0 : DIM + 1;
+    const uint32_t m0 = cls_mask<C0>(b0, b1, b2);
+    const uint32_t m1 = cls_mask<C0 + 1>(b0, b1, b2);
+    const uint32_t m2 = (DIM == 3) ? cls_mask<C0 + 2>(b0, b1, b2) : 0u;
+    // classes with dE <= 0: everything that is not a Bernoulli class
+    uint32_t acc = ~(m0 | m1 | m2);
+    acc |= (m0 & T.A[0]) | (m1 & T.A[1]) | (m2 & T.A[2]);
+    uint32_t D = (m0 & ~T.A[0]) | (m1 & ~T.A[1]) | (m2 & ~T.A[2]);
+    uint4 w = make_uint4(0, 0, 0, 0);
+    for (int k = 0; k < 32 && D != 0u; ++k) {
+        if ((k & 3) == 0) w = philox4x32_10((uint32_t)h, phase, (uint32_t)gw, ST_BITS + (uint32_t)(k >> 2), key);
+        const uint32_t R = u4_get(w, k & 3);
+        const uint32_t t = (T.W[k][0] & m0) | (T.W[k][1] & m1) | (T.W[k][2] & m2);
+        acc |= D & ~R & t;
+        D &= ~(R ^ t);
+    }
+    own[h] = S ^ acc;
+}
+
+// W tables from the per-replica acceptance tables (entry e = q + 2*DIM, q = 2*DIM - 2c)
+__global__ void k_eabits_wtab(const uint32_t* __restrict__ thr, int tbl, int dim, int ferro, int nw,
+                              WTab* __restrict__ wtab)
+{
+    const int gw = blockIdx.x;
+    if (gw >= nw) return;
+    const int t = threadIdx.x;  // 0..98: (level, class) or the 3 "always" words
+    const int z = 2 * dim;
+    if (t < 96) {
+        const int k = t / 3, j3 = t % 3;
+        if (dim == 2 && j3 == 2) {
+            wtab[gw].W[k][j3] = 0u;
+            return;
+        }
+        const int c = ferro ? j3 : dim + 1 + j3;
+        const int e = 2 * z - 2 * c;
+        uint32_t word = 0u;
+        for (int j = 0; j < 32; ++j) {
+            const uint32_t hi = thr[((int64_t)(32 * gw + j) * tbl + e) * 2];
+            const uint32_t lo = thr[((int64_t)(32 * gw + j) * tbl + e) * 2 + 1];
+            const uint32_t T32 = hi > 0xFFFFu ? 0xFFFFFFFFu : ((hi << 16) | lo);
+            word |= ((T32 >> (31 - k)) & 1u) << j;
+        }
+        wtab[gw].W[k][j3] = word;
+    } else if (t < 99) {
+        const int j3 = t - 96;
+        if (dim == 2 && j3 == 2) {
+            wtab[gw].A[j3] = 0u;
+            return;
+        }
+        const int c = ferro ? j3 : dim + 1 + j3;
+        const int e = 2 * z - 2 * c;
+        uint32_t word = 0u;
+        for (int j = 0; j < 32; ++j)
+            if (thr[((int64_t)(32 * gw + j) * tbl + e) * 2] > 0xFFFFu) word |= 1u << j;
+        wtab[gw].A[j3] = word;
+    }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(256) k_init_eabits(uint32_t* __restrict__ base, int64_t wstride, int L, int gw0,
+                                                     Key2 key)
+{
+    const int wl = blockIdx.y;
+    int64_t N = (int64_t)L * L;
+    if (DIM == 3) N *= L;
+    const int64_t half = N >> 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int64_t x0 = i % L, rr = i / L;
+    int cs = (int)x0 + (int)(rr % L);
+    if (DIM == 3) cs += (int)(rr / L);
+    const int colour = cs & 1;
+    uint32_t word = 0u;
+    for (int j = 0; j < 32; ++j) {
+        const uint4 q = philox4x32_10((uint32_t)(i >> 2), 0u, (uint32_t)(32 * (gw0 + wl) + j), ST_INIT, key);
+        if (!(u4_get(q, (int)(i & 3)) >> 31)) word |= 1u << j;
+    }
+    base[(int64_t)wl * wstride + colour * half + (i >> 1)] = word;
+}
+
+// per replica: isum[r*8+0] = sum over bonds b s_i s_j, isum[r*8+1] = sum s
+template <int DIM>
+__global__ void __launch_bounds__(256) k_measure_eabits(const uint32_t* __restrict__ base, int64_t wstride, int L,
+                                                        const int8_t* __restrict__ bonds,
+                                                        const int64_t* __restrict__ rep_bond_off, int gw0, int bpw,
+                                                        unsigned long long* __restrict__ isum)
+{
+    __shared__ int s_anti[32], s_down[32];
+    const int wl = blockIdx.x / bpw;
+    const int gw = gw0 + wl;
+    if (threadIdx.x < 32) {
+        s_anti[threadIdx.x] = 0;
+        s_down[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    int64_t N = (int64_t)L * L;
+    if (DIM == 3) N *= L;
+    const int64_t half = N >> 1;
+    const int64_t h = (int64_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
+    if (h < half) {
+        const uint32_t* c0 = base + (int64_t)wl * wstride;
+        const uint32_t* c1 = c0 + half;
+        const int8_t* bs = bonds + rep_bond_off[32 * gw];
+        int64_t nb[2 * DIM];
+        nbrs_h<DIM>(L, 0, h, nb);
+        const uint32_t S = c0[h];
+        int anti[32];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) anti[j] = 0;
+#pragma unroll
+        for (int sl = 0; sl < 2 * DIM; ++sl) {
+            const int d = sl >> 1;
+            const int b = (sl & 1) ? bs[(int64_t)DIM * half + (int64_t)d * half + nb[sl]] : bs[(int64_t)d * half + h];
+            const uint32_t a = S ^ c1[nb[sl]] ^ (b < 0 ? 0xFFFFFFFFu : 0u);
+#pragma unroll
+            for (int j = 0; j < 32; ++j) anti[j] += (a >> j) & 1u;
+        }
+        const uint32_t T2 = c1[h];
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            if (anti[j]) atomicAdd(&s_anti[j], anti[j]);
+            const int dn = (int)((S >> j) & 1u) + (int)((T2 >> j) & 1u);
+            if (dn) atomicAdd(&s_down[j], dn);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int64_t sites0 = ((int64_t)(blockIdx.x - wl * bpw) * blockDim.x);
+        int64_t n0 = half - sites0;
+        if (n0 > (int64_t)blockDim.x) n0 = blockDim.x;
+        // colour-0 sites of this block: n0, each with 2*DIM bonds; sites counted for M: 2*n0
+        const long long bsum = (long long)n0 * 2 * DIM - 2ll * s_anti[threadIdx.x];
+        const long long msum = 2ll * n0 - 2ll * s_down[threadIdx.x];
+        const int64_t r = 32ll * gw + threadIdx.x;
+        atomicAdd(isum + r * 8 + 0, (unsigned long long)bsum);
+        atomicAdd(isum + r * 8 + 1, (unsigned long long)msum);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_wtab_eabits(mq_state* st)
+{
+    const int nw = st->nrep / 32;
+    hipLaunchKernelGGL(k_eabits_wtab, dim3(nw), dim3(128), 0, st->ctx->stream, st->d_thr, st->tbl, st->lat[0]->dim,
+                       st->p.J < 0.0 ? 1 : 0, nw, reinterpret_cast<WTab*>(st->d_wtab));
+    return hipGetLastError();
+}
+
+size_t wtab_bytes() { return sizeof(WTab); }
+
+hipError_t launch_init_eabits(mq_state* st, const Group& g, Key2 key)
+{
+    const int L = g.lat->L;
+    const int nwl = g.nrep / 32;
+    uint32_t* b = reinterpret_cast<uint32_t*>(st->d_spins + g.off);
+    const dim3 grid((unsigned)((g.lat->N + 255) / 256), (unsigned)nwl);
+    if (g.lat->dim == 3)
+        hipLaunchKernelGGL(k_init_eabits<3>, grid, dim3(256), 0, st->ctx->stream, b, g.stride / 4, L, g.rep0 / 32, key);
+    else
+        hipLaunchKernelGGL(k_init_eabits<2>, grid, dim3(256), 0, st->ctx->stream, b, g.stride / 4, L, g.rep0 / 32, key);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_eabits(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key)
+{
+    const int L = g.lat->L;
+    const int nwl = g.nrep / 32;
+    const int64_t half = g.lat->N / 2;
+    const int bpw = (int)((half + 255) / 256);
+    uint32_t* b = reinterpret_cast<uint32_t*>(st->d_spins + g.off);
+    const WTab* wt = reinterpret_cast<const WTab*>(st->d_wtab);
+    const dim3 grid((unsigned)(bpw * nwl));
+    const bool ferro = st->p.J < 0.0;
+#define MQ_E(D, F)                                                                                              \
+    hipLaunchKernelGGL((k_sweep_eabits<D, F>), grid, dim3(256), 0, st->ctx->stream, b, g.stride / 4, L,          \
+                       st->d_bonds, st->d_rep_bond_off, wt, g.rep0 / 32, bpw, colour, phase, st->cur_dyn, key)
+    if (g.lat->dim == 3) {
+        if (ferro) MQ_E(3, true);
+        else MQ_E(3, false);
+    } else {
+        if (ferro) MQ_E(2, true);
+        else MQ_E(2, false);
+    }
+#undef MQ_E
+    return hipGetLastError();
+}
+
+hipError_t launch_measure_eabits(mq_state* st, const Group& g)
+{
+    const int L = g.lat->L;
+    const int nwl = g.nrep / 32;
+    const int64_t half = g.lat->N / 2;
+    const int bpw = (int)((half + 255) / 256);
+    const uint32_t* b = reinterpret_cast<const uint32_t*>(st->d_spins + g.off);
+    unsigned long long* is = reinterpret_cast<unsigned long long*>(st->d_isum);
+    if (g.lat->dim == 3)
+        hipLaunchKernelGGL(k_measure_eabits<3>, dim3((unsigned)(bpw * nwl)), dim3(256), 0, st->ctx->stream, b,
+                           g.stride / 4, L, st->d_bonds, st->d_rep_bond_off, g.rep0 / 32, bpw, is);
+    else
+        hipLaunchKernelGGL(k_measure_eabits<2>, dim3((unsigned)(bpw * nwl)), dim3(256), 0, st->ctx->stream, b,
+                           g.stride / 4, L, st->d_bonds, st->d_rep_bond_off, g.rep0 / 32, bpw, is);
+    return hipGetLastError();
+}

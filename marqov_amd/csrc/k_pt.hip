@@ -131,7 +131,8 @@ hipError_t energy_async(mq_state* st)
     if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     if ((e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     for (const auto& g : st->groups) {
-        e = st->msc ? launch_measure_msc(st, g)
+        e = st->eabits ? launch_measure_eabits(st, g)
+            : st->msc ? launch_measure_msc(st, g)
             : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
         if (e != hipSuccess) return e;
     }
@@ -175,7 +176,9 @@ hipError_t pt_swap_async(mq_state* st, int ntemp, int parity, uint32_t step, Key
     hipLaunchKernelGGL(k_pt_swap, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st->ctx->stream, ngroups, ntemp,
                        parity, step, st->cur_dyn, key, st->d_rep_of_temp, st->d_temp_of_rep, st->d_energy, st->d_ladder,
                        st->d_thr, st->tbl, st->d_wthr, st->d_betaf, st->d_betad, st->d_pt_acc);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && st->eabits) e = launch_wtab_eabits(st);  // temperatures moved between bits
+    return e;
 }
 
 }  // namespace

@@ -152,6 +152,8 @@ struct mq_state {
     int lds_ngroups = 0, lds_blocks = 0;
     size_t lds_need[2] = {0, 0};        // LDS bytes per block without / with couplings
     bool msc = false;                   // 1-bit multi-spin-coded Ising storage
+    bool eabits = false;                // EA: 32 replicas of one sample per uint32 word
+    void* d_wtab = nullptr;             // per word group: per-bit threshold words
     void* d_msc_cls = nullptr;          // per-replica Bernoulli classes of the MSC sweep
     int tune_block = 256;               // threads per block of the streaming sweep kernels
     int tune_xcd = 1;                   // XCD-aware block remap of the row-vectorised kernel
@@ -178,6 +180,11 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
 hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key);
 hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key);
 hipError_t prepare_lds(mq_state* st);
+hipError_t launch_wtab_eabits(mq_state* st);
+size_t wtab_bytes();
+hipError_t launch_init_eabits(mq_state* st, const Group& g, Key2 key);
+hipError_t launch_sweep_eabits(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
+hipError_t launch_measure_eabits(mq_state* st, const Group& g);
 bool msc_supported(const mq_lattice* lat);
 hipError_t msc_upload_classes(mq_state* st, const std::vector<uint32_t>& thr);
 hipError_t launch_sweep_msc(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);

@@ -505,3 +505,52 @@ EXPORT void orc_ising_msc_sweeps(int L, int8_t* s, const uint32_t* thi, const ui
             }
         }
 }
+
+/* -------------------------------------------- bit-sliced EA replica words */
+/* Replays the 32-replica word sweep: replica j of word group gw sits in bit j.
+ * Site h of colour c, replica j: q = s_j(i) sum_k b_k s_j(nbr_k); an "always"
+ * entry flips, otherwise U_j < T_j with U_j's bit 31-k = bit j of
+ * Philox(h, 2*sweep + colour, gw, 8 + k/4)[k % 4].  spins: 32 replicas x N,
+ * replica-major, natural order; thi/tlo: 32 tables of 4*dim+1 entries. */
+EXPORT void orc_ea_bits_sweeps(int dim, int L, int8_t* spins, const int8_t* bonds, const uint32_t* thi,
+                               const uint32_t* tlo, int nsweeps, uint64_t seed, uint32_t sweep0, uint32_t gw)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    const int z = 2 * dim, tbl = 2 * z + 1;
+    for (int t = 0; t < nsweeps; ++t)
+        for (int c = 0; c < 2; ++c) {
+            uint32_t phase = 2u * (sweep0 + (uint32_t)t) + (uint32_t)c;
+            for (int64_t i = 0; i < N; ++i) {
+                if (colour_of(dim, L, i) != c) continue;
+                uint32_t words[32];
+                int have = 0; /* number of 4-level blocks computed */
+                for (int j = 0; j < 32; ++j) {
+                    int8_t* s = spins + (int64_t)j * N;
+                    int hs = 0;
+                    for (int slot = 0; slot < z; ++slot)
+                        hs += bond_of(bonds, dim, L, i, slot) * s[orc_neighbour(dim, L, i, slot)];
+                    int e = s[i] * hs + z;
+                    uint32_t hi = thi[j * tbl + e], lo = tlo[j * tbl + e];
+                    int flip;
+                    if (hi > 0xFFFFu) {
+                        flip = 1;
+                    } else {
+                        uint32_t T = (hi << 16) | lo;
+                        flip = 0;
+                        for (int k = 0; k < 32; ++k) {
+                            while (have <= (k >> 2)) {
+                                uint32_t w[4];
+                                orc_philox_u64key((uint32_t)(i >> 1), phase, gw, 8u + (uint32_t)have, seed, w);
+                                for (int m = 0; m < 4; ++m) words[4 * have + m] = w[m];
+                                ++have;
+                            }
+                            uint32_t rb = (words[k] >> j) & 1u, tb = (T >> (31 - k)) & 1u;
+                            if (rb != tb) { flip = (rb == 0u); break; }
+                        }
+                    }
+                    if (flip) s[i] = (int8_t)-s[i];
+                }
+            }
+        }
+}

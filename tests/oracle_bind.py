@@ -37,6 +37,7 @@ _SIG = {
     "orc_csr_sweeps": (None, [_I64, _P, _P, _P, _I, _P, _P, _P, _P, _I, _I, _I, _I, _U64, _U32, _U32]),
     "orc_csr_wolff": (_I, [_I64, _P, _P, _P, _I, _I, _U32, _I, _U64, _U32, _U32, _P]),
     "orc_ising_msc_sweeps": (None, [_I, _P, _P, _P, _I, _U64, _U32, _U32]),
+    "orc_ea_bits_sweeps": (None, [_I, _I, _P, _P, _P, _P, _I, _U64, _U32, _U32]),
     "ref_run_regular": (_I, [_I, _I, _I, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_run_csr": (_I, [_I, _I64, _P, _P, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_mt19937_64_nth": (_U64, [_U64, _I64]),
@@ -201,4 +202,20 @@ def ising_msc_sweeps(L, s, beta, J, nsweeps, seed, sweep0, rep):
     hi, lo = thresholds_ising(beta, J, 6)
     s = np.ascontiguousarray(s, np.int8).copy()
     lib().orc_ising_msc_sweeps(L, p(s), p(hi), p(lo), nsweeps, seed, sweep0, rep)
+    return s
+
+
+def ea_bits_sweeps(dim, L, spins32, bonds, betas32, J, nsweeps, seed, sweep0, gw):
+    """spins32: (32, N) int8; betas32: 32 inverse temperatures (one per bit)."""
+    z = 2 * dim
+    his, los = [], []
+    for b in betas32:
+        hi, lo = thresholds_ising(b, J, z)
+        his.append(hi)
+        los.append(lo)
+    thi = np.ascontiguousarray(np.concatenate(his), np.uint32)
+    tlo = np.ascontiguousarray(np.concatenate(los), np.uint32)
+    s = np.ascontiguousarray(spins32, np.int8).copy()
+    lib().orc_ea_bits_sweeps(dim, L, p(s), p(np.ascontiguousarray(bonds, np.int8)), p(thi), p(tlo), nsweeps, seed,
+                             sweep0, gw)
     return s

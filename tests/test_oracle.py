@@ -211,3 +211,35 @@ def test_replay_acceptance_per_class(kind, J, beta):
         p = min(1.0, np.exp(-beta * (-2.0 * J * qv)))
         f = flipped[sel].mean()
         assert abs(f - p) <= 4 * np.sqrt(max(p * (1 - p), 1e-12) / n) + 1e-12, (kind, qv, f, p, n)
+
+
+def test_ea_bits_replay_acceptance_per_class():
+    """The 32-replica word rule: per (replica temperature, class) the flip fraction of
+    colour-0 sites in one sweep matches min(1, exp(-beta dE))."""
+    dim, L = 3, 16
+    N = L ** dim
+    betas = 0.5 * (3.3 / 0.5) ** (np.arange(32) / 31)
+    bonds = ob.init_bonds(dim, N, 0.5, 4, 0)
+    s0 = np.stack([ob.init_ising(N, 3, j) for j in range(32)])
+    s1 = ob.ea_bits_sweeps(dim, L, s0, bonds, betas, -1.0, 1, 77, 0, 0)
+    nb = exact.neighbour_table(dim, L)
+    B = exact.bond_table(dim, L, bonds)
+    idx = np.arange(N)
+    x = np.stack([(idx // L ** d) % L for d in range(3)], 1)
+    c0 = (x.sum(1) % 2) == 0
+    tot = {}
+    for j in (0, 10, 31):
+        q = s0[j].astype(np.int64) * (B * s0[j][nb]).sum(1)
+        fl = s1[j] != s0[j]
+        for qv in (2, 4, 6):
+            sel = c0 & (q == qv)
+            n = int(sel.sum())
+            if n < 100:
+                continue
+            p = np.exp(-betas[j] * 2.0 * qv)
+            f = fl[sel].mean()
+            assert abs(f - p) <= 4 * np.sqrt(max(p * (1 - p), 1e-12) / n) + 1e-12, (j, qv, f, p, n)
+            tot[(j, qv)] = n
+        sel = c0 & (q <= 0)
+        assert fl[sel].all()
+    assert len(tot) >= 5
