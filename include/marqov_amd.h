@@ -115,6 +115,9 @@ MQ_API int mq_state_get_beta(const mq_state* st, double* beta);
  * (BimodalPDF, src/lattice/util/prob_density.h:25-35; symmetric by construction
  * as RegularRandomBond makes them, src/lattice/regular_random_bond.h:38-54). */
 MQ_API int mq_state_random_bonds(mq_state* st, const int32_t* rep_sample, double p_plus, uint64_t seed);
+/* replica -> disorder sample map without drawing couplings (then set each
+ * sample with mq_state_set_bonds); replicas of one sample must share a lattice */
+MQ_API int mq_state_set_samples(mq_state* st, const int32_t* rep_sample);
 /* EA couplings from the host for one sample: regular lattices bonds[i*dim+d]
  * couples i and i+e_d; CSR lattices one value per CSR slot (must be symmetric). */
 MQ_API int mq_state_set_bonds(mq_state* st, int sample, const int8_t* bonds);

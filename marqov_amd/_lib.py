@@ -61,6 +61,7 @@ SIGNATURES = {
     "mq_state_get_beta": (_I, [_P, _P]),
     "mq_state_random_bonds": (_I, [_P, _P, _D, _U64]),
     "mq_state_set_bonds": (_I, [_P, _I, _P]),
+    "mq_state_set_samples": (_I, [_P, _P]),
     "mq_state_init": (_I, [_P, _U64]),
     "mq_state_upload": (_I, [_P, _I, _P, _I64]),
     "mq_state_download": (_I, [_P, _I, _P, _I64]),

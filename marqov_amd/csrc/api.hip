@@ -460,29 +460,44 @@ extern "C" int mq_state_get_beta(const mq_state* st, double* beta)
     return MQ_OK;
 }
 
+static int set_samples(mq_state* st, const int32_t* rep_sample)
+{
+    int ns = 0;
+    for (int r = 0; r < st->nrep; ++r) {
+        if (rep_sample[r] < 0) return mq_fail(st->ctx, MQ_ERR_ARG, "negative sample id");
+        ns = std::max(ns, rep_sample[r] + 1);
+    }
+    std::vector<mq_lattice*> lat_of(ns, nullptr);
+    for (int r = 0; r < st->nrep; ++r) {
+        auto*& l = lat_of[rep_sample[r]];
+        if (l && l != st->lat[r]) return mq_fail(st->ctx, MQ_ERR_ARG, "replicas of one sample must share a lattice");
+        l = st->lat[r];
+    }
+    if (st->eabits)
+        for (int r = 0; r < st->nrep; ++r)
+            if (rep_sample[r] != rep_sample[r - r % 32])
+                return mq_fail(st->ctx, MQ_ERR_ARG, "bit-sliced EA: the 32 replicas of a word must share a sample");
+    dfree(st->d_bonds);
+    st->rep_sample.assign(rep_sample, rep_sample + st->nrep);
+    st->nsamples = ns;
+    return MQ_OK;
+}
+
+extern "C" int mq_state_set_samples(mq_state* st, const int32_t* rep_sample)
+{
+    if (!st || !rep_sample) return MQ_ERR_ARG;
+    int rc = set_samples(st, rep_sample);
+    if (rc) return rc;
+    return ensure_bond_storage(st);
+}
+
 extern "C" int mq_state_random_bonds(mq_state* st, const int32_t* rep_sample, double p_plus, uint64_t seed)
 {
     if (!st) return MQ_ERR_ARG;
     if (!(p_plus >= 0.0 && p_plus <= 1.0)) return mq_fail(st->ctx, MQ_ERR_ARG, "p_plus must be in [0,1]");
     if (rep_sample) {
-        int ns = 0;
-        for (int r = 0; r < st->nrep; ++r) {
-            if (rep_sample[r] < 0) return mq_fail(st->ctx, MQ_ERR_ARG, "negative sample id");
-            ns = std::max(ns, rep_sample[r] + 1);
-        }
-        std::vector<mq_lattice*> lat_of(ns, nullptr);
-        for (int r = 0; r < st->nrep; ++r) {
-            auto*& l = lat_of[rep_sample[r]];
-            if (l && l != st->lat[r]) return mq_fail(st->ctx, MQ_ERR_ARG, "replicas of one sample must share a lattice");
-            l = st->lat[r];
-        }
-        if (st->eabits)
-            for (int r = 0; r < st->nrep; ++r)
-                if (rep_sample[r] != rep_sample[r - r % 32])
-                    return mq_fail(st->ctx, MQ_ERR_ARG, "bit-sliced EA: the 32 replicas of a word must share a sample");
-        dfree(st->d_bonds);
-        st->rep_sample.assign(rep_sample, rep_sample + st->nrep);
-        st->nsamples = ns;
+        int rc0 = set_samples(st, rep_sample);
+        if (rc0) return rc0;
     }
     int rc = ensure_bond_storage(st);
     if (rc) return rc;

@@ -11,10 +11,13 @@
 // keyed by original vertex ids and original CSR slots, so a CPU replay needs
 // only the colour of every vertex, not the renumbering.
 //
-//   Metropolis: counter (vertex, 64*sweep + colour, replica, METRO); Ising/EA
-//     compare word 0 with the 32-bit threshold of dE = -2 J q, q = s_i sum b_ij s_j;
-//     Potts proposes (w0 * Q) >> 32 and compares word 1; O(3)/Phi4 use words 0,1
-//     for the direction and word 2 for acceptance.
+//   Metropolis: Ising/EA draw one Philox block per vertex and 8 replicas:
+//     counter (vertex, 64*sweep + colour, replica >> 3, METRO), 16-bit uniform
+//     (replica & 7) against the high half of the 32-bit threshold of
+//     dE = -2 J q (q = s_i sum b_ij s_j), the low half from the TIE stream on a
+//     tie; Potts uses counter (vertex, phase, replica, METRO): word 0 proposes
+//     (w0 * Q) >> 32, word 1 is compared with the threshold; O(3)/Phi4 use words
+//     0,1 for the direction and word 2 for acceptance.
 //   Wolff bonds: the edge {u, v} is keyed by the original CSR slot of v in row u
 //     with u < v (its canonical slot), counter (slot >> 2, cluster, replica, WBOND).
 #include <algorithm>
@@ -77,10 +80,10 @@ __global__ void __launch_bounds__(512) k_csr_sweep_int8(int8_t* __restrict__ bas
     int8_t* s = base + r * rs;
     const int qmax = (tbl - 1) >> 1;
     const int a = rp[pos], b = rp[pos + 1];
-    const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
     const uint32_t* T = thr + (int64_t)grep * 2 * tbl;
     const int si = s[pos * ps];
     if (model == MQ_POTTS) {
+        const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
         const int snew = (int)(((uint64_t)w.x * (uint64_t)Q) >> 32);
         int dn = 0;
         for (int k = a; k < b; ++k) {
@@ -95,8 +98,15 @@ __global__ void __launch_bounds__(512) k_csr_sweep_int8(int8_t* __restrict__ bas
         const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
         for (int k = a; k < b; ++k) h += (BONDS ? (int)bb[k] : 1) * s[(int64_t)ci[k] * ps];
         const int e = si * h + qmax;
-        const uint64_t th = ((uint64_t)T[2 * e] << 16) | T[2 * e + 1];
-        if ((uint64_t)w.x < th) s[pos * ps] = (int8_t)(-si);
+        const uint4 w8 = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep >> 3, ST_METRO, key);
+        const int m = grep & 7;
+        const uint32_t hi16 = (u4_get(w8, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+        bool acc = hi16 < T[2 * e];
+        if (hi16 == T[2 * e]) {
+            const uint4 tw = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep >> 3, ST_TIE, key);
+            acc = ((u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu) < T[2 * e + 1];
+        }
+        if (acc) s[pos * ps] = (int8_t)(-si);
     }
 }
 
@@ -152,6 +162,88 @@ __global__ void __launch_bounds__(512) k_csr_sweep_o3(float4* __restrict__ base,
     bool acc = dE <= 0.f;
     if (!acc) acc = u01f(w.z) < __expf(-beta * dE);
     if (acc) s[pos * ps] = make_float4(sn.x, sn.y, sn.z, 0.f);
+}
+
+
+// Ising / EA on a graph, 8 replicas per lane (R % 8 == 0, rep0 % 8 == 0): the
+// lane reads its 8 replicas' spins of each neighbour as one 8-byte word, counts
+// down neighbours byte-wise ((word ^ bond sign) & 0x02 per byte, doubled count
+// <= 2*maxdeg <= 254), forms the table index e = q + qmax per byte, and accepts
+// with the 8 x 16-bit uniforms of one Philox block (tie -> TIE stream).  The
+// group's acceptance tables are staged in LDS.  Same decisions as the per-replica
+// kernel above.
+template <bool BONDS>
+__global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ base, int R,
+                                                          const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                          const int32_t* __restrict__ vert_of,
+                                                          const int8_t* __restrict__ bonds,
+                                                          const int64_t* __restrict__ rep_bond_off,
+                                                          const uint32_t* __restrict__ thr, int tbl, int64_t c0,
+                                                          int64_t nc, uint32_t phase, const uint32_t* __restrict__ dyn,
+                                                          int rep0, Key2 key)
+{
+    extern __shared__ uint32_t s_tab[];  // [R][tbl] high halves, then [R][tbl] low halves
+    if (dyn) phase += dyn[0];
+    for (int i = threadIdx.x; i < R * tbl; i += blockDim.x) {
+        s_tab[i] = thr[((int64_t)rep0 * tbl + i) * 2];
+        s_tab[R * tbl + i] = thr[((int64_t)rep0 * tbl + i) * 2 + 1];
+    }
+    __syncthreads();
+    const int oct = R >> 3;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc * oct) return;
+    const int64_t vl = t / oct;
+    const int o = (int)(t - vl * oct);
+    const int64_t pos = c0 + vl;
+    const int qmax = (tbl - 1) >> 1;
+    const int a = rp[pos], b = rp[pos + 1];
+    const int deg = b - a;
+    const int8_t* bb = BONDS ? bonds + rep_bond_off[rep0 + 8 * o] : nullptr;
+    uint32_t n0 = 0u, n1 = 0u;
+    for (int k = a; k < b; ++k) {
+        const uint2 v = *reinterpret_cast<const uint2*>(base + (int64_t)ci[k] * R + 8 * o);
+        const uint32_t f = (BONDS && bb[k] < 0) ? 0xFEFEFEFEu : 0u;  // -x = x ^ 0xFE for x = +-1
+        n0 += (v.x ^ f) & 0x02020202u;
+        n1 += (v.y ^ f) & 0x02020202u;
+    }
+    uint2* ownp = reinterpret_cast<uint2*>(base + pos * R + 8 * o);
+    const uint2 S = *ownp;
+    const uint32_t bias = (uint32_t)(qmax - deg) * 0x01010101u;
+    const uint32_t two_deg = (uint32_t)(2 * deg) * 0x01010101u;
+    auto eword = [&](uint32_t sw, uint32_t n2) {
+        const uint32_t mask = ((sw & 0x02020202u) >> 1) * 0xFFu;  // own spin down
+        return ((n2 & mask) | ((two_deg - n2) & ~mask)) + bias;     // 2*aligned - deg + qmax
+    };
+    const uint32_t E0 = eword(S.x, n0), E1 = eword(S.y, n1);
+    const int grep0 = rep0 + 8 * o;
+    const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep0 >> 3, ST_METRO, key);
+    uint32_t acc = 0u;
+    bool tie = false;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const uint32_t e = ((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu;
+        const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+        const uint32_t th = s_tab[(8 * o + m) * tbl + e];
+        const uint32_t tt = th > 0xFFFFu ? 0xFFFFu : th;  // "always" (65536) as 0xFFFF + lo 0x10000
+        acc |= (u < tt ? 1u : 0u) << m;
+        tie |= (u == tt);
+    }
+    if (tie) {
+        const uint4 tw = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep0 >> 3, ST_TIE, key);
+        acc = 0u;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const uint32_t e = ((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu;
+            const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+            const uint32_t lo = (u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+            const uint32_t th = s_tab[(8 * o + m) * tbl + e];
+            const uint32_t tt = th > 0xFFFFu ? 0xFFFFu : th;
+            const uint32_t tl = th > 0xFFFFu ? 0x10000u : s_tab[R * tbl + (8 * o + m) * tbl + e];
+            acc |= ((u < tt) || (u == tt && lo < tl) ? 1u : 0u) << m;
+        }
+    }
+    auto expand = [](uint32_t nib) { return ((nib * 0x00204081u) & 0x01010101u) * 0xFEu; };
+    *ownp = make_uint2(S.x ^ expand(acc & 15u), S.y ^ expand(acc >> 4));
 }
 
 template <typename T>
@@ -501,6 +593,25 @@ hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t p
                                st->p.proposal, J, d, lam, m, phase, st->cur_dyn, g.rep0, key);
     } else {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
+        const size_t tab = (size_t)g.nrep * st->tbl * 8;
+        bool oct_ok = model != MQ_POTTS && g.nrep % 8 == 0 && g.rep0 % 8 == 0 && l->maxdeg <= 127 && tab <= 32768 &&
+                      st->kernel_choice != 2;
+        if (oct_ok && bonds)  // the 8 replicas of a lane must share one coupling sample
+            for (int r = g.rep0; r < g.rep0 + g.nrep && oct_ok; ++r)
+                oct_ok = st->rep_sample[r] == st->rep_sample[r - r % 8];
+        if (oct_ok) {
+            const int64_t w8 = nc * (g.nrep / 8);
+            const dim3 g8((unsigned)((w8 + 255) / 256));
+            if (bonds)
+                hipLaunchKernelGGL(k_csr_sweep_ising8<true>, g8, dim3(256), tab, s, b, g.nrep, l->d_rp, l->d_ci,
+                                   l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc, phase,
+                                   st->cur_dyn, g.rep0, key);
+            else
+                hipLaunchKernelGGL(k_csr_sweep_ising8<false>, g8, dim3(256), tab, s, b, g.nrep, l->d_rp, l->d_ci,
+                                   l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc, phase,
+                                   st->cur_dyn, g.rep0, key);
+            return hipGetLastError();
+        }
         if (bonds)
             hipLaunchKernelGGL(k_csr_sweep_int8<true>, grid, dim3(blk), 0, s, b, (int64_t)1, (int64_t)g.nrep, g.nrep, l->d_rp,
                                l->d_ci, l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc,

@@ -148,6 +148,10 @@ class State:
         self.ctx.check(self.ctx.lib.mq_state_random_bonds(self.h, None if rs is None else _ptr(rs), p_plus, seed),
                        "mq_state_random_bonds")
 
+    def set_samples(self, rep_sample):
+        rs = np.ascontiguousarray(rep_sample, np.int32)
+        self.ctx.check(self.ctx.lib.mq_state_set_samples(self.h, _ptr(rs)), "mq_state_set_samples")
+
     def set_bonds(self, sample, bonds):
         b = np.ascontiguousarray(bonds, np.int8)
         self.ctx.check(self.ctx.lib.mq_state_set_bonds(self.h, sample, _ptr(b)), "mq_state_set_bonds")

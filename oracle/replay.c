@@ -390,8 +390,10 @@ EXPORT void orc_philox4x32_10_export(const uint32_t* ctr, const uint32_t* key, u
 /* ------------------------------------------------------------ CSR graphs */
 /* Colour-class sweeps on a general graph (original vertex numbering); the
  * colour of every vertex is an input (the library's greedy colouring).
- * counter (vertex, 64*sweep + colour, replica, METRO): Ising/EA accept iff
- * word 0 < T; Potts proposes (word0*Q)>>32 and accepts iff word 1 < T. */
+ * Ising/EA: counter (vertex, 64*sweep + colour, replica >> 3, METRO), 16-bit
+ * half (replica & 7) against the threshold's high half, TIE stream on equality;
+ * Potts: counter (vertex, phase, replica, METRO), proposes (word0*Q)>>32 and
+ * accepts iff word 1 < T. */
 EXPORT void orc_csr_sweeps(int64_t N, const int32_t* rp, const int32_t* ci, const int32_t* colour, int ncol,
                            int8_t* s, const int8_t* bonds, const uint32_t* thi, const uint32_t* tlo, int qmax,
                            int potts, int Q, int nsweeps, uint64_t seed, uint32_t sweep0, uint32_t rep)
@@ -402,9 +404,9 @@ EXPORT void orc_csr_sweeps(int64_t N, const int32_t* rp, const int32_t* ci, cons
             for (int64_t v = 0; v < N; ++v) {
                 if (colour[v] != c) continue;
                 uint32_t w[4];
-                orc_philox_u64key((uint32_t)v, phase, rep, ST_METRO, seed, w);
                 int si = s[v];
                 if (potts) {
+                    orc_philox_u64key((uint32_t)v, phase, rep, ST_METRO, seed, w);
                     int snew = (int)(((uint64_t)w[0] * (uint64_t)Q) >> 32);
                     int dn = 0;
                     for (int k = rp[v]; k < rp[v + 1]; ++k) dn += (s[ci[k]] == snew) - (s[ci[k]] == si);
@@ -415,8 +417,18 @@ EXPORT void orc_csr_sweeps(int64_t N, const int32_t* rp, const int32_t* ci, cons
                     int h = 0;
                     for (int k = rp[v]; k < rp[v + 1]; ++k) h += (bonds ? bonds[k] : 1) * s[ci[k]];
                     int e = si * h + qmax;
-                    uint64_t T = ((uint64_t)thi[e] << 16) | tlo[e];
-                    if ((uint64_t)w[0] < T) s[v] = (int8_t)(-si);
+                    /* one Philox block per vertex and 8 replicas: 16-bit half (rep & 7) */
+                    orc_philox_u64key((uint32_t)v, phase, rep >> 3, ST_METRO, seed, w);
+                    int m = (int)(rep & 7u);
+                    uint32_t hi16 = (w[m >> 1] >> (16 * (m & 1))) & 0xFFFFu;
+                    int acc;
+                    if (hi16 != thi[e]) acc = hi16 < thi[e];
+                    else {
+                        uint32_t tw[4];
+                        orc_philox_u64key((uint32_t)v, phase, rep >> 3, ST_TIE, seed, tw);
+                        acc = ((tw[m >> 1] >> (16 * (m & 1))) & 0xFFFFu) < tlo[e];
+                    }
+                    if (acc) s[v] = (int8_t)(-si);
                 }
             }
         }

@@ -103,3 +103,29 @@ def test_csr_regular_graph_observables(ctx):
     obs = st.measure()
     assert obs[0, 0] == pytest.approx(e, abs=1e-12)
     assert obs[0, 1] == pytest.approx(abs(int(s.astype(int).sum())) / 64, abs=1e-12)
+
+
+@pytest.mark.parametrize("model", ["ising", "ea"])
+def test_csr_octet_kernel_bit_exact(ctx, model):
+    """16 replicas on one graph select the 8-replicas-per-lane kernel."""
+    from marqov_amd import gpu
+    rp, ci = _graph(1200, 9)
+    N = len(rp) - 1
+    lat = _lat(ctx, rp, ci)
+    col = _colours(ctx, lat)
+    R = 16
+    st = gpu.State(ctx, model, [lat] * R, J=-1.0)
+    betas = np.linspace(0.3, 1.5, R)
+    st.set_beta(betas)
+    bonds = None
+    if model == "ea":
+        src = np.repeat(np.arange(N), np.diff(rp))
+        lo, hi = np.minimum(src, ci), np.maximum(src, ci)
+        bonds = np.where(((lo.astype(np.int64) * 7919 + hi * 104729) % 3) == 0, -1, 1).astype(np.int8)
+        st.set_samples(np.zeros(R, np.int32))  # one sample shared by all replicas
+        st.set_bonds(0, bonds)
+    st.init(seed=14)
+    st.metropolis(4, seed=15, sweep0=2)
+    for r in range(R):
+        exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(N, 14, r), betas[r], -1.0, 4, 15, 2, r, bonds=bonds)
+        assert np.array_equal(st.download(r), exp), r
