@@ -36,23 +36,25 @@ __device__ __forceinline__ uint32_t cls_mask(uint32_t b0, uint32_t b1, uint32_t 
 }
 
 template <int DIM>
-__device__ __forceinline__ void nbrs_h(int L, int colour, int64_t h, int64_t nb[2 * DIM])
+__device__ __forceinline__ void nbrs_h(int L, int colour, uint32_t h, uint32_t nb[2 * DIM])
 {
-    const int H = L >> 1;
-    const int64_t plane = (int64_t)H * L;
-    const int64_t row = h / H;
-    const int k = (int)(h - row * H);
-    const int x1 = (int)(row % L);
-    const int x2 = (DIM == 3) ? (int)(row / L) : 0;
-    const int p = (colour + x1 + x2) & 1;
-    const int64_t rb = h - k;
-    nb[0] = rb + (p ? ((k + 1 == H) ? 0 : k + 1) : k);
+    // N < 2^24 for these lattices: 32-bit indices and reciprocal division
+    const uint32_t H = (uint32_t)L >> 1;
+    const uint32_t plane = H * (uint32_t)L;
+    uint32_t k, x1;
+    const uint32_t row = fdiv_u24(h, H, 1.0f / (float)H, k);
+    uint32_t x2 = 0;
+    if (DIM == 3) x2 = fdiv_u24(row, (uint32_t)L, 1.0f / (float)L, x1);
+    else x1 = row;
+    const uint32_t p = ((uint32_t)colour + x1 + x2) & 1u;
+    const uint32_t rb = h - k;
+    nb[0] = rb + (p ? ((k + 1 == H) ? 0u : k + 1) : k);
     nb[1] = rb + (p ? k : ((k == 0) ? H - 1 : k - 1));
-    nb[2] = (x1 == L - 1) ? h - (int64_t)(L - 1) * H : h + H;
-    nb[3] = (x1 == 0) ? h + (int64_t)(L - 1) * H : h - H;
+    nb[2] = (x1 == (uint32_t)L - 1) ? h - ((uint32_t)L - 1) * H : h + H;
+    nb[3] = (x1 == 0) ? h + ((uint32_t)L - 1) * H : h - H;
     if (DIM == 3) {
-        nb[4] = (x2 == L - 1) ? h - (int64_t)(L - 1) * plane : h + plane;
-        nb[5] = (x2 == 0) ? h + (int64_t)(L - 1) * plane : h - plane;
+        nb[4] = (x2 == (uint32_t)L - 1) ? h - ((uint32_t)L - 1) * plane : h + plane;
+        nb[5] = (x2 == 0) ? h + ((uint32_t)L - 1) * plane : h - plane;
     }
 }
 
@@ -100,20 +102,21 @@ __global__ void __launch_bounds__(256) k_sweep_eabits(uint32_t* __restrict__ bas
     int64_t N = (int64_t)L * L;
     if (DIM == 3) N *= L;
     const int64_t half = N >> 1;
-    const int64_t h = (int64_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
-    if (h >= half) return;
+    const uint32_t h = (uint32_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
+    if (h >= (uint32_t)half) return;
+    const uint32_t hf = (uint32_t)half;
     uint32_t* own = base + (int64_t)wl * wstride + colour * half;
     const uint32_t* oth = base + (int64_t)wl * wstride + (1 - colour) * half;
     const int8_t* bs = bonds + rep_bond_off[32 * gw];
     const int8_t* bown = bs + (int64_t)colour * DIM * half;
     const int8_t* both = bs + (int64_t)(1 - colour) * DIM * half;
-    int64_t nb[2 * DIM];
+    uint32_t nb[2 * DIM];
     nbrs_h<DIM>(L, colour, h, nb);
     uint32_t n[2 * DIM];
 #pragma unroll
     for (int sl = 0; sl < 2 * DIM; ++sl) {
-        const int d = sl >> 1;
-        const int b = (sl & 1) ? both[(int64_t)d * half + nb[sl]] : bown[(int64_t)d * half + h];
+        const uint32_t d = (uint32_t)(sl >> 1);
+        const int b = (sl & 1) ? both[d * hf + nb[sl]] : bown[d * hf + h];
         n[sl] = oth[nb[sl]] ^ (b < 0 ? 0xFFFFFFFFu : 0u);
     }
     const uint32_t S = own[h];
@@ -130,7 +133,7 @@ __global__ void __launch_bounds__(256) k_sweep_eabits(uint32_t* __restrict__ bas
     uint32_t D = (m0 & ~T.A[0]) | (m1 & ~T.A[1]) | (m2 & ~T.A[2]);
     uint4 w = make_uint4(0, 0, 0, 0);
     for (int k = 0; k < 32 && D != 0u; ++k) {
-        if ((k & 3) == 0) w = philox4x32_10((uint32_t)h, phase, (uint32_t)gw, ST_BITS + (uint32_t)(k >> 2), key);
+        if ((k & 3) == 0) w = philox4x32_10(h, phase, (uint32_t)gw, ST_BITS + (uint32_t)(k >> 2), key);
         const uint32_t R = u4_get(w, k & 3);
         const uint32_t t = (T.W[k][0] & m0) | (T.W[k][1] & m1) | (T.W[k][2] & m2);
         acc |= D & ~R & t;
@@ -218,12 +221,12 @@ __global__ void __launch_bounds__(256) k_measure_eabits(const uint32_t* __restri
     int64_t N = (int64_t)L * L;
     if (DIM == 3) N *= L;
     const int64_t half = N >> 1;
-    const int64_t h = (int64_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
-    if (h < half) {
+    const uint32_t h = (uint32_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
+    if (h < (uint32_t)half) {
         const uint32_t* c0 = base + (int64_t)wl * wstride;
         const uint32_t* c1 = c0 + half;
         const int8_t* bs = bonds + rep_bond_off[32 * gw];
-        int64_t nb[2 * DIM];
+        uint32_t nb[2 * DIM];
         nbrs_h<DIM>(L, 0, h, nb);
         const uint32_t S = c0[h];
         int anti[32];

@@ -37,10 +37,22 @@ __device__ __forceinline__ RegGeom reg_geom(int dim, int L)
 template <int DIM>
 __device__ __forceinline__ void reg_nbrs(const RegGeom& G, int colour, int64_t h, int64_t nb[2 * DIM])
 {
-    const int64_t row = h / G.H;
-    const int k = (int)(h - row * G.H);
-    const int x1 = (int)(row % G.L);
-    const int x2 = (DIM == 3) ? (int)(row / G.L) : 0;
+    int64_t row;
+    int k, x1, x2 = 0;
+    if (G.half < (1 << 24)) {  // 32-bit indices, reciprocal division
+        uint32_t kk, xx1;
+        const uint32_t r32 = fdiv_u24((uint32_t)h, (uint32_t)G.H, 1.0f / (float)G.H, kk);
+        if (DIM == 3) x2 = (int)fdiv_u24(r32, (uint32_t)G.L, 1.0f / (float)G.L, xx1);
+        else xx1 = r32;
+        row = r32;
+        k = (int)kk;
+        x1 = (int)xx1;
+    } else {
+        row = h / G.H;
+        k = (int)(h - row * G.H);
+        x1 = (int)(row % G.L);
+        if (DIM == 3) x2 = (int)(row / G.L);
+    }
     const int p = (colour + x1 + x2) & 1;
     const int64_t rb = h - k;
     nb[0] = rb + (p ? ((k + 1 == G.H) ? 0 : k + 1) : k);

@@ -57,6 +57,23 @@ __host__ __device__ __forceinline__ uint32_t u4_get(const uint4& v, int i)
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }
 
+// n / d for n < 2^24 through a float reciprocal and one exact correction step
+// (the float quotient is within 1 of the true one there); rem gets n - q*d.
+__device__ __forceinline__ uint32_t fdiv_u24(uint32_t n, uint32_t d, float inv, uint32_t& rem)
+{
+    uint32_t q = (uint32_t)((float)n * inv);
+    int32_t r = (int32_t)(n - q * d);
+    if (r < 0) {
+        --q;
+        r += (int32_t)d;
+    } else if (r >= (int32_t)d) {
+        ++q;
+        r -= (int32_t)d;
+    }
+    rem = (uint32_t)r;
+    return q;
+}
+
 // [0,1) float from the top 24 bits
 __device__ __forceinline__ float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }
 
