@@ -372,6 +372,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
     if ((e = prepare_lds(st)) != hipSuccess) return fail(e);
     if (eabits && (e = hipMalloc(&st->d_wtab, wtab_bytes() * (size_t)(nrep / 32))) != hipSuccess) return fail(e);
+    if (eabits && (e = prepare_eabits(st)) != hipSuccess) return fail(e);
     *out = st;
     return MQ_OK;
 }
@@ -401,6 +402,7 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_dyn);
     if (st->d_msc_cls) (void)hipFree(st->d_msc_cls);
     if (st->d_wtab) (void)hipFree(st->d_wtab);
+    if (st->d_bits_groups) (void)hipFree(st->d_bits_groups);
     if (st->d_lds_groups) (void)hipFree(st->d_lds_groups);
     delete st;
 }
@@ -712,12 +714,18 @@ int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep
     for (int t = 0; t < nsweeps; ++t) {
         const uint64_t sw = sweep0 + (uint64_t)t;
         if (sw >= (1ull << 31)) return mq_fail(st->ctx, MQ_ERR_ARG, "sweep counter exceeds 2^31");
+        if (st->eabits) {  // one fused launch per colour over every lattice size
+            for (int c = 0; c < 2; ++c) {
+                hipError_t e = launch_sweep_eabits_all(st, c, (uint32_t)(2 * sw + c), key);
+                if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
+            }
+            continue;
+        }
         for (const auto& g : st->groups) {
             const int nc = g.lat->ncolors;
             for (int c = 0; c < nc; ++c) {
                 const uint32_t phase = (uint32_t)(sw * (uint64_t)(g.lat->kind == LAT_REGULAR ? 2 : 64) + c);
-                hipError_t e = st->eabits ? launch_sweep_eabits(st, g, c, phase, key)
-                               : st->msc ? launch_sweep_msc(st, g, c, phase, key)
+                hipError_t e = st->msc ? launch_sweep_msc(st, g, c, phase, key)
                                : (g.lat->kind == LAT_REGULAR) ? launch_sweep_regular(st, g, c, phase, key)
                                                               : launch_sweep_csr(st, g, c, phase, key);
                 if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));

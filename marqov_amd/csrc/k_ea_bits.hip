@@ -82,17 +82,31 @@ __device__ __forceinline__ void anti_count(uint32_t s, const uint32_t (&n)[2 * D
     }
 }
 
+// one launch covers every lattice size: block ranges per lattice group
+struct BitsGroup {
+    int64_t off;     // byte offset of the group's first word group
+    int64_t wstride; // words between word groups
+    int L, gw0, bpw, block0;
+};
+
 template <int DIM, bool FERRO>
-__global__ void __launch_bounds__(256) k_sweep_eabits(uint32_t* __restrict__ base, int64_t wstride, int L,
-                                                      const int8_t* __restrict__ bonds,
+__global__ void __launch_bounds__(256) k_sweep_eabits(char* __restrict__ spins, const BitsGroup* __restrict__ groups,
+                                                      int ngroups, const int8_t* __restrict__ bonds,
                                                       const int64_t* __restrict__ rep_bond_off,
-                                                      const WTab* __restrict__ wtab, int gw0, int bpw, int colour,
-                                                      uint32_t phase, const uint32_t* __restrict__ dyn, Key2 key)
+                                                      const WTab* __restrict__ wtab, int colour, uint32_t phase,
+                                                      const uint32_t* __restrict__ dyn, Key2 key)
 {
     __shared__ WTab T;
     if (dyn) phase += dyn[0];
-    const int wl = blockIdx.x / bpw;  // word group within this lattice group
-    const int gw = gw0 + wl;
+    int gi = 0;
+    while (gi + 1 < ngroups && (int)blockIdx.x >= groups[gi + 1].block0) ++gi;
+    const BitsGroup G = groups[gi];
+    const int L = G.L, bpw = G.bpw;
+    const int lb = (int)blockIdx.x - G.block0;
+    const int wl = lb / bpw;  // word group within this lattice group
+    const int gw = G.gw0 + wl;
+    uint32_t* base = reinterpret_cast<uint32_t*>(spins + G.off);
+    const int64_t wstride = G.wstride;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(wtab + gw);
         uint32_t* dst = reinterpret_cast<uint32_t*>(&T);
@@ -102,7 +116,7 @@ __global__ void __launch_bounds__(256) k_sweep_eabits(uint32_t* __restrict__ bas
     int64_t N = (int64_t)L * L;
     if (DIM == 3) N *= L;
     const int64_t half = N >> 1;
-    const uint32_t h = (uint32_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
+    const uint32_t h = (uint32_t)(lb - wl * bpw) * blockDim.x + threadIdx.x;
     if (h >= (uint32_t)half) return;
     const uint32_t hf = (uint32_t)half;
     uint32_t* own = base + (int64_t)wl * wstride + colour * half;
@@ -203,7 +217,24 @@ __global__ void __launch_bounds__(256) k_init_eabits(uint32_t* __restrict__ base
     base[(int64_t)wl * wstride + colour * half + (i >> 1)] = word;
 }
 
-// per replica: isum[r*8+0] = sum over bonds b s_i s_j, isum[r*8+1] = sum s
+// add the 3-bit per-lane counts (b2 b1 b0) into an NP-plane bit-sliced accumulator
+template <int NP>
+__device__ __forceinline__ void acc_add3(uint32_t (&A)[NP], uint32_t b0, uint32_t b1, uint32_t b2)
+{
+    uint32_t carry = 0u;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const uint32_t in = (p == 0) ? b0 : (p == 1) ? b1 : (p == 2) ? b2 : 0u;
+        const uint32_t sm = A[p] ^ in ^ carry;
+        carry = (A[p] & in) | (A[p] & carry) | (in & carry);
+        A[p] = sm;
+    }
+}
+
+// per replica: isum[r*8+0] = sum over bonds b s_i s_j, isum[r*8+1] = sum s.
+// A thread folds SPT colour-0 sites into bit-sliced per-bit counters (anti-
+// aligned bonds and down spins), then adds its 32 per-bit totals to LDS once.
+constexpr int SPT = 16;
 template <int DIM>
 __global__ void __launch_bounds__(256) k_measure_eabits(const uint32_t* __restrict__ base, int64_t wstride, int L,
                                                         const int8_t* __restrict__ bonds,
@@ -220,39 +251,46 @@ __global__ void __launch_bounds__(256) k_measure_eabits(const uint32_t* __restri
     __syncthreads();
     int64_t N = (int64_t)L * L;
     if (DIM == 3) N *= L;
-    const int64_t half = N >> 1;
-    const uint32_t h = (uint32_t)(blockIdx.x - wl * bpw) * blockDim.x + threadIdx.x;
-    if (h < (uint32_t)half) {
-        const uint32_t* c0 = base + (int64_t)wl * wstride;
-        const uint32_t* c1 = c0 + half;
-        const int8_t* bs = bonds + rep_bond_off[32 * gw];
+    const uint32_t half = (uint32_t)(N >> 1);
+    const uint32_t* c0 = base + (int64_t)wl * wstride;
+    const uint32_t* c1 = c0 + half;
+    const int8_t* bs = bonds + rep_bond_off[32 * gw];
+    uint32_t EA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // anti-aligned bonds: <= 6 * SPT per bit
+    uint32_t DA[6] = {0, 0, 0, 0, 0, 0};           // down spins: <= 2 * SPT per bit
+    const uint32_t first = (uint32_t)(blockIdx.x - wl * bpw) * blockDim.x * SPT + threadIdx.x;
+    for (int t = 0; t < SPT; ++t) {
+        const uint32_t h = first + (uint32_t)t * blockDim.x;  // coalesced across the block
+        if (h >= half) break;
         uint32_t nb[2 * DIM];
         nbrs_h<DIM>(L, 0, h, nb);
-        const uint32_t S = c0[h];
-        int anti[32];
-#pragma unroll
-        for (int j = 0; j < 32; ++j) anti[j] = 0;
+        uint32_t n[2 * DIM];
 #pragma unroll
         for (int sl = 0; sl < 2 * DIM; ++sl) {
-            const int d = sl >> 1;
-            const int b = (sl & 1) ? bs[(int64_t)DIM * half + (int64_t)d * half + nb[sl]] : bs[(int64_t)d * half + h];
-            const uint32_t a = S ^ c1[nb[sl]] ^ (b < 0 ? 0xFFFFFFFFu : 0u);
-#pragma unroll
-            for (int j = 0; j < 32; ++j) anti[j] += (a >> j) & 1u;
+            const uint32_t d = (uint32_t)(sl >> 1);
+            const int b = (sl & 1) ? bs[(uint32_t)DIM * half + d * half + nb[sl]] : bs[d * half + h];
+            n[sl] = c1[nb[sl]] ^ (b < 0 ? 0xFFFFFFFFu : 0u);
         }
+        const uint32_t S = c0[h];
+        uint32_t b0, b1, b2;
+        anti_count<DIM>(S, n, b0, b1, b2);
+        acc_add3<9>(EA, b0, b1, b2);
         const uint32_t T2 = c1[h];
+        acc_add3<6>(DA, S ^ T2, S & T2, 0u);  // S + T2 per bit: sum bit, carry bit
+    }
+    for (int j = 0; j < 32; ++j) {
+        int ca = 0, cd = 0;
 #pragma unroll
-        for (int j = 0; j < 32; ++j) {
-            if (anti[j]) atomicAdd(&s_anti[j], anti[j]);
-            const int dn = (int)((S >> j) & 1u) + (int)((T2 >> j) & 1u);
-            if (dn) atomicAdd(&s_down[j], dn);
-        }
+        for (int p = 0; p < 9; ++p) ca += (int)((EA[p] >> j) & 1u) << p;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) cd += (int)((DA[p] >> j) & 1u) << p;
+        if (ca) atomicAdd(&s_anti[j], ca);
+        if (cd) atomicAdd(&s_down[j], cd);
     }
     __syncthreads();
     if (threadIdx.x < 32) {
-        const int64_t sites0 = ((int64_t)(blockIdx.x - wl * bpw) * blockDim.x);
-        int64_t n0 = half - sites0;
-        if (n0 > (int64_t)blockDim.x) n0 = blockDim.x;
+        const int64_t sites0 = (int64_t)(blockIdx.x - wl * bpw) * blockDim.x * SPT;
+        int64_t n0 = (int64_t)half - sites0;
+        if (n0 > (int64_t)blockDim.x * SPT) n0 = (int64_t)blockDim.x * SPT;
         // colour-0 sites of this block: n0, each with 2*DIM bonds; sites counted for M: 2*n0
         const long long bsum = (long long)n0 * 2 * DIM - 2ll * s_anti[threadIdx.x];
         const long long msum = 2ll * n0 - 2ll * s_down[threadIdx.x];
@@ -287,20 +325,40 @@ hipError_t launch_init_eabits(mq_state* st, const Group& g, Key2 key)
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_eabits(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key)
+hipError_t prepare_eabits(mq_state* st)
 {
-    const int L = g.lat->L;
-    const int nwl = g.nrep / 32;
-    const int64_t half = g.lat->N / 2;
-    const int bpw = (int)((half + 255) / 256);
-    uint32_t* b = reinterpret_cast<uint32_t*>(st->d_spins + g.off);
+    std::vector<BitsGroup> gs;
+    int blocks = 0;
+    for (const auto& g : st->groups) {
+        BitsGroup d;
+        d.off = g.off;
+        d.wstride = g.stride / 4;
+        d.L = g.lat->L;
+        d.gw0 = g.rep0 / 32;
+        d.bpw = (int)((g.lat->N / 2 + 255) / 256);
+        d.block0 = blocks;
+        blocks += d.bpw * (g.nrep / 32);
+        gs.push_back(d);
+    }
+    if (gs.size() > 64) return hipErrorNotSupported;
+    hipError_t e = hipMalloc(&st->d_bits_groups, gs.size() * sizeof(BitsGroup));
+    if (e != hipSuccess) return e;
+    st->bits_ngroups = (int)gs.size();
+    st->bits_blocks = blocks;
+    return hipMemcpy(st->d_bits_groups, gs.data(), gs.size() * sizeof(BitsGroup), hipMemcpyHostToDevice);
+}
+
+// one launch per colour over all lattice groups
+hipError_t launch_sweep_eabits_all(mq_state* st, int colour, uint32_t phase, Key2 key)
+{
     const WTab* wt = reinterpret_cast<const WTab*>(st->d_wtab);
-    const dim3 grid((unsigned)(bpw * nwl));
+    const BitsGroup* dg = reinterpret_cast<const BitsGroup*>(st->d_bits_groups);
+    const dim3 grid((unsigned)st->bits_blocks);
     const bool ferro = st->p.J < 0.0;
 #define MQ_E(D, F)                                                                                              \
-    hipLaunchKernelGGL((k_sweep_eabits<D, F>), grid, dim3(256), 0, st->ctx->stream, b, g.stride / 4, L,          \
-                       st->d_bonds, st->d_rep_bond_off, wt, g.rep0 / 32, bpw, colour, phase, st->cur_dyn, key)
-    if (g.lat->dim == 3) {
+    hipLaunchKernelGGL((k_sweep_eabits<D, F>), grid, dim3(256), 0, st->ctx->stream, st->d_spins, dg,            \
+                       st->bits_ngroups, st->d_bonds, st->d_rep_bond_off, wt, colour, phase, st->cur_dyn, key)
+    if (st->lat[0]->dim == 3) {
         if (ferro) MQ_E(3, true);
         else MQ_E(3, false);
     } else {
@@ -316,7 +374,7 @@ hipError_t launch_measure_eabits(mq_state* st, const Group& g)
     const int L = g.lat->L;
     const int nwl = g.nrep / 32;
     const int64_t half = g.lat->N / 2;
-    const int bpw = (int)((half + 255) / 256);
+    const int bpw = (int)((half + 256 * SPT - 1) / (256 * SPT));
     const uint32_t* b = reinterpret_cast<const uint32_t*>(st->d_spins + g.off);
     unsigned long long* is = reinterpret_cast<unsigned long long*>(st->d_isum);
     if (g.lat->dim == 3)

@@ -171,6 +171,8 @@ struct mq_state {
     bool msc = false;                   // 1-bit multi-spin-coded Ising storage
     bool eabits = false;                // EA: 32 replicas of one sample per uint32 word
     void* d_wtab = nullptr;             // per word group: per-bit threshold words
+    void* d_bits_groups = nullptr;      // lattice-group table of the fused bit-sliced sweep
+    int bits_ngroups = 0, bits_blocks = 0;
     void* d_msc_cls = nullptr;          // per-replica Bernoulli classes of the MSC sweep
     int tune_block = 256;               // threads per block of the streaming sweep kernels
     int tune_xcd = 1;                   // XCD-aware block remap of the row-vectorised kernel
@@ -200,7 +202,8 @@ hipError_t prepare_lds(mq_state* st);
 hipError_t launch_wtab_eabits(mq_state* st);
 size_t wtab_bytes();
 hipError_t launch_init_eabits(mq_state* st, const Group& g, Key2 key);
-hipError_t launch_sweep_eabits(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
+hipError_t prepare_eabits(mq_state* st);
+hipError_t launch_sweep_eabits_all(mq_state* st, int colour, uint32_t phase, Key2 key);
 hipError_t launch_measure_eabits(mq_state* st, const Group& g);
 bool msc_supported(const mq_lattice* lat);
 hipError_t msc_upload_classes(mq_state* st, const std::vector<uint32_t>& thr);
