@@ -60,6 +60,14 @@ def timed(ctx, fn):
     return ms, time.perf_counter() - t0
 
 
+def timed_wolff(ctx, st, nclusters, seed, cluster0):
+    """One timed mq_wolff call; returns (ms, sizes) of exactly the clusters timed.  The copy
+    of the sizes (nrep x nclusters int64) is inside the timed region."""
+    box = {}
+    ms, _ = timed(ctx, lambda: box.setdefault("sizes", st.wolff(nclusters, seed, cluster0=cluster0, sizes=True)))
+    return ms, box["sizes"]
+
+
 def leg_c2_msc(gpu, ctx, L=256, R=32, steps=100):
     """configs[1] multi-spin-coded variant: 1 bit per spin, R replicas, exact bit-serial acceptance."""
     lat = gpu.RegularLattice(ctx, 3, L)
@@ -91,11 +99,11 @@ def leg_c1(gpu, ctx):
         st.init(seed=1)
         for t in range(10):
             st.emcs(10, 1, 1, t)
-        sizes = st.wolff(10, 1, cluster0=10 ** 6, sizes=True)
+        st.wolff(10, 1, cluster0=10 ** 6, sizes=False)
+        ms_w, sizes = timed_wolff(ctx, st, 10 * nstep, 1, 2 * 10 ** 6)
         mean_cl = float(sizes.mean())
-        ms_w, _ = timed(ctx, lambda: st.wolff(10 * nstep, 1, cluster0=2 * 10 ** 6, sizes=False))
         ms_m, _ = timed(ctx, lambda: st.metropolis(nstep, 1, sweep0=10 ** 5))
-        res[name] = {"replicas": nrep, "wolff_sites_per_ns": round(nrep * 10 * nstep * mean_cl / (ms_w * 1e6), 4),
+        res[name] = {"replicas": nrep, "wolff_sites_per_ns": round(float(sizes.sum()) / (ms_w * 1e6), 4),
                      "attempts_per_ns": round(nrep * 4096 * nstep / (ms_m * 1e6), 3),
                      "mean_cluster": round(mean_cl, 1),
                      "emcs_per_s": round(nrep * nstep / ((ms_w + ms_m) * 1e-3), 1)}
@@ -117,10 +125,10 @@ def leg_c3(gpu, ctx):
         st.init(seed=3)
         st.metropolis(5, 3, 0)
         ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
-        sizes = st.wolff(4, 3, cluster0=0, sizes=True)
-        ms_w, _ = timed(ctx, lambda: st.wolff(8, 3, cluster0=4, sizes=False))
+        st.wolff(4, 3, cluster0=0, sizes=False)
+        ms_w, sizes = timed_wolff(ctx, st, 16, 3, 4)
         res[name] = {"replicas": nrep, "attempts_per_ns": round(nrep * N * nstep / (ms * 1e6), 3),
-                     "wolff_sites_per_ns": round(nrep * 8 * float(sizes.mean()) / (ms_w * 1e6), 4),
+                     "wolff_sites_per_ns": round(float(sizes.sum()) / (ms_w * 1e6), 4),
                      "mean_cluster": round(float(sizes.mean()), 1),
                      "roofline_frac": round(48.0 * nrep * N * nstep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         if name == "batched":
@@ -201,9 +209,10 @@ def leg_c5(gpu, ctx):
             ms, _ = timed(ctx, lambda: st.metropolis(nstep, 5, sweep0=2))
             res[f"{model}_{name}_attempts_per_ns"] = round(nrep * n * nstep / (ms * 1e6), 3)
             if name == "batched":
-                sizes = st.wolff(2, 5, cluster0=0, sizes=True)
-                ms_w, _ = timed(ctx, lambda: st.wolff(4, 5, cluster0=2, sizes=False))
-                res[f"{model}_wolff_sites_per_ns"] = round(nrep * 4 * float(sizes.mean()) / (ms_w * 1e6), 4)
+                st.wolff(2, 5, cluster0=0, sizes=False)
+                ms_w, sizes = timed_wolff(ctx, st, 4, 5, 2)
+                res[f"{model}_wolff_sites_per_ns"] = round(float(sizes.sum()) / (ms_w * 1e6), 4)
+                res[f"{model}_wolff_mean_cluster"] = round(float(sizes.mean()), 1)
             del st
         res[f"{model}_batched_over_plain"] = round(res[f"{model}_batched_attempts_per_ns"] /
                                                    res[f"{model}_plain_attempts_per_ns"], 2)
@@ -283,9 +292,9 @@ def main():
         ms_p, _ = timed(ctx, lambda: sp.metropolis(k, seed=2, sweep0=4))
         line["plain"] = {"attempts_per_ns": round(k * N / (ms_p * 1e6), 3), "replicas": 1, "steps": k}
         line["batched_over_plain"] = round(value / line["plain"]["attempts_per_ns"], 3)
-        sizes = sp.wolff(4, 2, cluster0=0, sizes=True)
-        ms_w, _ = timed(ctx, lambda: sp.wolff(16, 2, cluster0=4, sizes=False))
-        line["plain"]["wolff_sites_per_ns"] = round(16 * float(sizes.mean()) / (ms_w * 1e6), 4)
+        sp.wolff(4, 2, cluster0=0, sizes=False)
+        ms_w, sizes = timed_wolff(ctx, sp, 16, 2, 4)
+        line["plain"]["wolff_sites_per_ns"] = round(float(sizes.sum()) / (ms_w * 1e6), 4)
         line["plain"]["wolff_mean_cluster"] = round(float(sizes.mean()), 1)
         del sp
         for name, fn in (("c2_msc", leg_c2_msc), ("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3),

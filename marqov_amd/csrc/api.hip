@@ -345,6 +345,10 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     st->spin_total = off;
     st->tbl = 2 * qmax_of(st) + 1;
     if (const char* kc = std::getenv("MARQOV_AMD_KERNELS")) st->kernel_choice = std::atoi(kc);
+    if (const char* wr = std::getenv("MARQOV_AMD_WOLFF_RING")) {
+        const int v = std::atoi(wr);
+        if (v >= 64 && v <= 2048 && (v & (v - 1)) == 0) st->wolff_ring = v;
+    }
     st->rep_sample.resize(nrep);
     std::iota(st->rep_sample.begin(), st->rep_sample.end(), 0);
     st->nsamples = nrep;

@@ -326,6 +326,37 @@ __global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ ba
 enum { CK_ISING = 0, CK_POTTS = 1, CK_CART = 2, CK_RDIR = 3 };
 
 template <int KIND>
+__device__ __forceinline__ float cproj(const float4& v, int axis, const float3& rd)
+{
+    return (KIND == CK_CART) ? (axis == 0 ? v.x : (axis == 1 ? v.y : v.z)) : (v.x * rd.x + v.y * rd.y + v.z * rd.z);
+}
+
+template <int KIND>
+__device__ __forceinline__ float4 creflect(float4 v, int axis, const float3& rd)
+{
+    if (KIND == CK_CART) {
+        if (axis == 0) v.x = -v.x;
+        else if (axis == 1) v.y = -v.y;
+        else v.z = -v.z;
+    } else {
+        const float d = v.x * rd.x + v.y * rd.y + v.z * rd.z;
+        v.x -= 2.f * d * rd.x;
+        v.y -= 2.f * d * rd.y;
+        v.z -= 2.f * d * rd.z;
+    }
+    return v;
+}
+
+// Frontier ring in LDS as in the regular-lattice kernel: entry pos written in
+// the level that starts at `head` sits in ring slot pos mod CWCAP when
+// pos - head < CWCAP, else in the global queue.
+constexpr int CWCAP = 4096;
+
+// Wolff on a CSR graph, one workgroup per replica, level-synchronous BFS.
+// int8 models claim sites with one atomic on the spin word (no bitmap);
+// O(3)/Phi4 keep the visited bitmap and carry the flipped spin's projection
+// with each frontier entry.  Bond randoms are keyed by the canonical edge id.
+template <int KIND>
 __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int64_t rs, int64_t ps, int64_t N,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                    const int32_t* __restrict__ edge, const int32_t* __restrict__ pos_of,
@@ -337,6 +368,8 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
 {
     constexpr bool VEC = (KIND == CK_CART || KIND == CK_RDIR);
     __shared__ int s_tail;
+    __shared__ int32_t s_ring[CWCAP];
+    __shared__ float s_proj[VEC ? CWCAP : 1];
     const int r = blockIdx.x;
     const int grep = rep0 + r;
     // element (position p) of this replica: sp8[p * ps] / sp4[p * ps]
@@ -353,81 +386,156 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
         int val = 0, newq = 0, axis = 0;
         float3 rd = make_float3(0.f, 0.f, 0.f);
         if (!VEC) {
-            val = sp8[seed * ps];
-            if (KIND == CK_POTTS) newq = (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q;
+            uint32_t sh;
+            val = (int8_t)(peek_word(sp8 + seed * ps, sh) >> sh);  // earlier clusters wrote with L2 atomics
+            newq = (KIND == CK_POTTS) ? (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q : -val;
         } else if (KIND == CK_CART) {
             axis = (int)(((uint64_t)sw.y * 3ull) >> 32);
         } else {
             rd = rnddir_c(sw.y, sw.z);
         }
         __syncthreads();
-        auto flip = [&](int64_t p) {
-            if (KIND == CK_ISING) sp8[p * ps] = (int8_t)(-val);
-            else if (KIND == CK_POTTS) sp8[p * ps] = (int8_t)newq;
-            else {
-                float4 v = sp4[p * ps];
-                if (KIND == CK_CART) {
-                    if (axis == 0) v.x = -v.x;
-                    else if (axis == 1) v.y = -v.y;
-                    else v.z = -v.z;
-                } else {
-                    const float d = v.x * rd.x + v.y * rd.y + v.z * rd.z;
-                    v.x -= 2.f * d * rd.x;
-                    v.y -= 2.f * d * rd.y;
-                    v.z -= 2.f * d * rd.z;
-                }
-                sp4[p * ps] = v;
-            }
-        };
         if (threadIdx.x == 0) {
-            flip(seed);
-            q[0] = (int32_t)seed;
-            atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+            if (KIND == CK_ISING) {
+                claim_ising(sp8 + seed * ps, val);
+            } else if (KIND == CK_POTTS) {
+                uint32_t sh;
+                claim_potts(sp8 + seed * ps, peek_word(sp8 + seed * ps, sh), val, newq);
+            } else {
+                const float4 v = creflect<KIND>(sp4[seed * ps], axis, rd);
+                sp4[seed * ps] = v;
+                s_proj[0] = cproj<KIND>(v, axis, rd);
+                atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+                q[0] = (int32_t)seed;
+            }
+            s_ring[0] = (int32_t)seed;
             s_tail = 1;
         }
         __syncthreads();
-        int head = 0;
+        int head = 0, hprev = 0;
         while (true) {
             const int tail = s_tail;
             __syncthreads();
             if (head >= tail) break;
             for (int idx = head + (int)threadIdx.x; idx < tail; idx += blockDim.x) {
-                const int64_t cur = q[idx];
+                const bool in_ring = idx - hprev < CWCAP;
+                const int64_t cur = in_ring ? s_ring[idx & (CWCAP - 1)] : q[idx];
                 float pc = 0.f;
-                if (VEC) {
-                    const float4 vc = sp4[cur * ps];
-                    pc = (KIND == CK_CART) ? (axis == 0 ? vc.x : (axis == 1 ? vc.y : vc.z))
-                                           : (vc.x * rd.x + vc.y * rd.y + vc.z * rd.z);
-                }
-                for (int k = rp[cur]; k < rp[cur + 1]; ++k) {
-                    const int64_t j = ci[k];
-                    const uint32_t eid = (uint32_t)edge[k];
-                    bool add;
-                    if (!VEC) {
-                        if (sp8[j * ps] != val) continue;
-                        const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
-                        add = u4_get(bw, (int)(eid & 3u)) < tw;
-                    } else {
-                        const float4 vj = sp4[j * ps];
-                        const float pj = (KIND == CK_CART) ? (axis == 0 ? vj.x : (axis == 1 ? vj.y : vj.z))
-                                                           : (vj.x * rd.x + vj.y * rd.y + vj.z * rd.z);
-                        const float cpl = J * pc * pj;
-                        if (!(cpl > 0.f)) continue;
-                        const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
-                        add = u01f(u4_get(bw, (int)(eid & 3u))) < -expm1f(-2.0f * beta * cpl);
+                if (VEC) pc = in_ring ? s_proj[idx & (CWCAP - 1)] : cproj<KIND>(sp4[cur * ps], axis, rd);
+                // Neighbours in chunks of CH: all claims of a chunk are issued before
+                // any result is tested, so their round trips overlap; new entries are
+                // placed with one LDS atomic per wavefront.
+                constexpr int CH = 8;
+                const int kb = rp[cur], ke = rp[cur + 1];
+                for (int k0 = kb; k0 < ke; k0 += CH) {
+                    int32_t jn[CH];
+                    uint32_t eid[CH];
+                    bool cand[CH], ok[CH];
+                    float pnew[CH];
+#pragma unroll
+                    for (int t = 0; t < CH; ++t) {
+                        const bool in = k0 + t < ke;
+                        jn[t] = in ? ci[k0 + t] : 0;
+                        eid[t] = in ? (uint32_t)edge[k0 + t] : 0u;
+                        cand[t] = in;
+                        pnew[t] = 0.f;
                     }
-                    if (!add) continue;
-                    const uint32_t bit = 1u << (j & 31);
-                    if (atomicOr(&vis[j >> 5], bit) & bit) continue;
-                    flip(j);
-                    q[atomicAdd(&s_tail, 1)] = (int32_t)j;
+                    if (KIND == CK_ISING) {
+                        uint32_t old[CH];
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) {
+                            if (!cand[t]) continue;
+                            const uint4 bw = philox4x32_10(eid[t] >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                            cand[t] = u4_get(bw, (int)(eid[t] & 3u)) < tw;
+                        }
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) old[t] = cand[t] ? claim_ising_issue(sp8 + jn[t] * ps, val) : 0u;
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) ok[t] = cand[t] && byte_at(old[t], sp8 + jn[t] * ps) == (int8_t)val;
+                    } else if (KIND == CK_POTTS) {
+                        uint32_t seen[CH], prev[CH];
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) {
+                            uint32_t sh;
+                            seen[t] = cand[t] ? peek_word(sp8 + jn[t] * ps, sh) : 0u;
+                        }
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) {
+                            if (!cand[t]) continue;
+                            cand[t] = false;
+                            if (byte_at(seen[t], sp8 + jn[t] * ps) == (int8_t)val) {
+                                const uint4 bw = philox4x32_10(eid[t] >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                                cand[t] = u4_get(bw, (int)(eid[t] & 3u)) < tw;
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) {
+                            prev[t] = seen[t];
+                            if (cand[t]) {
+                                uint32_t sh;
+                                uint32_t* w = spin_word(sp8 + jn[t] * ps, sh);
+                                prev[t] = atomicCAS(w, seen[t], (seen[t] & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) {
+                            ok[t] = cand[t] && prev[t] == seen[t];
+                            // the word changed under the first swap: retry from what it held
+                            if (cand[t] && !ok[t]) ok[t] = claim_potts(sp8 + jn[t] * ps, prev[t], val, newq);
+                        }
+                    } else {
+                        float4 vj[CH];
+                        uint32_t ob[CH];
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) vj[t] = cand[t] ? sp4[jn[t] * ps] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) {
+                            if (!cand[t]) continue;
+                            const float cpl = J * pc * cproj<KIND>(vj[t], axis, rd);
+                            cand[t] = false;
+                            if (cpl > 0.f) {
+                                const uint4 bw = philox4x32_10(eid[t] >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                                cand[t] = u01f(u4_get(bw, (int)(eid[t] & 3u))) < -expm1f(-2.0f * beta * cpl);
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < CH; ++t)
+                            ob[t] = cand[t] ? atomicOr(&vis[jn[t] >> 5], 1u << (jn[t] & 31)) : 0u;
+#pragma unroll
+                        for (int t = 0; t < CH; ++t) {
+                            ok[t] = cand[t] && !(ob[t] & (1u << (jn[t] & 31)));
+                            if (ok[t]) {
+                                // the claim succeeded, so nobody changed vj since it was read
+                                const float4 v = creflect<KIND>(vj[t], axis, rd);
+                                sp4[jn[t] * ps] = v;
+                                pnew[t] = cproj<KIND>(v, axis, rd);
+                            }
+                        }
+                    }
+                    uint32_t nok = 0;
+#pragma unroll
+                    for (int t = 0; t < CH; ++t) nok += ok[t] ? 1u : 0u;
+                    int pos = wave_reserve(&s_tail, nok);
+#pragma unroll
+                    for (int t = 0; t < CH; ++t) {
+                        if (!ok[t]) continue;
+                        if (pos - head < CWCAP) {
+                            s_ring[pos & (CWCAP - 1)] = jn[t];
+                            if (VEC) s_proj[pos & (CWCAP - 1)] = pnew[t];
+                        }
+                        if (VEC || pos - head >= CWCAP) q[pos] = jn[t];  // VEC: kept for the bitmap reset
+                        ++pos;
+                    }
                 }
             }
             __syncthreads();
+            hprev = head;
             head = tail;
         }
         const int size = s_tail;
-        for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+        if (VEC) {
+            for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+        }
         __syncthreads();
         if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
     }

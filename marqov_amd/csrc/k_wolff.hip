@@ -14,6 +14,8 @@
 // colour), Heisenberg.h:191-243 (random Cartesian component; add with
 // 1 - exp(-2 beta cpl), cpl = J s_cur'[a] s_j[a]), Phi4.h:182-244 (random
 // unit vector reflection s -= 2 (s.r) r, coupling (s_cur'.r)(s_j.r)).
+#include <algorithm>
+
 #include "mq_internal.hpp"
 
 namespace {
@@ -54,6 +56,37 @@ __device__ __forceinline__ float3 rnddir_w(uint32_t w0, uint32_t w1)
 
 __device__ __forceinline__ float comp(const float4& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
 
+template <int KIND>
+__device__ __forceinline__ float proj(const float4& v, int axis, const float3& rd)
+{
+    return (KIND == WK_CART) ? comp(v, axis) : (v.x * rd.x + v.y * rd.y + v.z * rd.z);
+}
+
+// Cartesian embedding: negate component axis; random direction r: s -= 2 (s.r) r.
+template <int KIND>
+__device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
+{
+    if (KIND == WK_CART) {
+        if (axis == 0) v.x = -v.x;
+        else if (axis == 1) v.y = -v.y;
+        else v.z = -v.z;
+    } else {
+        const float d = v.x * rd.x + v.y * rd.y + v.z * rd.z;
+        v.x -= 2.f * d * rd.x;
+        v.y -= 2.f * d * rd.y;
+        v.z -= 2.f * d * rd.z;
+    }
+    return v;
+}
+
+// Frontier ring in LDS: entry pos written during the level whose first entry
+// is `head` goes to ring slot pos mod WCAP when pos - head < WCAP, else to the
+// global queue.  Entries of the level being read lie in [head, tail) and the
+// ones being written in [tail, tail + WCAP - (tail - head)), so a slot is never
+// overwritten before it is read.  The reader of that level knows `head` as its
+// own `hprev`.
+constexpr int WCAP = 4096;
+
 template <int DIM, int KIND>
 __global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, int64_t stride, int L, int nclusters,
                                                        uint32_t cl0, int rep0, int Q,
@@ -65,11 +98,14 @@ __global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, 
 {
     constexpr bool VEC = (KIND == WK_CART || KIND == WK_RDIR);
     __shared__ int s_tail;
+    __shared__ int32_t s_ring[WCAP];
+    __shared__ float s_proj[VEC ? WCAP : 1];
     const int r = blockIdx.x;
     const int grep = rep0 + r;
-    int64_t N = (int64_t)L * L;
-    if (DIM == 3) N *= L;
-    const int64_t half = N >> 1;
+    const uint32_t uL = (uint32_t)L;
+    const uint32_t LL = uL * uL;
+    const uint32_t N = (DIM == 3) ? LL * uL : LL;  // <= 2^31 (checked at lattice creation)
+    const uint32_t half = N >> 1;
     int8_t* sp8 = reinterpret_cast<int8_t*>(base + r * stride);
     float4* sp4 = reinterpret_cast<float4*>(base + r * stride);
     int32_t* q = queue + rep_site_off[grep];
@@ -80,12 +116,14 @@ __global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, 
     for (int c = 0; c < nclusters; ++c) {
         const uint32_t ncl = cl0 + (uint32_t)c;
         const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
-        const int64_t seed = (int64_t)(((uint64_t)sw.x * (uint64_t)N) >> 32);
-        const int64_t seed_a = split_addr<DIM>(L, half, seed);
+        const uint32_t seed = (uint32_t)(((uint64_t)sw.x * (uint64_t)N) >> 32);
+        const uint32_t seed_a = (uint32_t)split_addr<DIM>(L, half, seed);
         int val = 0, newq = 0, axis = 0;
         float3 rd = make_float3(0.f, 0.f, 0.f);
         if (KIND == WK_ISING || KIND == WK_POTTS) {
-            val = sp8[seed_a];
+            // atomic load: earlier clusters changed bytes with L2 atomics only
+            uint32_t sh;
+            val = (int8_t)(peek_word(sp8 + seed_a, sh) >> sh);
             if (KIND == WK_POTTS) newq = (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q;
         } else if (KIND == WK_CART) {
             axis = (int)(((uint64_t)sw.y * 3ull) >> 32);
@@ -94,164 +132,307 @@ __global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, 
         }
         __syncthreads();  // every lane has read the seed's old value
         if (threadIdx.x == 0) {
-            if (KIND == WK_ISING) sp8[seed_a] = (int8_t)(-val);
-            else if (KIND == WK_POTTS) sp8[seed_a] = (int8_t)newq;
-            else {
-                float4 v = sp4[seed_a];
-                if (KIND == WK_CART) {
-                    if (axis == 0) v.x = -v.x;
-                    else if (axis == 1) v.y = -v.y;
-                    else v.z = -v.z;
-                } else {
-                    const float d = v.x * rd.x + v.y * rd.y + v.z * rd.z;
-                    v.x -= 2.f * d * rd.x;
-                    v.y -= 2.f * d * rd.y;
-                    v.z -= 2.f * d * rd.z;
-                }
+            if (KIND == WK_ISING) {
+                claim_ising(sp8 + seed_a, val);
+            } else if (KIND == WK_POTTS) {
+                uint32_t sh;
+                claim_potts(sp8 + seed_a, peek_word(sp8 + seed_a, sh), val, newq);
+            } else {
+                const float4 v = reflect<KIND>(sp4[seed_a], axis, rd);
                 sp4[seed_a] = v;
+                s_proj[0] = proj<KIND>(v, axis, rd);
+                atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+                q[0] = (int32_t)seed;
             }
-            q[0] = (int32_t)seed;
-            atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+            s_ring[0] = (int32_t)seed;
             s_tail = 1;
         }
         __syncthreads();
-        int head = 0;
+        int head = 0, hprev = 0;
         while (true) {
             const int tail = s_tail;
             __syncthreads();
             if (head >= tail) break;
             for (int idx = head + (int)threadIdx.x; idx < tail; idx += blockDim.x) {
-                const int64_t cur = q[idx];
-                float pc = 0.f;
-                if (VEC) {
-                    const float4 vc = sp4[split_addr<DIM>(L, half, cur)];
-                    pc = (KIND == WK_CART) ? comp(vc, axis) : (vc.x * rd.x + vc.y * rd.y + vc.z * rd.z);
-                }
-#pragma unroll
-                for (int sl = 0; sl < 2 * DIM; ++sl) {
-                    const int64_t j = nat_nbr<DIM>(L, cur, sl);
-                    const int64_t ja = split_addr<DIM>(L, half, j);
-                    const int64_t owner = (sl & 1) ? j : cur;
-                    const int64_t b = owner * DIM + (sl >> 1);
-                    bool add;
-                    if (!VEC) {
-                        if (sp8[ja] != val) continue;
-                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        add = u4_get(bw, (int)(b & 3)) < tw;
+                const bool in_ring = idx - hprev < WCAP;
+                const uint32_t cur = (uint32_t)(in_ring ? s_ring[idx & (WCAP - 1)] : q[idx]);
+                // coordinates of cur; its neighbours all have the other colour
+                uint32_t x[3] = {0u, 0u, 0u};
+                {
+                    const uint32_t r1 = cur / uL;
+                    x[0] = cur - r1 * uL;
+                    if (DIM == 3) {
+                        x[2] = r1 / uL;
+                        x[1] = r1 - x[2] * uL;
                     } else {
-                        const float4 vj = sp4[ja];
-                        const float pj = (KIND == WK_CART) ? comp(vj, axis) : (vj.x * rd.x + vj.y * rd.y + vj.z * rd.z);
-                        const float cpl = J * pc * pj;
-                        if (!(cpl > 0.f)) continue;
+                        x[1] = r1;
+                    }
+                }
+                const uint32_t nbase = ((x[0] + x[1] + x[2]) & 1u) ? 0u : half;
+                float pc = 0.f;
+                if (VEC) pc = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[split_addr<DIM>(L, half, cur)], axis, rd);
+                // All claims of this site are issued before any result is tested, so
+                // their round trips to L2 overlap; the new entries are then placed
+                // with one LDS atomic per wavefront.
+                constexpr int NS = 2 * DIM;
+                uint32_t jn[NS], jan[NS];
+                bool cand[NS], ok[NS];
+                float pnew[NS];
+#pragma unroll
+                for (int sl = 0; sl < NS; ++sl) {
+                    const int d = sl >> 1;
+                    const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
+                    uint32_t j;
+                    if ((sl & 1) == 0) j = (x[d] == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
+                    else j = (x[d] == 0) ? cur + (uL - 1) * pw : cur - pw;
+                    jn[sl] = j;
+                    jan[sl] = nbase + (j >> 1);
+                    pnew[sl] = 0.f;
+                }
+                if (KIND == WK_ISING) {
+                    // bond randoms first: the claim itself tests alignment
+                    uint32_t old[NS];
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const uint64_t b = (uint64_t)((sl & 1) ? jn[sl] : cur) * DIM + (uint64_t)(sl >> 1);
                         const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        add = u01f(u4_get(bw, (int)(b & 3))) < -expm1f(-2.0f * beta * cpl);
+                        cand[sl] = u4_get(bw, (int)(b & 3)) < tw;
                     }
-                    if (!add) continue;
-                    const uint32_t bit = 1u << (j & 31);
-                    const uint32_t old = atomicOr(&vis[j >> 5], bit);
-                    if (old & bit) continue;
-                    if (KIND == WK_ISING) sp8[ja] = (int8_t)(-val);
-                    else if (KIND == WK_POTTS) sp8[ja] = (int8_t)newq;
-                    else {
-                        float4 v = sp4[ja];
-                        if (KIND == WK_CART) {
-                            if (axis == 0) v.x = -v.x;
-                            else if (axis == 1) v.y = -v.y;
-                            else v.z = -v.z;
-                        } else {
-                            const float d = v.x * rd.x + v.y * rd.y + v.z * rd.z;
-                            v.x -= 2.f * d * rd.x;
-                            v.y -= 2.f * d * rd.y;
-                            v.z -= 2.f * d * rd.z;
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) old[sl] = cand[sl] ? claim_ising_issue(sp8 + jan[sl], val) : 0u;
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) ok[sl] = cand[sl] && byte_at(old[sl], sp8 + jan[sl]) == (int8_t)val;
+                } else if (KIND == WK_POTTS) {
+                    uint32_t seen[NS], prev[NS];
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        uint32_t sh;
+                        seen[sl] = peek_word(sp8 + jan[sl], sh);
+                    }
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        cand[sl] = false;
+                        if (byte_at(seen[sl], sp8 + jan[sl]) == (int8_t)val) {
+                            const uint64_t b = (uint64_t)((sl & 1) ? jn[sl] : cur) * DIM + (uint64_t)(sl >> 1);
+                            const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                            cand[sl] = u4_get(bw, (int)(b & 3)) < tw;
                         }
-                        sp4[ja] = v;
                     }
-                    const int pos = atomicAdd(&s_tail, 1);
-                    q[pos] = (int32_t)j;
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        prev[sl] = seen[sl];
+                        if (cand[sl]) {
+                            uint32_t sh;
+                            uint32_t* w = spin_word(sp8 + jan[sl], sh);
+                            prev[sl] = atomicCAS(w, seen[sl], (seen[sl] & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
+                        }
+                    }
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        ok[sl] = cand[sl] && prev[sl] == seen[sl];
+                        // the word changed under the first swap: retry from what it held
+                        if (cand[sl] && !ok[sl]) ok[sl] = claim_potts(sp8 + jan[sl], prev[sl], val, newq);
+                    }
+                } else {
+                    float4 vj[NS];
+                    uint32_t ob[NS];
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) vj[sl] = sp4[jan[sl]];
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const float cpl = J * pc * proj<KIND>(vj[sl], axis, rd);
+                        cand[sl] = false;
+                        if (cpl > 0.f) {
+                            const uint64_t b = (uint64_t)((sl & 1) ? jn[sl] : cur) * DIM + (uint64_t)(sl >> 1);
+                            const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                            cand[sl] = u01f(u4_get(bw, (int)(b & 3))) < -expm1f(-2.0f * beta * cpl);
+                        }
+                    }
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) ob[sl] = cand[sl] ? atomicOr(&vis[jn[sl] >> 5], 1u << (jn[sl] & 31)) : 0u;
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        ok[sl] = cand[sl] && !(ob[sl] & (1u << (jn[sl] & 31)));
+                        if (ok[sl]) {
+                            // the claim succeeded, so nobody changed vj since it was read
+                            const float4 v = reflect<KIND>(vj[sl], axis, rd);
+                            sp4[jan[sl]] = v;
+                            pnew[sl] = proj<KIND>(v, axis, rd);
+                        }
+                    }
+                }
+                uint32_t nok = 0;
+#pragma unroll
+                for (int sl = 0; sl < NS; ++sl) nok += ok[sl] ? 1u : 0u;
+                int pos = wave_reserve(&s_tail, nok);
+#pragma unroll
+                for (int sl = 0; sl < NS; ++sl) {
+                    if (!ok[sl]) continue;
+                    if (pos - head < WCAP) {
+                        s_ring[pos & (WCAP - 1)] = (int32_t)jn[sl];
+                        if (VEC) s_proj[pos & (WCAP - 1)] = pnew[sl];
+                    }
+                    if (VEC || pos - head >= WCAP) q[pos] = (int32_t)jn[sl];  // VEC: kept for the bitmap reset
+                    ++pos;
                 }
             }
             __syncthreads();
+            hprev = head;
             head = tail;
         }
         const int size = s_tail;
-        for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+        if (VEC) {
+            for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+        }
         __syncthreads();
         if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
     }
 }
 
 
-// LDS-resident variant for small int8 lattices (N <= 16384): the replica's
-// spins, its visited bitmap and a 16-bit cluster queue live in LDS for all
-// nclusters clusters; spins are written back once.  Same bond counters and
-// membership rule as k_wolff_regular, so the clusters are identical.
+// One wavefront per replica for small int8 lattices (N <= 16384): the
+// replica's spins and a 16-bit frontier ring (spilling to the global queue)
+// live in the wave's slice of LDS for all nclusters clusters and are written
+// back once.  A level is processed 64 frontier entries at a time in two
+// passes: (A) lanes take entries and list the neighbours that still hold the
+// cluster's old value (candidate bonds) in LDS; (B) lanes take candidates,
+// draw the bond's Philox uniform and claim the site.  Pass B runs one bond
+// per lane, so a narrow front costs one Philox evaluation per lane instead of
+// 2*DIM.  New entries are placed by wave-wide prefix counts; nothing
+// synchronises the workgroup.  Same bond counters and membership rule as
+// k_wolff_regular, so the clusters are identical.
 template <int DIM, int KIND>
-__global__ void __launch_bounds__(256) k_wolff_lds(char* __restrict__ base, int64_t stride, int L, int nclusters,
-                                                   uint32_t cl0, int rep0, int Q, const uint32_t* __restrict__ wthr,
-                                                   int64_t* __restrict__ csize, Key2 key)
+__global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int64_t stride, int L, int nrep,
+                                                    int nclusters, uint32_t cl0, int rep0, int Q,
+                                                    const uint32_t* __restrict__ wthr, int32_t* __restrict__ queue,
+                                                    const int64_t* __restrict__ rep_site_off,
+                                                    int64_t* __restrict__ csize, Key2 key, int cap, int rep_lds)
 {
+    constexpr int NCB = 64 * 2 * DIM;  // candidate list entries per pass A
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_tail;
-    const int r = blockIdx.x;
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int r = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
+    if (r >= nrep) return;  // whole wave; nothing below synchronises the workgroup
     const int grep = rep0 + r;
-    int N = L * L;
-    if (DIM == 3) N *= L;
-    const int half = N >> 1;
-    int8_t* S = reinterpret_cast<int8_t*>(smem);
-    uint32_t* vis = reinterpret_cast<uint32_t*>(smem + ((N + 15) & ~15));
-    uint16_t* q = reinterpret_cast<uint16_t*>(smem + ((N + 15) & ~15) + (((N + 31) / 32) * 4 + 15) / 16 * 16);
-    int8_t* gs = reinterpret_cast<int8_t*>(base + r * stride);
-    for (int i = threadIdx.x; i < N; i += blockDim.x) S[i] = gs[i];
-    for (int i = threadIdx.x; i < (N + 31) / 32; i += blockDim.x) vis[i] = 0u;
+    const uint32_t uL = (uint32_t)L;
+    const uint32_t LL = uL * uL;
+    const uint32_t N = (DIM == 3) ? LL * uL : LL;
+    const uint32_t half = N >> 1;
+    const float invL = 1.0f / (float)L;
+    unsigned char* mine = smem + (size_t)wave * rep_lds;
+    uint32_t* SW = reinterpret_cast<uint32_t*>(mine);
+    int8_t* S = reinterpret_cast<int8_t*>(mine);
+    uint32_t* cb_b = reinterpret_cast<uint32_t*>(mine + ((N + 15) & ~15u));  // bond id
+    uint16_t* cb_j = reinterpret_cast<uint16_t*>(cb_b + NCB);                 // natural index
+    uint16_t* cb_a = cb_j + NCB;                                              // colour-split address
+    uint16_t* ring = cb_a + NCB;
+    const uint32_t cmask = (uint32_t)cap - 1u;
+    uint32_t* gw = reinterpret_cast<uint32_t*>(base + r * stride);  // N is a multiple of 4 (L even)
+    int32_t* q = queue + rep_site_off[grep];
+    for (uint32_t i = lane; i < N / 4; i += 64) SW[i] = gw[i];
     const uint32_t tw = wthr[grep];
-    __syncthreads();
+    const uint64_t lt = __lanemask_lt();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+
     for (int c = 0; c < nclusters; ++c) {
         const uint32_t ncl = cl0 + (uint32_t)c;
         const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
-        const int seed = (int)(((uint64_t)sw.x * (uint64_t)N) >> 32);
-        const int seed_a = (int)split_addr<DIM>(L, half, seed);
+        const uint32_t seed = (uint32_t)(((uint64_t)sw.x * (uint64_t)N) >> 32);
+        const uint32_t seed_a = (uint32_t)split_addr<DIM>(L, half, seed);
         const int val = S[seed_a];
-        int newq = 0;
-        if (KIND == WK_POTTS) newq = (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            S[seed_a] = (KIND == WK_ISING) ? (int8_t)(-val) : (int8_t)newq;
-            q[0] = (uint16_t)seed;
-            vis[seed >> 5] |= 1u << (seed & 31);
-            s_tail = 1;
+        const int newq = (KIND == WK_POTTS) ? (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q : -val;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        if (lane == 0) {
+            S[seed_a] = (int8_t)newq;
+            ring[0] = (uint16_t)seed;
         }
-        __syncthreads();
-        int head = 0;
-        while (true) {
-            const int tail = s_tail;
-            __syncthreads();
-            if (head >= tail) break;
-            for (int idx = head + (int)threadIdx.x; idx < tail; idx += blockDim.x) {
-                const int cur = q[idx];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        int head = 0, hprev = 0, tail = 1;
+        while (head < tail) {
+            int next = tail;
+            for (int chunk = head; chunk < tail; chunk += 64) {
+                // pass A: candidate bonds of up to 64 frontier entries
+                const int idx = chunk + lane;
+                const bool act = idx < tail;
+                uint32_t cur = 0;
+                if (act) cur = (idx - hprev < cap) ? (uint32_t)ring[(uint32_t)idx & cmask] : (uint32_t)q[idx];
+                uint32_t x[3] = {0u, 0u, 0u};
+                {
+                    uint32_t rem;
+                    const uint32_t r1 = fdiv_u24(cur, uL, invL, rem);
+                    x[0] = rem;
+                    if (DIM == 3) x[2] = fdiv_u24(r1, uL, invL, x[1]);
+                    else x[1] = r1;
+                }
+                const uint32_t nbase = ((x[0] + x[1] + x[2]) & 1u) ? 0u : half;
+                int ncand = 0;
 #pragma unroll
                 for (int sl = 0; sl < 2 * DIM; ++sl) {
-                    const int j = (int)nat_nbr<DIM>(L, cur, sl);
-                    const int ja = (int)split_addr<DIM>(L, half, j);
-                    if (S[ja] != val) continue;
-                    const int owner = (sl & 1) ? j : cur;
-                    const uint32_t b = (uint32_t)owner * DIM + (uint32_t)(sl >> 1);
-                    const uint4 bw = philox4x32_10(b >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
-                    if (!(u4_get(bw, (int)(b & 3u)) < tw)) continue;
-                    const uint32_t bit = 1u << (j & 31);
-                    if (atomicOr(&vis[j >> 5], bit) & bit) continue;
-                    S[ja] = (KIND == WK_ISING) ? (int8_t)(-val) : (int8_t)newq;
-                    q[atomicAdd(&s_tail, 1)] = (uint16_t)j;
+                    const int d = sl >> 1;
+                    const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
+                    uint32_t j;
+                    if ((sl & 1) == 0) j = (x[d] == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
+                    else j = (x[d] == 0) ? cur + (uL - 1) * pw : cur - pw;
+                    const uint32_t ja = nbase + (j >> 1);
+                    const bool cand = act && S[ja] == val;
+                    const uint64_t m = __ballot(cand);
+                    if (cand) {
+                        const int p = ncand + __popcll(m & lt);
+                        cb_b[p] = ((sl & 1) ? j : cur) * (uint32_t)DIM + (uint32_t)d;
+                        cb_j[p] = (uint16_t)j;
+                        cb_a[p] = (uint16_t)ja;
+                    }
+                    ncand += __popcll(m);
                 }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                // pass B: one candidate bond per lane
+                for (int cc = 0; cc < ncand; cc += 64) {
+                    const int k = cc + lane;
+                    bool ok = false;
+                    uint32_t j = 0;
+                    if (k < ncand) {
+                        const uint32_t b = cb_b[k];
+                        j = cb_j[k];
+                        const uint32_t ja = cb_a[k];
+                        const uint4 bw = philox4x32_10(b >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                        if (u4_get(bw, (int)(b & 3u)) < tw) {
+                            const uint32_t sh = (ja & 3u) * 8u;
+                            if (KIND == WK_ISING) {
+                                const uint32_t mm = 0xFEu << sh;
+                                const uint32_t old = val > 0 ? atomicOr(&SW[ja >> 2], mm) : atomicAnd(&SW[ja >> 2], ~mm);
+                                ok = (int8_t)(old >> sh) == (int8_t)val;
+                            } else {
+                                uint32_t seen = SW[ja >> 2];
+                                while ((int8_t)(seen >> sh) == (int8_t)val) {
+                                    const uint32_t nw = (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh);
+                                    const uint32_t prev = atomicCAS(&SW[ja >> 2], seen, nw);
+                                    if (prev == seen) {
+                                        ok = true;
+                                        break;
+                                    }
+                                    seen = prev;
+                                }
+                            }
+                        }
+                    }
+                    const uint64_t m = __ballot(ok);
+                    if (ok) {
+                        const int pos = next + __popcll(m & lt);
+                        if (pos - head < cap) ring[(uint32_t)pos & cmask] = (uint16_t)j;
+                        else q[pos] = (int32_t)j;
+                    }
+                    next += __popcll(m);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             }
-            __syncthreads();
+            hprev = head;
             head = tail;
+            tail = next;
         }
-        const int size = s_tail;
-        for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
-        __syncthreads();
-        if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
+        if (lane == 0 && csize) csize[(int64_t)grep * nclusters + c] = tail;
     }
-    for (int i = threadIdx.x; i < N; i += blockDim.x) gs[i] = S[i];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    for (uint32_t i = lane; i < N / 4; i += 64) gw[i] = SW[i];
 }
 
 }  // namespace
@@ -272,10 +453,18 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
     const float J = (float)st->p.J;
     const int64_t N = g.lat->N;
     if ((kind == WK_ISING || kind == WK_POTTS) && N <= 16384 && st->kernel_choice == 0) {
-        const size_t lds = (size_t)((N + 15) & ~15ll) + (size_t)(((N + 31) / 32) * 4 + 15) / 16 * 16 + (size_t)N * 2;
-#define MQ_WL(D, K)                                                                                                   \
-    hipLaunchKernelGGL((k_wolff_lds<D, K>), grid, block, lds, s, base, g.stride, L, nclusters, cl0, g.rep0, st->p.q, \
-                       st->d_wthr, st->d_csize, key)
+        // ring: the whole cluster when N <= wolff_ring (512 by default), else
+        // wolff_ring entries with spill to the global queue
+        int cap = 64;
+        while (cap < N && cap < st->wolff_ring) cap <<= 1;
+        const int ncb = 64 * 2 * dim;  // candidate list: uint32 bond + two uint16 per entry
+        const int rep_lds = (int)(((N + 15) & ~15ll) + 8 * ncb + (2 * cap + 15) / 16 * 16);
+        const int wpb = std::max(1, std::min(4, 65536 / rep_lds));
+        const dim3 wgrid((unsigned)((g.nrep + wpb - 1) / wpb)), wblock((unsigned)(64 * wpb));
+        const size_t lds = (size_t)wpb * (size_t)rep_lds;
+#define MQ_WL(D, K)                                                                                                 \
+    hipLaunchKernelGGL((k_wolff_wave<D, K>), wgrid, wblock, lds, s, base, g.stride, L, g.nrep, nclusters, cl0,     \
+                       g.rep0, st->p.q, st->d_wthr, st->d_queue, st->d_rep_site_off, st->d_csize, key, cap, rep_lds)
         if (dim == 3) {
             if (kind == WK_ISING) MQ_WL(3, WK_ISING);
             else MQ_WL(3, WK_POTTS);

@@ -78,6 +78,90 @@ __device__ __forceinline__ uint32_t fdiv_u24(uint32_t n, uint32_t d, float inv, 
 __device__ __forceinline__ float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }
 
 // ---------------------------------------------------------------------------
+// Wolff cluster membership on int8 spins.  A cluster claims a site by changing
+// its spin byte with one atomic on the containing 32-bit word (spin buffers are
+// 256-byte aligned, so the word never leaves the buffer); the returned old
+// byte says whether this lane made the change.  During one cluster a byte only
+// ever goes from the cluster's old value to its new one, so "still equal to
+// the old value" is exactly "not yet in the cluster" and no visited bitmap is
+// needed.
+__device__ __forceinline__ uint32_t* spin_word(int8_t* p, uint32_t& sh)
+{
+    sh = ((uint32_t)(uintptr_t)p & 3u) * 8u;
+    return reinterpret_cast<uint32_t*>((uintptr_t)p & ~(uintptr_t)3);
+}
+
+// Ising bytes are +1 = 0x01 and -1 = 0xFF: OR 0xFE takes +1 to -1 and AND 0x01
+// takes -1 to +1, each leaving the other value unchanged.
+__device__ __forceinline__ bool claim_ising(int8_t* p, int val)
+{
+    uint32_t sh;
+    uint32_t* w = spin_word(p, sh);
+    const uint32_t m = 0xFEu << sh;
+    const uint32_t old = val > 0 ? atomicOr(w, m) : atomicAnd(w, ~m);
+    return (int8_t)(old >> sh) == (int8_t)val;
+}
+
+// Current word holding the byte at p (relaxed atomic load).
+__device__ __forceinline__ uint32_t peek_word(int8_t* p, uint32_t& sh)
+{
+    uint32_t* w = spin_word(p, sh);
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Issue-only forms: the caller collects the old words of several claims before
+// testing any of them, so their round trips overlap.
+__device__ __forceinline__ uint32_t claim_ising_issue(int8_t* p, int val)
+{
+    uint32_t sh;
+    uint32_t* w = spin_word(p, sh);
+    const uint32_t m = 0xFEu << sh;
+    return val > 0 ? atomicOr(w, m) : atomicAnd(w, ~m);
+}
+
+__device__ __forceinline__ int8_t byte_at(uint32_t w, const int8_t* p)
+{
+    return (int8_t)(w >> (((uint32_t)(uintptr_t)p & 3u) * 8u));
+}
+
+// Reserves n (< 16) consecutive entries for each active lane of the wavefront
+// with a single LDS atomic on *tail, placing the lanes in lane order; returns
+// this lane's first entry.  Every active lane of the wavefront must call it.
+__device__ __forceinline__ int wave_reserve(int* tail, uint32_t n)
+{
+    const uint64_t lt = __lanemask_lt();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint64_t m = __ballot((int)((n >> b) & 1u));
+        pre += __popcll(m & lt) << b;
+        tot += __popcll(m) << b;
+    }
+    const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+    int base = 0;
+    if ((int)__lane_id() == leader && tot > 0) base = atomicAdd(tail, tot);
+    base = __shfl(base, leader);
+    return base + pre;
+}
+
+// Potts relabel val -> newq by compare-and-swap, starting from a word seen
+// earlier; returns false once the byte no longer holds val.  Terminates: every
+// failed swap means the word changed, and each byte changes at most once per
+// cluster.
+__device__ __forceinline__ bool claim_potts(int8_t* p, uint32_t seen, int val, int newq)
+{
+    uint32_t sh;
+    uint32_t* w = spin_word(p, sh);
+    while ((int8_t)(seen >> sh) == (int8_t)val) {
+        const uint32_t nw = (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh);
+        const uint32_t prev = atomicCAS(w, seen, nw);
+        if (prev == seen) return true;
+        seen = prev;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
 struct mq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -178,6 +262,7 @@ struct mq_state {
     int tune_xcd = 1;                   // XCD-aware block remap of the row-vectorised kernel
     int lds_block = 256;                // threads per block of the LDS multi-sweep kernel
     int kernel_choice = 0;              // 0 auto, 1 streaming kernels only, 2 generic streaming kernel only  // force the generic kernels (parity of the specialised ones)
+    int wolff_ring = 512;               // LDS frontier ring entries of the small-lattice Wolff kernel (power of two, 64..2048)
 };
 
 // --------------------------------------------------------------------------

@@ -129,3 +129,50 @@ def test_csr_octet_kernel_bit_exact(ctx, model):
     for r in range(R):
         exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(N, 14, r), betas[r], -1.0, 4, 15, 2, r, bonds=bonds)
         assert np.array_equal(st.download(r), exp), r
+
+
+def _random_graph(n, mean_degree, seed):
+    """Erdos-Renyi-like simple undirected graph as sorted CSR (no geometry, so BFS
+    fronts grow geometrically)."""
+    rng = np.random.default_rng(seed)
+    m = int(n * mean_degree / 2)
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    keep = a != b
+    u = np.minimum(a[keep], b[keep]).astype(np.int64)
+    v = np.maximum(a[keep], b[keep]).astype(np.int64)
+    pairs = np.unique(u * n + v)
+    u, v = pairs // n, pairs % n
+    src = np.concatenate([u, v])
+    dst = np.concatenate([v, u])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, src + 1, 1)
+    return np.cumsum(rp).astype(np.int32), dst.astype(np.int32)
+
+
+@pytest.mark.parametrize("model", ["ising", "potts"])
+def test_csr_wolff_frontier_spill_bit_exact(ctx, model):
+    """A random graph of mean degree 16 in an ordered start with bond probability
+    1 - e^-2: BFS fronts of thousands of vertices overflow the 4096-entry LDS ring and
+    spill to the global queue; the clusters must still equal the CPU replay."""
+    from marqov_amd import gpu
+    n = 24000
+    rp, ci = _random_graph(n, 16, 5)
+    lat = _lat(ctx, rp, ci)
+    st = gpu.State(ctx, model, [lat, lat], J=-1.0, q=3)
+    beta = 1.0 if model == "ising" else 2.0
+    st.set_beta(beta)
+    starts = []
+    for r in range(2):
+        s0 = np.ones(n, np.int8) if model == "ising" else np.zeros(n, np.int8)
+        s0[::11] = -1 if model == "ising" else 1
+        st.upload(r, s0)
+        starts.append(s0)
+    sizes = st.wolff(2, seed=3, cluster0=0)
+    for r in range(2):
+        exp, esz = ob.csr_wolff(rp, ci, starts[r], beta, -1.0, 2, 3, 0, r, potts=(model == "potts"))
+        assert esz.max() > n // 2
+        assert np.array_equal(sizes[r], esz), (sizes[r], esz)
+        assert np.array_equal(st.download(r), exp)

@@ -271,3 +271,35 @@ def test_wolff_lds_and_streaming_bit_exact(ctx, kernels, model, dim, L, monkeypa
             exp, esz = ob.wolff_potts(dim, L, ob.init_potts(N, 3, 30, r), 3, beta, -1.0, 7, 31, 3, r)
         assert np.array_equal(sizes[r], esz), (r, sizes[r], esz)
         assert np.array_equal(st.download(r), exp), r
+
+
+@pytest.mark.parametrize("model", ["ising", "potts"])
+@pytest.mark.parametrize("kernels,ring,dim,L", [("0", "64", 2, 32), ("0", "64", 3, 16), ("1", "2048", 3, 64)])
+def test_wolff_frontier_spill_bit_exact(ctx, model, kernels, ring, dim, L, monkeypatch):
+    """Ordered starts at large beta grow near-percolating clusters whose BFS fronts
+    overflow the LDS frontier ring (64 entries forced in the small-lattice kernel, 4096 in
+    the streaming kernel at 3D L=64): entries spilled to the global queue must leave the
+    clusters equal to the CPU replay."""
+    from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
+    monkeypatch.setenv("MARQOV_AMD_WOLFF_RING", ring)
+    N = L ** dim
+    lat = gpu.RegularLattice(ctx, dim, L)
+    st = gpu.State(ctx, model, [lat] * 2, J=-1.0, q=3)
+    beta = 1.0 if model == "ising" else 2.0  # bond probability 1 - e^-2 for both
+    st.set_beta(beta)
+    starts = []
+    for r in range(2):
+        s0 = (np.ones(N, np.int8) if model == "ising" else np.zeros(N, np.int8))
+        s0[np.random.default_rng(40 + r).random(N) < 0.1] = -1 if model == "ising" else 2  # 10% defects
+        st.upload(r, s0)
+        starts.append(s0)
+    sizes = st.wolff(3, seed=17, cluster0=1)
+    for r in range(2):
+        if model == "ising":
+            exp, esz = ob.wolff_ising(dim, L, starts[r], beta, -1.0, 3, 17, 1, r)
+        else:
+            exp, esz = ob.wolff_potts(dim, L, starts[r], 3, beta, -1.0, 3, 17, 1, r)
+        assert esz.max() > N // 2  # the fronts are far wider than the ring
+        assert np.array_equal(sizes[r], esz), (r, sizes[r], esz)
+        assert np.array_equal(st.download(r), exp), r
