@@ -89,6 +89,22 @@ hipError_t zero64(void* p, int64_t n, hipStream_t s)
     return hipGetLastError();
 }
 
+// measurement staging: per replica isum[8], fsum[8] (bit patterns) and the
+// current beta, so one copy brings everything to the host
+__global__ void k_pack_obs(const unsigned long long* __restrict__ isum, const double* __restrict__ fsum,
+                           const double* __restrict__ betad, int nrep, uint64_t* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nrep * 17) return;
+    const int64_t r = i / 17;
+    const int k = (int)(i - r * 17);
+    uint64_t v;
+    if (k < 8) v = isum[r * 8 + k];
+    else if (k < 16) v = (uint64_t)__double_as_longlong(fsum[r * 8 + (k - 8)]);
+    else v = (uint64_t)__double_as_longlong(betad[r]);
+    out[i] = v;
+}
+
 int check_state(mq_state* st)
 {
     if (!st) return MQ_ERR_ARG;
@@ -402,6 +418,8 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_rep_of_temp);
     dfree(st->d_pt_acc);
     dfree(st->d_energy);
+    dfree(st->d_pack);
+    if (st->h_pack) (void)hipHostFree(st->h_pack);
     dfree(st->d_ladder);
     dfree(st->d_dyn);
     if (st->d_msc_cls) (void)hipFree(st->d_msc_cls);
@@ -795,9 +813,16 @@ extern "C" int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, ui
     return MQ_OK;
 }
 
-static int measure_sums(mq_state* st, std::vector<long long>& is, std::vector<double>& fs)
+// Runs the measurement kernels and brings isum, fsum and the current betas to
+// the pinned host buffer h_pack ([nrep][17] words) with one copy.
+static int measure_sums(mq_state* st)
 {
     hipStream_t s = st->ctx->stream;
+    if (!st->d_pack) {
+        const size_t n = (size_t)st->nrep * 17;
+        MQ_HIP(st->ctx, dalloc(&st->d_pack, n));
+        MQ_HIP(st->ctx, hipHostMalloc(reinterpret_cast<void**>(&st->h_pack), n * sizeof(uint64_t), hipHostMallocDefault));
+    }
     MQ_HIP(st->ctx, zero64(st->d_isum, (int64_t)st->nrep * 8, s));
     MQ_HIP(st->ctx, zero64(st->d_fsum, (int64_t)st->nrep * 8, s));
     for (const auto& g : st->groups) {
@@ -806,10 +831,12 @@ static int measure_sums(mq_state* st, std::vector<long long>& is, std::vector<do
                        : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
         if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
     }
-    is.resize((size_t)st->nrep * 8);
-    fs.resize((size_t)st->nrep * 8);
-    MQ_HIP(st->ctx, hipMemcpyAsync(is.data(), st->d_isum, is.size() * 8, hipMemcpyDeviceToHost, s));
-    MQ_HIP(st->ctx, hipMemcpyAsync(fs.data(), st->d_fsum, fs.size() * 8, hipMemcpyDeviceToHost, s));
+    const int64_t n = (int64_t)st->nrep * 17;
+    hipLaunchKernelGGL(k_pack_obs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<const unsigned long long*>(st->d_isum), st->d_fsum, st->d_betad, st->nrep,
+                       st->d_pack);
+    MQ_HIP(st->ctx, hipGetLastError());
+    MQ_HIP(st->ctx, hipMemcpyAsync(st->h_pack, st->d_pack, sizeof(uint64_t) * (size_t)n, hipMemcpyDeviceToHost, s));
     MQ_HIP(st->ctx, hipStreamSynchronize(s));
     return MQ_OK;
 }
@@ -818,17 +845,17 @@ extern "C" int mq_measure(mq_state* st, double* out)
 {
     if (!st || !out) return MQ_ERR_ARG;
     if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
-    std::vector<long long> is;
-    std::vector<double> fs;
-    int rc = measure_sums(st, is, fs);
-    if (rc) return rc;
-    std::vector<double> bcur(st->nrep);
-    rc = mq_state_get_beta(st, bcur.data());
+    int rc = measure_sums(st);
     if (rc) return rc;
     for (int r = 0; r < st->nrep; ++r) {
         const double N = (double)st->lat[r]->N;
-        const long long* I = &is[(size_t)r * 8];
-        const double* F = &fs[(size_t)r * 8];
+        const uint64_t* P = st->h_pack + (size_t)r * 17;
+        long long I[8];
+        double F[8];
+        std::memcpy(I, P, sizeof(I));
+        std::memcpy(F, P + 8, sizeof(F));
+        double bcur_r;
+        std::memcpy(&bcur_r, P + 16, sizeof(double));
         double e = 0, m = 0;
         if (st->p.model == MQ_POTTS) {
             e = st->p.J * (double)I[0] / N;
@@ -841,7 +868,7 @@ extern "C" int mq_measure(mq_state* st, double* out)
         } else {
             e = st->p.J * F[0] / N;
             if (st->p.model == MQ_PHI4) {
-                const double b = bcur[r];
+                const double b = bcur_r;
                 e += (st->p.mass / b) * F[4] / N + (st->p.lambda / b) * F[5] / N;
             }
             m = std::sqrt(F[1] * F[1] + F[2] * F[2] + F[3] * F[3]) / N;

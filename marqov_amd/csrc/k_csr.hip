@@ -2,8 +2,8 @@
 // Metropolis sweeps, frontier-BFS Wolff and FP64 observables on the device.
 //
 // Layout: vertices are renumbered so that every colour class is contiguous
-// (position = class offset + rank by original id); the device CSR holds
-// positions.  The replicas of one graph are interleaved: the spin of replica r
+// (position = class offset + breadth-first rank within the class); the device
+// CSR holds positions.  The replicas of one graph are interleaved: the spin of replica r
 // at position p sits at p * R + r (R replicas), so a wavefront sweeping one
 // vertex for 64 replicas makes 64-byte coalesced neighbour reads while the CSR
 // row and column indices are wave-uniform loads shared by all replicas.
@@ -59,6 +59,21 @@ __global__ void __launch_bounds__(256) k_csr_init(char* __restrict__ base, int64
     }
 }
 
+// Neighbours are visited in chunks of NBCH: the chunk's column indices are
+// loaded together, then its spins, so a vertex costs two dependent memory
+// round trips per chunk instead of two per neighbour.
+constexpr int NBCH = 8;
+
+__device__ __forceinline__ void gather_i8(const int8_t* s, int64_t ps, const int32_t* __restrict__ ci, int k0, int b,
+                                          int (&v)[NBCH])
+{
+    int nb[NBCH];
+#pragma unroll
+    for (int t = 0; t < NBCH; ++t) nb[t] = k0 + t < b ? ci[k0 + t] : -1;
+#pragma unroll
+    for (int t = 0; t < NBCH; ++t) v[t] = nb[t] >= 0 ? (int)s[(int64_t)nb[t] * ps] : 0;
+}
+
 // one thread per (vertex of the colour class, replica), replica fastest
 template <bool BONDS>
 __global__ void __launch_bounds__(512) k_csr_sweep_int8(int8_t* __restrict__ base, int64_t rs, int64_t ps, int R,
@@ -86,9 +101,12 @@ __global__ void __launch_bounds__(512) k_csr_sweep_int8(int8_t* __restrict__ bas
         const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
         const int snew = (int)(((uint64_t)w.x * (uint64_t)Q) >> 32);
         int dn = 0;
-        for (int k = a; k < b; ++k) {
-            const int v = s[(int64_t)ci[k] * ps];
-            dn += (v == snew) - (v == si);
+        for (int k0 = a; k0 < b; k0 += NBCH) {
+            int v[NBCH];
+            gather_i8(s, ps, ci, k0, b, v);
+#pragma unroll
+            for (int t = 0; t < NBCH; ++t)
+                if (k0 + t < b) dn += (v[t] == snew) - (v[t] == si);
         }
         const int e = dn + qmax;
         const uint64_t th = ((uint64_t)T[2 * e] << 16) | T[2 * e + 1];
@@ -96,7 +114,13 @@ __global__ void __launch_bounds__(512) k_csr_sweep_int8(int8_t* __restrict__ bas
     } else {
         int h = 0;
         const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
-        for (int k = a; k < b; ++k) h += (BONDS ? (int)bb[k] : 1) * s[(int64_t)ci[k] * ps];
+        for (int k0 = a; k0 < b; k0 += NBCH) {
+            int v[NBCH];
+            gather_i8(s, ps, ci, k0, b, v);
+#pragma unroll
+            for (int t = 0; t < NBCH; ++t)
+                if (k0 + t < b) h += (BONDS ? (int)bb[k0 + t] : 1) * v[t];
+        }
         const int e = si * h + qmax;
         const uint4 w8 = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep >> 3, ST_METRO, key);
         const int m = grep & 7;
@@ -132,12 +156,21 @@ __global__ void __launch_bounds__(512) k_csr_sweep_o3(float4* __restrict__ base,
     const int a = rp[pos], b = rp[pos + 1];
     float nx = 0.f, ny = 0.f, nz = 0.f;
     const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
-    for (int k = a; k < b; ++k) {
-        const float4 v = s[(int64_t)ci[k] * ps];
-        const float f = BONDS ? (float)bb[k] : 1.0f;
-        nx += f * v.x;
-        ny += f * v.y;
-        nz += f * v.z;
+    for (int k0 = a; k0 < b; k0 += NBCH) {
+        int nb[NBCH];
+        float4 v[NBCH];
+#pragma unroll
+        for (int t = 0; t < NBCH; ++t) nb[t] = k0 + t < b ? ci[k0 + t] : -1;
+#pragma unroll
+        for (int t = 0; t < NBCH; ++t) v[t] = nb[t] >= 0 ? s[(int64_t)nb[t] * ps] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int t = 0; t < NBCH; ++t) {
+            if (nb[t] < 0) continue;
+            const float f = BONDS ? (float)bb[k0 + t] : 1.0f;
+            nx += f * v[t].x;
+            ny += f * v[t].y;
+            nz += f * v[t].z;
+        }
     }
     const float4 so = s[pos * ps];
     const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
@@ -170,8 +203,9 @@ __global__ void __launch_bounds__(512) k_csr_sweep_o3(float4* __restrict__ base,
 // down neighbours byte-wise ((word ^ bond sign) & 0x02 per byte, doubled count
 // <= 2*maxdeg <= 254), forms the table index e = q + qmax per byte, and accepts
 // with the 8 x 16-bit uniforms of one Philox block (tie -> TIE stream).  The
-// group's acceptance tables are staged in LDS.  Same decisions as the per-replica
-// kernel above.
+// thresholds are read through the caches (a block sweeps only 256/(R/8)
+// vertices, too few to amortise staging all R tables in LDS).  Same decisions
+// as the per-replica kernel above.
 template <bool BONDS>
 __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ base, int R,
                                                           const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
@@ -182,13 +216,7 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
                                                           int64_t nc, uint32_t phase, const uint32_t* __restrict__ dyn,
                                                           int rep0, Key2 key)
 {
-    extern __shared__ uint32_t s_tab[];  // [R][tbl] high halves, then [R][tbl] low halves
     if (dyn) phase += dyn[0];
-    for (int i = threadIdx.x; i < R * tbl; i += blockDim.x) {
-        s_tab[i] = thr[((int64_t)rep0 * tbl + i) * 2];
-        s_tab[R * tbl + i] = thr[((int64_t)rep0 * tbl + i) * 2 + 1];
-    }
-    __syncthreads();
     const int oct = R >> 3;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nc * oct) return;
@@ -200,11 +228,21 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
     const int deg = b - a;
     const int8_t* bb = BONDS ? bonds + rep_bond_off[rep0 + 8 * o] : nullptr;
     uint32_t n0 = 0u, n1 = 0u;
-    for (int k = a; k < b; ++k) {
-        const uint2 v = *reinterpret_cast<const uint2*>(base + (int64_t)ci[k] * R + 8 * o);
-        const uint32_t f = (BONDS && bb[k] < 0) ? 0xFEFEFEFEu : 0u;  // -x = x ^ 0xFE for x = +-1
-        n0 += (v.x ^ f) & 0x02020202u;
-        n1 += (v.y ^ f) & 0x02020202u;
+    for (int k0 = a; k0 < b; k0 += NBCH) {
+        int nb[NBCH];
+        uint2 v[NBCH];
+#pragma unroll
+        for (int t = 0; t < NBCH; ++t) nb[t] = k0 + t < b ? ci[k0 + t] : -1;
+#pragma unroll
+        for (int t = 0; t < NBCH; ++t)
+            v[t] = nb[t] >= 0 ? *reinterpret_cast<const uint2*>(base + (int64_t)nb[t] * R + 8 * o) : make_uint2(0u, 0u);
+#pragma unroll
+        for (int t = 0; t < NBCH; ++t) {
+            if (nb[t] < 0) continue;
+            const uint32_t f = (BONDS && bb[k0 + t] < 0) ? 0xFEFEFEFEu : 0u;  // -x = x ^ 0xFE for x = +-1
+            n0 += (v[t].x ^ f) & 0x02020202u;
+            n1 += (v[t].y ^ f) & 0x02020202u;
+        }
     }
     uint2* ownp = reinterpret_cast<uint2*>(base + pos * R + 8 * o);
     const uint2 S = *ownp;
@@ -216,6 +254,7 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
     };
     const uint32_t E0 = eword(S.x, n0), E1 = eword(S.y, n1);
     const int grep0 = rep0 + 8 * o;
+    const uint32_t* T = thr + (int64_t)grep0 * tbl * 2;  // the lane's 8 replicas' (hi, lo) rows
     const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep0 >> 3, ST_METRO, key);
     uint32_t acc = 0u;
     bool tie = false;
@@ -223,7 +262,7 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
     for (int m = 0; m < 8; ++m) {
         const uint32_t e = ((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu;
         const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
-        const uint32_t th = s_tab[(8 * o + m) * tbl + e];
+        const uint32_t th = T[((int64_t)m * tbl + e) * 2];
         const uint32_t tt = th > 0xFFFFu ? 0xFFFFu : th;  // "always" (65536) as 0xFFFF + lo 0x10000
         acc |= (u < tt ? 1u : 0u) << m;
         tie |= (u == tt);
@@ -236,9 +275,9 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
             const uint32_t e = ((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu;
             const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
             const uint32_t lo = (u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
-            const uint32_t th = s_tab[(8 * o + m) * tbl + e];
+            const uint32_t th = T[((int64_t)m * tbl + e) * 2];
             const uint32_t tt = th > 0xFFFFu ? 0xFFFFu : th;
-            const uint32_t tl = th > 0xFFFFu ? 0x10000u : s_tab[R * tbl + (8 * o + m) * tbl + e];
+            const uint32_t tl = th > 0xFFFFu ? 0x10000u : T[((int64_t)m * tbl + e) * 2 + 1];
             acc |= ((u < tt) || (u == tt && lo < tl) ? 1u : 0u) << m;
         }
     }
@@ -268,11 +307,11 @@ __global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ ba
     __shared__ double red[8][4];
     const int r = blockIdx.y;
     const int grep = rep0 + r;
-    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (pos < N) {
+    const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
+    // grid-stride over positions: a few blocks per replica keep the atomics few
+    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < N; pos += (int64_t)gridDim.x * blockDim.x) {
         const int a = rp[pos], b = rp[pos + 1];
-        const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
         if (VEC) {
             const float4* s = reinterpret_cast<const float4*>(base) + r * rs;
             const float4 v = s[pos * ps];
@@ -283,12 +322,12 @@ __global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ ba
                 e += (BONDS ? (double)bb[k] : 1.0) * ((double)v.x * u.x + (double)v.y * u.y + (double)v.z * u.z);
             }
             const double n2 = (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z;
-            acc[0] = e;
-            acc[1] = v.x;
-            acc[2] = v.y;
-            acc[3] = v.z;
-            acc[4] = n2;
-            acc[5] = (n2 - 1) * (n2 - 1);
+            acc[0] += e;
+            acc[1] += v.x;
+            acc[2] += v.y;
+            acc[3] += v.z;
+            acc[4] += n2;
+            acc[5] += (n2 - 1) * (n2 - 1);
         } else {
             const int8_t* s = reinterpret_cast<const int8_t*>(base) + r * rs;
             const int si = s[pos * ps];
@@ -299,11 +338,13 @@ __global__ void __launch_bounds__(256) k_csr_measure(const char* __restrict__ ba
                 if (model == MQ_POTTS) e += (si == sj);
                 else e += (long long)(BONDS ? bb[k] : 1) * si * sj;
             }
-            acc[0] = (double)e;
+            acc[0] += (double)e;  // integer-valued doubles: exact below 2^53
             if (model == MQ_POTTS) {
-                for (int q = 0; q < Q && q < 7; ++q) acc[1 + q] = (si == q);
+#pragma unroll
+                for (int q = 0; q < 7; ++q)  // static indices keep acc in registers
+                    if (q < Q) acc[1 + q] += (si == q);
             } else {
-                acc[1] = si;
+                acc[1] += si;
             }
         }
     }
@@ -609,11 +650,36 @@ extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, co
     l->vert_of.resize(N);
     l->colour.assign(colour.begin(), colour.end());
     {
+        // Within a class, vertices are placed in breadth-first order (every
+        // component, seeds in id order): graph neighbours get nearby ranks, so
+        // the neighbours of a run of consecutive positions fall in a few short
+        // windows of the other classes and the gathers hit L2.  Results do not
+        // depend on this order (random numbers are keyed by original ids).
+        std::vector<int32_t> order;
+        order.reserve(N);
+        std::vector<char> seen(N, 0);
+        for (int64_t s0 = 0; s0 < N; ++s0) {
+            if (seen[s0]) continue;
+            seen[s0] = 1;
+            size_t head = order.size();
+            order.push_back((int32_t)s0);
+            while (head < order.size()) {
+                const int32_t u = order[head++];
+                for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) {
+                    const int32_t v = col_idx[k];
+                    if (!seen[v]) {
+                        seen[v] = 1;
+                        order.push_back(v);
+                    }
+                }
+            }
+        }
         std::vector<int64_t> next(l->color_off.begin(), l->color_off.end() - 1);
-        for (int64_t u = 0; u < N; ++u) {
+        for (int64_t t = 0; t < N; ++t) {
+            const int32_t u = order[t];
             const int64_t p = next[colour[u]]++;
             l->pos_of[u] = (int32_t)p;
-            l->vert_of[p] = (int32_t)u;
+            l->vert_of[p] = u;
         }
     }
     l->h_rp.assign(row_ptr, row_ptr + N + 1);
@@ -701,8 +767,7 @@ hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t p
                                st->p.proposal, J, d, lam, m, phase, st->cur_dyn, g.rep0, key);
     } else {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
-        const size_t tab = (size_t)g.nrep * st->tbl * 8;
-        bool oct_ok = model != MQ_POTTS && g.nrep % 8 == 0 && g.rep0 % 8 == 0 && l->maxdeg <= 127 && tab <= 32768 &&
+        bool oct_ok = model != MQ_POTTS && g.nrep % 8 == 0 && g.rep0 % 8 == 0 && l->maxdeg <= 127 &&
                       st->kernel_choice != 2;
         if (oct_ok && bonds)  // the 8 replicas of a lane must share one coupling sample
             for (int r = g.rep0; r < g.rep0 + g.nrep && oct_ok; ++r)
@@ -711,11 +776,11 @@ hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t p
             const int64_t w8 = nc * (g.nrep / 8);
             const dim3 g8((unsigned)((w8 + 255) / 256));
             if (bonds)
-                hipLaunchKernelGGL(k_csr_sweep_ising8<true>, g8, dim3(256), tab, s, b, g.nrep, l->d_rp, l->d_ci,
+                hipLaunchKernelGGL(k_csr_sweep_ising8<true>, g8, dim3(256), 0, s, b, g.nrep, l->d_rp, l->d_ci,
                                    l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc, phase,
                                    st->cur_dyn, g.rep0, key);
             else
-                hipLaunchKernelGGL(k_csr_sweep_ising8<false>, g8, dim3(256), tab, s, b, g.nrep, l->d_rp, l->d_ci,
+                hipLaunchKernelGGL(k_csr_sweep_ising8<false>, g8, dim3(256), 0, s, b, g.nrep, l->d_rp, l->d_ci,
                                    l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc, phase,
                                    st->cur_dyn, g.rep0, key);
             return hipGetLastError();
@@ -738,7 +803,9 @@ hipError_t launch_measure_csr(mq_state* st, const Group& g)
     const bool vec = st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4;
     const bool bonds = st->d_bonds != nullptr;
     hipStream_t s = st->ctx->stream;
-    const dim3 grid = grid_for(l->N, g.nrep);
+    const int64_t per_rep = std::max<int64_t>(1, std::min<int64_t>((l->N + 255) / 256,
+                                                                   (8ll * st->ctx->num_cus + g.nrep - 1) / g.nrep));
+    const dim3 grid((unsigned)per_rep, (unsigned)g.nrep, 1);
 #define MQ_M(B, V)                                                                                                  \
     hipLaunchKernelGGL((k_csr_measure<B, V>), grid, dim3(256), 0, s, st->d_spins + g.off, (int64_t)1,             \
                        (int64_t)g.nrep, l->d_rp,                                                                 \

@@ -586,11 +586,13 @@ __global__ void __launch_bounds__(256) k_measure_int8(const int8_t* __restrict__
     const int r = blockIdx.y;
     const int grep = rep0 + r;
     const RegGeom G = reg_geom(DIM, L);
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // grid-stride over the colour-0 half: a few blocks per replica, so the
+    // per-block atomics stay few
     long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (h < G.half) {
-        const int8_t* c0 = base + r * stride;
-        const int8_t* c1 = c0 + G.half;
+    const int8_t* c0 = base + r * stride;
+    const int8_t* c1 = c0 + G.half;
+    const int8_t* bs = BONDS ? bonds + rep_bond_off[grep] : nullptr;
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < G.half; h += (int64_t)gridDim.x * blockDim.x) {
         int64_t nb[2 * DIM];
         reg_nbrs<DIM>(G, 0, h, nb);
         const int s = c0[h];
@@ -598,26 +600,106 @@ __global__ void __launch_bounds__(256) k_measure_int8(const int8_t* __restrict__
             int eq = 0;
 #pragma unroll
             for (int sl = 0; sl < 2 * DIM; ++sl) eq += (c1[nb[sl]] == s);
-            acc[0] = eq;
-            for (int q = 0; q < Q && q < 7; ++q) acc[1 + q] = (s == q) + (c1[h] == q);
+            acc[0] += eq;
+            const int t = c1[h];
+#pragma unroll
+            for (int q = 0; q < 7; ++q)  // static indices keep acc in registers
+                if (q < Q) acc[1 + q] += (s == q) + (t == q);
         } else {
             int hs = 0;
 #pragma unroll
             for (int sl = 0; sl < 2 * DIM; ++sl) {
                 int b = 1;
                 if (BONDS) {
-                    const int8_t* bs = bonds + rep_bond_off[grep];
                     const int d = sl >> 1;
                     b = (sl & 1) ? bs[(int64_t)DIM * G.half + (int64_t)d * G.half + nb[sl]] : bs[(int64_t)d * G.half + h];
                 }
                 hs += b * c1[nb[sl]];
             }
-            acc[0] = s * hs;
-            acc[1] = s + c1[h];
+            acc[0] += s * hs;
+            acc[1] += s + c1[h];
         }
     }
     block_sum_atomic<long long, 8>(acc, reinterpret_cast<long long*>(isum) + (int64_t)grep * 8);
 }
+
+// Observables of 3D Ising without couplings, L a power of two >= 32, with the
+// loads of k_sweep_ising3_row16 for colour 0: a thread takes 16 colour-0 sites
+// of one row, forms 2*aligned per byte from the same SWAR neighbour sums (every
+// bond is seen once, from its colour-0 end), and counts the down spins of its
+// chunk in both colours.  isum[r*8] += sum over bonds s_i s_j, isum[r*8+1] +=
+// sum s.
+__global__ void __launch_bounds__(256) k_measure_ising3_row16(const int8_t* __restrict__ base, int64_t stride, int lgL,
+                                                              int rep0, int lgbpr, int lgit,
+                                                              unsigned long long* __restrict__ isum)
+{
+    const uint32_t lb = blockIdx.x;
+    const uint32_t r = lb >> lgbpr;
+    const uint32_t grep = (uint32_t)rep0 + r;
+    const uint32_t L = 1u << lgL;
+    const uint32_t H = L >> 1;
+    const uint32_t lgcpr = (uint32_t)lgL - 5u;
+    const uint32_t lgplane = 2u * (uint32_t)lgL - 1u;
+    const uint32_t plane = 1u << lgplane;
+    const uint32_t half = plane << lgL;
+    const int8_t* own = base + (int64_t)r * stride;
+    const int8_t* oth = own + half;
+    const uint32_t M2 = 0x02020202u;
+    long long v[2] = {0, 0};
+    // the block takes 2^lgit runs of 256 consecutive chunks, one atomic pair in the end
+    for (uint32_t it = 0; it < (1u << lgit); ++it) {
+        const uint32_t chunk = ((((lb - (r << lgbpr)) << lgit) + it) << 8) + threadIdx.x;
+        const uint32_t row = chunk >> lgcpr;
+        const uint32_t c = chunk & ((1u << lgcpr) - 1u);
+        const uint32_t y = row & (L - 1u);
+        const uint32_t z = row >> lgL;
+        const uint32_t p = (y + z) & 1u;
+        const uint32_t rb = row * H;
+        const uint32_t h0 = rb + (c << 4);
+        const uint4 S = *reinterpret_cast<const uint4*>(own + h0);
+        const uint4 O = *reinterpret_cast<const uint4*>(oth + h0);
+        const uint32_t cmax = (1u << lgcpr) - 1u;
+        const uint32_t xo = p ? rb + ((c == cmax) ? 0u : ((c + 1u) << 4)) : rb + ((c == 0u) ? H : (c << 4)) - 4u;
+        const uint32_t X = *reinterpret_cast<const uint32_t*>(oth + xo);
+        const uint32_t hym = (y == 0u) ? h0 + (L - 1u) * H : h0 - H;
+        const uint32_t hyp = (y == L - 1u) ? h0 - (L - 1u) * H : h0 + H;
+        const uint32_t hzm = (z == 0u) ? h0 + (L - 1u) * plane : h0 - plane;
+        const uint32_t hzp = (z == L - 1u) ? h0 - (L - 1u) * plane : h0 + plane;
+        const uint4 Ym = *reinterpret_cast<const uint4*>(oth + hym);
+        const uint4 Yp = *reinterpret_cast<const uint4*>(oth + hyp);
+        const uint4 Zm = *reinterpret_cast<const uint4*>(oth + hzm);
+        const uint4 Zp = *reinterpret_cast<const uint4*>(oth + hzp);
+        uint32_t xs0, xs1, xs2, xs3;
+        if (p) {
+            xs0 = __builtin_amdgcn_alignbyte(O.y, O.x, 1);
+            xs1 = __builtin_amdgcn_alignbyte(O.z, O.y, 1);
+            xs2 = __builtin_amdgcn_alignbyte(O.w, O.z, 1);
+            xs3 = __builtin_amdgcn_alignbyte(X, O.w, 1);
+        } else {
+            xs0 = __builtin_amdgcn_alignbyte(O.x, X, 3);
+            xs1 = __builtin_amdgcn_alignbyte(O.y, O.x, 3);
+            xs2 = __builtin_amdgcn_alignbyte(O.z, O.y, 3);
+            xs3 = __builtin_amdgcn_alignbyte(O.w, O.z, 3);
+        }
+        auto aligned2 = [&](uint32_t s, uint32_t o, uint32_t xs, uint32_t ym, uint32_t yp, uint32_t zm, uint32_t zp) {
+            const uint32_t n2 = (o & M2) + (xs & M2) + (ym & M2) + (yp & M2) + (zm & M2) + (zp & M2);
+            const uint32_t mask = ((s & M2) >> 1) * 0xFFu;
+            return (n2 & mask) | ((0x0C0C0C0Cu - n2) & ~mask);  // 2 * aligned neighbours, 0..12 per byte
+        };
+        const uint32_t a4 = aligned2(S.x, O.x, xs0, Ym.x, Yp.x, Zm.x, Zp.x) +
+                            aligned2(S.y, O.y, xs1, Ym.y, Yp.y, Zm.y, Zp.y) +
+                            aligned2(S.z, O.z, xs2, Ym.z, Yp.z, Zm.z, Zp.z) +
+                            aligned2(S.w, O.w, xs3, Ym.w, Yp.w, Zm.w, Zp.w);
+        const uint32_t t2 = (a4 & 0x00FF00FFu) + ((a4 >> 8) & 0x00FF00FFu);
+        // s_i h_i = aligned - anti-aligned = 2 aligned - 6 per site
+        v[0] += (long long)((t2 & 0xFFFFu) + (t2 >> 16)) - 96;
+        const int down = __popc(S.x & M2) + __popc(S.y & M2) + __popc(S.z & M2) + __popc(S.w & M2) +
+                         __popc(O.x & M2) + __popc(O.y & M2) + __popc(O.z & M2) + __popc(O.w & M2);
+        v[1] += 32 - 2 * down;
+    }
+    block_sum_atomic<long long, 2>(v, reinterpret_cast<long long*>(isum) + (int64_t)grep * 8);
+}
+
 
 // fsum[r*8 + 0] = sum over bonds b s_i.s_j, [1..3] = M, [4] = sum |s|^2, [5] = sum (|s|^2-1)^2
 template <int DIM, bool BONDS>
@@ -629,11 +711,11 @@ __global__ void __launch_bounds__(256) k_measure_f4(const float4* __restrict__ b
     const int r = blockIdx.y;
     const int grep = rep0 + r;
     const RegGeom G = reg_geom(DIM, L);
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (h < G.half) {
-        const float4* c0 = base + r * stride4;
-        const float4* c1 = c0 + G.half;
+    const float4* c0 = base + r * stride4;
+    const float4* c1 = c0 + G.half;
+    const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < G.half; h += (int64_t)gridDim.x * blockDim.x) {
         int64_t nb[2 * DIM];
         reg_nbrs<DIM>(G, 0, h, nb);
         const float4 s = c0[h];
@@ -644,20 +726,19 @@ __global__ void __launch_bounds__(256) k_measure_f4(const float4* __restrict__ b
             const float4 v = c1[nb[sl]];
             double b = 1.0;
             if (BONDS) {
-                const int8_t* bb = bonds + rep_bond_off[grep];
                 const int d = sl >> 1;
                 b = (sl & 1) ? bb[(int64_t)DIM * G.half + (int64_t)d * G.half + nb[sl]] : bb[(int64_t)d * G.half + h];
             }
             bs += b * ((double)s.x * v.x + (double)s.y * v.y + (double)s.z * v.z);
         }
-        acc[0] = bs;
-        acc[1] = (double)s.x + t.x;
-        acc[2] = (double)s.y + t.y;
-        acc[3] = (double)s.z + t.z;
+        acc[0] += bs;
+        acc[1] += (double)s.x + t.x;
+        acc[2] += (double)s.y + t.y;
+        acc[3] += (double)s.z + t.z;
         const double n2s = (double)s.x * s.x + (double)s.y * s.y + (double)s.z * s.z;
         const double n2t = (double)t.x * t.x + (double)t.y * t.y + (double)t.z * t.z;
-        acc[4] = n2s + n2t;
-        acc[5] = (n2s - 1) * (n2s - 1) + (n2t - 1) * (n2t - 1);
+        acc[4] += n2s + n2t;
+        acc[5] += (n2s - 1) * (n2s - 1) + (n2t - 1) * (n2t - 1);
     }
     block_sum_atomic<double, 6>(acc, fsum + (int64_t)grep * 8);
 }
@@ -797,7 +878,10 @@ hipError_t launch_measure_regular(mq_state* st, const Group& g)
     hipStream_t s = st->ctx->stream;
     const int model = st->p.model;
     const bool bonds = (model == MQ_EA_ISING) || (st->d_bonds != nullptr);
-    const dim3 grid = grid_for(half, 256, g.nrep);
+    // about 8 blocks per CU in total, at least one per replica
+    const int64_t per_rep = std::max<int64_t>(1, std::min<int64_t>((half + 255) / 256,
+                                                                   (8ll * st->ctx->num_cus + g.nrep - 1) / g.nrep));
+    const dim3 grid((unsigned)per_rep, (unsigned)g.nrep, 1);
     if (is_vec(model)) {
         const float4* b = reinterpret_cast<const float4*>(st->d_spins + g.off);
         const int64_t s4 = g.stride / 16;
@@ -811,6 +895,18 @@ hipError_t launch_measure_regular(mq_state* st, const Group& g)
     } else {
         const int8_t* b = reinterpret_cast<const int8_t*>(st->d_spins + g.off);
         unsigned long long* is = reinterpret_cast<unsigned long long*>(st->d_isum);
+        const bool pow2 = (L & (L - 1)) == 0;
+        if (model == MQ_ISING && dim == 3 && !bonds && pow2 && L >= 32 && L <= 1024 && st->kernel_choice != 2) {
+            int lgL = 0;
+            while ((1 << lgL) < L) ++lgL;
+            // (N/2)/16 chunks per replica = 2^(3 lgL - 5): blocks of 256 threads, up to
+            // 8 chunks per thread
+            const int lgit = std::min(3, 3 * lgL - 13);
+            const int lgbpr = 3 * lgL - 13 - lgit;
+            hipLaunchKernelGGL(k_measure_ising3_row16, dim3((unsigned)(((int64_t)1 << lgbpr) * g.nrep)), dim3(256), 0, s,
+                               b, g.stride, lgL, g.rep0, lgbpr, lgit, is);
+            return hipGetLastError();
+        }
         if (dim == 3) {
             if (bonds) hipLaunchKernelGGL((k_measure_int8<3, true>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds, st->d_rep_bond_off, L, model, st->p.q, g.rep0, is);
             else hipLaunchKernelGGL((k_measure_int8<3, false>), grid, dim3(256), 0, s, b, g.stride, st->d_bonds, st->d_rep_bond_off, L, model, st->p.q, g.rep0, is);

@@ -242,6 +242,8 @@ struct mq_state {
     int32_t* d_rep_of_temp = nullptr;  // [nrep] (grouped by ntemp)
     int32_t* d_pt_acc = nullptr;
     double* d_energy = nullptr;        // [nrep]
+    uint64_t* d_pack = nullptr;        // [nrep][17] measurement staging: isum[8], fsum[8] and beta as raw words
+    uint64_t* h_pack = nullptr;        // pinned host copy of d_pack
     std::vector<double> beta_ladder;   // per replica index at creation (temperature k)
     double* d_ladder = nullptr;
     int pt_ntemp = 0;

@@ -148,6 +148,24 @@ def test_measure_matches_oracle(ctx, model, dim, L):
         assert obs[r, 1] == pytest.approx(m, abs=1e-12)
 
 
+@pytest.mark.parametrize("L", [32, 64])
+def test_measure_row16_matches_oracle(ctx, L):
+    """3D Ising with L a power of two >= 32 is measured by the row-vectorised kernel:
+    bond sum and magnetisation must equal the oracle's from the downloaded spins."""
+    dim, N = 3, L ** 3
+    st = _state(ctx, "ising", dim, L, nrep=3, J=-1.0)
+    st.set_beta([0.1, 0.2216546, 0.5])
+    st.init(seed=12)
+    st.metropolis(4, seed=3)
+    obs = st.measure()
+    for r in range(3):
+        s = st.download(r)
+        e = -1.0 * ob.lib().orc_bond_sum_ising(dim, L, ob.p(s), None) / N
+        m = abs(int(s.astype(np.int64).sum())) / N
+        assert obs[r, 0] == pytest.approx(e, abs=1e-12), r
+        assert obs[r, 1] == pytest.approx(m, abs=1e-12), r
+
+
 @pytest.mark.parametrize("L,nsweeps", [(32, 6), (64, 2)])
 def test_ising3_row16_kernel_bit_exact(ctx, L, nsweeps):
     """L % 32 == 0 selects the row-vectorised 3D kernel; it must replay bit-exactly."""
