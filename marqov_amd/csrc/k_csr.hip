@@ -15,9 +15,10 @@
 //     counter (vertex, 64*sweep + colour, replica >> 3, METRO), 16-bit uniform
 //     (replica & 7) against the high half of the 32-bit threshold of
 //     dE = -2 J q (q = s_i sum b_ij s_j), the low half from the TIE stream on a
-//     tie; Potts uses counter (vertex, phase, replica, METRO): word 0 proposes
-//     (w0 * Q) >> 32, word 1 is compared with the threshold; O(3)/Phi4 use words
-//     0,1 for the direction and word 2 for acceptance.
+//     tie; Potts proposes (w * Q) >> 32 with word (replica & 3) of counter
+//     (vertex, phase, replica >> 2, PROPOSE) and accepts with the Ising rule's
+//     16-bit halves; O(3)/Phi4 use counter (vertex, phase, replica, METRO),
+//     words 0,1 for the direction and word 2 for acceptance.
 //   Wolff bonds: the edge {u, v} is keyed by the original CSR slot of v in row u
 //     with u < v (its canonical slot), counter (slot >> 2, cluster, replica, WBOND).
 #include <algorithm>
@@ -64,14 +65,22 @@ __global__ void __launch_bounds__(256) k_csr_init(char* __restrict__ base, int64
 // round trips per chunk instead of two per neighbour.
 constexpr int NBCH = 8;
 
+// Slots past the end of the row repeat the row's last slot, so every load of
+// a chunk is unconditional (the compiler then issues them back to back);
+// callers mask those slots out of the sums.
+__device__ __forceinline__ void chunk_cols(const int32_t* __restrict__ ci, int k0, int b, int (&nb)[NBCH])
+{
+#pragma unroll
+    for (int t = 0; t < NBCH; ++t) nb[t] = ci[min(k0 + t, b - 1)];
+}
+
 __device__ __forceinline__ void gather_i8(const int8_t* s, int64_t ps, const int32_t* __restrict__ ci, int k0, int b,
                                           int (&v)[NBCH])
 {
     int nb[NBCH];
+    chunk_cols(ci, k0, b, nb);
 #pragma unroll
-    for (int t = 0; t < NBCH; ++t) nb[t] = k0 + t < b ? ci[k0 + t] : -1;
-#pragma unroll
-    for (int t = 0; t < NBCH; ++t) v[t] = nb[t] >= 0 ? (int)s[(int64_t)nb[t] * ps] : 0;
+    for (int t = 0; t < NBCH; ++t) v[t] = (int)s[(int64_t)nb[t] * ps];
 }
 
 // one thread per (vertex of the colour class, replica), replica fastest
@@ -98,36 +107,43 @@ __global__ void __launch_bounds__(512) k_csr_sweep_int8(int8_t* __restrict__ bas
     const uint32_t* T = thr + (int64_t)grep * 2 * tbl;
     const int si = s[pos * ps];
     if (model == MQ_POTTS) {
-        const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep, ST_METRO, key);
-        const int snew = (int)(((uint64_t)w.x * (uint64_t)Q) >> 32);
+        const uint32_t v = (uint32_t)vert_of[pos];
+        const uint4 pw = philox4x32_10(v, phase, (uint32_t)grep >> 2, ST_PROPOSE, key);
+        const int snew = (int)(((uint64_t)u4_get(pw, grep & 3) * (uint64_t)Q) >> 32);
         int dn = 0;
         for (int k0 = a; k0 < b; k0 += NBCH) {
             int v[NBCH];
             gather_i8(s, ps, ci, k0, b, v);
 #pragma unroll
-            for (int t = 0; t < NBCH; ++t)
-                if (k0 + t < b) dn += (v[t] == snew) - (v[t] == si);
+            for (int t = 0; t < NBCH; ++t) dn += (k0 + t < b) ? (v[t] == snew) - (v[t] == si) : 0;
         }
         const int e = dn + qmax;
-        const uint64_t th = ((uint64_t)T[2 * e] << 16) | T[2 * e + 1];
-        if ((uint64_t)w.y < th) s[pos * ps] = (int8_t)snew;
+        const uint4 w8 = philox4x32_10(v, phase, (uint32_t)grep >> 3, ST_METRO, key);
+        const int m = grep & 7;
+        const uint32_t hi16 = (u4_get(w8, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+        bool acc = hi16 < T[2 * e];
+        if (hi16 == T[2 * e]) {
+            const uint4 tw = philox4x32_10(v, phase, (uint32_t)grep >> 3, ST_TIE, key);
+            acc = ((u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu) < T[2 * e + 1];
+        }
+        if (acc) s[pos * ps] = (int8_t)snew;
     } else {
+        const uint32_t v = (uint32_t)vert_of[pos];
+        const uint4 w8 = philox4x32_10(v, phase, (uint32_t)grep >> 3, ST_METRO, key);  // independent of the gathers
         int h = 0;
         const int8_t* bb = BONDS ? bonds + rep_bond_off[grep] : nullptr;
         for (int k0 = a; k0 < b; k0 += NBCH) {
             int v[NBCH];
             gather_i8(s, ps, ci, k0, b, v);
 #pragma unroll
-            for (int t = 0; t < NBCH; ++t)
-                if (k0 + t < b) h += (BONDS ? (int)bb[k0 + t] : 1) * v[t];
+            for (int t = 0; t < NBCH; ++t) h += (k0 + t < b) ? (BONDS ? (int)bb[min(k0 + t, b - 1)] : 1) * v[t] : 0;
         }
         const int e = si * h + qmax;
-        const uint4 w8 = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep >> 3, ST_METRO, key);
         const int m = grep & 7;
         const uint32_t hi16 = (u4_get(w8, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
         bool acc = hi16 < T[2 * e];
         if (hi16 == T[2 * e]) {
-            const uint4 tw = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep >> 3, ST_TIE, key);
+            const uint4 tw = philox4x32_10(v, phase, (uint32_t)grep >> 3, ST_TIE, key);
             acc = ((u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu) < T[2 * e + 1];
         }
         if (acc) s[pos * ps] = (int8_t)(-si);
@@ -159,14 +175,12 @@ __global__ void __launch_bounds__(512) k_csr_sweep_o3(float4* __restrict__ base,
     for (int k0 = a; k0 < b; k0 += NBCH) {
         int nb[NBCH];
         float4 v[NBCH];
+        chunk_cols(ci, k0, b, nb);
 #pragma unroll
-        for (int t = 0; t < NBCH; ++t) nb[t] = k0 + t < b ? ci[k0 + t] : -1;
-#pragma unroll
-        for (int t = 0; t < NBCH; ++t) v[t] = nb[t] >= 0 ? s[(int64_t)nb[t] * ps] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int t = 0; t < NBCH; ++t) v[t] = s[(int64_t)nb[t] * ps];
 #pragma unroll
         for (int t = 0; t < NBCH; ++t) {
-            if (nb[t] < 0) continue;
-            const float f = BONDS ? (float)bb[k0 + t] : 1.0f;
+            const float f = (k0 + t < b) ? (BONDS ? (float)bb[min(k0 + t, b - 1)] : 1.0f) : 0.0f;
             nx += f * v[t].x;
             ny += f * v[t].y;
             nz += f * v[t].z;
@@ -224,28 +238,31 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
     const int o = (int)(t - vl * oct);
     const int64_t pos = c0 + vl;
     const int qmax = (tbl - 1) >> 1;
+    const int grep0 = rep0 + 8 * o;
+    // own spins, vertex id and the Philox block do not depend on the gathers:
+    // issue them first so their latency overlaps the neighbour loads
+    uint2* ownp = reinterpret_cast<uint2*>(base + pos * R + 8 * o);
+    const uint2 S = *ownp;
+    const uint32_t vid = (uint32_t)vert_of[pos];
+    const uint4 w = philox4x32_10(vid, phase, (uint32_t)grep0 >> 3, ST_METRO, key);
     const int a = rp[pos], b = rp[pos + 1];
     const int deg = b - a;
-    const int8_t* bb = BONDS ? bonds + rep_bond_off[rep0 + 8 * o] : nullptr;
+    const int8_t* bb = BONDS ? bonds + rep_bond_off[grep0] : nullptr;
     uint32_t n0 = 0u, n1 = 0u;
     for (int k0 = a; k0 < b; k0 += NBCH) {
         int nb[NBCH];
         uint2 v[NBCH];
+        chunk_cols(ci, k0, b, nb);
 #pragma unroll
-        for (int t = 0; t < NBCH; ++t) nb[t] = k0 + t < b ? ci[k0 + t] : -1;
-#pragma unroll
-        for (int t = 0; t < NBCH; ++t)
-            v[t] = nb[t] >= 0 ? *reinterpret_cast<const uint2*>(base + (int64_t)nb[t] * R + 8 * o) : make_uint2(0u, 0u);
+        for (int t = 0; t < NBCH; ++t) v[t] = *reinterpret_cast<const uint2*>(base + (int64_t)nb[t] * R + 8 * o);
 #pragma unroll
         for (int t = 0; t < NBCH; ++t) {
-            if (nb[t] < 0) continue;
-            const uint32_t f = (BONDS && bb[k0 + t] < 0) ? 0xFEFEFEFEu : 0u;  // -x = x ^ 0xFE for x = +-1
-            n0 += (v[t].x ^ f) & 0x02020202u;
-            n1 += (v[t].y ^ f) & 0x02020202u;
+            const uint32_t in = (k0 + t < b) ? 0x02020202u : 0u;
+            const uint32_t f = (BONDS && bb[min(k0 + t, b - 1)] < 0) ? 0xFEFEFEFEu : 0u;  // -x = x ^ 0xFE for x = +-1
+            n0 += (v[t].x ^ f) & in;
+            n1 += (v[t].y ^ f) & in;
         }
     }
-    uint2* ownp = reinterpret_cast<uint2*>(base + pos * R + 8 * o);
-    const uint2 S = *ownp;
     const uint32_t bias = (uint32_t)(qmax - deg) * 0x01010101u;
     const uint32_t two_deg = (uint32_t)(2 * deg) * 0x01010101u;
     auto eword = [&](uint32_t sw, uint32_t n2) {
@@ -253,22 +270,21 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
         return ((n2 & mask) | ((two_deg - n2) & ~mask)) + bias;     // 2*aligned - deg + qmax
     };
     const uint32_t E0 = eword(S.x, n0), E1 = eword(S.y, n1);
-    const int grep0 = rep0 + 8 * o;
     const uint32_t* T = thr + (int64_t)grep0 * tbl * 2;  // the lane's 8 replicas' (hi, lo) rows
-    const uint4 w = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep0 >> 3, ST_METRO, key);
+    uint32_t thv[8];  // all 8 table reads in flight together
+#pragma unroll
+    for (int m = 0; m < 8; ++m) thv[m] = T[((int64_t)m * tbl + (((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu)) * 2];
     uint32_t acc = 0u;
     bool tie = false;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const uint32_t e = ((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu;
         const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
-        const uint32_t th = T[((int64_t)m * tbl + e) * 2];
-        const uint32_t tt = th > 0xFFFFu ? 0xFFFFu : th;  // "always" (65536) as 0xFFFF + lo 0x10000
+        const uint32_t tt = thv[m] > 0xFFFFu ? 0xFFFFu : thv[m];  // "always" (65536) as 0xFFFF + lo 0x10000
         acc |= (u < tt ? 1u : 0u) << m;
         tie |= (u == tt);
     }
     if (tie) {
-        const uint4 tw = philox4x32_10((uint32_t)vert_of[pos], phase, (uint32_t)grep0 >> 3, ST_TIE, key);
+        const uint4 tw = philox4x32_10(vid, phase, (uint32_t)grep0 >> 3, ST_TIE, key);
         acc = 0u;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
@@ -283,6 +299,96 @@ __global__ void __launch_bounds__(256) k_csr_sweep_ising8(int8_t* __restrict__ b
     }
     auto expand = [](uint32_t nib) { return ((nib * 0x00204081u) & 0x01010101u) * 0xFEu; };
     *ownp = make_uint2(S.x ^ expand(acc & 15u), S.y ^ expand(acc >> 4));
+}
+
+// q-state Potts on a graph, 8 replicas per lane (R % 8 == 0, rep0 % 8 == 0,
+// maxdeg <= 127): the lane proposes 8 new colours from two PROPOSE blocks,
+// counts per byte the neighbours equal to the new colour (cn) and to the old
+// one (co) with a byte-wise zero test of (word ^ colours) (colours < 128, so
+// x + 0x7F has its byte's top bit clear exactly when the byte is 0), forms
+// e = cn + (qmax - co) per byte and accepts with the 16-bit halves of one
+// METRO block (tie -> TIE stream).  Same decisions as k_csr_sweep_int8.
+__global__ void __launch_bounds__(256) k_csr_sweep_potts8(int8_t* __restrict__ base, int R,
+                                                          const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                          const int32_t* __restrict__ vert_of,
+                                                          const uint32_t* __restrict__ thr, int tbl, int64_t c0,
+                                                          int64_t nc, int Q, uint32_t phase,
+                                                          const uint32_t* __restrict__ dyn, int rep0, Key2 key)
+{
+    if (dyn) phase += dyn[0];
+    const int oct = R >> 3;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc * oct) return;
+    const int64_t vl = t / oct;
+    const int o = (int)(t - vl * oct);
+    const int64_t pos = c0 + vl;
+    const int qmax = (tbl - 1) >> 1;
+    const int grep0 = rep0 + 8 * o;
+    const uint32_t v = (uint32_t)vert_of[pos];
+    // proposals of the lane's 8 replicas (bytes, replica grep0 + m in byte m)
+    uint32_t SN[2];
+#pragma unroll
+    for (int hblk = 0; hblk < 2; ++hblk) {
+        const uint4 pw = philox4x32_10(v, phase, (uint32_t)(grep0 >> 2) + (uint32_t)hblk, ST_PROPOSE, key);
+        uint32_t word = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) word |= (uint32_t)(((uint64_t)u4_get(pw, k) * (uint64_t)Q) >> 32) << (8 * k);
+        SN[hblk] = word;
+    }
+    uint2* ownp = reinterpret_cast<uint2*>(base + pos * R + 8 * o);
+    const uint2 S = *ownp;
+    const int a = rp[pos], b = rp[pos + 1];
+    const uint32_t K7 = 0x7F7F7F7Fu, H8 = 0x80808080u;
+    auto eqb = [&](uint32_t x) { return (~(x + K7) & H8) >> 7; };  // 0x01 per zero byte (bytes < 0x80)
+    uint32_t cn0 = 0u, cn1 = 0u, co0 = 0u, co1 = 0u;
+    for (int k0 = a; k0 < b; k0 += NBCH) {
+        int nb[NBCH];
+        uint2 w[NBCH];
+        chunk_cols(ci, k0, b, nb);
+#pragma unroll
+        for (int tt = 0; tt < NBCH; ++tt) w[tt] = *reinterpret_cast<const uint2*>(base + (int64_t)nb[tt] * R + 8 * o);
+#pragma unroll
+        for (int tt = 0; tt < NBCH; ++tt) {
+            const uint32_t in = (k0 + tt < b) ? 0xFFFFFFFFu : 0u;
+            cn0 += eqb(w[tt].x ^ SN[0]) & in;
+            cn1 += eqb(w[tt].y ^ SN[1]) & in;
+            co0 += eqb(w[tt].x ^ S.x) & in;
+            co1 += eqb(w[tt].y ^ S.y) & in;
+        }
+    }
+    const uint32_t qb = (uint32_t)qmax * 0x01010101u;  // co <= deg <= qmax per byte: no borrow
+    const uint32_t E0 = cn0 + (qb - co0), E1 = cn1 + (qb - co1);
+    const uint4 wm = philox4x32_10(v, phase, (uint32_t)grep0 >> 3, ST_METRO, key);
+    const uint32_t* T = thr + (int64_t)grep0 * tbl * 2;
+    uint32_t thv[8];  // all 8 table reads in flight together
+#pragma unroll
+    for (int m = 0; m < 8; ++m) thv[m] = T[((int64_t)m * tbl + (((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu)) * 2];
+    uint32_t acc = 0u;
+    bool tie = false;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const uint32_t u = (u4_get(wm, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+        const uint32_t tt = thv[m] > 0xFFFFu ? 0xFFFFu : thv[m];  // "always" (65536) as 0xFFFF + lo 0x10000
+        acc |= (u < tt ? 1u : 0u) << m;
+        tie |= (u == tt);
+    }
+    if (tie) {
+        const uint4 tw = philox4x32_10(v, phase, (uint32_t)grep0 >> 3, ST_TIE, key);
+        acc = 0u;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const uint32_t e = ((m < 4 ? E0 : E1) >> (8 * (m & 3))) & 0xFFu;
+            const uint32_t u = (u4_get(wm, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+            const uint32_t lo = (u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+            const uint32_t th = T[((int64_t)m * tbl + e) * 2];
+            const uint32_t tt = th > 0xFFFFu ? 0xFFFFu : th;
+            const uint32_t tl = th > 0xFFFFu ? 0x10000u : T[((int64_t)m * tbl + e) * 2 + 1];
+            acc |= ((u < tt) || (u == tt && lo < tl) ? 1u : 0u) << m;
+        }
+    }
+    auto bytemask = [](uint32_t nib) { return ((nib * 0x00204081u) & 0x01010101u) * 0xFFu; };
+    const uint32_t M0 = bytemask(acc & 15u), M1 = bytemask(acc >> 4);
+    *ownp = make_uint2((S.x & ~M0) | (SN[0] & M0), (S.y & ~M1) | (SN[1] & M1));
 }
 
 template <typename T>
@@ -767,7 +873,7 @@ hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t p
                                st->p.proposal, J, d, lam, m, phase, st->cur_dyn, g.rep0, key);
     } else {
         int8_t* b = reinterpret_cast<int8_t*>(st->d_spins + g.off);
-        bool oct_ok = model != MQ_POTTS && g.nrep % 8 == 0 && g.rep0 % 8 == 0 && l->maxdeg <= 127 &&
+        bool oct_ok = g.nrep % 8 == 0 && g.rep0 % 8 == 0 && l->maxdeg <= 127 &&
                       st->kernel_choice != 2;
         if (oct_ok && bonds)  // the 8 replicas of a lane must share one coupling sample
             for (int r = g.rep0; r < g.rep0 + g.nrep && oct_ok; ++r)
@@ -775,7 +881,10 @@ hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t p
         if (oct_ok) {
             const int64_t w8 = nc * (g.nrep / 8);
             const dim3 g8((unsigned)((w8 + 255) / 256));
-            if (bonds)
+            if (model == MQ_POTTS)
+                hipLaunchKernelGGL(k_csr_sweep_potts8, g8, dim3(256), 0, s, b, g.nrep, l->d_rp, l->d_ci, l->d_vert_of,
+                                   st->d_thr, st->tbl, c0, nc, st->p.q, phase, st->cur_dyn, g.rep0, key);
+            else if (bonds)
                 hipLaunchKernelGGL(k_csr_sweep_ising8<true>, g8, dim3(256), 0, s, b, g.nrep, l->d_rp, l->d_ci,
                                    l->d_vert_of, st->d_bonds, st->d_rep_bond_off, st->d_thr, st->tbl, c0, nc, phase,
                                    st->cur_dyn, g.rep0, key);

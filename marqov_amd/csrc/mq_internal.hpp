@@ -19,6 +19,7 @@ enum : uint32_t {
     ST_WBOND = 4,
     ST_PT = 5,
     ST_DISORDER = 6,
+    ST_PROPOSE = 7,
 };
 
 enum { LAT_REGULAR = 0, LAT_CSR = 1 };

@@ -115,3 +115,30 @@ def enumerate_potts(dim, L, Q, beta, k=-1.0):
     w = np.exp(-beta * (E - E.min()))
     Z = w.sum()
     return {"e": float((w * E).sum() / Z / N), "m": float((w * m).sum() / Z)}
+
+
+def enumerate_graph(rp, ci, beta, J=-1.0, Q=0, bonds=None):
+    """Exact <E/N> and <m^2> (Ising, Q = 0: E = J sum_edges b s_i s_j) or <E/N> and
+    <m_Potts> (Q >= 2: E = J * #equal edges) on a small graph given as symmetric CSR;
+    each edge appears in both rows and is counted once."""
+    rp = np.asarray(rp, np.int64)
+    ci = np.asarray(ci, np.int64)
+    N = len(rp) - 1
+    src = np.repeat(np.arange(N), np.diff(rp))
+    keep = src < ci
+    a, b = src[keep], ci[keep]
+    w_b = np.ones(len(a)) if bonds is None else np.asarray(bonds, np.float64)[keep]
+    if Q == 0:
+        states = ((np.arange(2 ** N)[:, None] >> np.arange(N)[None, :]) & 1).astype(np.int64) * 2 - 1
+        E = J * (states[:, a] * states[:, b] * w_b[None, :]).sum(1)
+        M = states.sum(1) / N
+        w = np.exp(-beta * (E - E.min()))
+        Z = w.sum()
+        return {"e": float((w * E).sum() / Z / N), "m2": float((w * M ** 2).sum() / Z)}
+    states = np.array(list(itertools.product(range(Q), repeat=N)), np.int64)
+    E = J * (states[:, a] == states[:, b]).sum(1)
+    counts = np.stack([(states == q).sum(1) for q in range(Q)], 1)
+    m = (Q * counts.max(1) - N) / (N * (Q - 1))
+    w = np.exp(-beta * (E - E.min()))
+    Z = w.sum()
+    return {"e": float((w * E).sum() / Z / N), "m": float((w * m).sum() / Z)}

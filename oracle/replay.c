@@ -42,7 +42,7 @@
 #include <string.h>
 #include "philox.h"
 
-enum { ST_INIT = 0, ST_METRO = 1, ST_TIE = 2, ST_WSEED = 3, ST_WBOND = 4, ST_PT = 5, ST_DISORDER = 6 };
+enum { ST_INIT = 0, ST_METRO = 1, ST_TIE = 2, ST_WSEED = 3, ST_WBOND = 4, ST_PT = 5, ST_DISORDER = 6, ST_PROPOSE = 7 };
 
 #define EXPORT __attribute__((visibility("default")))
 
@@ -406,13 +406,27 @@ EXPORT void orc_csr_sweeps(int64_t N, const int32_t* rp, const int32_t* ci, cons
                 uint32_t w[4];
                 int si = s[v];
                 if (potts) {
-                    orc_philox_u64key((uint32_t)v, phase, rep, ST_METRO, seed, w);
-                    int snew = (int)(((uint64_t)w[0] * (uint64_t)Q) >> 32);
+                    /* proposal: word (rep & 3) of the PROPOSE block of 4 replicas;
+                     * acceptance: 16-bit half (rep & 7) of the METRO block of 8
+                     * replicas against the high half of T, the TIE stream's half
+                     * against the low half on equality (= the 32-bit test u < T) */
+                    uint32_t pw[4];
+                    orc_philox_u64key((uint32_t)v, phase, rep >> 2, ST_PROPOSE, seed, pw);
+                    int snew = (int)(((uint64_t)pw[rep & 3u] * (uint64_t)Q) >> 32);
                     int dn = 0;
                     for (int k = rp[v]; k < rp[v + 1]; ++k) dn += (s[ci[k]] == snew) - (s[ci[k]] == si);
                     int e = dn + qmax;
-                    uint64_t T = ((uint64_t)thi[e] << 16) | tlo[e];
-                    if ((uint64_t)w[1] < T) s[v] = (int8_t)snew;
+                    orc_philox_u64key((uint32_t)v, phase, rep >> 3, ST_METRO, seed, w);
+                    int m = (int)(rep & 7u);
+                    uint32_t hi16 = (w[m >> 1] >> (16 * (m & 1))) & 0xFFFFu;
+                    int acc;
+                    if (hi16 != thi[e]) acc = hi16 < thi[e];
+                    else {
+                        uint32_t tw[4];
+                        orc_philox_u64key((uint32_t)v, phase, rep >> 3, ST_TIE, seed, tw);
+                        acc = ((tw[m >> 1] >> (16 * (m & 1))) & 0xFFFFu) < tlo[e];
+                    }
+                    if (acc) s[v] = (int8_t)snew;
                 } else {
                     int h = 0;
                     for (int k = rp[v]; k < rp[v + 1]; ++k) h += (bonds ? bonds[k] : 1) * s[ci[k]];

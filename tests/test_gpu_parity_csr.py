@@ -105,16 +105,16 @@ def test_csr_regular_graph_observables(ctx):
     assert obs[0, 1] == pytest.approx(abs(int(s.astype(int).sum())) / 64, abs=1e-12)
 
 
-@pytest.mark.parametrize("model", ["ising", "ea"])
+@pytest.mark.parametrize("model", ["ising", "ea", "potts"])
 def test_csr_octet_kernel_bit_exact(ctx, model):
-    """16 replicas on one graph select the 8-replicas-per-lane kernel."""
+    """16 replicas on one graph select the 8-replicas-per-lane kernels."""
     from marqov_amd import gpu
     rp, ci = _graph(1200, 9)
     N = len(rp) - 1
     lat = _lat(ctx, rp, ci)
     col = _colours(ctx, lat)
     R = 16
-    st = gpu.State(ctx, model, [lat] * R, J=-1.0)
+    st = gpu.State(ctx, model, [lat] * R, J=-1.0, q=3)
     betas = np.linspace(0.3, 1.5, R)
     st.set_beta(betas)
     bonds = None
@@ -127,7 +127,10 @@ def test_csr_octet_kernel_bit_exact(ctx, model):
     st.init(seed=14)
     st.metropolis(4, seed=15, sweep0=2)
     for r in range(R):
-        exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(N, 14, r), betas[r], -1.0, 4, 15, 2, r, bonds=bonds)
+        if model == "potts":
+            exp = ob.csr_sweeps(rp, ci, col, ob.init_potts(N, 3, 14, r), betas[r], -1.0, 4, 15, 2, r, potts=True)
+        else:
+            exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(N, 14, r), betas[r], -1.0, 4, 15, 2, r, bonds=bonds)
         assert np.array_equal(st.download(r), exp), r
 
 
