@@ -825,9 +825,10 @@ static int measure_sums(mq_state* st)
     }
     MQ_HIP(st->ctx, zero64(st->d_isum, (int64_t)st->nrep * 8, s));
     MQ_HIP(st->ctx, zero64(st->d_fsum, (int64_t)st->nrep * 8, s));
+    if (st->eabits) MQ_HIP(st->ctx, launch_measure_eabits_all(st));
     for (const auto& g : st->groups) {
-        hipError_t e = st->eabits ? launch_measure_eabits(st, g)
-                       : st->msc ? launch_measure_msc(st, g)
+        if (st->eabits) break;
+        hipError_t e = st->msc ? launch_measure_msc(st, g)
                        : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
         if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
     }

@@ -231,35 +231,91 @@ __device__ __forceinline__ void acc_add3(uint32_t (&A)[NP], uint32_t b0, uint32_
     }
 }
 
-// per replica: isum[r*8+0] = sum over bonds b s_i s_j, isum[r*8+1] = sum s.
-// A thread folds SPT colour-0 sites into bit-sliced per-bit counters (anti-
-// aligned bonds and down spins), then adds its 32 per-bit totals to LDS once.
-constexpr int SPT = 16;
+// Bit-sliced add of Y into X over NP planes (values that overflow NP planes are
+// not produced by the callers).
+template <int NP>
+__device__ __forceinline__ void bs_add(uint32_t (&X)[NP], const uint32_t (&Y)[NP])
+{
+    uint32_t carry = 0u;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const uint32_t x = X[p], y = Y[p];
+        X[p] = x ^ y ^ carry;
+        carry = (x & y) | (x & carry) | (y & carry);
+    }
+}
+
+// Reduce-scatter of bit-sliced per-bit counts across the 64 lanes of a
+// wavefront: after five exchange steps (partners lane ^ 16, 8, 4, 2, 1, each
+// lane keeping the half of the 32 bit positions its own lane bit selects) lane
+// l holds the sum over its 32-lane half of bit position l & 31; the two halves
+// are added as integers.  Returns that count.
+template <int NP>
+__device__ __forceinline__ int bs_reduce_scatter(uint32_t (&A)[NP])
+{
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int m = 16; m >= 1; m >>= 1) {
+        // bit positions j with (j & m) != 0
+        const uint32_t hi = m == 16 ? 0xFFFF0000u : m == 8 ? 0xFF00FF00u : m == 4 ? 0xF0F0F0F0u : m == 2 ? 0xCCCCCCCCu : 0xAAAAAAAAu;
+        const uint32_t keep = (lane & m) ? hi : ~hi;
+        uint32_t Y[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            Y[p] = (uint32_t)__shfl_xor((int)A[p], m, 64) & keep;
+            A[p] &= keep;
+        }
+        bs_add<NP>(A, Y);
+    }
+    const int j = lane & 31;
+    int c = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) c += (int)((A[p] >> j) & 1u) << p;
+    return c + __shfl_xor(c, 32, 64);
+}
+
+// Per replica: isum[r*8+0] += sum over bonds b s_i s_j, isum[r*8+1] += sum s.
+// One launch over every lattice size; a wavefront takes up to 64*MSPL colour-0
+// sites of one word of 32 replicas (table entries ngroups.. of the group
+// table: bpw = wavefronts per word, block0 = first wavefront).  Lanes fold
+// their sites into bit-sliced per-bit counters of anti-aligned bonds and down
+// spins, the wavefront reduce-scatters them so lane j holds replica j's
+// totals, and lanes 0..31 add them with one atomic each.
+constexpr int MSPL = 8;  // sites per lane
 template <int DIM>
-__global__ void __launch_bounds__(256) k_measure_eabits(const uint32_t* __restrict__ base, int64_t wstride, int L,
+__global__ void __launch_bounds__(256) k_measure_eabits(const char* __restrict__ spins,
+                                                        const BitsGroup* __restrict__ groups, int ngroups, int nwaves,
                                                         const int8_t* __restrict__ bonds,
-                                                        const int64_t* __restrict__ rep_bond_off, int gw0, int bpw,
+                                                        const int64_t* __restrict__ rep_bond_off,
                                                         unsigned long long* __restrict__ isum)
 {
-    __shared__ int s_anti[32], s_down[32];
-    const int wl = blockIdx.x / bpw;
-    const int gw = gw0 + wl;
-    if (threadIdx.x < 32) {
-        s_anti[threadIdx.x] = 0;
-        s_down[threadIdx.x] = 0;
-    }
-    __syncthreads();
-    int64_t N = (int64_t)L * L;
-    if (DIM == 3) N *= L;
-    const uint32_t half = (uint32_t)(N >> 1);
-    const uint32_t* c0 = base + (int64_t)wl * wstride;
+    constexpr int NE = 7 + 5;  // anti-aligned: <= 2*DIM*MSPL per lane (6 bits), +5 for the exchange steps
+    constexpr int ND = 5 + 5;  // down spins: <= 2*MSPL per lane
+    const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (wave >= nwaves) return;  // whole wavefront; nothing below synchronises the workgroup
+    const int lane = (int)(threadIdx.x & 63);
+    int gi = 0;
+    while (gi + 1 < ngroups && groups[ngroups + gi + 1].block0 <= wave) ++gi;
+    const BitsGroup G = groups[ngroups + gi];
+    const int local = wave - G.block0;
+    const int wl = local / G.bpw;
+    const int chunk = local - wl * G.bpw;
+    const int gw = G.gw0 + wl;
+    const int L = G.L;
+    uint32_t N = (uint32_t)L * (uint32_t)L;
+    if (DIM == 3) N *= (uint32_t)L;
+    const uint32_t half = N >> 1;
+    const uint32_t* c0 = reinterpret_cast<const uint32_t*>(spins + G.off) + (int64_t)wl * G.wstride;
     const uint32_t* c1 = c0 + half;
     const int8_t* bs = bonds + rep_bond_off[32 * gw];
-    uint32_t EA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // anti-aligned bonds: <= 6 * SPT per bit
-    uint32_t DA[6] = {0, 0, 0, 0, 0, 0};           // down spins: <= 2 * SPT per bit
-    const uint32_t first = (uint32_t)(blockIdx.x - wl * bpw) * blockDim.x * SPT + threadIdx.x;
-    for (int t = 0; t < SPT; ++t) {
-        const uint32_t h = first + (uint32_t)t * blockDim.x;  // coalesced across the block
+    const uint32_t first = (uint32_t)chunk * 64u * MSPL;
+    uint32_t EA[NE], DA[ND];
+#pragma unroll
+    for (int p = 0; p < NE; ++p) EA[p] = 0u;
+#pragma unroll
+    for (int p = 0; p < ND; ++p) DA[p] = 0u;
+    for (int t = 0; t < MSPL; ++t) {
+        const uint32_t h = first + (uint32_t)t * 64u + (uint32_t)lane;  // coalesced across the wavefront
         if (h >= half) break;
         uint32_t nb[2 * DIM];
         nbrs_h<DIM>(L, 0, h, nb);
@@ -273,30 +329,18 @@ __global__ void __launch_bounds__(256) k_measure_eabits(const uint32_t* __restri
         const uint32_t S = c0[h];
         uint32_t b0, b1, b2;
         anti_count<DIM>(S, n, b0, b1, b2);
-        acc_add3<9>(EA, b0, b1, b2);
+        acc_add3<NE - 5>(reinterpret_cast<uint32_t(&)[NE - 5]>(EA), b0, b1, b2);
         const uint32_t T2 = c1[h];
-        acc_add3<6>(DA, S ^ T2, S & T2, 0u);  // S + T2 per bit: sum bit, carry bit
+        acc_add3<ND - 5>(reinterpret_cast<uint32_t(&)[ND - 5]>(DA), S ^ T2, S & T2, 0u);  // S + T2 per bit
     }
-    for (int j = 0; j < 32; ++j) {
-        int ca = 0, cd = 0;
-#pragma unroll
-        for (int p = 0; p < 9; ++p) ca += (int)((EA[p] >> j) & 1u) << p;
-#pragma unroll
-        for (int p = 0; p < 6; ++p) cd += (int)((DA[p] >> j) & 1u) << p;
-        if (ca) atomicAdd(&s_anti[j], ca);
-        if (cd) atomicAdd(&s_down[j], cd);
-    }
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        const int64_t sites0 = (int64_t)(blockIdx.x - wl * bpw) * blockDim.x * SPT;
-        int64_t n0 = (int64_t)half - sites0;
-        if (n0 > (int64_t)blockDim.x * SPT) n0 = (int64_t)blockDim.x * SPT;
-        // colour-0 sites of this block: n0, each with 2*DIM bonds; sites counted for M: 2*n0
-        const long long bsum = (long long)n0 * 2 * DIM - 2ll * s_anti[threadIdx.x];
-        const long long msum = 2ll * n0 - 2ll * s_down[threadIdx.x];
-        const int64_t r = 32ll * gw + threadIdx.x;
-        atomicAdd(isum + r * 8 + 0, (unsigned long long)bsum);
-        atomicAdd(isum + r * 8 + 1, (unsigned long long)msum);
+    const int anti = bs_reduce_scatter<NE>(EA);
+    const int down = bs_reduce_scatter<ND>(DA);
+    if (lane < 32) {
+        const int64_t rem = (int64_t)half - first;
+        const int64_t nsites = rem < 64 * MSPL ? rem : 64 * MSPL;  // colour-0 sites of this wavefront
+        const int64_t r = 32ll * gw + lane;
+        atomicAdd(isum + r * 8 + 0, (unsigned long long)(nsites * 2 * DIM - 2ll * anti));
+        atomicAdd(isum + r * 8 + 1, (unsigned long long)(2ll * nsites - 2ll * down));
     }
 }
 
@@ -341,10 +385,22 @@ hipError_t prepare_eabits(mq_state* st)
         gs.push_back(d);
     }
     if (gs.size() > 64) return hipErrorNotSupported;
+    const size_t ng = gs.size();
+    // measurement entries: bpw = wavefronts per word, block0 = first wavefront
+    int waves = 0;
+    for (size_t i = 0; i < ng; ++i) {
+        const Group& g = st->groups[i];
+        BitsGroup d = gs[i];
+        d.bpw = (int)((g.lat->N / 2 + 64 * MSPL - 1) / (64 * MSPL));
+        d.block0 = waves;
+        waves += d.bpw * (g.nrep / 32);
+        gs.push_back(d);
+    }
     hipError_t e = hipMalloc(&st->d_bits_groups, gs.size() * sizeof(BitsGroup));
     if (e != hipSuccess) return e;
-    st->bits_ngroups = (int)gs.size();
+    st->bits_ngroups = (int)ng;
     st->bits_blocks = blocks;
+    st->bits_mwaves = waves;
     return hipMemcpy(st->d_bits_groups, gs.data(), gs.size() * sizeof(BitsGroup), hipMemcpyHostToDevice);
 }
 
@@ -369,19 +425,17 @@ hipError_t launch_sweep_eabits_all(mq_state* st, int colour, uint32_t phase, Key
     return hipGetLastError();
 }
 
-hipError_t launch_measure_eabits(mq_state* st, const Group& g)
+// one fused launch over every lattice group
+hipError_t launch_measure_eabits_all(mq_state* st)
 {
-    const int L = g.lat->L;
-    const int nwl = g.nrep / 32;
-    const int64_t half = g.lat->N / 2;
-    const int bpw = (int)((half + 256 * SPT - 1) / (256 * SPT));
-    const uint32_t* b = reinterpret_cast<const uint32_t*>(st->d_spins + g.off);
+    const BitsGroup* dg = reinterpret_cast<const BitsGroup*>(st->d_bits_groups);
+    const dim3 grid((unsigned)((st->bits_mwaves + 3) / 4));
     unsigned long long* is = reinterpret_cast<unsigned long long*>(st->d_isum);
-    if (g.lat->dim == 3)
-        hipLaunchKernelGGL(k_measure_eabits<3>, dim3((unsigned)(bpw * nwl)), dim3(256), 0, st->ctx->stream, b,
-                           g.stride / 4, L, st->d_bonds, st->d_rep_bond_off, g.rep0 / 32, bpw, is);
+    if (st->lat[0]->dim == 3)
+        hipLaunchKernelGGL(k_measure_eabits<3>, grid, dim3(256), 0, st->ctx->stream, st->d_spins, dg, st->bits_ngroups,
+                           st->bits_mwaves, st->d_bonds, st->d_rep_bond_off, is);
     else
-        hipLaunchKernelGGL(k_measure_eabits<2>, dim3((unsigned)(bpw * nwl)), dim3(256), 0, st->ctx->stream, b,
-                           g.stride / 4, L, st->d_bonds, st->d_rep_bond_off, g.rep0 / 32, bpw, is);
+        hipLaunchKernelGGL(k_measure_eabits<2>, grid, dim3(256), 0, st->ctx->stream, st->d_spins, dg, st->bits_ngroups,
+                           st->bits_mwaves, st->d_bonds, st->d_rep_bond_off, is);
     return hipGetLastError();
 }

@@ -130,9 +130,12 @@ hipError_t energy_async(mq_state* st)
     hipError_t e;
     if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     if ((e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
+    if (st->eabits) {
+        if ((e = launch_measure_eabits_all(st)) != hipSuccess) return e;
+    }
     for (const auto& g : st->groups) {
-        e = st->eabits ? launch_measure_eabits(st, g)
-            : st->msc ? launch_measure_msc(st, g)
+        if (st->eabits) break;
+        e = st->msc ? launch_measure_msc(st, g)
             : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
         if (e != hipSuccess) return e;
     }

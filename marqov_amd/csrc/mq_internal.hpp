@@ -260,6 +260,7 @@ struct mq_state {
     void* d_wtab = nullptr;             // per word group: per-bit threshold words
     void* d_bits_groups = nullptr;      // lattice-group table of the fused bit-sliced sweep
     int bits_ngroups = 0, bits_blocks = 0;
+    int bits_mwaves = 0;                // wavefronts of the fused bit-sliced measurement (table entries ngroups..2*ngroups-1)
     void* d_msc_cls = nullptr;          // per-replica Bernoulli classes of the MSC sweep
     int tune_block = 256;               // threads per block of the streaming sweep kernels
     int tune_xcd = 1;                   // XCD-aware block remap of the row-vectorised kernel
@@ -292,7 +293,7 @@ size_t wtab_bytes();
 hipError_t launch_init_eabits(mq_state* st, const Group& g, Key2 key);
 hipError_t prepare_eabits(mq_state* st);
 hipError_t launch_sweep_eabits_all(mq_state* st, int colour, uint32_t phase, Key2 key);
-hipError_t launch_measure_eabits(mq_state* st, const Group& g);
+hipError_t launch_measure_eabits_all(mq_state* st);
 bool msc_supported(const mq_lattice* lat);
 hipError_t msc_upload_classes(mq_state* st, const std::vector<uint32_t>& thr);
 hipError_t launch_sweep_msc(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);

@@ -52,19 +52,24 @@ def test_eabits_init_and_sweeps_bit_exact(ctx, J):
         assert np.array_equal(st.download(r), exp[r // 32][r % 32]), r
 
 
-def test_eabits_measure(ctx):
-    st, reps = _state(ctx, [6])
+@pytest.mark.parametrize("Ls,words", [([6], 1), ([4, 8, 12, 24], 2)])
+def test_eabits_measure(ctx, Ls, words):
+    """One fused launch measures every lattice size; at L=24 a word spans 14
+    wavefronts whose per-replica totals are combined with atomics."""
+    st, reps = _state(ctx, Ls, words_per_L=words)
     st.metropolis(2, seed=21, sweep0=0)
     obs = st.measure()
-    L = 6
-    N = L ** 3
-    nb = exact.neighbour_table(DIM, L)
-    B = exact.bond_table(DIM, L, ob.init_bonds(DIM, N, 0.5, 4, 0))
-    for r in (0, 5, 31):
-        s = st.download(r).astype(np.int64)
-        e = -1.0 * (s[:, None] * s[nb] * B).sum() / 2.0 / N
-        assert obs[r, 0] == pytest.approx(e, abs=1e-12)
-        assert obs[r, 1] == pytest.approx(abs(s.sum()) / N, abs=1e-12)
+    for w in range(len(reps) // 32):
+        L = reps[32 * w].L
+        N = L ** 3
+        nb = exact.neighbour_table(DIM, L)
+        B = exact.bond_table(DIM, L, ob.init_bonds(DIM, N, 0.5, 4, w))
+        for j in (0, 5, 31):
+            r = 32 * w + j
+            s = st.download(r).astype(np.int64)
+            e = -1.0 * (s[:, None] * s[nb] * B).sum() / 2.0 / N
+            assert obs[r, 0] == pytest.approx(e, abs=1e-12), r
+            assert obs[r, 1] == pytest.approx(abs(s.sum()) / N, abs=1e-12), r
 
 
 def test_eabits_pt_and_graph(ctx):
