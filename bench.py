@@ -9,7 +9,7 @@ every replica; value = spin-flip attempts/ns over the timed steps.
 
 Beside it on the same JSON line (skip with --quick):
   plain        the same sweep, one replica per launch
-  c1_2d_L64    configs[0] schedule (10 Wolff + 1 sweep per EMCS), batched vs plain
+  c1_2d_L64    2D Ising L=64 EMCS schedule (10 Wolff + 1 sweep per EMCS), batched vs plain
   c3_heis_L128 Heisenberg FP32 sweeps + Wolff, batched vs plain
   c4_ea_batch  512 +-J samples x 32 temperatures, L in {8,12,16,24}, HIP-graph
                captured {sweep, PT swap}, batched vs plain
@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--replicas", type=int, default=32)
     ap.add_argument("--L", type=int, default=256)
-    ap.add_argument("--quick", action="store_true", help="headline only (no configs[0],[2..4] legs)")
+    ap.add_argument("--quick", action="store_true", help="headline only (no configuration legs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--legs", default="", help="comma-separated subset of the extra legs to run")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -69,7 +69,7 @@ def timed_wolff(ctx, st, nclusters, seed, cluster0):
 
 
 def leg_c2_msc(gpu, ctx, L=256, R=32, steps=100):
-    """configs[1] multi-spin-coded variant: 1 bit per spin, R replicas, exact bit-serial acceptance."""
+    """3D Ising L=256, multi-spin-coded variant: 1 bit per spin, R replicas, exact bit-serial acceptance."""
     lat = gpu.RegularLattice(ctx, 3, L)
     N = L ** 3
     res = {}
@@ -90,10 +90,10 @@ def leg_c2_msc(gpu, ctx, L=256, R=32, steps=100):
 
 
 def leg_c1(gpu, ctx):
-    """2D Ising L=64 beta_c, EMCS = 10 Wolff + 1 sweep."""
+    """2D Ising L=64 at beta_c, EMCS = 10 Wolff + 1 sweep."""
     lat = gpu.RegularLattice(ctx, 2, 64)
     res = {}
-    for name, nrep, nstep in (("batched", 1024, 20), ("plain", 1, 20)):
+    for name, nrep, nstep in (("batched", 4096, 20), ("plain", 1, 20)):
         st = gpu.State(ctx, "ising", [lat] * nrep, J=-1.0)
         st.set_beta(0.4406868)
         st.init(seed=1)
@@ -114,7 +114,7 @@ def leg_c1(gpu, ctx):
 
 
 def leg_c3(gpu, ctx):
-    """configs[2]: 3D Heisenberg L=128 beta=0.693, FP32 float4 spins, small-angle proposal."""
+    """3D Heisenberg L=128 beta=0.693, FP32 float4 spins, small-angle proposal."""
     L, beta = 128, 0.693
     lat = gpu.RegularLattice(ctx, 3, L)
     N = L ** 3
@@ -154,7 +154,7 @@ def leg_c3(gpu, ctx):
 
 
 def leg_c4(gpu, ctx):
-    """512 EA samples (128 each of L=8,12,16,24) x 32 temperatures, HIP-graph
+    """EA +-J batch: 512 samples (128 each of L=8,12,16,24) x 32 temperatures, HIP-graph
     captured {10 sweeps, energy, PT swap}; bit-sliced words (32 temperatures of a sample per
     uint32) and int8 spins, against one L=24 replica per launch."""
     Ls = [8, 12, 16, 24]
@@ -194,7 +194,7 @@ def leg_c4(gpu, ctx):
 
 
 def leg_c5(gpu, ctx):
-    """2D RGG N=2^20, mean degree 6, Ising and Potts-3, 64 replicas vs 1."""
+    """2D random geometric graph N=2^20, mean degree 6, Ising and Potts-3, 64 replicas vs 1."""
     from marqov_amd import graphs
     n = 1 << 20
     rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n), 5)

@@ -1,4 +1,4 @@
-// C++ driver over include/marqov_amd.hpp: the configs[0] schedule (2D Ising,
+// C++ driver over include/marqov_amd.hpp: the EMCS schedule of 2D Ising L=64 (2D Ising,
 // L=64, J=-1, beta_c, 10 Wolff clusters + 1 checkerboard sweep per EMCS) on a
 // batch of replicas, as Core::wrmploop/gameloop run it (core.h:1001-1050).
 // Prints one JSON line with the mean energy per site, its standard error over

@@ -1,4 +1,4 @@
-"""The C++ host API (include/marqov_amd.hpp) drives the configs[0] schedule:
+"""The C++ host API (include/marqov_amd.hpp) drives the 2D Ising L=64 EMCS schedule:
 2D Ising L=64 at beta_c, 10 Wolff + 1 sweep per EMCS, energy vs Kaufman's exact value."""
 import json
 import os

@@ -81,7 +81,7 @@ def test_potts3_2d_vs_restatement(ctx):
 
 
 def test_ising_c1_energy_vs_kaufman(ctx):
-    """configs[0] schedule (10 Wolff + 1 sweep per EMCS) at L=64, beta_c, many replicas."""
+    """2D Ising EMCS schedule (10 Wolff + 1 sweep per EMCS) at L=64, beta_c, many replicas."""
     L, beta = 64, 0.4406868
     obs, _ = gpu_series(ctx, "ising", 2, L, beta, 256, 200, 300, ncluster=10, nmetro=1, J=-1.0)
     e = obs[:, :, 0].mean(0)
@@ -104,7 +104,7 @@ def test_heisenberg_renormalize_keeps_unit_length(ctx):
 
 
 def test_full_size_c2_properties(ctx):
-    """configs[1] size (3D L=256): measured energy equals the energy of the downloaded
+    """3D Ising at L=256: measured energy equals the energy of the downloaded
     state; a sweep at beta = 0 accepts every move, so it flips every spin exactly once."""
     from marqov_amd import gpu
     L = 256

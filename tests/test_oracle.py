@@ -174,7 +174,7 @@ def test_restatement_matches_enumeration(model):
 
 
 def test_restatement_c1_energy_vs_kaufman():
-    """2D L=64 at beta_c, 10 Wolff + 1 sweep per EMCS, vs the exact torus energy."""
+    """2D Ising L=64 at beta_c, 10 Wolff + 1 sweep per EMCS, vs the exact torus energy."""
     L, beta = 64, 0.4406868
     _, obs, cl = ob.ref_run(ob.ISING, 2, L, beta, ncluster=10, nmetro=1, nwarm=300, nmeas=3000, seed=1)
     me, se = _mean_err(obs[:, 0])
