@@ -222,7 +222,7 @@ def leg_c5(gpu, ctx):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is design-only",
+        print(json.dumps({"error": "marqov_amd runs on one GPU; sharding across GPUs is designed but not implemented",
                           "n_gpus": a.gpus}))
         sys.exit(2)
     if a.calibrate_copy:

@@ -2,7 +2,7 @@
  * oracle/philox.h -- CPU Philox4x32-10 (test infrastructure only).
  *
  * This file belongs to the oracle: only tests/, the smoke check and the
- * bench's cpu_baseline leg may use it, and only as the checker.
+ * bench's cpu_baseline leg may use it, and only to check results.
  *
  * Philox4x32-10 as published by Salmon, Moraes, Dror and Shaw,
  * "Parallel random numbers: as easy as 1, 2, 3" (SC'11), Random123 1.09:

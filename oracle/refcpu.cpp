@@ -3,7 +3,7 @@
  * hot path (test infrastructure; also the CPU baseline the bench times).
  *
  * Only tests/, the smoke check and the bench's cpu_baseline leg may load this
- * library, and only as the checker / the timed CPU baseline.
+ * library, and only to check results or as the timed CPU baseline.
  *
  * Restated semantics, each following the cited reference code:
  *  - RNGCache over std::mt19937_64: 1024 cached 64-bit words per refill,

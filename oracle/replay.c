@@ -2,7 +2,7 @@
  * oracle/replay.c -- CPU replay of the GPU update schedule (test infrastructure).
  *
  * Only tests/, the smoke check and the bench's cpu_baseline leg may load this
- * library, and only as the checker; the product path never calls it.
+ * library, and only to check results; the product path never calls it.
  *
  * What it restates, per function:
  *  - orc_thresholds_*: the integer acceptance tables the library builds on the

@@ -1,4 +1,4 @@
-"""ctypes binding of the oracle (oracle/liboracle.so): the checker, never the product."""
+"""ctypes binding of the oracle (oracle/liboracle.so): used to check results, never by the product."""
 import ctypes
 import os
 import subprocess
