@@ -136,20 +136,23 @@ def leg_c3(gpu, ctx):
             res[name]["e"] = float(obs[:, 0].mean())
         del st
     res["batched_over_plain"] = round(res["batched"]["attempts_per_ns"] / res["plain"]["attempts_per_ns"], 2)
-    # finite-size-scaling shape: many small FP32 lattices (Heisenberg L=16) in one launch vs one per launch
-    lat16 = gpu.RegularLattice(ctx, 3, 16)
-    small = {}
-    for name, nrep, nstep in (("batched", 512, 20), ("plain", 1, 20)):
-        st = gpu.State(ctx, "heisenberg", [lat16] * nrep, J=-1.0, proposal="small", delta=0.6)
-        st.set_beta(beta)
-        st.init(seed=3)
-        st.metropolis(5, 3, 0)
-        ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
-        small[name] = round(nrep * 16 ** 3 * nstep / (ms * 1e6), 3)
-        del st
-    res["L16_batch"] = {"replicas": 512, "batched_attempts_per_ns": small["batched"],
-                        "plain_attempts_per_ns": small["plain"],
-                        "batched_over_plain": round(small["batched"] / small["plain"], 1)}
+    # finite-size-scaling shape: as many FP32 replicas as fill ~256 MiB in one launch vs one per
+    # launch, L = 16, 32, 64 (a single L=128 replica already saturates HBM)
+    for Ls, nrep in ((16, 4096), (32, 512), (64, 64)):
+        latS = gpu.RegularLattice(ctx, 3, Ls)
+        small = {}
+        for name, nr, nstep in (("batched", nrep, 20), ("plain", 1, 20)):
+            st = gpu.State(ctx, "heisenberg", [latS] * nr, J=-1.0, proposal="small", delta=0.6)
+            st.set_beta(beta)
+            st.init(seed=3)
+            st.metropolis(5, 3, 0)
+            ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
+            small[name] = round(nr * Ls ** 3 * nstep / (ms * 1e6), 3)
+            del st
+        res[f"L{Ls}_batch"] = {"replicas": nrep, "batched_attempts_per_ns": small["batched"],
+                               "plain_attempts_per_ns": small["plain"],
+                               "batched_over_plain": round(small["batched"] / small["plain"], 1),
+                               "roofline_frac": round(48.0 * small["batched"] / HBM_PEAK_GBS, 4)}
     return res
 
 

@@ -36,8 +36,8 @@ def candidates(state):
     for kernels, block, xcd in itertools.product((0, 1), (64, 128, 256, 512), (0, 1)):
         if kernels == 0 and small:
             continue  # the LDS kernel ignores block / xcd: tuned below
-        if xcd == 0 and not (getattr(lat, "dim", 0) == 3 and state.model == "ising"):
-            continue  # only the row-vectorised 3D kernels remap blocks
+        if xcd == 0 and not (getattr(lat, "dim", 0) == 3 and state.model in ("ising", "heisenberg", "phi4")):
+            continue  # only the row-vectorised 3D Ising and the FP32 kernels remap blocks
         out.append({"kernels": kernels, "block": block, "xcd_remap": xcd, "lds_block": 256})
     if small:
         for lb in (128, 256, 512):

@@ -490,13 +490,25 @@ __global__ void __launch_bounds__(512) k_sweep_o3(float4* __restrict__ base, int
                                                   const int64_t* __restrict__ rep_bond_off,
                                                   const float* __restrict__ betaf, int L, int model, int proposal,
                                                   float J, float delta, float lambda, float mass, int colour,
-                                                  uint32_t phase, const uint32_t* __restrict__ dyn, int rep0, Key2 key)
+                                                  uint32_t phase, const uint32_t* __restrict__ dyn, int rep0, int xcd,
+                                                  Key2 key)
 {
     if (dyn) phase += dyn[0];
-    const int r = blockIdx.y;
+    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs in launch
+    // order; giving each XCD a contiguous run of tiles keeps the z+-1 planes of
+    // a tile in the same L2
+    uint32_t bx = blockIdx.x, by = blockIdx.y;
+    const uint32_t nblk = gridDim.x * gridDim.y;
+    if (xcd && (nblk & 7u) == 0u) {
+        const uint32_t lin = blockIdx.x + blockIdx.y * gridDim.x;
+        const uint32_t lb = (lin & 7u) * (nblk >> 3) + (lin >> 3);
+        by = lb / gridDim.x;
+        bx = lb - by * gridDim.x;
+    }
+    const int r = (int)by;
     const int grep = rep0 + r;
     const RegGeom G = reg_geom(DIM, L);
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t h = (int64_t)bx * blockDim.x + threadIdx.x;
     if (h >= G.half) return;
     float4* own = base + r * stride4 + colour * G.half;
     const float4* oth = base + r * stride4 + (1 - colour) * G.half;
@@ -853,19 +865,21 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
         if (dim == 3) {
             if (bonds)
                 hipLaunchKernelGGL((k_sweep_o3<3, true>), grid, dim3(blk), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
-                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0, key);
+                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0,
+                                   st->tune_xcd, key);
             else
                 hipLaunchKernelGGL((k_sweep_o3<3, false>), grid, dim3(blk), 0, s, b, s4, st->d_bonds,
                                    st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
-                                   colour, phase, st->cur_dyn, g.rep0, key);
+                                   colour, phase, st->cur_dyn, g.rep0, st->tune_xcd, key);
         } else {
             if (bonds)
                 hipLaunchKernelGGL((k_sweep_o3<2, true>), grid, dim3(blk), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off,
-                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0, key);
+                                   st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass, colour, phase, st->cur_dyn, g.rep0,
+                                   st->tune_xcd, key);
             else
                 hipLaunchKernelGGL((k_sweep_o3<2, false>), grid, dim3(blk), 0, s, b, s4, st->d_bonds,
                                    st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
-                                   colour, phase, st->cur_dyn, g.rep0, key);
+                                   colour, phase, st->cur_dyn, g.rep0, st->tune_xcd, key);
         }
     }
     return hipGetLastError();

@@ -29,6 +29,26 @@ def test_all_shapes_bit_exact(ctx, dim, L):
         assert all(np.array_equal(a, b) for a, b in zip(out, ref)), shape
 
 
+@pytest.mark.parametrize("model", ["heisenberg", "potts"])
+def test_all_shapes_bit_exact_other_models(ctx, model):
+    """The FP32 and Potts sweeps give the same spins under every launch shape
+    (threads per block, XCD-aware tile order)."""
+    from marqov_amd import autotune, gpu
+    lat = gpu.RegularLattice(ctx, 3, 32)
+    kw = dict(J=-1.0, q=3, proposal="small", delta=0.6)
+    ref = None
+    for shape in autotune.candidates(gpu.State(ctx, model, [lat] * 8, **kw)):
+        st = gpu.State(ctx, model, [lat] * 8, **kw)
+        st.set_tuning(**shape)
+        st.set_beta(0.6)
+        st.init(seed=4)
+        st.metropolis(3, seed=8, sweep0=0)
+        out = [st.download(r) for r in (0, 7)]
+        if ref is None:
+            ref = out
+        assert all(np.array_equal(a, b) for a, b in zip(out, ref)), shape
+
+
 def test_tune_caches_choice(ctx, tmp_path):
     from marqov_amd import autotune, gpu
     lat = gpu.RegularLattice(ctx, 3, 64)

@@ -146,3 +146,25 @@ def test_fp64_observable_accumulation(ctx, model):
         m = np.linalg.norm(s.sum(0)) / N
         assert abs(obs[r, 0] - e) <= 1e-12 * max(1.0, abs(e)), (obs[r, 0], e)
         assert abs(obs[r, 1] - m) <= 1e-12, (obs[r, 1], m)
+
+
+@pytest.mark.parametrize("model,L", [("heisenberg", 16), ("phi4", 16), ("heisenberg", 64)])
+def test_batched_equals_plain(ctx, model, L):
+    """Batching changes the launch, not the arithmetic: replica 0 of a batch and a
+    one-replica state run the same FP32 sweeps and Wolff clusters to bit-identical
+    spins, and their FP64 observables agree to summation round-off.  So the batched
+    path's observable error is the plain path's."""
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, 3, L)
+    kw = dict(J=-1.0, proposal="small", delta=0.6, lam=4.5, mass=1.0)
+    nb = 8 if L == 64 else 64
+    states = [gpu.State(ctx, model, [lat] * nb, **kw), gpu.State(ctx, model, [lat], **kw)]
+    for st in states:
+        st.set_beta(0.693)
+        st.init(seed=5)
+        st.metropolis(6, seed=9)
+        st.wolff(3, seed=9, cluster0=0)
+        st.metropolis(2, seed=9, sweep0=6)
+    assert np.array_equal(states[0].download(0), states[1].download(0))
+    ob_b, ob_p = states[0].measure()[0], states[1].measure()[0]
+    assert np.allclose(ob_b, ob_p, rtol=1e-12, atol=1e-14), (ob_b, ob_p)
