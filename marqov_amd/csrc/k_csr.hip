@@ -504,7 +504,7 @@ constexpr int CWCAP = 4096;
 // O(3)/Phi4 keep the visited bitmap and carry the flipped spin's projection
 // with each frontier entry.  Bond randoms are keyed by the canonical edge id.
 template <int KIND>
-__global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int64_t rs, int64_t ps, int64_t N,
+__global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int64_t rs, int64_t ps, int64_t N,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                    const int32_t* __restrict__ edge, const int32_t* __restrict__ pos_of,
                                                    int nclusters, uint32_t cl0, int rep0, int Q,
@@ -564,114 +564,88 @@ __global__ void __launch_bounds__(256) k_csr_wolff(char* __restrict__ base, int6
             const int tail = s_tail;
             __syncthreads();
             if (head >= tail) break;
-            for (int idx = head + (int)threadIdx.x; idx < tail; idx += blockDim.x) {
-                const bool in_ring = idx - hprev < CWCAP;
-                const int64_t cur = in_ring ? s_ring[idx & (CWCAP - 1)] : q[idx];
+            // Each wavefront takes 64 frontier entries, scans their degrees, and then
+            // runs one lane per edge: a lane finds its entry by a binary search over
+            // the scan, loads the far end, draws the edge's Philox uniform and
+            // claims.  A narrow level costs one Philox evaluation per lane instead
+            // of one per neighbour in a row; new entries are placed with one LDS
+            // atomic per wavefront.
+            const int lane = (int)(threadIdx.x & 63u);
+            const int nwv = (int)(blockDim.x >> 6);
+            for (int wb = head + (int)(threadIdx.x >> 6) * 64; wb < tail; wb += nwv * 64) {
+                const int idx = wb + lane;
+                int32_t cur = 0;
+                int kb = 0, deg = 0;
                 float pc = 0.f;
-                if (VEC) pc = in_ring ? s_proj[idx & (CWCAP - 1)] : cproj<KIND>(sp4[cur * ps], axis, rd);
-                // Neighbours in chunks of CH: all claims of a chunk are issued before
-                // any result is tested, so their round trips overlap; new entries are
-                // placed with one LDS atomic per wavefront.
-                constexpr int CH = 8;
-                const int kb = rp[cur], ke = rp[cur + 1];
-                for (int k0 = kb; k0 < ke; k0 += CH) {
-                    int32_t jn[CH];
-                    uint32_t eid[CH];
-                    bool cand[CH], ok[CH];
-                    float pnew[CH];
+                if (idx < tail) {
+                    const bool in_ring = idx - hprev < CWCAP;
+                    cur = in_ring ? s_ring[idx & (CWCAP - 1)] : q[idx];
+                    kb = rp[cur];
+                    deg = rp[cur + 1] - kb;
+                    if (VEC) pc = in_ring ? s_proj[idx & (CWCAP - 1)] : cproj<KIND>(sp4[(int64_t)cur * ps], axis, rd);
+                }
+                int inc = deg;
 #pragma unroll
-                    for (int t = 0; t < CH; ++t) {
-                        const bool in = k0 + t < ke;
-                        jn[t] = in ? ci[k0 + t] : 0;
-                        eid[t] = in ? (uint32_t)edge[k0 + t] : 0u;
-                        cand[t] = in;
-                        pnew[t] = 0.f;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int v = __shfl_up(inc, o);
+                    if (lane >= o) inc += v;
+                }
+                const int total = __shfl(inc, 63);
+                const int pre = inc - deg;
+                for (int t0 = 0; t0 < total; t0 += 64) {  // wavefront-uniform
+                    const int t = t0 + lane;
+                    // owner: the last lane whose scan start is <= t (its degree is then > t - start)
+                    int e = 0;
+#pragma unroll
+                    for (int stp = 32; stp > 0; stp >>= 1) {
+                        const int pe = __shfl(pre, e + stp);
+                        if (pe <= t) e += stp;
                     }
-                    if (KIND == CK_ISING) {
-                        uint32_t old[CH];
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) {
-                            if (!cand[t]) continue;
-                            const uint4 bw = philox4x32_10(eid[t] >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
-                            cand[t] = u4_get(bw, (int)(eid[t] & 3u)) < tw;
-                        }
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) old[t] = cand[t] ? claim_ising_issue(sp8 + jn[t] * ps, val) : 0u;
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) ok[t] = cand[t] && byte_at(old[t], sp8 + jn[t] * ps) == (int8_t)val;
-                    } else if (KIND == CK_POTTS) {
-                        uint32_t seen[CH], prev[CH];
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) {
+                    const int k = __shfl(kb, e) + (t - __shfl(pre, e));
+                    const float epc = VEC ? __shfl(pc, e) : 0.f;
+                    bool ok = false;
+                    int32_t j = 0;
+                    float pnew = 0.f;
+                    if (t < total) {
+                        j = ci[k];
+                        const uint32_t eid = (uint32_t)edge[k];
+                        if (KIND == CK_ISING) {
+                            // the claim itself tests alignment
+                            const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                            if (u4_get(bw, (int)(eid & 3u)) < tw) ok = claim_ising(sp8 + (int64_t)j * ps, val);
+                        } else if (KIND == CK_POTTS) {
                             uint32_t sh;
-                            seen[t] = cand[t] ? peek_word(sp8 + jn[t] * ps, sh) : 0u;
-                        }
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) {
-                            if (!cand[t]) continue;
-                            cand[t] = false;
-                            if (byte_at(seen[t], sp8 + jn[t] * ps) == (int8_t)val) {
-                                const uint4 bw = philox4x32_10(eid[t] >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
-                                cand[t] = u4_get(bw, (int)(eid[t] & 3u)) < tw;
+                            const uint32_t seen = peek_word(sp8 + (int64_t)j * ps, sh);  // in flight during the Philox rounds
+                            const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                            if ((int8_t)(seen >> sh) == (int8_t)val && u4_get(bw, (int)(eid & 3u)) < tw) {
+                                uint32_t* w = spin_word(sp8 + (int64_t)j * ps, sh);
+                                const uint32_t prev = atomicCAS(w, seen, (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
+                                // the word changed under the first swap: retry from what it held
+                                ok = (prev == seen) || claim_potts(sp8 + (int64_t)j * ps, prev, val, newq);
                             }
-                        }
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) {
-                            prev[t] = seen[t];
-                            if (cand[t]) {
-                                uint32_t sh;
-                                uint32_t* w = spin_word(sp8 + jn[t] * ps, sh);
-                                prev[t] = atomicCAS(w, seen[t], (seen[t] & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
-                            }
-                        }
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) {
-                            ok[t] = cand[t] && prev[t] == seen[t];
-                            // the word changed under the first swap: retry from what it held
-                            if (cand[t] && !ok[t]) ok[t] = claim_potts(sp8 + jn[t] * ps, prev[t], val, newq);
-                        }
-                    } else {
-                        float4 vj[CH];
-                        uint32_t ob[CH];
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) vj[t] = cand[t] ? sp4[jn[t] * ps] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) {
-                            if (!cand[t]) continue;
-                            const float cpl = J * pc * cproj<KIND>(vj[t], axis, rd);
-                            cand[t] = false;
-                            if (cpl > 0.f) {
-                                const uint4 bw = philox4x32_10(eid[t] >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
-                                cand[t] = u01f(u4_get(bw, (int)(eid[t] & 3u))) < -expm1f(-2.0f * beta * cpl);
-                            }
-                        }
-#pragma unroll
-                        for (int t = 0; t < CH; ++t)
-                            ob[t] = cand[t] ? atomicOr(&vis[jn[t] >> 5], 1u << (jn[t] & 31)) : 0u;
-#pragma unroll
-                        for (int t = 0; t < CH; ++t) {
-                            ok[t] = cand[t] && !(ob[t] & (1u << (jn[t] & 31)));
-                            if (ok[t]) {
-                                // the claim succeeded, so nobody changed vj since it was read
-                                const float4 v = creflect<KIND>(vj[t], axis, rd);
-                                sp4[jn[t] * ps] = v;
-                                pnew[t] = cproj<KIND>(v, axis, rd);
+                        } else {
+                            const float4 vj = sp4[(int64_t)j * ps];  // in flight during the Philox rounds
+                            const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                            const float cpl = J * epc * cproj<KIND>(vj, axis, rd);
+                            if (cpl > 0.f && u01f(u4_get(bw, (int)(eid & 3u))) < -expm1f(-2.0f * beta * cpl)) {
+                                const uint32_t ob = atomicOr(&vis[j >> 5], 1u << (j & 31));
+                                ok = !(ob & (1u << (j & 31)));
+                                if (ok) {
+                                    // the claim succeeded, so nobody changed vj since it was read
+                                    const float4 v = creflect<KIND>(vj, axis, rd);
+                                    sp4[(int64_t)j * ps] = v;
+                                    pnew = cproj<KIND>(v, axis, rd);
+                                }
                             }
                         }
                     }
-                    uint32_t nok = 0;
-#pragma unroll
-                    for (int t = 0; t < CH; ++t) nok += ok[t] ? 1u : 0u;
-                    int pos = wave_reserve(&s_tail, nok);
-#pragma unroll
-                    for (int t = 0; t < CH; ++t) {
-                        if (!ok[t]) continue;
+                    const int pos = wave_reserve(&s_tail, ok ? 1u : 0u);
+                    if (ok) {
                         if (pos - head < CWCAP) {
-                            s_ring[pos & (CWCAP - 1)] = jn[t];
-                            if (VEC) s_proj[pos & (CWCAP - 1)] = pnew[t];
+                            s_ring[pos & (CWCAP - 1)] = j;
+                            if (VEC) s_proj[pos & (CWCAP - 1)] = pnew;
                         }
-                        if (VEC || pos - head >= CWCAP) q[pos] = jn[t];  // VEC: kept for the bitmap reset
-                        ++pos;
+                        if (VEC || pos - head >= CWCAP) q[pos] = j;  // VEC: kept for the bitmap reset
                     }
                 }
             }
@@ -938,7 +912,7 @@ hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_
     hipStream_t s = st->ctx->stream;
     const float J = (float)st->p.J;
 #define MQ_W(K)                                                                                                   \
-    hipLaunchKernelGGL((k_csr_wolff<K>), dim3(g.nrep), dim3(256), 0, s, st->d_spins + g.off, (int64_t)1,          \
+    hipLaunchKernelGGL((k_csr_wolff<K>), dim3(g.nrep), dim3(wolff_block(st, g.nrep)), 0, s, st->d_spins + g.off, (int64_t)1,          \
                        (int64_t)g.nrep, l->N,                                                                    \
                        l->d_rp, l->d_ci, l->d_edge, l->d_pos_of, nclusters, cl0, g.rep0, st->p.q, st->d_wthr,    \
                        st->d_betaf, J, st->d_queue, st->d_visit, st->d_rep_site_off, st->d_csize, key)

@@ -88,7 +88,7 @@ __device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
 constexpr int WCAP = 4096;
 
 template <int DIM, int KIND>
-__global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, int64_t stride, int L, int nclusters,
+__global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base, int64_t stride, int L, int nclusters,
                                                        uint32_t cl0, int rep0, int Q,
                                                        const uint32_t* __restrict__ wthr,
                                                        const float* __restrict__ betaf, float J,
@@ -153,7 +153,16 @@ __global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, 
             const int tail = s_tail;
             __syncthreads();
             if (head >= tail) break;
-            for (int idx = head + (int)threadIdx.x; idx < tail; idx += blockDim.x) {
+            // One lane per (frontier site, bond slot): a narrow level then costs one
+            // Philox evaluation and one claim round trip per lane instead of 2*DIM
+            // Philox evaluations in a row; new entries are placed with one LDS
+            // atomic per wavefront.
+            constexpr int NS = 2 * DIM;
+            const int ntask = (tail - head) * NS;
+            for (int t = (int)threadIdx.x; t < ntask; t += (int)blockDim.x) {
+                const int e = t / NS;
+                const int sl = t - e * NS;
+                const int idx = head + e;
                 const bool in_ring = idx - hprev < WCAP;
                 const uint32_t cur = (uint32_t)(in_ring ? s_ring[idx & (WCAP - 1)] : q[idx]);
                 // coordinates of cur; its neighbours all have the other colour
@@ -169,111 +178,53 @@ __global__ void __launch_bounds__(256) k_wolff_regular(char* __restrict__ base, 
                     }
                 }
                 const uint32_t nbase = ((x[0] + x[1] + x[2]) & 1u) ? 0u : half;
-                float pc = 0.f;
-                if (VEC) pc = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[split_addr<DIM>(L, half, cur)], axis, rd);
-                // All claims of this site are issued before any result is tested, so
-                // their round trips to L2 overlap; the new entries are then placed
-                // with one LDS atomic per wavefront.
-                constexpr int NS = 2 * DIM;
-                uint32_t jn[NS], jan[NS];
-                bool cand[NS], ok[NS];
-                float pnew[NS];
-#pragma unroll
-                for (int sl = 0; sl < NS; ++sl) {
-                    const int d = sl >> 1;
-                    const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
-                    uint32_t j;
-                    if ((sl & 1) == 0) j = (x[d] == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
-                    else j = (x[d] == 0) ? cur + (uL - 1) * pw : cur - pw;
-                    jn[sl] = j;
-                    jan[sl] = nbase + (j >> 1);
-                    pnew[sl] = 0.f;
-                }
+                const int d = sl >> 1;
+                const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
+                const uint32_t xd = d == 0 ? x[0] : (d == 1 ? x[1] : x[2]);
+                uint32_t j;
+                if ((sl & 1) == 0) j = (xd == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
+                else j = (xd == 0) ? cur + (uL - 1) * pw : cur - pw;
+                const uint32_t jan = nbase + (j >> 1);
+                const uint64_t b = (uint64_t)((sl & 1) ? j : cur) * DIM + (uint64_t)d;
+                bool ok = false;
+                float pnew = 0.f;
                 if (KIND == WK_ISING) {
-                    // bond randoms first: the claim itself tests alignment
-                    uint32_t old[NS];
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) {
-                        const uint64_t b = (uint64_t)((sl & 1) ? jn[sl] : cur) * DIM + (uint64_t)(sl >> 1);
-                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        cand[sl] = u4_get(bw, (int)(b & 3)) < tw;
-                    }
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) old[sl] = cand[sl] ? claim_ising_issue(sp8 + jan[sl], val) : 0u;
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) ok[sl] = cand[sl] && byte_at(old[sl], sp8 + jan[sl]) == (int8_t)val;
+                    // the claim itself tests alignment
+                    const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                    if (u4_get(bw, (int)(b & 3)) < tw) ok = claim_ising(sp8 + jan, val);
                 } else if (KIND == WK_POTTS) {
-                    uint32_t seen[NS], prev[NS];
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) {
-                        uint32_t sh;
-                        seen[sl] = peek_word(sp8 + jan[sl], sh);
-                    }
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) {
-                        cand[sl] = false;
-                        if (byte_at(seen[sl], sp8 + jan[sl]) == (int8_t)val) {
-                            const uint64_t b = (uint64_t)((sl & 1) ? jn[sl] : cur) * DIM + (uint64_t)(sl >> 1);
-                            const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                            cand[sl] = u4_get(bw, (int)(b & 3)) < tw;
-                        }
-                    }
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) {
-                        prev[sl] = seen[sl];
-                        if (cand[sl]) {
-                            uint32_t sh;
-                            uint32_t* w = spin_word(sp8 + jan[sl], sh);
-                            prev[sl] = atomicCAS(w, seen[sl], (seen[sl] & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
-                        }
-                    }
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) {
-                        ok[sl] = cand[sl] && prev[sl] == seen[sl];
+                    uint32_t sh;
+                    const uint32_t seen = peek_word(sp8 + jan, sh);  // in flight during the Philox rounds
+                    const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                    if ((int8_t)(seen >> sh) == (int8_t)val && u4_get(bw, (int)(b & 3)) < tw) {
+                        uint32_t* w = spin_word(sp8 + jan, sh);
+                        const uint32_t prev = atomicCAS(w, seen, (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
                         // the word changed under the first swap: retry from what it held
-                        if (cand[sl] && !ok[sl]) ok[sl] = claim_potts(sp8 + jan[sl], prev[sl], val, newq);
+                        ok = (prev == seen) || claim_potts(sp8 + jan, prev, val, newq);
                     }
                 } else {
-                    float4 vj[NS];
-                    uint32_t ob[NS];
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) vj[sl] = sp4[jan[sl]];
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) {
-                        const float cpl = J * pc * proj<KIND>(vj[sl], axis, rd);
-                        cand[sl] = false;
-                        if (cpl > 0.f) {
-                            const uint64_t b = (uint64_t)((sl & 1) ? jn[sl] : cur) * DIM + (uint64_t)(sl >> 1);
-                            const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                            cand[sl] = u01f(u4_get(bw, (int)(b & 3))) < -expm1f(-2.0f * beta * cpl);
-                        }
-                    }
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) ob[sl] = cand[sl] ? atomicOr(&vis[jn[sl] >> 5], 1u << (jn[sl] & 31)) : 0u;
-#pragma unroll
-                    for (int sl = 0; sl < NS; ++sl) {
-                        ok[sl] = cand[sl] && !(ob[sl] & (1u << (jn[sl] & 31)));
-                        if (ok[sl]) {
+                    const float pc = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[split_addr<DIM>(L, half, cur)], axis, rd);
+                    const float4 vj = sp4[jan];  // in flight during the Philox rounds
+                    const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                    const float cpl = J * pc * proj<KIND>(vj, axis, rd);
+                    if (cpl > 0.f && u01f(u4_get(bw, (int)(b & 3))) < -expm1f(-2.0f * beta * cpl)) {
+                        const uint32_t ob = atomicOr(&vis[j >> 5], 1u << (j & 31));
+                        ok = !(ob & (1u << (j & 31)));
+                        if (ok) {
                             // the claim succeeded, so nobody changed vj since it was read
-                            const float4 v = reflect<KIND>(vj[sl], axis, rd);
-                            sp4[jan[sl]] = v;
-                            pnew[sl] = proj<KIND>(v, axis, rd);
+                            const float4 v = reflect<KIND>(vj, axis, rd);
+                            sp4[jan] = v;
+                            pnew = proj<KIND>(v, axis, rd);
                         }
                     }
                 }
-                uint32_t nok = 0;
-#pragma unroll
-                for (int sl = 0; sl < NS; ++sl) nok += ok[sl] ? 1u : 0u;
-                int pos = wave_reserve(&s_tail, nok);
-#pragma unroll
-                for (int sl = 0; sl < NS; ++sl) {
-                    if (!ok[sl]) continue;
+                int pos = wave_reserve(&s_tail, ok ? 1u : 0u);
+                if (ok) {
                     if (pos - head < WCAP) {
-                        s_ring[pos & (WCAP - 1)] = (int32_t)jn[sl];
-                        if (VEC) s_proj[pos & (WCAP - 1)] = pnew[sl];
+                        s_ring[pos & (WCAP - 1)] = (int32_t)j;
+                        if (VEC) s_proj[pos & (WCAP - 1)] = pnew;
                     }
-                    if (VEC || pos - head >= WCAP) q[pos] = (int32_t)jn[sl];  // VEC: kept for the bitmap reset
-                    ++pos;
+                    if (VEC || pos - head >= WCAP) q[pos] = (int32_t)j;  // VEC: kept for the bitmap reset
                 }
             }
             __syncthreads();
@@ -447,7 +398,7 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
     else if (model == MQ_HEISENBERG && st->p.embedding == MQ_EMBED_CARTESIAN) kind = WK_CART;
     else if (model == MQ_HEISENBERG || model == MQ_PHI4) kind = WK_RDIR;
     else return hipErrorInvalidValue;
-    const dim3 grid(g.nrep), block(256);
+    const dim3 grid(g.nrep), block(wolff_block(st, g.nrep));
     hipStream_t s = st->ctx->stream;
     char* base = st->d_spins + g.off;
     const float J = (float)st->p.J;

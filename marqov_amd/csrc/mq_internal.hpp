@@ -267,6 +267,7 @@ struct mq_state {
     int lds_block = 256;                // threads per block of the LDS multi-sweep kernel
     int kernel_choice = 0;              // 0 auto, 1 streaming kernels only, 2 generic streaming kernel only  // force the generic kernels (parity of the specialised ones)
     int wolff_ring = 512;               // LDS frontier ring entries of the small-lattice Wolff kernel (power of two, 64..2048)
+    int wolff_block = 0;                // threads per replica of the global-memory Wolff kernels (0 = automatic)
 };
 
 // --------------------------------------------------------------------------
@@ -304,6 +305,17 @@ hipError_t launch_init_csr(mq_state* st, const Group& g, Key2 key);
 hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
 hipError_t launch_measure_csr(mq_state* st, const Group& g);
 hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
+
+// Threads of the workgroup that grows one replica's clusters in global memory.
+// A wide frontier level is worked through blockDim entries at a time, each
+// waiting on its claims' round trips, so with no more replicas than CUs (one
+// workgroup per CU either way) the widest block keeps the most claims in
+// flight; with more replicas, 256-thread blocks let several share a CU.
+inline int wolff_block(const mq_state* st, int nwg)
+{
+    if (st->wolff_block > 0) return st->wolff_block;
+    return nwg <= st->ctx->num_cus ? 1024 : 256;
+}
 
 hipError_t launch_energy_all(mq_state* st);
 hipError_t launch_pt_swap(mq_state* st, int ntemp, int parity, uint32_t step, Key2 key);

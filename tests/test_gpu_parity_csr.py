@@ -72,9 +72,11 @@ def test_csr_ea_bonds_bit_exact(ctx):
     assert np.array_equal(st.download(0), exp)
 
 
+@pytest.mark.parametrize("block", ["256", "1024"])
 @pytest.mark.parametrize("model,beta", [("ising", 0.9), ("potts", 1.6)])
-def test_csr_wolff_bit_exact(ctx, model, beta):
+def test_csr_wolff_bit_exact(ctx, model, beta, block, monkeypatch):
     from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
     rp, ci = _graph(2000, 7)
     N = len(rp) - 1
     lat = _lat(ctx, rp, ci)
@@ -155,12 +157,15 @@ def _random_graph(n, mean_degree, seed):
     return np.cumsum(rp).astype(np.int32), dst.astype(np.int32)
 
 
+@pytest.mark.parametrize("block", ["256", "1024"])
 @pytest.mark.parametrize("model", ["ising", "potts"])
-def test_csr_wolff_frontier_spill_bit_exact(ctx, model):
+def test_csr_wolff_frontier_spill_bit_exact(ctx, model, block, monkeypatch):
     """A random graph of mean degree 16 in an ordered start with bond probability
     1 - e^-2: BFS fronts of thousands of vertices overflow the 4096-entry LDS ring and
-    spill to the global queue; the clusters must still equal the CPU replay."""
+    spill to the global queue (with 256- and 1024-thread workgroups); the clusters must
+    still equal the CPU replay."""
     from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
     n = 24000
     rp, ci = _random_graph(n, 16, 5)
     lat = _lat(ctx, rp, ci)

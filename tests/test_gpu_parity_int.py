@@ -292,15 +292,17 @@ def test_wolff_lds_and_streaming_bit_exact(ctx, kernels, model, dim, L, monkeypa
 
 
 @pytest.mark.parametrize("model", ["ising", "potts"])
-@pytest.mark.parametrize("kernels,ring,dim,L", [("0", "64", 2, 32), ("0", "64", 3, 16), ("1", "2048", 3, 64)])
-def test_wolff_frontier_spill_bit_exact(ctx, model, kernels, ring, dim, L, monkeypatch):
+@pytest.mark.parametrize("kernels,ring,dim,L,block", [("0", "64", 2, 32, "0"), ("0", "64", 3, 16, "0"),
+                                                      ("1", "2048", 3, 64, "256"), ("1", "2048", 3, 64, "1024")])
+def test_wolff_frontier_spill_bit_exact(ctx, model, kernels, ring, dim, L, block, monkeypatch):
     """Ordered starts at large beta grow near-percolating clusters whose BFS fronts
     overflow the LDS frontier ring (64 entries forced in the small-lattice kernel, 4096 in
-    the streaming kernel at 3D L=64): entries spilled to the global queue must leave the
-    clusters equal to the CPU replay."""
+    the streaming kernel at 3D L=64, with 256- and 1024-thread workgroups): entries spilled
+    to the global queue must leave the clusters equal to the CPU replay."""
     from marqov_amd import gpu
     monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
     monkeypatch.setenv("MARQOV_AMD_WOLFF_RING", ring)
+    monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
     N = L ** dim
     lat = gpu.RegularLattice(ctx, dim, L)
     st = gpu.State(ctx, model, [lat] * 2, J=-1.0, q=3)

@@ -168,3 +168,36 @@ def test_batched_equals_plain(ctx, model, L):
     assert np.array_equal(states[0].download(0), states[1].download(0))
     ob_b, ob_p = states[0].measure()[0], states[1].measure()[0]
     assert np.allclose(ob_b, ob_p, rtol=1e-12, atol=1e-14), (ob_b, ob_p)
+
+
+@pytest.mark.parametrize("model,embedding,lattice", [("heisenberg", "cartesian", "regular"),
+                                                     ("phi4", "random", "regular"),
+                                                     ("heisenberg", "random", "graph")])
+def test_vec_wolff_block_invariant(ctx, model, embedding, lattice, monkeypatch):
+    """The O(3)/Phi4 cluster is the connected component of activated bonds, whose
+    activation depends only on the two end spins before the flip and on the bond's
+    Philox uniform; so the FP32 spins after Wolff moves are bit-identical whether the
+    workgroup growing a cluster has 256 or 1024 threads."""
+    from marqov_amd import gpu
+    if lattice == "regular":
+        lat = gpu.RegularLattice(ctx, 3, 16)
+    else:
+        from marqov_amd import graphs
+        n = 4096
+        rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n, 8.0), 11)
+        lat = gpu.CsrLattice(ctx, rp, ci)
+    out = {}
+    for block in ("256", "1024"):
+        monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
+        st = gpu.State(ctx, model, [lat] * 3, J=-1.0, proposal="small", delta=0.6, embedding=embedding)
+        st.set_beta(0.9 if lattice == "regular" else 1.2)
+        st.init(seed=21)
+        sizes = []
+        for t in range(4):
+            st.metropolis(3, seed=22, sweep0=3 * t)
+            sizes.append(st.wolff(5, seed=23, cluster0=5 * t))
+        out[block] = (np.concatenate(sizes, axis=1), [st.download(r) for r in range(3)])
+    assert out["256"][0].max() > 20, out["256"][0]
+    assert np.array_equal(out["256"][0], out["1024"][0])
+    for r in range(3):
+        assert np.array_equal(out["256"][1][r], out["1024"][1][r]), r
