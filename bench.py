@@ -9,6 +9,7 @@ every replica; value = spin-flip attempts/ns over the timed steps.
 
 Beside it on the same JSON line (skip with --quick):
   plain        the same sweep, one replica per launch
+  wolff        Wolff clusters on the thermalised 32-replica state vs one replica
   c1_2d_L64    2D Ising L=64 EMCS schedule (10 Wolff + 1 sweep per EMCS), batched vs plain
   c3_heis_L128 Heisenberg FP32 sweeps + Wolff, batched vs plain
   c4_ea_batch  512 +-J samples x 32 temperatures, L in {8,12,16,24}, HIP-graph
@@ -16,7 +17,8 @@ Beside it on the same JSON line (skip with --quick):
   c5_rgg       2D random geometric graph N=2^20, Ising and Potts-3, 64 replicas vs 1
   roofline     the headline sweep kernel's algorithmic HBM rate (3 B/attempt)
   cpu_baseline the reference-faithful CPU restatement (random-site Metropolis,
-               mt19937_64), one replica per thread on this host's cores
+               mt19937_64; and its stack Wolff on 2D L=64), one replica per thread
+               on this host's cores
 """
 import argparse
 import json
@@ -66,6 +68,27 @@ def timed_wolff(ctx, st, nclusters, seed, cluster0):
     box = {}
     ms, _ = timed(ctx, lambda: box.setdefault("sizes", st.wolff(nclusters, seed, cluster0=cluster0, sizes=True)))
     return ms, box["sizes"]
+
+
+def wolff_rate(ctx, st, rounds, nsw, ncl, seed, ncalls, sweep0, cluster0=0):
+    """Wolff throughput on a thermalised state: `rounds` x (nsw sweeps + ncl clusters)
+    untimed, then `ncalls` timed calls of ncl clusters per replica.  A call lasts as long
+    as its slowest replica's clusters, so this is the batch's real rate.  Returns
+    (sites/ns, mean cluster size of the timed clusters)."""
+    c, sw = cluster0, sweep0
+    for _ in range(rounds):
+        st.metropolis(nsw, seed, sweep0=sw)
+        sw += nsw
+        st.wolff(ncl, seed, cluster0=c, sizes=False)
+        c += ncl
+    ms_tot, sites, n = 0.0, 0, 0
+    for _ in range(ncalls):
+        ms, sizes = timed_wolff(ctx, st, ncl, seed, c)
+        c += ncl
+        ms_tot += ms
+        sites += int(sizes.sum())
+        n += sizes.size
+    return sites / (ms_tot * 1e6), sites / n
 
 
 def leg_c2_msc(gpu, ctx, L=256, R=32, steps=100):
@@ -125,17 +148,17 @@ def leg_c3(gpu, ctx):
         st.init(seed=3)
         st.metropolis(5, 3, 0)
         ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
-        st.wolff(4, 3, cluster0=0, sizes=False)
-        ms_w, sizes = timed_wolff(ctx, st, 16, 3, 4)
+        w_rate, w_mean = wolff_rate(ctx, st, 30, 10, 10, 3, 4, sweep0=5 + nstep)
         res[name] = {"replicas": nrep, "attempts_per_ns": round(nrep * N * nstep / (ms * 1e6), 3),
-                     "wolff_sites_per_ns": round(float(sizes.sum()) / (ms_w * 1e6), 4),
-                     "mean_cluster": round(float(sizes.mean()), 1),
+                     "wolff_sites_per_ns": round(w_rate, 4),
+                     "mean_cluster": round(w_mean, 1),
                      "roofline_frac": round(48.0 * nrep * N * nstep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         if name == "batched":
             obs = st.measure()
             res[name]["e"] = float(obs[:, 0].mean())
         del st
     res["batched_over_plain"] = round(res["batched"]["attempts_per_ns"] / res["plain"]["attempts_per_ns"], 2)
+    res["wolff"] = "after 30 x (10 sweeps + 10 clusters); 4 timed calls of 10 clusters per replica"
     # finite-size-scaling shape: as many FP32 replicas as fill ~256 MiB in one launch vs one per
     # launch, L = 16, 32, 64 (a single L=128 replica already saturates HBM)
     for Ls, nrep in ((16, 4096), (32, 512), (64, 64)):
@@ -212,13 +235,13 @@ def leg_c5(gpu, ctx):
             ms, _ = timed(ctx, lambda: st.metropolis(nstep, 5, sweep0=2))
             res[f"{model}_{name}_attempts_per_ns"] = round(nrep * n * nstep / (ms * 1e6), 3)
             if name == "batched":
-                st.wolff(2, 5, cluster0=0, sizes=False)
-                ms_w, sizes = timed_wolff(ctx, st, 4, 5, 2)
-                res[f"{model}_wolff_sites_per_ns"] = round(float(sizes.sum()) / (ms_w * 1e6), 4)
-                res[f"{model}_wolff_mean_cluster"] = round(float(sizes.mean()), 1)
+                w_rate, w_mean = wolff_rate(ctx, st, 20, 4, 4, 5, 4, sweep0=2 + nstep)
+                res[f"{model}_wolff_sites_per_ns"] = round(w_rate, 4)
+                res[f"{model}_wolff_mean_cluster"] = round(w_mean, 1)
             del st
         res[f"{model}_batched_over_plain"] = round(res[f"{model}_batched_attempts_per_ns"] /
                                                    res[f"{model}_plain_attempts_per_ns"], 2)
+    res["wolff"] = "64 replicas after 20 x (4 sweeps + 4 clusters); 4 timed calls of 4 clusters per replica"
     return res
 
 
@@ -256,6 +279,10 @@ def main():
     bytes_per_launch = 3.0 * R * (N // 2)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     obs = st.measure()
+    wolff_b = None
+    if not a.quick:
+        log("headline state: Wolff clusters")
+        wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 3, sweep0=a.warmup + a.steps)
     del st
 
     line = {
@@ -295,10 +322,13 @@ def main():
         ms_p, _ = timed(ctx, lambda: sp.metropolis(k, seed=2, sweep0=4))
         line["plain"] = {"attempts_per_ns": round(k * N / (ms_p * 1e6), 3), "replicas": 1, "steps": k}
         line["batched_over_plain"] = round(value / line["plain"]["attempts_per_ns"], 3)
-        sp.wolff(4, 2, cluster0=0, sizes=False)
-        ms_w, sizes = timed_wolff(ctx, sp, 16, 2, 4)
-        line["plain"]["wolff_sites_per_ns"] = round(float(sizes.sum()) / (ms_w * 1e6), 4)
-        line["plain"]["wolff_mean_cluster"] = round(float(sizes.mean()), 1)
+        w_rate, w_mean = wolff_rate(ctx, sp, 20, 4, 10, 2, 3, sweep0=4 + k)
+        line["plain"]["wolff_sites_per_ns"] = round(w_rate, 4)
+        line["plain"]["wolff_mean_cluster"] = round(w_mean, 1)
+        line["wolff"] = {"sites_per_ns": round(wolff_b[0], 4), "mean_cluster": round(wolff_b[1], 1),
+                         "replicas": R, "plain_sites_per_ns": round(w_rate, 4),
+                         "batched_over_plain": round(wolff_b[0] / w_rate, 2),
+                         "schedule": "after 20 x (4 sweeps + 10 clusters); 3 timed calls of 10 clusters per replica"}
         del sp
         for name, fn in (("c2_msc", leg_c2_msc), ("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3),
                          ("c4_ea_batch", leg_c4), ("c5_rgg", leg_c5)):
@@ -322,6 +352,13 @@ def main():
                                 "kind": "port", "per_core": round(float(out[1]), 5),
                                 "sample": f"{nth} threads x (1 warm-up + 3 timed) random-site Metropolis sweeps "
                                           f"of 3D Ising L={L}, one replica per thread, {out[2]:.1f} s timed"}
+        # Wolff on the 2D L=64 critical Ising lattice, the configuration the CPU reference runs
+        ow = np.zeros(3)
+        ob.lib().ref_bench_regular(1, 2, 64, -1.0, 2, 0.4406868, 500, 20000, nth, 1, 1, ob.p(ow))
+        line["cpu_baseline"]["wolff_2d_L64"] = {
+            "value": round(float(ow[0]), 5), "unit": "sites/ns", "cores": nth, "per_core": round(float(ow[1]), 5),
+            "sample": f"{nth} threads x (500 warm-up + 20000 timed) stack Wolff clusters of 2D Ising L=64 at "
+                      f"beta_c, one replica per thread, {ow[2]:.1f} s timed"}
     print(json.dumps(line), flush=True)
 
 
