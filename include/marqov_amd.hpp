@@ -105,6 +105,8 @@ inline mq_model_params params_of(const Heisenberg& m)
     p.embedding = m.embedding;
     return p;
 }
+// A model chosen at run time (q, couplings and proposals read from a configuration).
+inline mq_model_params params_of(const mq_model_params& p) { return p; }
 inline mq_model_params params_of(const Phi4& m)
 {
     mq_model_params p{};
@@ -137,6 +139,7 @@ public:
     Batch& operator=(const Batch&) = delete;
     mq_state* get() const { return h_; }
     int nrep() const { return nrep_; }
+    void check(int rc, const char* what) const { ctx_.check(rc, what); }
 
     void set_beta(const std::vector<double>& beta) { ctx_.check(mq_state_set_beta(h_, beta.data()), "mq_state_set_beta"); }
     void init(uint64_t seed) { ctx_.check(mq_state_init(h_, seed), "mq_state_init"); }
@@ -187,8 +190,7 @@ public:
     std::vector<double> move(int ncluster, int nmetro)
     {
         std::vector<double> mean(b_.nrep());
-        int rc = mq_emcs(b_.get(), ncluster, nmetro, seed_, step_++, mean.data());
-        if (rc != MQ_OK) throw Error("mq_emcs failed");
+        b_.check(mq_emcs(b_.get(), ncluster, nmetro, seed_, step_++, mean.data()), "mq_emcs");
         return mean;
     }
     uint64_t steps() const { return step_; }

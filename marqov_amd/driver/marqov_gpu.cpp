@@ -1,0 +1,455 @@
+// marqov_gpu -- runs a MARQOV .ini configuration on one MI355X.
+//
+//   marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run]
+//
+// Reads <Hamiltonian>.ini from the configuration directory (default ./config,
+// as check_registry_availability, src/libmarqov/util/startup.h:69-80) with the
+// reference's registry rules and runs what the reference's drivers run for it
+// (demo/{Ising,Heisenberg,Phi4,Potts,IsingRGG}.cpp with RegularLatticeLoop,
+// src/libmarqov/util/regularlatticeloop.h:11-136):
+//   * parameter sets are the Cartesian product of the model's lists, the
+//     first list varying fastest (cart_prod, util/cartprod.h); Ising,
+//     Heisenberg, Potts<q>, EdwardsAnderson-Ising and IsingRGG take (beta, J),
+//     Phi4 takes (beta, lambda, mass) with J = -1 (demo/Phi4.cpp:70-79);
+//   * for each L of the list: ncluster = int(nclusteramp * L^nclusterexp), and
+//     every parameter set runs `rep` replicas (one value, or one per L) with
+//     repid 0..rep-1 (replicator, util/replicate.h:34-52);
+//   * a simulation runs 10 * (warmupsteps/10) EMCS, then 10 * (measuresteps/10)
+//     EMCS each followed by one measurement (Core::wrmploop / gameloop,
+//     core.h:1001-1050); an EMCS is ncluster Wolff moves and nmetro sweeps
+//     (emcs.h:62-109).
+// "Potts" picks q from PottsSelect.ini [Which] q as demo/Potts.cpp does.
+//
+// All replicas of one L and one set of non-temperature parameters run as one
+// batch on the GPU (one mq_state): each EMCS step is a handful of launches for
+// the whole batch.  Periodic lattices with dimension 2 or 3 and even L use the
+// checkerboard kernels; odd L and dimension 4 use the graph kernels on the
+// same lattice as a CSR graph.  IsingRGG builds a periodic random geometric
+// graph of L^dim points with mean degree 6 per repid (demo/IsingRGG.cpp).
+// EdwardsAnderson-Ising draws +-1 couplings with probability 1/2 each
+// (BimodalPDF, src/lattice/util/prob_density.h:25-35), one disorder sample per
+// repid shared by all temperatures.
+//
+// Output (HDF5 is not part of this path): one text file per simulation,
+// <out>/<L>/beta<beta>_<repid>.txt (the reference's outpath and defaultfilter
+// name, util/filters.h:28-37, with .txt for .h5), holding a commented header
+// and one row per measurement: step, E/N, |m|, m^2, m^4, mean cluster size of
+// that EMCS.  <out>/<L>/summary.txt holds per-simulation means.
+#include <sys/stat.h>
+
+#include <cerrno>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lattices.hpp"
+#include "marqov_amd.hpp"
+#include "registry.hpp"
+
+using namespace marqov_amd;
+
+namespace {
+
+struct Options {
+    std::string ham;
+    std::string config = "./config";
+    std::string out;
+    uint64_t seed = 1;
+    int device = 0;
+    bool dry = false;
+};
+
+struct Spec {
+    std::string block;  // configuration block and file stem, e.g. "Potts3"
+    int model = MQ_ISING;
+    int q = 2;
+    bool rgg = false;
+};
+
+struct ParamSet {
+    double beta;
+    double J, lambda, mass;
+};
+
+struct Sim {
+    ParamSet p;
+    int repid;
+    int id;
+};
+
+struct Group {  // one mq_state: sims sharing everything but beta and repid
+    std::vector<Sim> sims;
+};
+
+uint64_t mix(uint64_t a, uint64_t b)
+{
+    uint64_t z = a ^ (b + 0x9E3779B97F4A7C15ull + (a << 6) + (a >> 2));
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void make_dir(const std::string& path)
+{
+    std::string acc;
+    for (std::size_t i = 0; i <= path.size(); ++i) {
+        if (i == path.size() || path[i] == '/') {
+            if (!acc.empty() && mkdir(acc.c_str(), 0755) != 0 && errno != EEXIST)
+                throw std::runtime_error("cannot create directory " + acc);
+        }
+        if (i < path.size()) acc += path[i];
+    }
+}
+
+// beta<beta>_<repid>, beta formatted by std::to_string as the reference's
+// defaultfilter does (util/filters.h:28-37); with several groups of
+// non-temperature parameters (the reference would overwrite one file with the
+// next) the group index is appended.
+std::string sim_name(double beta, int repid, int group, int ngroups)
+{
+    return "beta" + std::to_string(beta) + "_" + std::to_string(repid) +
+           (ngroups > 1 ? "_g" + std::to_string(group) : "");
+}
+
+Spec spec_of(const std::string& ham, const Registry& reg)
+{
+    Spec s;
+    s.block = ham;
+    if (ham == "Ising" || ham == "Ising4d") {
+        s.model = MQ_ISING;
+    } else if (ham == "IsingRGG") {
+        s.model = MQ_ISING;
+        s.rgg = true;
+    } else if (ham == "Heisenberg") {
+        s.model = MQ_HEISENBERG;
+    } else if (ham == "Phi4") {
+        s.model = MQ_PHI4;
+    } else if (ham == "EdwardsAnderson-Ising") {
+        s.model = MQ_EA_ISING;
+    } else if (ham == "Potts") {
+        s.q = reg.get<int>("PottsSelect.ini", "Which", "q");
+        s.block = "Potts" + std::to_string(s.q);
+        s.model = MQ_POTTS;
+    } else if (ham.rfind("Potts", 0) == 0) {
+        s.model = MQ_POTTS;
+        s.q = reg.get<int>(ham + ".ini", ham, "q");
+    } else {
+        throw std::runtime_error("no GPU update path for Hamiltonian '" + ham +
+                                 "' (Ising, Ising4d, IsingRGG, Heisenberg, Phi4, Potts, Potts<q>, EdwardsAnderson-Ising)");
+    }
+    if (s.model == MQ_POTTS && (s.q < 2 || s.q > 8)) throw std::runtime_error("Potts q must be in 2..8");
+    return s;
+}
+
+// cart_prod: the first list varies fastest
+std::vector<ParamSet> parameter_sets(const Spec& s, const Registry& reg)
+{
+    const std::string f = s.block + ".ini";
+    const auto beta = reg.get<std::vector<double>>(f, s.block, "beta");
+    std::vector<ParamSet> out;
+    if (s.model == MQ_PHI4) {
+        const auto lambda = reg.get<std::vector<double>>(f, s.block, "lambda");
+        const auto mass = reg.get<std::vector<double>>(f, s.block, "mass");
+        for (double m : mass)
+            for (double l : lambda)
+                for (double b : beta) out.push_back({b, -1.0, l, m});
+    } else {
+        const auto J = reg.get<std::vector<double>>(f, s.block, "J");
+        for (double j : J)
+            for (double b : beta) out.push_back({b, j, 0.0, 0.0});
+    }
+    return out;
+}
+
+mq_model_params model_params(const Spec& s, const ParamSet& p)
+{
+    mq_model_params m{};
+    m.model = s.model;
+    m.J = p.J;
+    m.q = s.q;
+    m.lambda = p.lambda;
+    m.mass = p.mass;
+    m.proposal = MQ_PROPOSE_FRESH;      // SVInitializer<double>: fresh uniform direction
+    m.delta = 0.5;
+    m.embedding = MQ_EMBED_CARTESIAN;   // Wolff<Heisenberg>: random Cartesian component
+    m.spin_bits = 8;
+    return m;
+}
+
+struct Plan {
+    int L, dim, nrep, ncluster, nmetro, nwarm, nmeas;
+    std::string lattice;  // "regular", "csr-hypercube", "rgg"
+    std::vector<Group> groups;
+    std::string outpath;
+};
+
+void print_usage()
+{
+    std::fprintf(stderr,
+                 "usage: marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run]\n"
+                 "  Hamiltonian: Ising, Ising4d, IsingRGG, Heisenberg, Phi4, Potts (q from PottsSelect.ini),\n"
+                 "               Potts<q>, EdwardsAnderson-Ising\n");
+}
+
+void write_sim(const Plan& pl, const Spec& s, const Sim& sim, uint64_t seed, const std::vector<double>& obs,
+               const std::vector<double>& cl, int nrep, int r, const std::string& path)
+{
+    FILE* fp = std::fopen(path.c_str(), "w");
+    if (!fp) throw std::runtime_error("cannot write " + path);
+    std::fprintf(fp, "# marqov_gpu %s L=%d dim=%d lattice=%s\n", s.block.c_str(), pl.L, pl.dim, pl.lattice.c_str());
+    std::fprintf(fp, "# beta=%.17g J=%.17g", sim.p.beta, sim.p.J);
+    if (s.model == MQ_PHI4) std::fprintf(fp, " lambda=%.17g mass=%.17g", sim.p.lambda, sim.p.mass);
+    if (s.model == MQ_POTTS) std::fprintf(fp, " q=%d", s.q);
+    std::fprintf(fp, " repid=%d id=%d seed=%llu\n", sim.repid, sim.id, (unsigned long long)seed);
+    std::fprintf(fp, "# ncluster=%d nmetro=%d warmup=%d measure=%d\n", pl.ncluster, pl.nmetro, pl.nwarm, pl.nmeas);
+    std::fprintf(fp, "# step E M M2 M4 clustersize\n");
+    for (int t = 0; t < pl.nmeas; ++t) {
+        const double* o = &obs[((size_t)t * nrep + r) * MQ_NOBS];
+        std::fprintf(fp, "%d %.17g %.17g %.17g %.17g %.17g\n", pl.nwarm + t, o[0], o[1], o[2], o[3],
+                     cl[(size_t)t * nrep + r]);
+    }
+    std::fclose(fp);
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    Options o;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto next = [&](const char* what) -> std::string {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "marqov_gpu: %s needs a value\n", what);
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "--config") o.config = next("--config");
+        else if (a == "--out") o.out = next("--out");
+        else if (a == "--seed") o.seed = std::strtoull(next("--seed").c_str(), nullptr, 10);
+        else if (a == "--device") o.device = std::atoi(next("--device").c_str());
+        else if (a == "--dry-run") o.dry = true;
+        else if (a == "-h" || a == "--help") {
+            print_usage();
+            return 0;
+        } else if (!a.empty() && a[0] == '-') {
+            std::fprintf(stderr, "marqov_gpu: unknown option %s\n", a.c_str());
+            print_usage();
+            return 2;
+        } else if (o.ham.empty()) o.ham = a;
+        else {
+            print_usage();
+            return 2;
+        }
+    }
+    if (o.ham.empty()) {
+        print_usage();
+        return 2;
+    }
+    try {
+        Registry reg(o.config, ".ini");
+        const Spec s = spec_of(o.ham, reg);
+        const std::string f = s.block + ".ini";
+        if (!reg.has_file(f)) throw RegistryError("no " + f + " in " + o.config);
+        const auto nL = reg.get<std::vector<int>>(f, s.block, "L");
+        const int dim = reg.get<int>(f, s.block, "dim");
+        auto nrep = reg.get<std::vector<int>>(f, s.block, "rep");
+        if (nrep.size() == 1) nrep.resize(nL.size(), nrep[0]);
+        if (nrep.size() != nL.size()) throw RegistryError("rep needs one value or one per L in " + f);
+        const double amp = reg.get<double>(f, "MC", "nclusteramp");
+        const int cexp = reg.get<int>(f, "MC", "nclusterexp");
+        const int nmetro = reg.get<int>(f, "MC", "nmetro");
+        const int warm = reg.get<int>(f, "MC", "warmupsteps");
+        const int meas = reg.get<int>(f, "MC", "measuresteps");
+        const std::string outdir = o.out.empty() ? reg.get<std::string>(f, "IO", "outdir") : o.out;
+        const auto psets = parameter_sets(s, reg);
+
+        // group parameter sets by everything but beta (one mq_state each)
+        std::vector<std::vector<int>> by_group;
+        {
+            std::map<std::vector<double>, int> gid;
+            for (int k = 0; k < (int)psets.size(); ++k) {
+                const std::vector<double> key = {psets[k].J, psets[k].lambda, psets[k].mass};
+                auto it = gid.find(key);
+                if (it == gid.end()) {
+                    it = gid.emplace(key, (int)by_group.size()).first;
+                    by_group.emplace_back();
+                }
+                by_group[it->second].push_back(k);
+            }
+        }
+
+        std::vector<Plan> plans;
+        int id = 0;
+        for (std::size_t li = 0; li < nL.size(); ++li) {
+            Plan pl;
+            pl.L = nL[li];
+            pl.dim = dim;
+            pl.nrep = nrep[li];
+            pl.ncluster = (int)(amp * std::pow((double)pl.L, (double)cexp));
+            pl.nmetro = nmetro;
+            pl.nwarm = 10 * (warm / 10);
+            pl.nmeas = 10 * (meas / 10);
+            pl.lattice = s.rgg ? "rgg" : ((dim == 2 || dim == 3) && pl.L % 2 == 0 && pl.L >= 4) ? "regular" : "csr-hypercube";
+            pl.outpath = outdir + "/" + std::to_string(pl.L) + "/";
+            // simulation ids in the reference's order: parameter sets, then replicas
+            std::map<int, Sim> sims;
+            std::vector<int> gof(psets.size());
+            for (std::size_t g = 0; g < by_group.size(); ++g)
+                for (int k : by_group[g]) gof[k] = (int)g;
+            pl.groups.assign(by_group.size(), Group{});
+            for (std::size_t k = 0; k < psets.size(); ++k)
+                for (int r = 0; r < nrep[li]; ++r) pl.groups[gof[k]].sims.push_back({psets[k], r, id++});
+            plans.push_back(pl);
+        }
+
+        if (o.dry) {
+            for (const auto& pl : plans) {
+                std::printf("{\"L\": %d, \"dim\": %d, \"lattice\": \"%s\", \"ncluster\": %d, \"nmetro\": %d, "
+                            "\"warmup\": %d, \"measure\": %d, \"out\": \"%s\", \"groups\": [",
+                            pl.L, pl.dim, pl.lattice.c_str(), pl.ncluster, pl.nmetro, pl.nwarm, pl.nmeas,
+                            pl.outpath.c_str());
+                for (std::size_t g = 0; g < pl.groups.size(); ++g) {
+                    std::printf("%s[", g ? ", " : "");
+                    for (std::size_t k = 0; k < pl.groups[g].sims.size(); ++k) {
+                        const Sim& x = pl.groups[g].sims[k];
+                        std::printf("%s{\"id\": %d, \"beta\": %.17g, \"J\": %.17g, \"lambda\": %.17g, \"mass\": %.17g, "
+                                    "\"repid\": %d, \"name\": \"%s\"}",
+                                    k ? ", " : "", x.id, x.p.beta, x.p.J, x.p.lambda, x.p.mass, x.repid,
+                                    sim_name(x.p.beta, x.repid, (int)g, (int)pl.groups.size()).c_str());
+                    }
+                    std::printf("]");
+                }
+                std::printf("], \"model\": \"%s\", \"q\": %d}\n", s.block.c_str(), s.q);
+            }
+            return 0;
+        }
+
+        Context ctx(o.device);
+        make_dir(outdir);
+        for (const auto& pl : plans) {
+            make_dir(pl.outpath);
+            std::printf("\nL = %d (%s, %d groups, ncluster %d, nmetro %d)\n", pl.L, pl.lattice.c_str(),
+                        (int)pl.groups.size(), pl.ncluster, pl.nmetro);
+            std::unique_ptr<RegularLattice> reglat;
+            std::vector<std::unique_ptr<GraphLattice>> graphs;
+            if (pl.lattice == "regular") {
+                reglat.reset(new RegularLattice(ctx, pl.dim, pl.L));
+            } else if (pl.lattice == "csr-hypercube") {
+                const Csr g = hypercube_csr(pl.dim, pl.L);
+                graphs.emplace_back(new GraphLattice(ctx, g.row_ptr, g.col_idx));
+            } else {
+                int64_t n = 1;
+                for (int d = 0; d < pl.dim; ++d) n *= pl.L;
+                const double r = rgg_radius(pl.dim, n, 6.0);
+                for (int rep = 0; rep < pl.nrep; ++rep) {
+                    const Csr g = rgg_periodic(pl.dim, n, r, mix(mix(o.seed, 0x5267u), (uint64_t)pl.L * 1000003u + rep));
+                    graphs.emplace_back(new GraphLattice(ctx, g.row_ptr, g.col_idx));
+                }
+            }
+            FILE* sum = std::fopen((pl.outpath + "summary.txt").c_str(), "w");
+            if (!sum) throw std::runtime_error("cannot write " + pl.outpath + "summary.txt");
+            std::fprintf(sum, "# name beta J lambda mass repid <E> err(E) <|m|> <m^2> <m^4> <clustersize>\n");
+            for (std::size_t g = 0; g < pl.groups.size(); ++g) {
+                // RGG: replicas of one graph must be adjacent, so order by repid, then beta
+                std::vector<Sim> sims = pl.groups[g].sims;
+                if (pl.lattice == "rgg")
+                    std::stable_sort(sims.begin(), sims.end(), [](const Sim& a, const Sim& b) { return a.repid < b.repid; });
+                const int nr = (int)sims.size();
+                std::vector<const RegularLattice*> rl;
+                std::vector<const GraphLattice*> gl;
+                for (const Sim& x : sims) {
+                    if (reglat) rl.push_back(reglat.get());
+                    else gl.push_back(pl.lattice == "rgg" ? graphs[x.repid].get() : graphs[0].get());
+                }
+                const mq_model_params mp = model_params(s, sims[0].p);
+                std::unique_ptr<Batch> b(reglat ? new Batch(ctx, mp, rl) : new Batch(ctx, mp, gl));
+                std::vector<double> beta(nr);
+                for (int r = 0; r < nr; ++r) beta[r] = sims[r].p.beta;
+                b->set_beta(beta);
+                const uint64_t seed = mix(mix(o.seed, (uint64_t)pl.L), g);
+                b->init(seed);
+                if (s.model == MQ_EA_ISING) {
+                    std::vector<int32_t> sample(nr);
+                    for (int r = 0; r < nr; ++r) sample[r] = sims[r].repid;
+                    const uint64_t bseed = mix(o.seed, 0xEA00u + (uint64_t)pl.L);
+                    if (reglat) {
+                        b->random_bonds(sample, 0.5, bseed);
+                    } else {
+                        // graphs: +-1 per undirected edge from a hash of (sample, edge), so
+                        // both CSR slots of an edge carry the same coupling
+                        ctx.check(mq_state_set_samples(b->get(), sample.data()), "mq_state_set_samples");
+                        const Csr g = hypercube_csr(pl.dim, pl.L);
+                        const uint64_t n = (uint64_t)g.nsites();
+                        std::vector<int8_t> bonds(g.col_idx.size());
+                        for (int sm = 0; sm < pl.nrep; ++sm) {
+                            const uint64_t ks = mix(bseed, (uint64_t)sm);
+                            for (uint64_t i = 0; i < n; ++i)
+                                for (int32_t k = g.row_ptr[i]; k < g.row_ptr[i + 1]; ++k) {
+                                    const uint64_t j = (uint64_t)g.col_idx[k];
+                                    const uint64_t e = std::min(i, j) * n + std::max(i, j);
+                                    bonds[k] = (mix(ks, e) >> 63) ? (int8_t)1 : (int8_t)-1;
+                                }
+                            ctx.check(mq_state_set_bonds(b->get(), sm, bonds.data()), "mq_state_set_bonds");
+                        }
+                    }
+                }
+                GpuEMCS emcs(*b, seed);
+                const auto t0 = std::chrono::steady_clock::now();
+                std::printf("|");
+                for (int k = 0; k < 10; ++k) {
+                    for (int i = 0; i < pl.nwarm / 10; ++i) emcs.move(pl.ncluster, pl.nmetro);
+                    std::printf(".");
+                    std::fflush(stdout);
+                }
+                std::printf("|");
+                std::vector<double> obs((size_t)pl.nmeas * nr * MQ_NOBS), cl((size_t)pl.nmeas * nr);
+                for (int k = 0, t = 0; k < 10; ++k) {
+                    for (int i = 0; i < pl.nmeas / 10; ++i, ++t) {
+                        const std::vector<double> mc = emcs.move(pl.ncluster, pl.nmetro);
+                        ctx.check(mq_measure(b->get(), &obs[(size_t)t * nr * MQ_NOBS]), "mq_measure");
+                        for (int r = 0; r < nr; ++r) cl[(size_t)t * nr + r] = mc[r];
+                    }
+                    std::printf(".");
+                    std::fflush(stdout);
+                }
+                const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::printf("| group %d: %d simulations, %d EMCS each, %.2f s\n", (int)g, nr, pl.nwarm + pl.nmeas, secs);
+                for (int r = 0; r < nr; ++r) {
+                    const Sim& x = sims[r];
+                    const std::string name = sim_name(x.p.beta, x.repid, (int)g, (int)pl.groups.size());
+                    write_sim(pl, s, x, seed, obs, cl, nr, r, pl.outpath + name + ".txt");
+                    double se = 0, se2 = 0, sm = 0, sm2 = 0, sm4 = 0, sc = 0;
+                    for (int t = 0; t < pl.nmeas; ++t) {
+                        const double* v = &obs[((size_t)t * nr + r) * MQ_NOBS];
+                        se += v[0];
+                        se2 += v[0] * v[0];
+                        sm += v[1];
+                        sm2 += v[2];
+                        sm4 += v[3];
+                        sc += cl[(size_t)t * nr + r];
+                    }
+                    const double n = pl.nmeas > 0 ? pl.nmeas : 1;
+                    const double me = se / n;
+                    const double err = pl.nmeas > 1 ? std::sqrt(std::max(0.0, se2 / n - me * me) / (n - 1)) : 0.0;
+                    std::fprintf(sum, "%s %.10g %.10g %.10g %.10g %d %.10g %.3g %.10g %.10g %.10g %.6g\n", name.c_str(),
+                                 x.p.beta, x.p.J, x.p.lambda, x.p.mass, x.repid, me, err, sm / n, sm2 / n, sm4 / n,
+                                 sc / n);
+                }
+            }
+            std::fclose(sum);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "marqov_gpu: %s\n", e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,197 @@
+"""The .ini front-end (marqov_amd/bin/marqov_gpu): configuration parsing and run
+plans on the CPU (--dry-run touches no GPU), its graph builders against brute
+force, and full runs on the GPU checked against exact enumeration."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import exact_path  # noqa: F401
+import exact
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "marqov_amd", "bin", "marqov_gpu")
+CONF = os.path.join(ROOT, "tests", "data", "config")
+LCHK = os.path.join(ROOT, "tests", "cpp", "lattice_check")
+
+
+def plan(ham, conf=CONF):
+    r = subprocess.run([BIN, ham, "--config", conf, "--dry-run"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    return [json.loads(line) for line in r.stdout.splitlines() if line.strip()]
+
+
+def test_plan_ising():
+    p = plan("Ising")
+    assert [x["L"] for x in p] == [4, 3]
+    assert [x["lattice"] for x in p] == ["regular", "csr-hypercube"]  # odd L is not bipartite
+    for x in p:
+        # ncluster = int(2 * L^0); 10 * (205 // 10) warm-up and 10 * (4009 // 10) measured EMCS
+        assert (x["ncluster"], x["nmetro"], x["warmup"], x["measure"]) == (2, 1, 200, 4000)
+        assert x["out"] == f"out_ising/{x['L']}/"
+        assert len(x["groups"]) == 1
+    sims = p[0]["groups"][0] + p[1]["groups"][0]
+    assert [s["id"] for s in sims] == list(range(48))
+    # parameter sets first (beta fastest), replicas inner
+    first = p[0]["groups"][0]
+    assert [s["beta"] for s in first[:9]] == [0.3] * 8 + [0.4406868]
+    assert [s["repid"] for s in first[:9]] == list(range(8)) + [0]
+    assert first[8]["name"] == "beta0.440687_0"
+
+
+def test_plan_potts_select_and_cart_prod():
+    p = plan("Potts")  # q from PottsSelect.ini
+    assert len(p) == 1 and p[0]["model"] == "Potts3" and p[0]["q"] == 3
+    assert p[0]["ncluster"] == 2  # int(0.5 * 4^1)
+    g0, g1 = p[0]["groups"]  # one state per J; beta varies fastest inside
+    assert [s["J"] for s in g0] == [-1.0] * 8 and [s["J"] for s in g1] == [-0.5] * 8
+    assert [s["beta"] for s in g0] == [0.8] * 4 + [1.1] * 4
+    assert [s["id"] for s in g0 + g1] == list(range(16))
+    assert g1[0]["name"] == "beta0.800000_0_g1"
+    assert plan("Potts3")[0]["groups"] == p[0]["groups"]
+
+
+def test_plan_phi4_and_rep_per_L():
+    p = plan("Phi4")
+    assert [x["L"] for x in p] == [6, 8]
+    assert [x["ncluster"] for x in p] == [3, 4]
+    # (beta, lambda, mass): beta fastest; one group per (lambda, mass); rep = 2 for L=6, 3 for L=8
+    assert [len(x["groups"]) for x in p] == [2, 2]
+    assert [len(g) for g in p[0]["groups"]] == [4, 4] and [len(g) for g in p[1]["groups"]] == [6, 6]
+    assert {s["lambda"] for s in p[0]["groups"][0]} == {4.5} and {s["lambda"] for s in p[0]["groups"][1]} == {3.0}
+    assert all(s["J"] == -1.0 for g in p[0]["groups"] for s in g)
+
+
+def test_plan_ea_and_rgg():
+    p = plan("EdwardsAnderson-Ising")
+    assert [x["lattice"] for x in p] == ["csr-hypercube", "regular"]
+    assert all(x["ncluster"] == 0 for x in p)
+    p = plan("IsingRGG")
+    assert p[0]["lattice"] == "rgg" and p[0]["ncluster"] == 8
+
+
+def test_registry_errors():
+    r = subprocess.run([BIN, "Ising", "--config", os.path.join(ROOT, "tests", "data", "config_noend"), "--dry-run"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "File not recognized as Config File" in r.stderr
+    r = subprocess.run([BIN, "XXZAntiferro", "--config", CONF, "--dry-run"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "no GPU update path" in r.stderr
+    r = subprocess.run([BIN, "Heisenberg", "--config", "/nonexistent", "--dry-run"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "Configuration Directory not found" in r.stderr
+
+
+def _read_csr(lines, off):
+    rp = np.array(lines[off].split(), np.int64)
+    ci = np.array(lines[off + 1].split(), np.int64) if off + 1 < len(lines) else np.zeros(0, np.int64)
+    return rp, ci
+
+
+@pytest.mark.parametrize("dim,L", [(2, 3), (3, 5), (4, 3)])
+def test_hypercube_csr_reference_order(dim, L):
+    out = subprocess.run([LCHK, "cube", str(dim), str(L)], capture_output=True, text=True, check=True).stdout
+    lines = out.splitlines()
+    rp, ci = _read_csr(lines, 1)
+    N = L ** dim
+    assert list(rp) == list(range(0, 2 * dim * N + 1, 2 * dim))
+    for i in range(N):
+        x = [(i // L ** d) % L for d in range(dim)]
+        want = []
+        for d in range(dim):  # +x_d then -x_d, as RegularHypercubic
+            for s in (1, -1):
+                y = list(x)
+                y[d] = (y[d] + s) % L
+                want.append(sum(y[k] * L ** k for k in range(dim)))
+        assert list(ci[rp[i]:rp[i + 1]]) == want, i
+
+
+@pytest.mark.parametrize("dim,n,r", [(2, 400, 0.09), (3, 500, 0.16), (2, 60, 0.3), (3, 40, 0.45)])
+def test_rgg_matches_brute_force(dim, n, r):
+    out = subprocess.run([LCHK, "rgg", str(dim), str(n), repr(r), "7"], capture_output=True, text=True,
+                         check=True).stdout
+    lines = out.splitlines()
+    pts = np.array(lines[1].split(), np.float64).reshape(n, dim)
+    rp, ci = _read_csr(lines, 2)
+    d = np.abs(pts[:, None, :] - pts[None, :, :])
+    d = np.minimum(d, 1.0 - d)
+    adj = ((d * d).sum(-1) < r * r) & ~np.eye(n, dtype=bool)
+    for i in range(n):
+        assert list(ci[rp[i]:rp[i + 1]]) == list(np.nonzero(adj[i])[0]), i
+
+
+def _run(ham, out, seed=5):
+    r = subprocess.run([BIN, ham, "--config", CONF, "--out", str(out), "--seed", str(seed)], capture_output=True,
+                       text=True, timeout=110)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+
+
+def _series(path):
+    hdr = [line for line in open(path) if line.startswith("#")]
+    data = np.loadtxt(path)
+    return hdr, data
+
+
+def _hypercube_graph(dim, L):
+    out = subprocess.run([LCHK, "cube", str(dim), str(L)], capture_output=True, text=True, check=True).stdout
+    rp, ci = _read_csr(out.splitlines(), 1)
+    return rp.astype(np.int32), ci.astype(np.int32)
+
+
+@pytest.mark.gpu
+def test_gpu_ising_ini_vs_exact(tmp_path):
+    """The whole front-end on the GPU: Ising.ini runs 2D Ising on a 4x4 torus
+    (checkerboard kernels) and a 3x3 torus (graph kernels), 8 replicas at each of
+    3 temperatures, 2 Wolff clusters + 1 sweep per EMCS; <E/N> and <m^2> of each
+    temperature agree with exact enumeration within 4 standard errors over the
+    replicas."""
+    _run("Ising", tmp_path)
+    for L in (4, 3):
+        rp, ci = _hypercube_graph(2, L)
+        for beta in (0.3, 0.4406868, 0.7):
+            e_rep, m2_rep = [], []
+            for rep in range(8):
+                hdr, d = _series(tmp_path / str(L) / f"beta{beta:.6f}_{rep}.txt")
+                assert d.shape == (4000, 6), d.shape
+                assert np.allclose(d[:, 0], np.arange(200, 4200))
+                assert np.all(d[:, 5] >= 1.0)
+                e_rep.append(d[:, 1].mean())
+                m2_rep.append(d[:, 3].mean())
+            ex = exact.enumerate_ising(2, 4, beta, J=-1.0) if L == 4 else exact.enumerate_graph(rp, ci, beta, J=-1.0)
+            ee, m2e = ex["e"], ex["m2"]
+            for got, want, what in ((e_rep, ee, "E"), (m2_rep, m2e, "m2")):
+                g = np.asarray(got)
+                se = g.std(ddof=1) / np.sqrt(len(g))
+                print(f"L={L} beta={beta} {what}: {g.mean():.5f} +- {se:.5f} exact {want:.5f}", flush=True)
+                assert abs(g.mean() - want) < 4 * se + 1e-9, (L, beta, what, g.mean(), se, want)
+    summ = (tmp_path / "4" / "summary.txt").read_text().splitlines()
+    assert len(summ) == 1 + 24
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ham,dirs", [("Potts", ["4"]), ("Phi4", ["6", "8"]), ("Heisenberg", ["6"]),
+                                      ("EdwardsAnderson-Ising", ["3", "4"]), ("IsingRGG", ["8"])])
+def test_gpu_ini_models_run(tmp_path, ham, dirs):
+    """Every supported Hamiltonian runs from its .ini file: one series file per
+    simulation with the planned number of rows and finite observables, energies
+    falling with beta."""
+    _run(ham, tmp_path)
+    p = plan(ham)
+    for x, dname in zip(p, dirs):
+        for g, grp in enumerate(x["groups"]):
+            for s in grp:
+                _, d = _series(tmp_path / dname / f"{s['name']}.txt")
+                assert d.shape == (x["measure"], 6)
+                assert np.all(np.isfinite(d[:, 1:5]))
+                if x["ncluster"] == 0:
+                    assert np.all(np.isnan(d[:, 5]))
+        lines = [line.split() for line in (tmp_path / dname / "summary.txt").read_text().splitlines()[1:]]
+        by_beta = {}
+        for f in lines:
+            by_beta.setdefault(float(f[1]), []).append(float(f[6]))
+        betas = sorted(by_beta)
+        means = [np.mean(by_beta[b]) for b in betas]
+        if ham != "Phi4":  # Phi4's on-site terms scale with 1/beta
+            assert all(a >= b - 0.05 for a, b in zip(means, means[1:])), means
