@@ -169,6 +169,11 @@ public:
         return acc;
     }
     template <class T>
+    void upload(int rep, const std::vector<T>& v)
+    {
+        ctx_.check(mq_state_upload(h_, rep, v.data(), (int64_t)(v.size() * sizeof(T))), "mq_state_upload");
+    }
+    template <class T>
     std::vector<T> download(int rep, std::size_t nsites, int per_site)
     {
         std::vector<T> v(nsites * per_site);
@@ -183,10 +188,12 @@ private:
 };
 
 // One elementary Monte Carlo step of every replica (emcs.h:69-109).
-// `step` numbers the EMCS so that every step draws fresh Philox counters.
+// `step` numbers the EMCS so that every step draws fresh Philox counters; a
+// run resumed from a checkpoint passes the number of EMCS already done, and
+// then continues the same trajectory.
 class GpuEMCS {
 public:
-    GpuEMCS(Batch& batch, uint64_t seed) : b_(batch), seed_(seed) {}
+    GpuEMCS(Batch& batch, uint64_t seed, uint64_t step0 = 0) : b_(batch), seed_(seed), step_(step0) {}
     std::vector<double> move(int ncluster, int nmetro)
     {
         std::vector<double> mean(b_.nrep());
@@ -198,7 +205,7 @@ public:
 private:
     Batch& b_;
     uint64_t seed_;
-    uint64_t step_ = 0;
+    uint64_t step_;
 };
 
 }  // namespace marqov_amd

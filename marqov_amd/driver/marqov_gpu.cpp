@@ -1,6 +1,6 @@
 // marqov_gpu -- runs a MARQOV .ini configuration on one MI355X.
 //
-//   marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run]
+//   marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run] [--fresh]
 //
 // Reads <Hamiltonian>.ini from the configuration directory (default ./config,
 // as check_registry_availability, src/libmarqov/util/startup.h:69-80) with the
@@ -29,6 +29,16 @@
 // EdwardsAnderson-Ising draws +-1 couplings with probability 1/2 each
 // (BimodalPDF, src/lattice/util/prob_density.h:25-35), one disorder sample per
 // repid shared by all temperatures.
+//
+// Checkpoints: after its game loop every GPU batch writes
+// <out>/<L>/checkpoint_g<group>.bin (the spins of every simulation in the
+// reference's site order, the EMCS count, seeds and the plan).  When the file is
+// present the next run of the same configuration resumes, as MARQOV resumes from
+// an existing HDF5 dump (core.h:543-607, 650-700): it loads the spins, skips the
+// warm-up (wrmploop runs only at step 0, core.h:1034-1050) and appends another
+// game loop as the next block.  Philox counters are keyed by the EMCS number, so a
+// resumed trajectory is exactly the one an uninterrupted run would have made.
+// --fresh ignores and overwrites existing checkpoints.
 //
 // Output (HDF5 is not part of this path): one text file per simulation,
 // <out>/<L>/beta<beta>_<repid>.txt (the reference's outpath and defaultfilter
@@ -64,6 +74,7 @@ struct Options {
     uint64_t seed = 1;
     int device = 0;
     bool dry = false;
+    bool fresh = false;
 };
 
 struct Spec {
@@ -190,19 +201,95 @@ struct Plan {
     std::string outpath;
 };
 
+// ---- checkpoints
+constexpr uint64_t CKPT_MAGIC = 0x3130544B50434D51ull;  // "QMCPKT01"
+
+struct CkptHeader {
+    uint64_t magic;
+    int32_t model, q, dim, L, nrep, per_site, block, lattice;
+    uint64_t base_seed, seed, step;
+    double J, lambda, mass;
+};
+
+bool file_exists(const std::string& p)
+{
+    struct stat sb;
+    return stat(p.c_str(), &sb) == 0;
+}
+
+int lattice_code(const std::string& l) { return l == "regular" ? 0 : (l == "csr-hypercube" ? 1 : 2); }
+
+void write_ckpt(const std::string& path, const CkptHeader& h, const std::vector<Sim>& sims,
+                const std::vector<std::vector<char>>& spins)
+{
+    const std::string tmp = path + ".tmp";
+    FILE* fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) throw std::runtime_error("cannot write " + tmp);
+    bool ok = std::fwrite(&h, sizeof(h), 1, fp) == 1;
+    for (const Sim& x : sims) {
+        ok = ok && std::fwrite(&x.p.beta, sizeof(double), 1, fp) == 1;
+        ok = ok && std::fwrite(&x.repid, sizeof(int), 1, fp) == 1;
+    }
+    for (const auto& v : spins) ok = ok && std::fwrite(v.data(), 1, v.size(), fp) == v.size();
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) throw std::runtime_error("cannot write " + path);
+}
+
+// Reads a checkpoint and checks that it belongs to this batch.
+CkptHeader read_ckpt(const std::string& path, const CkptHeader& want, const std::vector<Sim>& sims, std::size_t bytes,
+                     std::vector<std::vector<char>>& spins)
+{
+    FILE* fp = std::fopen(path.c_str(), "rb");
+    if (!fp) throw std::runtime_error("cannot read " + path);
+    CkptHeader h{};
+    bool ok = std::fread(&h, sizeof(h), 1, fp) == 1 && h.magic == CKPT_MAGIC;
+    auto mismatch = [&](const std::string& what) {
+        std::fclose(fp);
+        throw std::runtime_error(path + " does not belong to this configuration (" + what +
+                                 "); remove it or run with --fresh");
+    };
+    if (!ok) mismatch("not a checkpoint");
+    if (h.model != want.model || h.q != want.q || h.dim != want.dim || h.L != want.L || h.nrep != want.nrep ||
+        h.per_site != want.per_site || h.lattice != want.lattice || h.J != want.J || h.lambda != want.lambda ||
+        h.mass != want.mass)
+        mismatch("model, lattice or simulations differ");
+    if (h.base_seed != want.base_seed) mismatch("written with --seed " + std::to_string(h.base_seed));
+    for (const Sim& x : sims) {
+        double b;
+        int rep;
+        if (std::fread(&b, sizeof(double), 1, fp) != 1 || std::fread(&rep, sizeof(int), 1, fp) != 1) mismatch("short file");
+        if (b != x.p.beta || rep != x.repid) mismatch("temperatures or replicas differ");
+    }
+    spins.assign(sims.size(), std::vector<char>(bytes));
+    for (auto& v : spins)
+        if (std::fread(v.data(), 1, bytes, fp) != bytes) mismatch("short file");
+    std::fclose(fp);
+    return h;
+}
+
 void print_usage()
 {
     std::fprintf(stderr,
-                 "usage: marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run]\n"
+                 "usage: marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run] [--fresh]\n"
                  "  Hamiltonian: Ising, Ising4d, IsingRGG, Heisenberg, Phi4, Potts (q from PottsSelect.ini),\n"
                  "               Potts<q>, EdwardsAnderson-Ising\n");
 }
 
 void write_sim(const Plan& pl, const Spec& s, const Sim& sim, uint64_t seed, const std::vector<double>& obs,
-               const std::vector<double>& cl, int nrep, int r, const std::string& path)
+               const std::vector<double>& cl, int nrep, int r, const std::string& path, int block, uint64_t step0)
 {
-    FILE* fp = std::fopen(path.c_str(), "w");
+    FILE* fp = std::fopen(path.c_str(), block > 0 ? "a" : "w");
     if (!fp) throw std::runtime_error("cannot write " + path);
+    if (block > 0) {  // a resumed run appends its game loop as the next block
+        std::fprintf(fp, "# block %d (resumed after %llu EMCS)\n", block, (unsigned long long)step0);
+        for (int t = 0; t < pl.nmeas; ++t) {
+            const double* o = &obs[((size_t)t * nrep + r) * MQ_NOBS];
+            std::fprintf(fp, "%llu %.17g %.17g %.17g %.17g %.17g\n", (unsigned long long)(step0 + t), o[0], o[1],
+                         o[2], o[3], cl[(size_t)t * nrep + r]);
+        }
+        std::fclose(fp);
+        return;
+    }
     std::fprintf(fp, "# marqov_gpu %s L=%d dim=%d lattice=%s\n", s.block.c_str(), pl.L, pl.dim, pl.lattice.c_str());
     std::fprintf(fp, "# beta=%.17g J=%.17g", sim.p.beta, sim.p.J);
     if (s.model == MQ_PHI4) std::fprintf(fp, " lambda=%.17g mass=%.17g", sim.p.lambda, sim.p.mass);
@@ -237,6 +324,7 @@ int main(int argc, char** argv)
         else if (a == "--seed") o.seed = std::strtoull(next("--seed").c_str(), nullptr, 10);
         else if (a == "--device") o.device = std::atoi(next("--device").c_str());
         else if (a == "--dry-run") o.dry = true;
+        else if (a == "--fresh") o.fresh = true;
         else if (a == "-h" || a == "--help") {
             print_usage();
             return 0;
@@ -357,7 +445,7 @@ int main(int argc, char** argv)
             }
             FILE* sum = std::fopen((pl.outpath + "summary.txt").c_str(), "w");
             if (!sum) throw std::runtime_error("cannot write " + pl.outpath + "summary.txt");
-            std::fprintf(sum, "# name beta J lambda mass repid <E> err(E) <|m|> <m^2> <m^4> <clustersize>\n");
+            std::fprintf(sum, "# name beta J lambda mass repid <E> err(E) <|m|> <m^2> <m^4> <clustersize> block\n");
             for (std::size_t g = 0; g < pl.groups.size(); ++g) {
                 // RGG: replicas of one graph must be adjacent, so order by repid, then beta
                 std::vector<Sim> sims = pl.groups[g].sims;
@@ -376,7 +464,24 @@ int main(int argc, char** argv)
                 for (int r = 0; r < nr; ++r) beta[r] = sims[r].p.beta;
                 b->set_beta(beta);
                 const uint64_t seed = mix(mix(o.seed, (uint64_t)pl.L), g);
-                b->init(seed);
+                const int per_site = (s.model == MQ_HEISENBERG || s.model == MQ_PHI4) ? 3 : 1;
+                const std::size_t nsites = reglat ? reglat->size() : gl[0]->size();
+                const std::size_t bytes = nsites * (per_site == 3 ? 3 * sizeof(float) : 1);
+                CkptHeader ck{CKPT_MAGIC, s.model, s.q, pl.dim, pl.L, nr, per_site, 0, lattice_code(pl.lattice),
+                              o.seed, seed, 0, sims[0].p.J, sims[0].p.lambda, sims[0].p.mass};
+                const std::string ckpath = pl.outpath + "checkpoint_g" + std::to_string(g) + ".bin";
+                const bool resume = !o.fresh && file_exists(ckpath);
+                uint64_t step0 = 0;
+                int block = 0;
+                if (resume) {
+                    std::vector<std::vector<char>> sp;
+                    const CkptHeader h = read_ckpt(ckpath, ck, sims, bytes, sp);
+                    step0 = h.step;
+                    block = h.block + 1;
+                    for (int r = 0; r < nr; ++r) b->upload(r, sp[r]);
+                } else {
+                    b->init(seed);
+                }
                 if (s.model == MQ_EA_ISING) {
                     std::vector<int32_t> sample(nr);
                     for (int r = 0; r < nr; ++r) sample[r] = sims[r].repid;
@@ -402,15 +507,20 @@ int main(int argc, char** argv)
                         }
                     }
                 }
-                GpuEMCS emcs(*b, seed);
+                GpuEMCS emcs(*b, seed, step0);
                 const auto t0 = std::chrono::steady_clock::now();
-                std::printf("|");
-                for (int k = 0; k < 10; ++k) {
-                    for (int i = 0; i < pl.nwarm / 10; ++i) emcs.move(pl.ncluster, pl.nmetro);
-                    std::printf(".");
-                    std::fflush(stdout);
+                if (!resume) {  // the warm-up runs only for a fresh start (wrmploop, core.h:1034-1050)
+                    std::printf("|");
+                    for (int k = 0; k < 10; ++k) {
+                        for (int i = 0; i < pl.nwarm / 10; ++i) emcs.move(pl.ncluster, pl.nmetro);
+                        std::printf(".");
+                        std::fflush(stdout);
+                    }
+                    std::printf("|");
+                } else {
+                    std::printf("resuming block %d after %llu EMCS |", block, (unsigned long long)step0);
                 }
-                std::printf("|");
+                const uint64_t meas0 = emcs.steps();
                 std::vector<double> obs((size_t)pl.nmeas * nr * MQ_NOBS), cl((size_t)pl.nmeas * nr);
                 for (int k = 0, t = 0; k < 10; ++k) {
                     for (int i = 0; i < pl.nmeas / 10; ++i, ++t) {
@@ -422,11 +532,22 @@ int main(int argc, char** argv)
                     std::fflush(stdout);
                 }
                 const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                std::printf("| group %d: %d simulations, %d EMCS each, %.2f s\n", (int)g, nr, pl.nwarm + pl.nmeas, secs);
+                std::printf("| group %d: %d simulations, %llu EMCS each, %.2f s\n", (int)g, nr,
+                            (unsigned long long)(emcs.steps() - step0), secs);
+                {
+                    std::vector<std::vector<char>> sp(nr);
+                    for (int r = 0; r < nr; ++r) {
+                        sp[r].resize(bytes);
+                        ctx.check(mq_state_download(b->get(), r, sp[r].data(), (int64_t)bytes), "mq_state_download");
+                    }
+                    ck.step = emcs.steps();
+                    ck.block = block;
+                    write_ckpt(ckpath, ck, sims, sp);
+                }
                 for (int r = 0; r < nr; ++r) {
                     const Sim& x = sims[r];
                     const std::string name = sim_name(x.p.beta, x.repid, (int)g, (int)pl.groups.size());
-                    write_sim(pl, s, x, seed, obs, cl, nr, r, pl.outpath + name + ".txt");
+                    write_sim(pl, s, x, seed, obs, cl, nr, r, pl.outpath + name + ".txt", block, meas0);
                     double se = 0, se2 = 0, sm = 0, sm2 = 0, sm4 = 0, sc = 0;
                     for (int t = 0; t < pl.nmeas; ++t) {
                         const double* v = &obs[((size_t)t * nr + r) * MQ_NOBS];
@@ -440,9 +561,9 @@ int main(int argc, char** argv)
                     const double n = pl.nmeas > 0 ? pl.nmeas : 1;
                     const double me = se / n;
                     const double err = pl.nmeas > 1 ? std::sqrt(std::max(0.0, se2 / n - me * me) / (n - 1)) : 0.0;
-                    std::fprintf(sum, "%s %.10g %.10g %.10g %.10g %d %.10g %.3g %.10g %.10g %.10g %.6g\n", name.c_str(),
+                    std::fprintf(sum, "%s %.10g %.10g %.10g %.10g %d %.10g %.3g %.10g %.10g %.10g %.6g %d\n", name.c_str(),
                                  x.p.beta, x.p.J, x.p.lambda, x.p.mass, x.repid, me, err, sm / n, sm2 / n, sm4 / n,
-                                 sc / n);
+                                 sc / n, block);
                 }
             }
             std::fclose(sum);

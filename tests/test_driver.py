@@ -195,3 +195,49 @@ def test_gpu_ini_models_run(tmp_path, ham, dirs):
         means = [np.mean(by_beta[b]) for b in betas]
         if ham != "Phi4":  # Phi4's on-site terms scale with 1/beta
             assert all(a >= b - 0.05 for a, b in zip(means, means[1:])), means
+
+
+def _rows(path):
+    """Data rows of a series file as text (bit-exact comparison), and the block headers."""
+    rows, blocks = [], []
+    for line in open(path):
+        if line.startswith("# block"):
+            blocks.append(line)
+        elif not line.startswith("#"):
+            rows.append(line)
+    return rows, blocks
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ham,Ls", [("Ising", ["8", "5"]), ("Heisenberg", ["6"])])
+def test_gpu_ini_resume_is_exact(tmp_path, ham, Ls):
+    """Checkpoint/resume: a run of 2 x 200 measured EMCS, stopped after the first 200
+    and resumed from the checkpoint (no second warm-up, next block appended), writes
+    exactly the rows of one uninterrupted run of 400 measured EMCS: the Philox
+    counters are keyed by the EMCS number, so the resumed trajectory is the same."""
+    once, twice = tmp_path / "once", tmp_path / "twice"
+    for d, out in (("config_resume_once", once), ("config_resume_twice", twice)):
+        conf = os.path.join(ROOT, "tests", "data", d)
+        runs = 1 if d.endswith("once") else 2
+        for _ in range(runs):
+            r = subprocess.run([BIN, ham, "--config", conf, "--out", str(out), "--seed", "11"], capture_output=True,
+                               text=True, timeout=110)
+            print(r.stdout[-800:], r.stderr[-800:], flush=True)
+            assert r.returncode == 0, r.stderr
+    p = plan(ham, os.path.join(ROOT, "tests", "data", "config_resume_once"))
+    nfiles = 0
+    for x, L in zip(p, Ls):
+        for grp in x["groups"]:
+            for s in grp:
+                a, ablocks = _rows(once / L / f"{s['name']}.txt")
+                b, bblocks = _rows(twice / L / f"{s['name']}.txt")
+                assert len(a) == 400 and not ablocks
+                assert len(bblocks) == 1 and "resumed after" in bblocks[0]
+                assert a == b, (L, s["name"])
+                nfiles += 1
+    assert nfiles > 0
+    # a different seed does not silently continue somebody else's chain
+    conf = os.path.join(ROOT, "tests", "data", "config_resume_twice")
+    r = subprocess.run([BIN, ham, "--config", conf, "--out", str(twice), "--seed", "12"], capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 1 and "--seed 11" in r.stderr
