@@ -323,3 +323,33 @@ def test_wolff_frontier_spill_bit_exact(ctx, model, kernels, ring, dim, L, block
         assert esz.max() > N // 2  # the fronts are far wider than the ring
         assert np.array_equal(sizes[r], esz), (r, sizes[r], esz)
         assert np.array_equal(st.download(r), exp), r
+
+
+@pytest.mark.parametrize("model,dim,L", [("ising", 2, 64), ("ising", 3, 16), ("potts", 2, 32)])
+def test_wolff_wave_chained_calls_bit_exact(ctx, model, dim, L):
+    """Ordered starts with 10% defects at bond probability 1 - e^-1 grow large clusters
+    in the small-lattice kernel; three calls of four clusters each must match the CPU
+    replay cluster by cluster and spin by spin (state carried across launches)."""
+    from marqov_amd import gpu
+    N = L ** dim
+    lat = gpu.RegularLattice(ctx, dim, L)
+    st = gpu.State(ctx, model, [lat] * 3, J=-1.0, q=3)
+    beta = 0.5 if model == "ising" else 1.0
+    st.set_beta(beta)
+    cur = []
+    for r in range(3):
+        s0 = (np.ones(N, np.int8) if model == "ising" else np.zeros(N, np.int8))
+        s0[np.random.default_rng(70 + r).random(N) < 0.1] = -1 if model == "ising" else 2
+        st.upload(r, s0)
+        cur.append(s0)
+    for call in range(3):
+        sizes = st.wolff(4, seed=19, cluster0=4 * call)
+        for r in range(3):
+            if model == "ising":
+                cur[r], esz = ob.wolff_ising(dim, L, cur[r], beta, -1.0, 4, 19, 4 * call, r)
+            else:
+                cur[r], esz = ob.wolff_potts(dim, L, cur[r], 3, beta, -1.0, 4, 19, 4 * call, r)
+            assert np.array_equal(sizes[r], esz), (call, r, sizes[r], esz)
+            assert np.array_equal(st.download(r), cur[r]), (call, r)
+        if call == 0:
+            assert sizes.max() * 8 > N
