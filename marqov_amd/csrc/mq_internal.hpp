@@ -64,13 +64,10 @@ __device__ __forceinline__ uint32_t fdiv_u24(uint32_t n, uint32_t d, float inv, 
 {
     uint32_t q = (uint32_t)((float)n * inv);
     int32_t r = (int32_t)(n - q * d);
-    if (r < 0) {
-        --q;
-        r += (int32_t)d;
-    } else if (r >= (int32_t)d) {
-        ++q;
-        r -= (int32_t)d;
-    }
+    // branch-free correction by one either way
+    const bool lo = r < 0, hi = r >= (int32_t)d;
+    q = q - (lo ? 1u : 0u) + (hi ? 1u : 0u);
+    r = r + (lo ? (int32_t)d : 0) - (hi ? (int32_t)d : 0);
     rem = (uint32_t)r;
     return q;
 }
