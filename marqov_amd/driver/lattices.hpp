@@ -12,12 +12,21 @@
 //   image searches across faces, edges and corners
 //   (src/lattice/random_geometric_graph.h:29-168).  Here a cell list with
 //   cells of side >= r finds the same pairs; neighbours are sorted by index.
+// * read_csv_graph: GraphFromCSV's file format (src/lattice/graph_from_csv.h:32-57,
+//   lattice/util/io.h:57-105): first line the number of coordinates per
+//   vertex, second line skipped, then one line per vertex with that many
+//   coordinates and the vertex's neighbours, tab-separated; neighbour indices
+//   are shifted by idxoffs.  Reading stops at the end of the file or at an
+//   empty line.
 #pragma once
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <random>
+#include <sstream>
+#include <string>
 #include <stdexcept>
 #include <vector>
 
@@ -135,6 +144,46 @@ inline Csr rgg_periodic(int dim, int64_t n, double r, uint64_t seed)
     for (int64_t i = 0; i < n; ++i) g.row_ptr[i + 1] = g.row_ptr[i] + (int32_t)nb[i].size();
     g.col_idx.reserve(g.row_ptr[n]);
     for (int64_t i = 0; i < n; ++i) g.col_idx.insert(g.col_idx.end(), nb[i].begin(), nb[i].end());
+    return g;
+}
+
+inline Csr read_csv_graph(const std::string& path, int idxoffs = 0)
+{
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot read graph file " + path);
+    std::string row;
+    if (!std::getline(in, row)) throw std::runtime_error("empty graph file " + path);
+    int ncoords = 0;
+    try {
+        ncoords = std::stoi(row);
+    } catch (const std::exception&) {
+        throw std::runtime_error("graph file " + path + ": first line must be the number of coordinates");
+    }
+    std::getline(in, row);  // second header line
+    std::vector<std::vector<int32_t>> nb;
+    while (std::getline(in, row)) {
+        if (!row.empty() && row.back() == '\r') row.pop_back();
+        if (row.empty()) break;
+        std::istringstream ss(row);
+        std::string w;
+        int k = 0;
+        std::vector<int32_t> v;
+        while (std::getline(ss, w, '\t')) {
+            if (k++ < ncoords) continue;
+            v.push_back((int32_t)(std::stol(w) - idxoffs));
+        }
+        nb.push_back(std::move(v));
+    }
+    const int64_t n = (int64_t)nb.size();
+    Csr g;
+    g.row_ptr.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int32_t j : nb[i])
+            if (j < 0 || j >= n) throw std::runtime_error("graph file " + path + ": neighbour index out of range");
+        g.row_ptr[i + 1] = g.row_ptr[i] + (int32_t)nb[i].size();
+    }
+    g.col_idx.reserve(g.row_ptr[n]);
+    for (const auto& v : nb) g.col_idx.insert(g.col_idx.end(), v.begin(), v.end());
     return g;
 }
 

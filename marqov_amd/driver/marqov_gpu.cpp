@@ -1,6 +1,7 @@
 // marqov_gpu -- runs a MARQOV .ini configuration on one MI355X.
 //
 //   marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run] [--fresh]
+//              [--graph FILE.csv [--idxoffs K]]
 //
 // Reads <Hamiltonian>.ini from the configuration directory (default ./config,
 // as check_registry_availability, src/libmarqov/util/startup.h:69-80) with the
@@ -26,6 +27,10 @@
 // checkerboard kernels; odd L and dimension 4 use the graph kernels on the
 // same lattice as a CSR graph.  IsingRGG builds a periodic random geometric
 // graph of L^dim points with mean degree 6 per repid (demo/IsingRGG.cpp).
+// --graph runs the configuration on a graph read in GraphFromCSV's format
+// (src/lattice/graph_from_csv.h:32-57, lattice/util/io.h:57-105) instead of the
+// L list: one batch per parameter group on that graph, output under <out>/graph/,
+// ncluster = int(nclusteramp * n^(nclusterexp/dim)) for n vertices.
 // EdwardsAnderson-Ising draws +-1 couplings with probability 1/2 each
 // (BimodalPDF, src/lattice/util/prob_density.h:25-35), one disorder sample per
 // repid shared by all temperatures.
@@ -75,6 +80,8 @@ struct Options {
     int device = 0;
     bool dry = false;
     bool fresh = false;
+    std::string graph;  // GraphFromCSV file replacing the hypercubes
+    int idxoffs = 0;
 };
 
 struct Spec {
@@ -196,7 +203,9 @@ mq_model_params model_params(const Spec& s, const ParamSet& p)
 
 struct Plan {
     int L, dim, nrep, ncluster, nmetro, nwarm, nmeas;
-    std::string lattice;  // "regular", "csr-hypercube", "rgg"
+    int64_t nsites;
+    std::string lattice;  // "regular", "csr-hypercube", "rgg", "csv"
+    std::string source;   // graph file of a "csv" plan
     std::vector<Group> groups;
     std::string outpath;
 };
@@ -217,7 +226,10 @@ bool file_exists(const std::string& p)
     return stat(p.c_str(), &sb) == 0;
 }
 
-int lattice_code(const std::string& l) { return l == "regular" ? 0 : (l == "csr-hypercube" ? 1 : 2); }
+int lattice_code(const std::string& l)
+{
+    return l == "regular" ? 0 : (l == "csr-hypercube" ? 1 : (l == "rgg" ? 2 : 3));
+}
 
 void write_ckpt(const std::string& path, const CkptHeader& h, const std::vector<Sim>& sims,
                 const std::vector<std::vector<char>>& spins)
@@ -271,6 +283,7 @@ void print_usage()
 {
     std::fprintf(stderr,
                  "usage: marqov_gpu <Hamiltonian> [--config DIR] [--out DIR] [--seed S] [--device D] [--dry-run] [--fresh]\n"
+                 "                  [--graph FILE.csv [--idxoffs K]]\n"
                  "  Hamiltonian: Ising, Ising4d, IsingRGG, Heisenberg, Phi4, Potts (q from PottsSelect.ini),\n"
                  "               Potts<q>, EdwardsAnderson-Ising\n");
 }
@@ -290,7 +303,8 @@ void write_sim(const Plan& pl, const Spec& s, const Sim& sim, uint64_t seed, con
         std::fclose(fp);
         return;
     }
-    std::fprintf(fp, "# marqov_gpu %s L=%d dim=%d lattice=%s\n", s.block.c_str(), pl.L, pl.dim, pl.lattice.c_str());
+    std::fprintf(fp, "# marqov_gpu %s L=%d dim=%d lattice=%s sites=%lld%s%s\n", s.block.c_str(), pl.L, pl.dim,
+                 pl.lattice.c_str(), (long long)pl.nsites, pl.source.empty() ? "" : " graph=", pl.source.c_str());
     std::fprintf(fp, "# beta=%.17g J=%.17g", sim.p.beta, sim.p.J);
     if (s.model == MQ_PHI4) std::fprintf(fp, " lambda=%.17g mass=%.17g", sim.p.lambda, sim.p.mass);
     if (s.model == MQ_POTTS) std::fprintf(fp, " q=%d", s.q);
@@ -325,6 +339,8 @@ int main(int argc, char** argv)
         else if (a == "--device") o.device = std::atoi(next("--device").c_str());
         else if (a == "--dry-run") o.dry = true;
         else if (a == "--fresh") o.fresh = true;
+        else if (a == "--graph") o.graph = next("--graph");
+        else if (a == "--idxoffs") o.idxoffs = std::atoi(next("--idxoffs").c_str());
         else if (a == "-h" || a == "--help") {
             print_usage();
             return 0;
@@ -375,19 +391,35 @@ int main(int argc, char** argv)
             }
         }
 
+        Csr csv;
+        if (!o.graph.empty()) {
+            if (s.rgg) throw std::runtime_error("--graph replaces the random geometric graphs of IsingRGG; use Ising");
+            csv = read_csv_graph(o.graph, o.idxoffs);
+        }
         std::vector<Plan> plans;
         int id = 0;
-        for (std::size_t li = 0; li < nL.size(); ++li) {
+        const std::size_t nplans = o.graph.empty() ? nL.size() : 1;
+        for (std::size_t li = 0; li < nplans; ++li) {
             Plan pl;
-            pl.L = nL[li];
             pl.dim = dim;
             pl.nrep = nrep[li];
-            pl.ncluster = (int)(amp * std::pow((double)pl.L, (double)cexp));
             pl.nmetro = nmetro;
             pl.nwarm = 10 * (warm / 10);
             pl.nmeas = 10 * (meas / 10);
-            pl.lattice = s.rgg ? "rgg" : ((dim == 2 || dim == 3) && pl.L % 2 == 0 && pl.L >= 4) ? "regular" : "csr-hypercube";
-            pl.outpath = outdir + "/" + std::to_string(pl.L) + "/";
+            if (o.graph.empty()) {
+                pl.L = nL[li];
+                pl.nsites = (int64_t)std::llround(std::pow((double)pl.L, (double)dim));
+                pl.ncluster = (int)(amp * std::pow((double)pl.L, (double)cexp));
+                pl.lattice = s.rgg ? "rgg" : ((dim == 2 || dim == 3) && pl.L % 2 == 0 && pl.L >= 4) ? "regular" : "csr-hypercube";
+                pl.outpath = outdir + "/" + std::to_string(pl.L) + "/";
+            } else {
+                pl.L = 0;
+                pl.nsites = csv.nsites();
+                pl.ncluster = (int)(amp * std::pow((double)pl.nsites, (double)cexp / (double)std::max(dim, 1)));
+                pl.lattice = "csv";
+                pl.source = o.graph;
+                pl.outpath = outdir + "/graph/";
+            }
             // simulation ids in the reference's order: parameter sets, then replicas
             std::map<int, Sim> sims;
             std::vector<int> gof(psets.size());
@@ -401,10 +433,10 @@ int main(int argc, char** argv)
 
         if (o.dry) {
             for (const auto& pl : plans) {
-                std::printf("{\"L\": %d, \"dim\": %d, \"lattice\": \"%s\", \"ncluster\": %d, \"nmetro\": %d, "
-                            "\"warmup\": %d, \"measure\": %d, \"out\": \"%s\", \"groups\": [",
-                            pl.L, pl.dim, pl.lattice.c_str(), pl.ncluster, pl.nmetro, pl.nwarm, pl.nmeas,
-                            pl.outpath.c_str());
+                std::printf("{\"L\": %d, \"dim\": %d, \"nsites\": %lld, \"lattice\": \"%s\", \"ncluster\": %d, "
+                            "\"nmetro\": %d, \"warmup\": %d, \"measure\": %d, \"out\": \"%s\", \"groups\": [",
+                            pl.L, pl.dim, (long long)pl.nsites, pl.lattice.c_str(), pl.ncluster, pl.nmetro, pl.nwarm,
+                            pl.nmeas, pl.outpath.c_str());
                 for (std::size_t g = 0; g < pl.groups.size(); ++g) {
                     std::printf("%s[", g ? ", " : "");
                     for (std::size_t k = 0; k < pl.groups[g].sims.size(); ++k) {
@@ -425,15 +457,17 @@ int main(int argc, char** argv)
         make_dir(outdir);
         for (const auto& pl : plans) {
             make_dir(pl.outpath);
-            std::printf("\nL = %d (%s, %d groups, ncluster %d, nmetro %d)\n", pl.L, pl.lattice.c_str(),
+            std::printf("\nL = %d, %lld sites (%s, %d groups, ncluster %d, nmetro %d)\n", pl.L, (long long)pl.nsites,
+                        pl.lattice.c_str(),
                         (int)pl.groups.size(), pl.ncluster, pl.nmetro);
             std::unique_ptr<RegularLattice> reglat;
             std::vector<std::unique_ptr<GraphLattice>> graphs;
+            Csr hg;  // the plan's one graph (hypercube or file), kept for EA couplings
             if (pl.lattice == "regular") {
                 reglat.reset(new RegularLattice(ctx, pl.dim, pl.L));
-            } else if (pl.lattice == "csr-hypercube") {
-                const Csr g = hypercube_csr(pl.dim, pl.L);
-                graphs.emplace_back(new GraphLattice(ctx, g.row_ptr, g.col_idx));
+            } else if (pl.lattice == "csr-hypercube" || pl.lattice == "csv") {
+                hg = pl.lattice == "csv" ? csv : hypercube_csr(pl.dim, pl.L);
+                graphs.emplace_back(new GraphLattice(ctx, hg.row_ptr, hg.col_idx));
             } else {
                 int64_t n = 1;
                 for (int d = 0; d < pl.dim; ++d) n *= pl.L;
@@ -467,7 +501,8 @@ int main(int argc, char** argv)
                 const int per_site = (s.model == MQ_HEISENBERG || s.model == MQ_PHI4) ? 3 : 1;
                 const std::size_t nsites = reglat ? reglat->size() : gl[0]->size();
                 const std::size_t bytes = nsites * (per_site == 3 ? 3 * sizeof(float) : 1);
-                CkptHeader ck{CKPT_MAGIC, s.model, s.q, pl.dim, pl.L, nr, per_site, 0, lattice_code(pl.lattice),
+                const int32_t lsize = pl.lattice == "csv" ? (int32_t)pl.nsites : pl.L;
+                CkptHeader ck{CKPT_MAGIC, s.model, s.q, pl.dim, lsize, nr, per_site, 0, lattice_code(pl.lattice),
                               o.seed, seed, 0, sims[0].p.J, sims[0].p.lambda, sims[0].p.mass};
                 const std::string ckpath = pl.outpath + "checkpoint_g" + std::to_string(g) + ".bin";
                 const bool resume = !o.fresh && file_exists(ckpath);
@@ -492,7 +527,7 @@ int main(int argc, char** argv)
                         // graphs: +-1 per undirected edge from a hash of (sample, edge), so
                         // both CSR slots of an edge carry the same coupling
                         ctx.check(mq_state_set_samples(b->get(), sample.data()), "mq_state_set_samples");
-                        const Csr g = hypercube_csr(pl.dim, pl.L);
+                        const Csr& g = hg;
                         const uint64_t n = (uint64_t)g.nsites();
                         std::vector<int8_t> bonds(g.col_idx.size());
                         for (int sm = 0; sm < pl.nrep; ++sm) {

@@ -1,6 +1,7 @@
 // Prints the graphs built by marqov_amd/driver/lattices.hpp for the tests:
 //   lattice_check cube <dim> <L>           -> "n nnz", row_ptr, col_idx
 //   lattice_check rgg <dim> <n> <r> <seed> -> "n nnz", the points, row_ptr, col_idx
+//   lattice_check csv <path> [idxoffs]      -> "n nnz", row_ptr, col_idx
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -43,6 +44,10 @@ int main(int argc, char** argv)
         std::printf("\n");
         return 0;
     }
-    std::fprintf(stderr, "usage: lattice_check cube <dim> <L> | rgg <dim> <n> <r> <seed>\n");
+    if (argc >= 3 && !std::strcmp(argv[1], "csv")) {
+        dump(read_csv_graph(argv[2], argc >= 4 ? std::atoi(argv[3]) : 0));
+        return 0;
+    }
+    std::fprintf(stderr, "usage: lattice_check cube <dim> <L> | rgg <dim> <n> <r> <seed> | csv <path> [idxoffs]\n");
     return 2;
 }

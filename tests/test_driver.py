@@ -121,6 +121,41 @@ def test_rgg_matches_brute_force(dim, n, r):
         assert list(ci[rp[i]:rp[i + 1]]) == list(np.nonzero(adj[i])[0]), i
 
 
+def test_csv_graph_reader_matches_python(tmp_path):
+    """GraphFromCSV format: the C++ reader of the front-end and the Python reader agree,
+    with 1-based neighbour indices shifted back by idxoffs."""
+    from marqov_amd import graphs
+    rp, ci, pts = graphs.rgg2d(500, graphs.rgg_radius(500), 9)
+    path = str(tmp_path / "g.csv")
+    graphs.write_csv_graph(path, rp, ci, pts)
+    out = subprocess.run([LCHK, "csv", path], capture_output=True, text=True, check=True).stdout.splitlines()
+    rp2, ci2 = _read_csr(out, 1)
+    assert np.array_equal(rp2, rp) and np.array_equal(ci2, ci)
+    path1 = str(tmp_path / "g1.csv")
+    graphs.write_csv_graph(path1, rp, ci + 1, pts)
+    out = subprocess.run([LCHK, "csv", path1, "1"], capture_output=True, text=True, check=True).stdout.splitlines()
+    rp3, ci3 = _read_csr(out, 1)
+    assert np.array_equal(rp3, rp) and np.array_equal(ci3, ci)
+    rp4, ci4, _ = graphs.read_csv_graph(path1, idxoffs=1)
+    assert np.array_equal(rp4, rp) and np.array_equal(ci4, ci)
+
+
+def test_plan_csv_graph(tmp_path):
+    from marqov_amd import graphs
+    rp, ci = graphs.regular_csr(2, 3)
+    path = str(tmp_path / "t33.csv")
+    graphs.write_csv_graph(path, rp, ci)
+    r = subprocess.run([BIN, "Ising", "--config", CONF, "--graph", path, "--dry-run"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    p = [json.loads(line) for line in r.stdout.splitlines()]
+    assert len(p) == 1 and p[0]["lattice"] == "csv" and p[0]["nsites"] == 9 and p[0]["out"] == "out_ising/graph/"
+    assert p[0]["ncluster"] == 2 and len(p[0]["groups"][0]) == 24
+    r = subprocess.run([BIN, "IsingRGG", "--config", CONF, "--graph", path, "--dry-run"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1 and "--graph replaces" in r.stderr
+
+
 def _run(ham, out, seed=5):
     r = subprocess.run([BIN, ham, "--config", CONF, "--out", str(out), "--seed", str(seed)], capture_output=True,
                        text=True, timeout=110)
@@ -241,3 +276,31 @@ def test_gpu_ini_resume_is_exact(tmp_path, ham, Ls):
     r = subprocess.run([BIN, ham, "--config", conf, "--out", str(twice), "--seed", "12"], capture_output=True,
                        text=True, timeout=110)
     assert r.returncode == 1 and "--seed 11" in r.stderr
+
+
+@pytest.mark.gpu
+def test_gpu_ini_csv_graph_vs_exact(tmp_path):
+    """--graph: Ising.ini's temperatures and replicas run on a GraphFromCSV file (the
+    3x3 torus written with 1-based indices, read with idxoffs 1); <E/N> and <m^2>
+    agree with exact enumeration of that graph within 4 standard errors."""
+    from marqov_amd import graphs
+    rp, ci = graphs.regular_csr(2, 3)
+    path = str(tmp_path / "t33.csv")
+    graphs.write_csv_graph(path, rp, ci + 1)
+    out = tmp_path / "out"
+    r = subprocess.run([BIN, "Ising", "--config", CONF, "--out", str(out), "--graph", path, "--idxoffs", "1"],
+                       capture_output=True, text=True, timeout=110)
+    print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    assert r.returncode == 0, r.stderr
+    for beta in (0.3, 0.4406868, 0.7):
+        e_rep, m2_rep = [], []
+        for rep in range(8):
+            hdr, d = _series(out / "graph" / f"beta{beta:.6f}_{rep}.txt")
+            assert d.shape == (4000, 6) and "lattice=csv" in hdr[0]
+            e_rep.append(d[:, 1].mean())
+            m2_rep.append(d[:, 3].mean())
+        ex = exact.enumerate_graph(rp, ci, beta, J=-1.0)
+        for got, want in ((e_rep, ex["e"]), (m2_rep, ex["m2"])):
+            g = np.asarray(got)
+            se = g.std(ddof=1) / np.sqrt(len(g))
+            assert abs(g.mean() - want) < 4 * se + 1e-9, (beta, g.mean(), se, want)
