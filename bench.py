@@ -16,9 +16,10 @@ Beside it on the same JSON line (skip with --quick):
                captured {sweep, PT swap}, batched vs plain
   c5_rgg       2D random geometric graph N=2^20, Ising and Potts-3, 64 replicas vs 1
   roofline     the headline sweep kernel's algorithmic HBM rate (3 B/attempt)
-  cpu_baseline the reference-faithful CPU restatement (random-site Metropolis,
-               mt19937_64; and its stack Wolff on 2D L=64), one replica per thread
-               on this host's cores
+  cpu_baseline the reference's own Metropolis move compiled from its headers
+               (oracle/_ref; the reference-faithful CPU restatement when that is not
+               built) and the restatement's stack Wolff on 2D L=64, one simulation
+               per thread on this host's cores
 """
 import argparse
 import json
@@ -346,12 +347,33 @@ def main():
         import oracle_bind as ob
         ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         nth = max(1, min(16, ncpu))
-        out = np.zeros(3)
-        ob.lib().ref_bench_regular(1, 3, L, -1.0, 2, beta, 1, 3, nth, 2, 0, ob.p(out))
-        line["cpu_baseline"] = {"value": round(float(out[0]), 5), "unit": "attempts/ns", "cores": nth,
-                                "kind": "port", "per_core": round(float(out[1]), 5),
-                                "sample": f"{nth} threads x (1 warm-up + 3 timed) random-site Metropolis sweeps "
-                                          f"of 3D Ising L={L}, one replica per thread, {out[2]:.1f} s timed"}
+        ref_bin = os.path.join(ROOT, "oracle", "_ref", "ref_metropolis")
+        ref = None
+        if os.path.exists(ref_bin):
+            # the reference's own Metropolis<H, RegularLattice>::move, compiled from its headers
+            import subprocess
+            try:
+                r = subprocess.run([ref_bin, "bench", "ising", "3", str(L), repr(beta), "-1.0", "1", "3", "2", str(nth)],
+                                   capture_output=True, text=True, timeout=600)
+                if r.returncode == 0:
+                    ref = json.loads(r.stdout.strip().splitlines()[-1])
+            except (OSError, ValueError, subprocess.TimeoutExpired):
+                ref = None
+        if ref is not None:
+            line["cpu_baseline"] = {"value": round(float(ref["attempts_per_ns"]), 5), "unit": "attempts/ns",
+                                    "cores": nth, "kind": "reference",
+                                    "per_core": round(float(ref["per_thread"]), 5),
+                                    "sample": f"{nth} threads x (1 warm-up + 3 timed) sweeps of the reference's "
+                                              f"Metropolis<Ising, RegularLattice>::move (random site, mt19937_64 "
+                                              f"RNGCache) on 3D Ising L={L}, one simulation per thread, "
+                                              f"{ref['seconds']:.1f} s timed"}
+        else:
+            out = np.zeros(3)
+            ob.lib().ref_bench_regular(1, 3, L, -1.0, 2, beta, 1, 3, nth, 2, 0, ob.p(out))
+            line["cpu_baseline"] = {"value": round(float(out[0]), 5), "unit": "attempts/ns", "cores": nth,
+                                    "kind": "port", "per_core": round(float(out[1]), 5),
+                                    "sample": f"{nth} threads x (1 warm-up + 3 timed) random-site Metropolis sweeps "
+                                              f"of 3D Ising L={L}, one replica per thread, {out[2]:.1f} s timed"}
         # Wolff on the 2D L=64 critical Ising lattice, the configuration the CPU reference runs
         ow = np.zeros(3)
         ob.lib().ref_bench_regular(1, 2, 64, -1.0, 2, 0.4406868, 500, 20000, nth, 1, 1, ob.p(ow))

@@ -17,12 +17,21 @@
  * HDF5-bound core.h and are not built here.
  *
  * usage: ref_metropolis ising|heis DIM L BETA J NSWEEPS SEED
- * prints the final state, one site per line, in site-index order.
+ *   prints the final state, one site per line, in site-index order.
+ * usage: ref_metropolis bench ising|heis DIM L BETA J NWARM NSWEEPS SEED NTHREADS
+ *   times the reference's move as the CPU baseline: NTHREADS independent
+ *   simulations, one per thread (the CXX11Scheduler default of one simulation per
+ *   hardware thread, marqovscheduler.h:281-286), NWARM untimed then NSWEEPS
+ *   timed sweeps of N random-site moves each; prints one JSON line with the
+ *   aggregate and per-thread attempts/ns.
  */
 #include <array>
+#include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "util/svmath.h"
@@ -62,8 +71,58 @@ static void run(int dim, int L, double beta, double J, int nsweeps, unsigned lon
     }
 }
 
+template <class SV>
+static void bench(int dim, int L, double beta, double J, int nwarm, int nsweeps, unsigned long long seed, int nth,
+                  bool ising)
+{
+    RegularLattice lat(L, dim);
+    const std::size_t N = lat.size();
+    std::vector<double> secs(nth, 0.0);
+    std::atomic<int> ready{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t)
+        th.emplace_back([&, t]() {
+            UserHam<SV> ham(J);
+            RNGCache<std::mt19937_64> rng(seed + (unsigned long long)t);
+            std::vector<SV> s(N);
+            for (std::size_t i = 0; i < N; ++i) {
+                if (ising) s[i][0] = (rng.real() > 0.5) ? 1 : -1;
+                else set_to_rnddir(rng, s[i]);
+            }
+            auto sweep = [&]() {
+                for (std::size_t i = 0; i < N; ++i)
+                    MARQOV::Metropolis<UserHam<SV>, RegularLattice>::move(ham, lat, s, rng, beta, rng.integer(N));
+            };
+            for (int w = 0; w < nwarm; ++w) sweep();
+            ready++;
+            while (ready.load() < nth) std::this_thread::yield();
+            const auto a = std::chrono::steady_clock::now();
+            for (int w = 0; w < nsweeps; ++w) sweep();
+            secs[t] = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        });
+    for (auto& x : th) x.join();
+    double wall = 0.0, per = 0.0;
+    for (double x : secs) {
+        wall = x > wall ? x : wall;
+        per += (double)N * nsweeps / (x * 1e9);
+    }
+    std::printf("{\"attempts_per_ns\": %.6g, \"per_thread\": %.6g, \"threads\": %d, \"seconds\": %.3f, \"sites\": %zu}\n",
+                (double)N * nsweeps * nth / (wall * 1e9), per / nth, nth, wall, N);
+}
+
 int main(int argc, char** argv)
 {
+    if (argc == 11 && std::strcmp(argv[1], "bench") == 0) {
+        const bool ising = std::strcmp(argv[2], "ising") == 0;
+        const int dim = std::atoi(argv[3]), L = std::atoi(argv[4]);
+        const double beta = std::atof(argv[5]), J = std::atof(argv[6]);
+        const int nwarm = std::atoi(argv[7]), nsweeps = std::atoi(argv[8]);
+        const unsigned long long seed = std::strtoull(argv[9], nullptr, 10);
+        const int nth = std::atoi(argv[10]) > 0 ? std::atoi(argv[10]) : 1;
+        if (ising) bench<std::array<int, 1>>(dim, L, beta, J, nwarm, nsweeps, seed, nth, true);
+        else bench<std::array<double, 3>>(dim, L, beta, J, nwarm, nsweeps, seed, nth, false);
+        return 0;
+    }
     if (argc != 8) {
         std::fprintf(stderr, "usage: %s ising|heis DIM L BETA J NSWEEPS SEED\n", argv[0]);
         return 2;
