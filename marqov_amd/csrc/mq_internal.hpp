@@ -34,15 +34,25 @@ struct Key2 {
     uint32_t k0, k1;
 };
 
-__host__ __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, Key2 key)
+// a ^ b ^ k in one VALU instruction: v_bitop3_b32 with truth table 0x96 (odd
+// parity of the three inputs); k is wave-uniform (the round key) and stays in
+// an SGPR.  The compiler otherwise emits two v_xor_b32 per XOR3 here.
+__device__ __forceinline__ uint32_t xor3_k(uint32_t a, uint32_t b, uint32_t k)
+{
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(k));
+    return r;
+}
+
+__device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, Key2 key)
 {
     uint32_t k0 = key.k0, k1 = key.k1;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3_k((uint32_t)(p1 >> 32), c1, k0);
+        const uint32_t n2 = xor3_k((uint32_t)(p0 >> 32), c3, k1);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
