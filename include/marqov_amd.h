@@ -159,6 +159,15 @@ MQ_API int mq_renormalize(mq_state* st);
  * chain is the same).  accepted (host, may be NULL) gets the number of
  * accepted swaps per group. */
 MQ_API int mq_pt_swap(mq_state* st, int ntemp, int parity, uint64_t seed, uint64_t step, int32_t* accepted);
+/* Replica exchange along an explicit plan: the reference's scheduler offers
+ * the pairs of a user-made plan (createPTplan, ptstep,
+ * src/libmarqov/marqovscheduler.h:202-211,374-413).  pairs[2p], pairs[2p+1]
+ * (p < npairs) are temperature slots 0..ntemp-1 offered in every group with the
+ * acceptance of mq_pt_swap; the pairs of one call must be disjoint, a pair of a
+ * slot with itself offers nothing.  The uniform of pair (a < b) is keyed by a,
+ * so the neighbour pairs (k, k+1) of one parity reproduce mq_pt_swap exactly. */
+MQ_API int mq_pt_swap_pairs(mq_state* st, int ntemp, int npairs, const int32_t* pairs, uint64_t seed, uint64_t step,
+                            int32_t* accepted);
 /* temperature index currently held by each replica (after swaps) */
 MQ_API int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep);
 

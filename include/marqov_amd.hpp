@@ -15,6 +15,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "marqov_amd.h"
@@ -166,6 +167,20 @@ public:
     {
         std::vector<int32_t> acc(nrep_ / ntemp);
         ctx_.check(mq_pt_swap(h_, ntemp, parity, seed, step, acc.data()), "mq_pt_swap");
+        return acc;
+    }
+    // one step of a PT plan: disjoint temperature-slot pairs, offered in every group
+    std::vector<int32_t> pt_swap_pairs(int ntemp, const std::vector<std::pair<int32_t, int32_t>>& pairs, uint64_t seed,
+                                       uint64_t step)
+    {
+        std::vector<int32_t> flat;
+        for (const auto& p : pairs) {
+            flat.push_back(p.first);
+            flat.push_back(p.second);
+        }
+        std::vector<int32_t> acc(nrep_ / ntemp);
+        ctx_.check(mq_pt_swap_pairs(h_, ntemp, (int)pairs.size(), flat.data(), seed, step, acc.data()),
+                   "mq_pt_swap_pairs");
         return acc;
     }
     template <class T>

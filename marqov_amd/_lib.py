@@ -72,6 +72,7 @@ SIGNATURES = {
     "mq_energy": (_I, [_P, _P]),
     "mq_renormalize": (_I, [_P]),
     "mq_pt_swap": (_I, [_P, _I, _I, _U64, _U64, _P]),
+    "mq_pt_swap_pairs": (_I, [_P, _I, _I, _P, _U64, _U64, _P]),
     "mq_pt_get_temps": (_I, [_P, _P]),
     "mq_state_set_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
     "mq_state_get_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),

@@ -421,6 +421,7 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_temp_of_rep);
     dfree(st->d_rep_of_temp);
     dfree(st->d_pt_acc);
+    dfree(st->d_pt_pairs);
     dfree(st->d_energy);
     dfree(st->d_pack);
     if (st->h_pack) (void)hipHostFree(st->h_pack);

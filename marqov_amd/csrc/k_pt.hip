@@ -47,6 +47,45 @@ __device__ __forceinline__ double mh_ratio(double en)
     return 0.0;
 }
 
+// One offered exchange between temperature slots ka < kb of group g: the
+// replicas a, b holding them swap temperatures with probability
+// min(1, calcMHratio((beta_ka - beta_kb)(E_a - E_b))); the uniform is the
+// 53-bit Philox draw keyed by (ka, step, g).
+__device__ __forceinline__ void pt_offer(int g, int ntemp, int ka, int kb, uint32_t step, Key2 key,
+                                         int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
+                                         const double* __restrict__ energy, const double* __restrict__ ladder,
+                                         uint32_t* __restrict__ thr, int tbl, uint32_t* __restrict__ wthr,
+                                         float* __restrict__ betaf, double* __restrict__ betad, int32_t* __restrict__ acc)
+{
+    const int64_t sa = (int64_t)g * ntemp + ka, sb = (int64_t)g * ntemp + kb;
+    const int a = rep_of_temp[sa], b = rep_of_temp[sb];
+    const double en = (ladder[sa] - ladder[sb]) * (energy[a] - energy[b]);
+    const double ratio = mh_ratio(en);
+    const uint4 w = philox4x32_10((uint32_t)ka, step, (uint32_t)g, ST_PT, key);
+    const double u = (double)((((uint64_t)w.x << 32) | w.y) >> 11) * 0x1.0p-53;
+    if (!(u < fmin(1.0, ratio))) return;
+    rep_of_temp[sa] = b;
+    rep_of_temp[sb] = a;
+    temp_of_rep[a] = kb;
+    temp_of_rep[b] = ka;
+    for (int e = 0; e < 2 * tbl; ++e) {
+        const uint32_t x = thr[(int64_t)a * 2 * tbl + e];
+        thr[(int64_t)a * 2 * tbl + e] = thr[(int64_t)b * 2 * tbl + e];
+        thr[(int64_t)b * 2 * tbl + e] = x;
+    }
+    const uint32_t ww = wthr[a];
+    wthr[a] = wthr[b];
+    wthr[b] = ww;
+    const float bf = betaf[a];
+    betaf[a] = betaf[b];
+    betaf[b] = bf;
+    const double bd = betad[a];
+    betad[a] = betad[b];
+    betad[b] = bd;
+    atomicAdd(acc + g, 1);
+}
+
+// neighbouring temperatures (k, k+1), k = parity mod 2, in every group
 __global__ void k_pt_swap(int ngroups, int ntemp, int parity, uint32_t step, const uint32_t* __restrict__ dyn, Key2 key,
                           int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
                           const double* __restrict__ energy, const double* __restrict__ ladder,
@@ -64,32 +103,23 @@ __global__ void k_pt_swap(int ngroups, int ntemp, int parity, uint32_t step, con
     const int g = (int)(t / npairs);
     const int k = parity + 2 * (int)(t - (int64_t)g * npairs);
     if (k + 1 >= ntemp) return;
-    const int64_t sa = (int64_t)g * ntemp + k, sb = sa + 1;
-    const int a = rep_of_temp[sa], b = rep_of_temp[sb];
-    const double en = (ladder[sa] - ladder[sb]) * (energy[a] - energy[b]);
-    const double ratio = mh_ratio(en);
-    const uint4 w = philox4x32_10((uint32_t)k, step, (uint32_t)g, ST_PT, key);
-    const double u = (double)((((uint64_t)w.x << 32) | w.y) >> 11) * 0x1.0p-53;
-    if (!(u < fmin(1.0, ratio))) return;
-    rep_of_temp[sa] = b;
-    rep_of_temp[sb] = a;
-    temp_of_rep[a] = k + 1;
-    temp_of_rep[b] = k;
-    for (int e = 0; e < 2 * tbl; ++e) {
-        const uint32_t x = thr[(int64_t)a * 2 * tbl + e];
-        thr[(int64_t)a * 2 * tbl + e] = thr[(int64_t)b * 2 * tbl + e];
-        thr[(int64_t)b * 2 * tbl + e] = x;
-    }
-    const uint32_t ww = wthr[a];
-    wthr[a] = wthr[b];
-    wthr[b] = ww;
-    const float bf = betaf[a];
-    betaf[a] = betaf[b];
-    betaf[b] = bf;
-    const double bd = betad[a];
-    betad[a] = betad[b];
-    betad[b] = bd;
-    atomicAdd(acc + g, 1);
+    pt_offer(g, ntemp, k, k + 1, step, key, rep_of_temp, temp_of_rep, energy, ladder, thr, tbl, wthr, betaf, betad,
+             acc);
+}
+
+// a plan of npairs disjoint slot pairs (lower slot first), the same in every group
+__global__ void k_pt_swap_pairs(int ngroups, int ntemp, int npairs, const int32_t* __restrict__ pairs, uint32_t step,
+                                Key2 key, int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
+                                const double* __restrict__ energy, const double* __restrict__ ladder,
+                                uint32_t* __restrict__ thr, int tbl, uint32_t* __restrict__ wthr,
+                                float* __restrict__ betaf, double* __restrict__ betad, int32_t* __restrict__ acc)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ngroups * npairs) return;
+    const int g = (int)(t / npairs);
+    const int p = (int)(t - (int64_t)g * npairs);
+    pt_offer(g, ntemp, pairs[2 * p], pairs[2 * p + 1], step, key, rep_of_temp, temp_of_rep, energy, ladder, thr, tbl,
+             wthr, betaf, betad, acc);
 }
 
 __global__ void k_advance(uint32_t* dyn, uint32_t dphase, uint32_t dstep)
@@ -198,6 +228,60 @@ extern "C" int mq_pt_swap(mq_state* st, int ntemp, int parity, uint64_t seed, ui
     MQ_HIP(st->ctx, zero_fill(st->d_pt_acc, (st->nrep + 1) / 2, s));
     MQ_HIP(st->ctx, energy_async(st));
     MQ_HIP(st->ctx, pt_swap_async(st, ntemp, parity, (uint32_t)step, key));
+    if (accepted) {
+        MQ_HIP(st->ctx, hipMemcpyAsync(accepted, st->d_pt_acc, sizeof(int32_t) * (st->nrep / ntemp),
+                                       hipMemcpyDeviceToHost, s));
+    }
+    MQ_HIP(st->ctx, hipStreamSynchronize(s));
+    return MQ_OK;
+}
+
+extern "C" int mq_pt_swap_pairs(mq_state* st, int ntemp, int npairs, const int32_t* pairs, uint64_t seed, uint64_t step,
+                                int32_t* accepted)
+{
+    if (!st || npairs < 0 || (npairs > 0 && !pairs)) return MQ_ERR_ARG;
+    if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
+    int rc = ensure_pt(st, ntemp);
+    if (rc) return rc;
+    // validate and normalise: slots in range, lower slot first, every slot at most once;
+    // a pair of a slot with itself offers nothing (as the reference's ptstep, which
+    // skips self swaps) and is dropped
+    std::vector<int32_t> plan;
+    std::vector<char> used((size_t)ntemp, 0);
+    for (int p = 0; p < npairs; ++p) {
+        int a = pairs[2 * p], b = pairs[2 * p + 1];
+        if (a < 0 || b < 0 || a >= ntemp || b >= ntemp)
+            return mq_fail(st->ctx, MQ_ERR_ARG, "PT pair slot outside 0..ntemp-1");
+        if (a == b) continue;
+        if (a > b) std::swap(a, b);
+        if (used[(size_t)a] || used[(size_t)b])
+            return mq_fail(st->ctx, MQ_ERR_ARG, "PT pairs of one step must be disjoint");
+        used[(size_t)a] = used[(size_t)b] = 1;
+        plan.push_back(a);
+        plan.push_back(b);
+    }
+    const int np = (int)plan.size() / 2;
+    const Key2 key{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    hipStream_t s = st->ctx->stream;
+    MQ_HIP(st->ctx, zero_fill(st->d_pt_acc, (st->nrep + 1) / 2, s));
+    if (np > 0) {
+        if ((int)st->pt_pairs_cap < np) {
+            if (st->d_pt_pairs) MQ_HIP(st->ctx, hipFree(st->d_pt_pairs));
+            st->d_pt_pairs = nullptr;
+            MQ_HIP(st->ctx, hipMalloc(&st->d_pt_pairs, sizeof(int32_t) * 2 * (size_t)np));
+            st->pt_pairs_cap = np;
+        }
+        MQ_HIP(st->ctx, hipMemcpyAsync(st->d_pt_pairs, plan.data(), sizeof(int32_t) * 2 * (size_t)np,
+                                       hipMemcpyHostToDevice, s));
+        MQ_HIP(st->ctx, energy_async(st));
+        const int ngroups = st->nrep / ntemp;
+        const int64_t work = (int64_t)ngroups * np;
+        hipLaunchKernelGGL(k_pt_swap_pairs, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ngroups, ntemp, np,
+                           st->d_pt_pairs, (uint32_t)step, key, st->d_rep_of_temp, st->d_temp_of_rep, st->d_energy,
+                           st->d_ladder, st->d_thr, st->tbl, st->d_wthr, st->d_betaf, st->d_betad, st->d_pt_acc);
+        MQ_HIP(st->ctx, hipGetLastError());
+        if (st->eabits) MQ_HIP(st->ctx, launch_wtab_eabits(st));
+    }
     if (accepted) {
         MQ_HIP(st->ctx, hipMemcpyAsync(accepted, st->d_pt_acc, sizeof(int32_t) * (st->nrep / ntemp),
                                        hipMemcpyDeviceToHost, s));

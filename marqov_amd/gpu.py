@@ -199,6 +199,14 @@ class State:
         self.ctx.check(self.ctx.lib.mq_pt_swap(self.h, ntemp, parity, seed, step, _ptr(acc)), "mq_pt_swap")
         return acc
 
+    def pt_swap_pairs(self, ntemp, pairs, seed, step):
+        """Offer the disjoint temperature-slot pairs of one step of a PT plan in every group."""
+        pr = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        acc = np.zeros(self.nrep // ntemp, np.int32)
+        self.ctx.check(self.ctx.lib.mq_pt_swap_pairs(self.h, ntemp, len(pr), _ptr(pr), seed, step, _ptr(acc)),
+                       "mq_pt_swap_pairs")
+        return acc
+
     def pt_temps(self):
         t = np.zeros(self.nrep, np.int32)
         self.ctx.check(self.ctx.lib.mq_pt_get_temps(self.h, _ptr(t)), "mq_pt_get_temps")

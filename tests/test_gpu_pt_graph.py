@@ -63,6 +63,70 @@ class Replay:
                     acc[g] += 1
         return acc
 
+    def swap_pairs(self, pairs, seed, step):
+        """One step of a PT plan (createPTplan/ptstep, marqovscheduler.h:202-211,374-413):
+        the disjoint slot pairs, lower slot first, offered in every group."""
+        acc = np.zeros(NS, np.int32)
+        for g in range(NS):
+            for ka, kb in pairs:
+                if ka == kb:
+                    continue
+                ka, kb = min(ka, kb), max(ka, kb)
+                sa, sb = g * NT + ka, g * NT + kb
+                a, b = self.rot[sa], self.rot[sb]
+                en = (BETAS[ka] - BETAS[kb]) * (self.energy(a) - self.energy(b))
+                ratio = 0.0 if en <= np.log(np.finfo(float).tiny) else (
+                    np.finfo(float).max if en > np.log(np.finfo(float).max) else np.exp(en))
+                w = ob.philox([ka, step & 0xffffffff, g, 5], [seed & 0xffffffff, seed >> 32])
+                u = (((w[0] << 32) | w[1]) >> 11) * 2.0 ** -53
+                if u < min(1.0, ratio):
+                    self.rot[sa], self.rot[sb] = b, a
+                    self.tor[a], self.tor[b] = kb, ka
+                    acc[g] += 1
+        return acc
+
+
+def test_pt_pair_plan_bit_exact(ctx):
+    """Random plans of disjoint pairs (neighbours and far temperatures, either order, a
+    self pair) swap exactly as the CPU replay decides."""
+    st = _ea_state(ctx)
+    rp = Replay()
+    rng = np.random.default_rng(12)
+    for step in range(8):
+        st.metropolis(2, seed=7, sweep0=2 * step)
+        rp.sweeps(2, 7, 2 * step)
+        perm = rng.permutation(NT)
+        pairs = [(int(perm[2 * i]), int(perm[2 * i + 1])) for i in range(int(rng.integers(1, NT // 2 + 1)))]
+        if step == 3:
+            pairs.append((int(perm[-1]), int(perm[-1])))  # a self pair offers nothing
+        acc = st.pt_swap_pairs(NT, pairs, seed=9, step=step)
+        eacc = rp.swap_pairs(pairs, 9, step)
+        assert np.array_equal(acc, eacc), (step, pairs, acc, eacc)
+        assert np.array_equal(st.pt_temps(), rp.tor), step
+    for r in range(NS * NT):
+        assert np.array_equal(st.download(r), rp.s[r]), r
+
+
+def test_pt_pairs_neighbours_equal_even_odd(ctx):
+    """Neighbour pairs (k, k+1) of one parity as a plan reproduce mq_pt_swap exactly."""
+    a, b = _ea_state(ctx), _ea_state(ctx)
+    for step in range(6):
+        for st in (a, b):
+            st.metropolis(2, seed=7, sweep0=2 * step)
+        par = step & 1
+        acc_a = a.pt_swap(NT, par, seed=9, step=step)
+        acc_b = b.pt_swap_pairs(NT, [(k + 1, k) for k in range(par, NT - 1, 2)], seed=9, step=step)
+        assert np.array_equal(acc_a, acc_b) and np.array_equal(a.pt_temps(), b.pt_temps()), step
+
+
+def test_pt_pair_plan_rejects_bad_plans(ctx):
+    from marqov_amd.gpu import MarqovError
+    st = _ea_state(ctx)
+    with pytest.raises(MarqovError, match="disjoint"):
+        st.pt_swap_pairs(NT, [(0, 1), (1, 2)], seed=1, step=0)
+    with pytest.raises(MarqovError, match="outside"):
+        st.pt_swap_pairs(NT, [(0, NT)], seed=1, step=0)
+
 
 def test_pt_swaps_bit_exact(ctx):
     st = _ea_state(ctx)
