@@ -87,6 +87,10 @@ __device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
 // own `hprev`.
 constexpr int WCAP = 4096;
 
+// LDS bytes of one replica's spins in the small-lattice kernel: a byte per
+// site, or a bit per site when packed (16-byte multiples)
+__host__ __device__ inline uint32_t wave_spin_bytes(uint32_t N, bool pack) { return pack ? (N / 8 + 15) & ~15u : (N + 15) & ~15u; }
+
 template <int DIM, int KIND>
 __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base, int64_t stride, int L, int nclusters,
                                                        uint32_t cl0, int rep0, int Q,
@@ -254,7 +258,11 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
 // k_wolff_regular, so the clusters are identical.  (Evaluating every bond of
 // the lattice up front instead of pass B was measured at +5% on the 2D L=64
 // critical batch: the level bookkeeping, not the Philox draws, sets the pace.)
-template <int DIM, int KIND>
+// PACK (Ising, N a multiple of 32): the replica's spins sit in LDS as one bit
+// per site (set = down), 1/8 of the bytes, so a CU holds up to twice as many
+// replicas' waves and a batch larger than the CUs' share of the byte layout
+// stays resident; a claim is then an atomic OR / AND on the site's bit.
+template <int DIM, int KIND, bool PACK>
 __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int64_t stride, int L, int nrep,
                                                     int nclusters, uint32_t cl0, int rep0, int Q,
                                                     const uint32_t* __restrict__ wthr, int32_t* __restrict__ queue,
@@ -273,16 +281,29 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
     const uint32_t half = N >> 1;
     const float invL = 1.0f / (float)L;
     unsigned char* mine = smem + (size_t)wave * rep_lds;
-    uint32_t* SW = reinterpret_cast<uint32_t*>(mine);
+    uint32_t* SW = reinterpret_cast<uint32_t*>(mine);  // PACK: bit h of word h>>5 = site address h is down
     int8_t* S = reinterpret_cast<int8_t*>(mine);
-    uint32_t* cb_b = reinterpret_cast<uint32_t*>(mine + ((N + 15) & ~15u));  // bond id
+    uint32_t* cb_b = reinterpret_cast<uint32_t*>(mine + wave_spin_bytes(N, PACK));  // bond id
     uint16_t* cb_j = reinterpret_cast<uint16_t*>(cb_b + NCB);                 // natural index
     uint16_t* cb_a = cb_j + NCB;                                              // colour-split address
     uint16_t* ring = cb_a + NCB;
     const uint32_t cmask = (uint32_t)cap - 1u;
     uint32_t* gw = reinterpret_cast<uint32_t*>(base + r * stride);  // N is a multiple of 4 (L even)
     int32_t* q = queue + rep_site_off[grep];
-    for (uint32_t i = lane; i < N / 4; i += 64) SW[i] = gw[i];
+    if (PACK) {
+        // 32 spin bytes -> one word: bit 1 of each byte (set iff -1) gathered by a multiply
+        const uint4* g4 = reinterpret_cast<const uint4*>(base + r * stride);
+        for (uint32_t w = lane; w < N / 32; w += 64) {
+            const uint4 a = g4[2 * w], b = g4[2 * w + 1];
+            const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            uint32_t word = 0u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) word |= ((((d[k] >> 1) & 0x01010101u) * 0x08102040u) >> 27) << (4 * k);
+            SW[w] = word;
+        }
+    } else {
+        for (uint32_t i = lane; i < N / 4; i += 64) SW[i] = gw[i];
+    }
     const uint32_t tw = wthr[grep];
     const uint64_t lt = __lanemask_lt();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -292,11 +313,13 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
         const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
         const uint32_t seed = (uint32_t)(((uint64_t)sw.x * (uint64_t)N) >> 32);
         const uint32_t seed_a = (uint32_t)split_addr<DIM>(L, half, seed);
-        const int val = S[seed_a];
+        const int val = PACK ? (((SW[seed_a >> 5] >> (seed_a & 31u)) & 1u) ? -1 : 1) : (int)S[seed_a];
         const int newq = (KIND == WK_POTTS) ? (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q : -val;
+        const int vbit = val < 0 ? 1 : 0;  // PACK: the old value's bit
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         if (lane == 0) {
-            S[seed_a] = (int8_t)newq;
+            if (PACK) SW[seed_a >> 5] ^= 1u << (seed_a & 31u);
+            else S[seed_a] = (int8_t)newq;
             ring[0] = (uint16_t)seed;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -337,10 +360,10 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
                     jav[sl] = nbase + (j >> 1);
                 }
 #pragma unroll
-                for (int sl = 0; sl < NS; ++sl) sv[sl] = S[jav[sl]];
+                for (int sl = 0; sl < NS; ++sl) sv[sl] = PACK ? (int)((SW[jav[sl] >> 5] >> (jav[sl] & 31u)) & 1u) : (int)S[jav[sl]];
 #pragma unroll
                 for (int sl = 0; sl < NS; ++sl) {
-                    const bool cand = act && sv[sl] == val;
+                    const bool cand = act && sv[sl] == (PACK ? vbit : val);
                     const uint64_t m = __ballot(cand);
                     if (cand) {
                         const int p = ncand + __popcll(m & lt);
@@ -363,7 +386,11 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
                         const uint4 bw = philox4x32_10(b >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
                         if (u4_get(bw, (int)(b & 3u)) < tw) {
                             const uint32_t sh = (ja & 3u) * 8u;
-                            if (KIND == WK_ISING) {
+                            if (PACK) {
+                                const uint32_t mb = 1u << (ja & 31u);
+                                const uint32_t old = vbit ? atomicAnd(&SW[ja >> 5], ~mb) : atomicOr(&SW[ja >> 5], mb);
+                                ok = ((old & mb) != 0u) == (vbit != 0);
+                            } else if (KIND == WK_ISING) {
                                 const uint32_t mm = 0xFEu << sh;
                                 const uint32_t old = val > 0 ? atomicOr(&SW[ja >> 2], mm) : atomicAnd(&SW[ja >> 2], ~mm);
                                 ok = (int8_t)(old >> sh) == (int8_t)val;
@@ -398,7 +425,20 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
         if (lane == 0 && csize) csize[(int64_t)grep * nclusters + c] = tail;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    for (uint32_t i = lane; i < N / 4; i += 64) gw[i] = SW[i];
+    if (PACK) {
+        // one word -> 32 spin bytes: +1 = 0x01, -1 = 0xFF
+        uint4* g4 = reinterpret_cast<uint4*>(base + r * stride);
+        for (uint32_t w = lane; w < N / 32; w += 64) {
+            const uint32_t word = SW[w];
+            uint32_t d[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) d[k] = (((((word >> (4 * k)) & 15u) * 0x00204081u) & 0x01010101u) * 0xFEu) | 0x01010101u;
+            g4[2 * w] = make_uint4(d[0], d[1], d[2], d[3]);
+            g4[2 * w + 1] = make_uint4(d[4], d[5], d[6], d[7]);
+        }
+    } else {
+        for (uint32_t i = lane; i < N / 4; i += 64) gw[i] = SW[i];
+    }
 }
 
 }  // namespace
@@ -424,19 +464,28 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
         int cap = 64;
         while (cap < N && cap < st->wolff_ring) cap <<= 1;
         const int ncb = 64 * 2 * dim;  // candidate list: uint32 bond + two uint16 per entry
-        const int rep_lds = (int)(((N + 15) & ~15ll) + 8 * ncb + (2 * cap + 15) / 16 * 16);
+        const bool pack = kind == WK_ISING && N % 32 == 0;
+        const int rep_lds = (int)(wave_spin_bytes((uint32_t)N, pack) + 8 * ncb + (2 * cap + 15) / 16 * 16);
         const int wpb = std::max(1, std::min(4, 65536 / rep_lds));
         const dim3 wgrid((unsigned)((g.nrep + wpb - 1) / wpb)), wblock((unsigned)(64 * wpb));
         const size_t lds = (size_t)wpb * (size_t)rep_lds;
-#define MQ_WL(D, K)                                                                                                 \
-    hipLaunchKernelGGL((k_wolff_wave<D, K>), wgrid, wblock, lds, s, base, g.stride, L, g.nrep, nclusters, cl0,     \
+#define MQ_WL(D, K, P)                                                                                              \
+    hipLaunchKernelGGL((k_wolff_wave<D, K, P>), wgrid, wblock, lds, s, base, g.stride, L, g.nrep, nclusters, cl0,  \
                        g.rep0, st->p.q, st->d_wthr, st->d_queue, st->d_rep_site_off, st->d_csize, key, cap, rep_lds)
         if (dim == 3) {
-            if (kind == WK_ISING) MQ_WL(3, WK_ISING);
-            else MQ_WL(3, WK_POTTS);
+            if (kind == WK_ISING) {
+                if (pack) MQ_WL(3, WK_ISING, true);
+                else MQ_WL(3, WK_ISING, false);
+            } else {
+                MQ_WL(3, WK_POTTS, false);
+            }
         } else {
-            if (kind == WK_ISING) MQ_WL(2, WK_ISING);
-            else MQ_WL(2, WK_POTTS);
+            if (kind == WK_ISING) {
+                if (pack) MQ_WL(2, WK_ISING, true);
+                else MQ_WL(2, WK_ISING, false);
+            } else {
+                MQ_WL(2, WK_POTTS, false);
+            }
         }
 #undef MQ_WL
         return hipGetLastError();
