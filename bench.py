@@ -34,6 +34,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per headline launch (3D L=256, 32 replicas) by launch shape (block, XCD remap), from
+# rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (profiles/k_sweep_ising3_row16_3d_L256_R32.md)
+HEADLINE_TRAFFIC = {(64, 0): 872.6e6, (64, 1): 807.6e6, (128, 0): 839.1e6, (128, 1): 807.6e6,
+                    (256, 0): 822.3e6, (256, 1): 807.5e6}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -44,6 +49,8 @@ def parse():
     ap.add_argument("--L", type=int, default=256)
     ap.add_argument("--quick", action="store_true", help="headline only (no configuration legs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="headline with the default launch shape (no autotuner)")
+    ap.add_argument("--shape", default="", help="headline launch shape BLOCK,XCD_REMAP instead of the autotuner's")
     ap.add_argument("--legs", default="", help="comma-separated subset of the extra legs to run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM traffic)")
@@ -278,9 +285,23 @@ def main():
     st = gpu.State(ctx, "ising", [lat] * R, J=-1.0)
     st.set_beta(beta)
     st.init(seed=2)
+    tuning = None
+    if a.shape:
+        blk, xcd = (int(x) for x in a.shape.split(","))
+        tuning = {"kernels": 0, "block": blk, "xcd_remap": xcd, "lds_block": 256}
+        st.set_tuning(**tuning)
+    elif not a.no_tune:
+        # launch shape from the project's autotuner (its timing sweeps use counters far from
+        # the run's, so they only add warm-up to this chain); cached per box in a scratch file
+        from marqov_amd import autotune
+        shape, _ = autotune.tune(ctx, st, cache_path=os.path.join("/tmp", f"marqov_amd_tuning_{os.getuid()}.json"),
+                                 force=True, sweeps=6)
+        tuning = shape
     st.metropolis(a.warmup, seed=2, sweep0=0)
     ms, wall = timed(ctx, lambda: st.metropolis(a.steps, seed=2, sweep0=a.warmup))
     value = a.steps * R * N / (ms * 1e6)
+    tn = st.get_tuning()
+    shape_key = (tn["block"], tn["xcd_remap"])
     launch_ms = ms / (2 * a.steps)
     bytes_per_launch = 3.0 * R * (N // 2)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
@@ -306,13 +327,14 @@ def main():
         "dtype": "int8",
         "data": "synthetic (Philox-seeded random start, seed 2)",
         "config": {"workload": f"3D Ising L={L} periodic J=-1 beta={beta} int8 checkerboard Metropolis, "
-                               f"{R} replicas per launch", "replicas": R, "L": L, "sites": N},
+                               f"{R} replicas per launch", "replicas": R, "L": L, "sites": N,
+                   "launch_shape": tuning if tuning is not None else "default (block 256, XCD remap)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     # HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE of this configuration,
-                     # scaled by a 1 GiB copy (factors 2.0 / 1.0): 539.1 MB read + 268.4 MB written
-                     "traffic": 807.5e6 if (L, R) == (256, 32) else None,
-                     "traffic_source": "profiles/k_sweep_ising3_row16_3d_L256_R32.md",
+                     # HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE of this configuration
+                     # and launch shape, scaled by a 1 GiB copy (factors 2.0 / 1.0)
+                     "traffic": HEADLINE_TRAFFIC.get(shape_key) if (L, R) == (256, 32) else None,
+                     "traffic_source": "profiles/k_sweep_ising3_row16_3d_L256_R32.md (by launch shape)",
                      "kernel": "k_sweep_ising3_row16", "launch_ms": launch_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "observables": {"e_mean": float(obs[:, 0].mean()), "m_abs_mean": float(obs[:, 1].mean())},
