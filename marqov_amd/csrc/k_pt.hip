@@ -6,10 +6,11 @@
 // double range clamped); accept iff u < min(1, ratio).  The reference swaps the
 // state spaces of the two simulations; here the two replicas exchange their
 // temperatures (acceptance tables, beta) instead, which moves 2 table rows
-// rather than two lattices and leaves the Markov chain unchanged.  Pairs are
-// neighbouring temperatures (k, k+1) with k = parity mod 2 inside each group of
-// ntemp replicas (deterministic even/odd plan instead of the reference's random
-// pair plan, createPTplan marqovscheduler.h:202-211).  For models whose
+// rather than two lattices and leaves the Markov chain unchanged.  mq_pt_swap
+// offers neighbouring temperatures (k, k+1) with k = parity mod 2 inside each
+// group of ntemp replicas; mq_pt_swap_pairs offers the disjoint slot pairs of one
+// step of a caller's plan, such as one drawn by the reference's createPTplan
+// (marqovscheduler.h:202-211).  For models whose
 // Hamiltonian depends on beta only through on-site terms scaled by 1/beta
 // (Phi4, Phi4.h:110-124), beta_x E_x(y) = beta_x J I(y) + O(y), so only the
 // interaction sum I enters endiff.
@@ -92,10 +93,7 @@ __global__ void k_pt_swap(int ngroups, int ntemp, int parity, uint32_t step, con
                           uint32_t* __restrict__ thr, int tbl, uint32_t* __restrict__ wthr, float* __restrict__ betaf,
                           double* __restrict__ betad, int32_t* __restrict__ acc)
 {
-    if (dyn) {
-        step += dyn[1];
-        if (parity < 0) parity = (int)(step & 1u);
-    }
+    if (dyn) step += dyn[1];
     if (parity < 0) parity = (int)(step & 1u);
     const int npairs = (ntemp - parity) / 2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -21,6 +21,7 @@
 #pragma once
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <fstream>
@@ -169,8 +170,18 @@ inline Csr read_csv_graph(const std::string& path, int idxoffs = 0)
         int k = 0;
         std::vector<int32_t> v;
         while (std::getline(ss, w, '\t')) {
-            if (k++ < ncoords) continue;
-            v.push_back((int32_t)(std::stol(w) - idxoffs));
+            if (k++ < ncoords || w.empty()) continue;
+            std::size_t used = 0;
+            long long x = 0;
+            try {
+                x = std::stoll(w, &used);
+            } catch (const std::exception&) {
+                used = 0;
+            }
+            if (used != w.size() || x - idxoffs < INT32_MIN || x - idxoffs > INT32_MAX)
+                throw std::runtime_error("graph file " + path + ": bad neighbour index '" + w + "' on vertex " +
+                                         std::to_string(nb.size()));
+            v.push_back((int32_t)(x - idxoffs));
         }
         nb.push_back(std::move(v));
     }

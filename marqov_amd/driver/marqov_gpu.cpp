@@ -374,6 +374,11 @@ int main(int argc, char** argv)
         const int warm = reg.get<int>(f, "MC", "warmupsteps");
         const int meas = reg.get<int>(f, "MC", "measuresteps");
         const std::string outdir = o.out.empty() ? reg.get<std::string>(f, "IO", "outdir") : o.out;
+        if (dim < 1) throw RegistryError("dim must be >= 1 in " + f);
+        for (int r : nrep)
+            if (r < 1) throw RegistryError("rep must be >= 1 in " + f);
+        if (nmetro < 0 || warm < 0 || meas < 0 || amp < 0.0)
+            throw RegistryError("nmetro, warmupsteps, measuresteps and nclusteramp must be >= 0 in " + f);
         const auto psets = parameter_sets(s, reg);
 
         // group parameter sets by everything but beta (one mq_state each)
@@ -421,7 +426,6 @@ int main(int argc, char** argv)
                 pl.outpath = outdir + "/graph/";
             }
             // simulation ids in the reference's order: parameter sets, then replicas
-            std::map<int, Sim> sims;
             std::vector<int> gof(psets.size());
             for (std::size_t g = 0; g < by_group.size(); ++g)
                 for (int k : by_group[g]) gof[k] = (int)g;
@@ -477,8 +481,9 @@ int main(int argc, char** argv)
                     graphs.emplace_back(new GraphLattice(ctx, g.row_ptr, g.col_idx));
                 }
             }
-            FILE* sum = std::fopen((pl.outpath + "summary.txt").c_str(), "w");
-            if (!sum) throw std::runtime_error("cannot write " + pl.outpath + "summary.txt");
+            std::unique_ptr<FILE, int (*)(FILE*)> sumf(std::fopen((pl.outpath + "summary.txt").c_str(), "w"), &std::fclose);
+            if (!sumf) throw std::runtime_error("cannot write " + pl.outpath + "summary.txt");
+            FILE* sum = sumf.get();
             std::fprintf(sum, "# name beta J lambda mass repid <E> err(E) <|m|> <m^2> <m^4> <clustersize> block\n");
             for (std::size_t g = 0; g < pl.groups.size(); ++g) {
                 // RGG: replicas of one graph must be adjacent, so order by repid, then beta
@@ -601,7 +606,6 @@ int main(int argc, char** argv)
                                  sc / n, block);
                 }
             }
-            std::fclose(sum);
         }
         return 0;
     } catch (const std::exception& e) {
