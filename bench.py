@@ -149,13 +149,18 @@ def leg_c1(gpu, ctx):
     return res
 
 
+C3_REPLICAS = int(os.environ.get("MARQOV_BENCH_C3_REPLICAS", "256"))
+
+
 def leg_c3(gpu, ctx):
-    """3D Heisenberg L=128 beta=0.693, FP32 float4 spins, small-angle proposal."""
+    """3D Heisenberg L=128 beta=0.693, FP32 float4 spins, small-angle proposal; C3_REPLICAS
+    replicas (8 GiB of spins at 256: far beyond the 256 MB last-level cache, so the sweep
+    rate is an HBM rate, and one workgroup per replica grows 256 clusters at a time) vs 1."""
     L, beta = 128, 0.693
     lat = gpu.RegularLattice(ctx, 3, L)
     N = L ** 3
     res = {}
-    for name, nrep, nstep in (("batched", 8, 20), ("plain", 1, 20)):
+    for name, nrep, nstep in (("batched", C3_REPLICAS, 20), ("plain", 1, 20)):
         st = gpu.State(ctx, "heisenberg", [lat] * nrep, J=-1.0, proposal="small", delta=0.6, embedding="cartesian")
         st.set_beta(beta)
         st.init(seed=3)

@@ -365,6 +365,10 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         const int v = std::atoi(wr);
         if (v >= 64 && v <= 2048 && (v & (v - 1)) == 0) st->wolff_ring = v;
     }
+    if (const char* wv = std::getenv("MARQOV_AMD_WOLFF_VSET")) {
+        const int v = std::atoi(wv);
+        if (v >= 0 && v <= MQ_VSET / 2) st->wolff_vset = v;
+    }
     if (const char* wb = std::getenv("MARQOV_AMD_WOLFF_BLOCK")) {
         const int v = std::atoi(wb);
         if (v == 256 || v == 512 || v == 1024) st->wolff_block = v;

@@ -495,14 +495,19 @@ __device__ __forceinline__ float4 creflect(float4 v, int axis, const float3& rd)
 }
 
 // Frontier ring in LDS as in the regular-lattice kernel: entry pos written in
-// the level that starts at `head` sits in ring slot pos mod CWCAP when
-// pos - head < CWCAP, else in the global queue.
+// the level that starts at `head` sits in ring slot pos mod CAP when
+// pos - head < CAP, else in the global queue (CAP = CWCAP for int8 models,
+// CWCAP/2 for O(3)/Phi4).
 constexpr int CWCAP = 4096;
 
 // Wolff on a CSR graph, one workgroup per replica, level-synchronous BFS.
-// int8 models claim sites with one atomic on the spin word (no bitmap);
-// O(3)/Phi4 keep the visited bitmap and carry the flipped spin's projection
-// with each frontier entry.  Bond randoms are keyed by the canonical edge id.
+// int8 models claim sites with one atomic on the spin word; O(3)/Phi4 keep
+// the visited set in LDS while the cluster cannot outgrow vlim sites (then in
+// the global bitmap) and carry the flipped spin's projection with each
+// frontier entry.  A claimed site's row start and degree are loaded together
+// with the claim and kept with its ring entry, so a level waits on two
+// dependent global round trips (neighbour index, claim) instead of three.
+// Bond randoms are keyed by the canonical edge id.
 template <int KIND>
 __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int64_t rs, int64_t ps, int64_t N,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
@@ -511,12 +516,17 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                                                    const uint32_t* __restrict__ wthr, const float* __restrict__ betaf,
                                                    float J, int32_t* __restrict__ queue, uint32_t* __restrict__ visit,
                                                    const int64_t* __restrict__ rep_site_off, int64_t* __restrict__ csize,
-                                                   Key2 key)
+                                                   int maxdeg, int vlim, Key2 key)
 {
     constexpr bool VEC = (KIND == CK_CART || KIND == CK_RDIR);
+    constexpr int CAP = VEC ? CWCAP / 2 : CWCAP;  // ring entries (40 / 32 KB of LDS with the set)
     __shared__ int s_tail;
-    __shared__ int32_t s_ring[CWCAP];
-    __shared__ float s_proj[VEC ? CWCAP : 1];
+    // a ring entry is the site's row (start, degree) and, for O(3)/Phi4, its
+    // flipped projection; the site index itself is needed only when spilled
+    __shared__ int32_t s_kb[CAP];
+    __shared__ int32_t s_deg[CAP];
+    __shared__ float s_proj[VEC ? CAP : 1];
+    __shared__ uint32_t s_vset[VEC ? MQ_VSET : 1];
     const int r = blockIdx.x;
     const int grep = rep0 + r;
     // element (position p) of this replica: sp8[p * ps] / sp4[p * ps]
@@ -526,10 +536,13 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
     uint32_t* vis = visit + (rep_site_off[grep] >> 5);
     const uint32_t tw = VEC ? 0u : wthr[grep];
     const float beta = VEC ? betaf[grep] : 0.f;
+    if (VEC)
+        for (int i = threadIdx.x; i < MQ_VSET; i += blockDim.x) s_vset[i] = 0xFFFFFFFFu;
     for (int c = 0; c < nclusters; ++c) {
         const uint32_t ncl = cl0 + (uint32_t)c;
         const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
         const int64_t seed = pos_of[(int64_t)(((uint64_t)sw.x * (uint64_t)N) >> 32)];
+        bool glob = vlim == 0;  // O(3)/Phi4 membership in the global bitmap (block-uniform)
         int val = 0, newq = 0, axis = 0;
         float3 rd = make_float3(0.f, 0.f, 0.f);
         if (!VEC) {
@@ -552,10 +565,12 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                 const float4 v = creflect<KIND>(sp4[seed * ps], axis, rd);
                 sp4[seed * ps] = v;
                 s_proj[0] = cproj<KIND>(v, axis, rd);
-                atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+                if (glob) atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+                else vset_insert(s_vset, (uint32_t)seed);
                 q[0] = (int32_t)seed;
             }
-            s_ring[0] = (int32_t)seed;
+            s_kb[0] = rp[seed];
+            s_deg[0] = rp[seed + 1] - rp[seed];
             s_tail = 1;
         }
         __syncthreads();
@@ -564,6 +579,16 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
             const int tail = s_tail;
             __syncthreads();
             if (head >= tail) break;
+            if (VEC && !glob && (int64_t)tail + (int64_t)(tail - head) * maxdeg > (int64_t)vlim) {
+                // this level could take the cluster past vlim sites: move the members
+                // (all listed in q) to the global bitmap and claim there from now on
+                for (int idx = threadIdx.x; idx < tail; idx += blockDim.x) {
+                    const uint32_t m = (uint32_t)q[idx];
+                    atomicOr(&vis[m >> 5], 1u << (m & 31));
+                }
+                glob = true;
+                __syncthreads();
+            }
             // Each wavefront takes 64 frontier entries, scans their degrees, and then
             // runs one lane per edge: a lane finds its entry by a binary search over
             // the scan, loads the far end, draws the edge's Philox uniform and
@@ -574,15 +599,20 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
             const int nwv = (int)(blockDim.x >> 6);
             for (int wb = head + (int)(threadIdx.x >> 6) * 64; wb < tail; wb += nwv * 64) {
                 const int idx = wb + lane;
-                int32_t cur = 0;
                 int kb = 0, deg = 0;
                 float pc = 0.f;
                 if (idx < tail) {
-                    const bool in_ring = idx - hprev < CWCAP;
-                    cur = in_ring ? s_ring[idx & (CWCAP - 1)] : q[idx];
-                    kb = rp[cur];
-                    deg = rp[cur + 1] - kb;
-                    if (VEC) pc = in_ring ? s_proj[idx & (CWCAP - 1)] : cproj<KIND>(sp4[(int64_t)cur * ps], axis, rd);
+                    if (idx - hprev < CAP) {
+                        const int sl = idx & (CAP - 1);
+                        kb = s_kb[sl];
+                        deg = s_deg[sl];
+                        if (VEC) pc = s_proj[sl];
+                    } else {
+                        const int32_t cur = q[idx];
+                        kb = rp[cur];
+                        deg = rp[cur + 1] - kb;
+                        if (VEC) pc = cproj<KIND>(sp4[(int64_t)cur * ps], axis, rd);
+                    }
                 }
                 int inc = deg;
 #pragma unroll
@@ -604,11 +634,15 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                     const int k = __shfl(kb, e) + (t - __shfl(pre, e));
                     const float epc = VEC ? __shfl(pc, e) : 0.f;
                     bool ok = false;
-                    int32_t j = 0;
+                    int32_t j = 0, jk0 = 0, jk1 = 0;
                     float pnew = 0.f;
                     if (t < total) {
                         j = ci[k];
                         const uint32_t eid = (uint32_t)edge[k];
+                        // j's row, kept with its ring entry if j joins: in flight with the
+                        // spin load and the claim
+                        jk0 = rp[j];
+                        jk1 = rp[j + 1];
                         if (KIND == CK_ISING) {
                             // the claim itself tests alignment
                             const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
@@ -628,8 +662,12 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                             const uint4 bw = philox4x32_10(eid >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
                             const float cpl = J * epc * cproj<KIND>(vj, axis, rd);
                             if (cpl > 0.f && u01f(u4_get(bw, (int)(eid & 3u))) < -expm1f(-2.0f * beta * cpl)) {
-                                const uint32_t ob = atomicOr(&vis[j >> 5], 1u << (j & 31));
-                                ok = !(ob & (1u << (j & 31)));
+                                if (glob) {
+                                    const uint32_t ob = atomicOr(&vis[j >> 5], 1u << (j & 31));
+                                    ok = !(ob & (1u << (j & 31)));
+                                } else {
+                                    ok = vset_insert(s_vset, (uint32_t)j);
+                                }
                                 if (ok) {
                                     // the claim succeeded, so nobody changed vj since it was read
                                     const float4 v = creflect<KIND>(vj, axis, rd);
@@ -641,11 +679,13 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                     }
                     const int pos = wave_reserve(&s_tail, ok ? 1u : 0u);
                     if (ok) {
-                        if (pos - head < CWCAP) {
-                            s_ring[pos & (CWCAP - 1)] = j;
-                            if (VEC) s_proj[pos & (CWCAP - 1)] = pnew;
+                        if (pos - head < CAP) {
+                            const int sl = pos & (CAP - 1);
+                            s_kb[sl] = jk0;
+                            s_deg[sl] = jk1 - jk0;
+                            if (VEC) s_proj[sl] = pnew;
                         }
-                        if (VEC || pos - head >= CWCAP) q[pos] = j;  // VEC: kept for the bitmap reset
+                        if (VEC || pos - head >= CAP) q[pos] = j;  // VEC: every member, for the bitmap
                     }
                 }
             }
@@ -655,7 +695,10 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
         }
         const int size = s_tail;
         if (VEC) {
-            for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+            if (glob)
+                for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+            if (vlim > 0)
+                for (int i = threadIdx.x; i < MQ_VSET; i += blockDim.x) s_vset[i] = 0xFFFFFFFFu;
         }
         __syncthreads();
         if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
@@ -915,7 +958,8 @@ hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_
     hipLaunchKernelGGL((k_csr_wolff<K>), dim3(g.nrep), dim3(wolff_block(st, g.nrep)), 0, s, st->d_spins + g.off, (int64_t)1,          \
                        (int64_t)g.nrep, l->N,                                                                    \
                        l->d_rp, l->d_ci, l->d_edge, l->d_pos_of, nclusters, cl0, g.rep0, st->p.q, st->d_wthr,    \
-                       st->d_betaf, J, st->d_queue, st->d_visit, st->d_rep_site_off, st->d_csize, key)
+                       st->d_betaf, J, st->d_queue, st->d_visit, st->d_rep_site_off, st->d_csize, l->maxdeg,   \
+                       st->wolff_vset, key)
     if (model == MQ_ISING) MQ_W(CK_ISING);
     else if (model == MQ_POTTS) MQ_W(CK_POTTS);
     else if (model == MQ_HEISENBERG && st->p.embedding == MQ_EMBED_CARTESIAN) MQ_W(CK_CART);

@@ -6,8 +6,11 @@
 // the bond to an in-cluster neighbour is active; activation uses one Philox
 // uniform per bond keyed by (bond = owner*dim + d, cluster number, replica),
 // so the cluster is the connected component of active bonds and does not
-// depend on visit order.  Membership is an atomicOr on a per-replica bitmap,
-// cleared again from the cluster's own queue (O(cluster), not O(N)).
+// depend on visit order.  Ising and Potts sites join with one atomic on their
+// spin byte.  O(3)/Phi4 membership is an LDS hash set (an LDS compare-and-swap
+// per claim, no global round trip) while the cluster cannot outgrow vlim
+// sites, then an atomicOr on a per-replica bitmap, cleared again from the
+// cluster's own queue (O(cluster), not O(N)).
 //
 // Reference semantics: Ising.h:122-169 (aligned neighbours, p = -expm1(2 beta J)),
 // Potts.h:189-322 (neighbours equal to the old colour, relabel to a different
@@ -98,12 +101,13 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                                                        const float* __restrict__ betaf, float J,
                                                        int32_t* __restrict__ queue, uint32_t* __restrict__ visit,
                                                        const int64_t* __restrict__ rep_site_off,
-                                                       int64_t* __restrict__ csize, Key2 key)
+                                                       int64_t* __restrict__ csize, int vlim, Key2 key)
 {
     constexpr bool VEC = (KIND == WK_CART || KIND == WK_RDIR);
     __shared__ int s_tail;
     __shared__ int32_t s_ring[WCAP];
     __shared__ float s_proj[VEC ? WCAP : 1];
+    __shared__ uint32_t s_vset[VEC ? MQ_VSET : 1];
     const int r = blockIdx.x;
     const int grep = rep0 + r;
     const uint32_t uL = (uint32_t)L;
@@ -116,12 +120,17 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
     uint32_t* vis = visit + (rep_site_off[grep] >> 5);
     const uint32_t tw = VEC ? 0u : wthr[grep];
     const float beta = VEC ? betaf[grep] : 0.f;
+    if (VEC)
+        for (int i = threadIdx.x; i < MQ_VSET; i += blockDim.x) s_vset[i] = 0xFFFFFFFFu;
 
     for (int c = 0; c < nclusters; ++c) {
         const uint32_t ncl = cl0 + (uint32_t)c;
         const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
         const uint32_t seed = (uint32_t)(((uint64_t)sw.x * (uint64_t)N) >> 32);
         const uint32_t seed_a = (uint32_t)split_addr<DIM>(L, half, seed);
+        // O(3)/Phi4 membership: the LDS set while the cluster can stay within vlim
+        // sites, the global bitmap after that (block-uniform)
+        bool glob = vlim == 0;
         int val = 0, newq = 0, axis = 0;
         float3 rd = make_float3(0.f, 0.f, 0.f);
         if (KIND == WK_ISING || KIND == WK_POTTS) {
@@ -145,7 +154,8 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                 const float4 v = reflect<KIND>(sp4[seed_a], axis, rd);
                 sp4[seed_a] = v;
                 s_proj[0] = proj<KIND>(v, axis, rd);
-                atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+                if (glob) atomicOr(&vis[seed >> 5], 1u << (seed & 31));
+                else vset_insert(s_vset, seed);
                 q[0] = (int32_t)seed;
             }
             s_ring[0] = (int32_t)seed;
@@ -160,9 +170,21 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
             // One lane per (frontier site, bond slot): a narrow level then costs one
             // Philox evaluation and one claim round trip per lane instead of 2*DIM
             // Philox evaluations in a row; new entries are placed with one LDS
-            // atomic per wavefront.
+            // atomic per wavefront.  (Two or four tasks per lane with their claims
+            // issued together measured 13% / 24% slower on 3D Ising L=256: the
+            // workgroup's wavefronts already overlap the round trips.)
             constexpr int NS = 2 * DIM;
             const int ntask = (tail - head) * NS;
+            if (VEC && !glob && tail + ntask > vlim) {
+                // this level could take the cluster past vlim sites: move the members
+                // (all listed in q) to the global bitmap and claim there from now on
+                for (int idx = threadIdx.x; idx < tail; idx += blockDim.x) {
+                    const uint32_t m = (uint32_t)q[idx];
+                    atomicOr(&vis[m >> 5], 1u << (m & 31));
+                }
+                glob = true;
+                __syncthreads();
+            }
             for (int t = (int)threadIdx.x; t < ntask; t += (int)blockDim.x) {
                 const int e = t / NS;
                 const int sl = t - e * NS;
@@ -212,8 +234,12 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                     const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
                     const float cpl = J * pc * proj<KIND>(vj, axis, rd);
                     if (cpl > 0.f && u01f(u4_get(bw, (int)(b & 3))) < -expm1f(-2.0f * beta * cpl)) {
-                        const uint32_t ob = atomicOr(&vis[j >> 5], 1u << (j & 31));
-                        ok = !(ob & (1u << (j & 31)));
+                        if (glob) {
+                            const uint32_t ob = atomicOr(&vis[j >> 5], 1u << (j & 31));
+                            ok = !(ob & (1u << (j & 31)));
+                        } else {
+                            ok = vset_insert(s_vset, j);
+                        }
                         if (ok) {
                             // the claim succeeded, so nobody changed vj since it was read
                             const float4 v = reflect<KIND>(vj, axis, rd);
@@ -228,7 +254,7 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                         s_ring[pos & (WCAP - 1)] = (int32_t)j;
                         if (VEC) s_proj[pos & (WCAP - 1)] = pnew;
                     }
-                    if (VEC || pos - head >= WCAP) q[pos] = (int32_t)j;  // VEC: kept for the bitmap reset
+                    if (VEC || pos - head >= WCAP) q[pos] = (int32_t)j;  // VEC: every member, for the bitmap
                 }
             }
             __syncthreads();
@@ -237,7 +263,10 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
         }
         const int size = s_tail;
         if (VEC) {
-            for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+            if (glob)
+                for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
+            if (vlim > 0)
+                for (int i = threadIdx.x; i < MQ_VSET; i += blockDim.x) s_vset[i] = 0xFFFFFFFFu;
         }
         __syncthreads();
         if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
@@ -493,7 +522,7 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
 #define MQ_W(D, K)                                                                                               \
     hipLaunchKernelGGL((k_wolff_regular<D, K>), grid, block, 0, s, base, g.stride, L, nclusters, cl0, g.rep0,  \
                        st->p.q, st->d_wthr, st->d_betaf, J, st->d_queue, st->d_visit, st->d_rep_site_off,       \
-                       st->d_csize, key)
+                       st->d_csize, st->wolff_vset, key)
     if (dim == 3) {
         if (kind == WK_ISING) MQ_W(3, WK_ISING);
         else if (kind == WK_POTTS) MQ_W(3, WK_POTTS);

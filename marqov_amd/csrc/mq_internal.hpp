@@ -95,18 +95,23 @@ __device__ __forceinline__ float u01f(uint32_t w) { return (float)(w >> 8) * 0x1
 // needed.
 __device__ __forceinline__ uint32_t* spin_word(int8_t* p, uint32_t& sh)
 {
-    sh = ((uint32_t)(uintptr_t)p & 3u) * 8u;
-    return reinterpret_cast<uint32_t*>((uintptr_t)p & ~(uintptr_t)3);
+    // offset arithmetic on p itself (not an integer round trip), so the compiler
+    // keeps p's global address space and emits global rather than flat accesses
+    const uint32_t o = (uint32_t)(uintptr_t)p & 3u;
+    sh = o * 8u;
+    return reinterpret_cast<uint32_t*>(p - o);
 }
 
 // Ising bytes are +1 = 0x01 and -1 = 0xFF: OR 0xFE takes +1 to -1 and AND 0x01
-// takes -1 to +1, each leaving the other value unchanged.
+// takes -1 to +1, each leaving the other value unchanged.  val (the cluster's
+// old value) must be the same in every lane of the wavefront: the direction is
+// then a scalar branch rather than two masked atomics in a row.
 __device__ __forceinline__ bool claim_ising(int8_t* p, int val)
 {
     uint32_t sh;
     uint32_t* w = spin_word(p, sh);
     const uint32_t m = 0xFEu << sh;
-    const uint32_t old = val > 0 ? atomicOr(w, m) : atomicAnd(w, ~m);
+    const uint32_t old = __builtin_amdgcn_readfirstlane(val) > 0 ? atomicOr(w, m) : atomicAnd(w, ~m);
     return (int8_t)(old >> sh) == (int8_t)val;
 }
 
@@ -115,21 +120,6 @@ __device__ __forceinline__ uint32_t peek_word(int8_t* p, uint32_t& sh)
 {
     uint32_t* w = spin_word(p, sh);
     return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Issue-only forms: the caller collects the old words of several claims before
-// testing any of them, so their round trips overlap.
-__device__ __forceinline__ uint32_t claim_ising_issue(int8_t* p, int val)
-{
-    uint32_t sh;
-    uint32_t* w = spin_word(p, sh);
-    const uint32_t m = 0xFEu << sh;
-    return val > 0 ? atomicOr(w, m) : atomicAnd(w, ~m);
-}
-
-__device__ __forceinline__ int8_t byte_at(uint32_t w, const int8_t* p)
-{
-    return (int8_t)(w >> (((uint32_t)(uintptr_t)p & 3u) * 8u));
 }
 
 // Reserves n (< 16) consecutive entries for each active lane of the wavefront
@@ -150,6 +140,23 @@ __device__ __forceinline__ int wave_reserve(int* tail, uint32_t n)
     if ((int)__lane_id() == leader && tot > 0) base = atomicAdd(tail, tot);
     base = __shfl(base, leader);
     return base + pre;
+}
+
+// Visited set of a growing O(3)/Phi4 cluster in LDS: open addressing with
+// linear probing over MQ_VSET site indices (empty = 0xFFFFFFFF), Fibonacci
+// hashing of the index.  Callers keep at most MQ_VSET/2 members, so a free slot
+// always exists and probe runs stay short.  Returns true when j was not a
+// member and is one now; concurrent inserts of one j let exactly one lane win.
+constexpr int MQ_VSET = 4096;
+__device__ __forceinline__ bool vset_insert(uint32_t* set, uint32_t j)
+{
+    uint32_t h = (j * 2654435769u) >> 20;
+    while (true) {
+        const uint32_t old = atomicCAS(&set[h], 0xFFFFFFFFu, j);
+        if (old == 0xFFFFFFFFu) return true;
+        if (old == j) return false;
+        h = (h + 1u) & (uint32_t)(MQ_VSET - 1);
+    }
 }
 
 // Potts relabel val -> newq by compare-and-swap, starting from a word seen
@@ -277,6 +284,7 @@ struct mq_state {
     int kernel_choice = 0;              // 0 auto, 1 streaming kernels only, 2 generic streaming kernel only  // force the generic kernels (parity of the specialised ones)
     int wolff_ring = 512;               // LDS frontier ring entries of the small-lattice Wolff kernel (power of two, 64..2048)
     int wolff_block = 0;                // threads per replica of the global-memory Wolff kernels (0 = automatic)
+    int wolff_vset = 2048;              // O(3)/Phi4 Wolff: clusters up to this size keep their visited set in LDS (0..MQ_VSET/2)
 };
 
 // --------------------------------------------------------------------------
@@ -317,13 +325,15 @@ hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_
 
 // Threads of the workgroup that grows one replica's clusters in global memory.
 // A wide frontier level is worked through blockDim entries at a time, each
-// waiting on its claims' round trips, so with no more replicas than CUs (one
-// workgroup per CU either way) the widest block keeps the most claims in
-// flight; with more replicas, 256-thread blocks let several share a CU.
+// waiting on its claims' round trips, so with few replicas the widest block
+// keeps the most claims in flight.  From half as many replicas as CUs on,
+// 256-thread blocks are faster (measured on 3D Heisenberg L=128: 1024 threads
+// lead at 64 and 128 replicas, 0.16 / 0.27 sites/ns against 0.13 / 0.24, and
+// fall to 0.20 against 0.43 at 256 replicas on 256 CUs).
 inline int wolff_block(const mq_state* st, int nwg)
 {
     if (st->wolff_block > 0) return st->wolff_block;
-    return nwg <= st->ctx->num_cus ? 1024 : 256;
+    return 2 * nwg <= st->ctx->num_cus ? 1024 : 256;
 }
 
 hipError_t launch_energy_all(mq_state* st);

@@ -201,3 +201,42 @@ def test_vec_wolff_block_invariant(ctx, model, embedding, lattice, monkeypatch):
     assert np.array_equal(out["256"][0], out["1024"][0])
     for r in range(3):
         assert np.array_equal(out["256"][1][r], out["1024"][1][r]), r
+
+
+@pytest.mark.parametrize("model,embedding,lattice", [("heisenberg", "cartesian", "regular"),
+                                                     ("phi4", "random", "regular"),
+                                                     ("heisenberg", "random", "graph"),
+                                                     ("phi4", "cartesian", "graph")])
+def test_vec_wolff_visited_set_invariant(ctx, model, embedding, lattice, monkeypatch):
+    """O(3)/Phi4 clusters keep their visited set in LDS while they cannot outgrow
+    MARQOV_AMD_WOLFF_VSET sites and move it to the global bitmap when a level could
+    pass that limit.  Membership does not depend on where it is recorded, so the spins
+    and cluster sizes are bit-identical with the set off (0), with a tiny limit (16: nearly
+    every cluster moves mid-growth) and with the default (2048)."""
+    from marqov_amd import gpu
+    if lattice == "regular":
+        lat = gpu.RegularLattice(ctx, 3, 16)
+    else:
+        from marqov_amd import graphs
+        n = 4096
+        rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n, 8.0), 11)
+        lat = gpu.CsrLattice(ctx, rp, ci)
+    out = {}
+    for vset in ("0", "16", "2048"):
+        monkeypatch.setenv("MARQOV_AMD_WOLFF_VSET", vset)
+        st = gpu.State(ctx, model, [lat] * 3, J=-1.0, proposal="small", delta=0.6, embedding=embedding)
+        st.set_beta(0.9 if lattice == "regular" else 1.2)
+        st.init(seed=31)
+        sizes = []
+        for t in range(4):
+            st.metropolis(3, seed=32, sweep0=3 * t)
+            sizes.append(st.wolff(6, seed=33, cluster0=6 * t))
+        out[vset] = (np.concatenate(sizes, axis=1), [st.download(r) for r in range(3)])
+    s0 = out["0"][0]
+    print(f"[vset] {model}/{embedding}/{lattice}: clusters > 16: {(s0 > 16).sum()} of {s0.size}, "
+          f"largest {s0.max()}", flush=True)
+    assert s0.max() > 20, s0  # the limit-16 run moved at least one cluster mid-growth
+    for vset in ("16", "2048"):
+        assert np.array_equal(out[vset][0], s0), vset
+        for r in range(3):
+            assert np.array_equal(out[vset][1][r], out["0"][1][r]), (vset, r)
