@@ -177,9 +177,10 @@ def leg_c3(gpu, ctx):
         del st
     res["batched_over_plain"] = round(res["batched"]["attempts_per_ns"] / res["plain"]["attempts_per_ns"], 2)
     res["wolff"] = "after 30 x (10 sweeps + 10 clusters); 4 timed calls of 10 clusters per replica"
-    # finite-size-scaling shape: as many FP32 replicas as fill ~256 MiB in one launch vs one per
-    # launch, L = 16, 32, 64 (a single L=128 replica already saturates HBM)
-    for Ls, nrep in ((16, 4096), (32, 512), (64, 64)):
+    # finite-size-scaling shape: as many FP32 replicas as fill 2 GiB (8x the 256 MB last-level
+    # cache) in one launch vs one per launch, L = 16, 32, 64 (a single L=128 replica already
+    # saturates HBM)
+    for Ls, nrep in ((16, 32768), (32, 4096), (64, 512)):
         latS = gpu.RegularLattice(ctx, 3, Ls)
         small = {}
         for name, nr, nstep in (("batched", nrep, 20), ("plain", 1, 20)):
