@@ -18,6 +18,13 @@
  *
  * usage: ref_metropolis ising|heis DIM L BETA J NSWEEPS SEED
  *   prints the final state, one site per line, in site-index order.
+ * usage: ref_metropolis phi4 DIM L BETA LAMBDA MASS NSWEEPS SEED
+ *   the same for the Phi4 model (J = -1, Phi4.h:93), declared as a user model:
+ *   the reference's Standard_Interaction and Onsite_Quadratic(mass/beta) terms,
+ *   a user on-site term lambda/beta (|phi|^2 - 1)^2 (the model Phi4.h:69-74 and
+ *   :110-124 define) and the radial proposal phi + 0.5 r as the Initializer
+ *   specialisation (Phi4.h:20-34, :169), so the reference's Metropolis move
+ *   assembles the on-site energy difference (metropolishelpers.h:651-683).
  * usage: ref_metropolis bench ising|heis DIM L BETA J NWARM NSWEEPS SEED NTHREADS
  *   times the reference's move as the CPU baseline: NTHREADS independent
  *   simulations, one per thread (the CXX11Scheduler default of one simulation per
@@ -26,6 +33,7 @@
  *   aggregate and per-thread attempts/ns.
  */
 #include <array>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -48,6 +56,59 @@ struct UserHam {
     explicit UserHam(double j) : J(j), interactions{{new Standard_Interaction<SV>(j)}} {}
     ~UserHam() { delete interactions[0]; }
 };
+
+typedef std::array<double, 3> Phi4SV;
+
+class UserFourthMinusOne : public OnSite<Phi4SV, double>
+{
+public:
+    explicit UserFourthMinusOne(double h) : OnSite<Phi4SV, double>(h) {}
+    double get(const Phi4SV& phi) override { return std::pow(dot(phi, phi) - 1.0, 2); }
+};
+
+struct UserPhi4 {
+    static constexpr int SymD = 3;
+    typedef Phi4SV StateVector;
+    std::array<Standard_Interaction<Phi4SV>*, 1> interactions;
+    Onsite_Quadratic<Phi4SV> quadratic;
+    UserFourthMinusOne fourth;
+    std::vector<OnSite<Phi4SV, double>*> onsite;
+    UserPhi4(double beta, double lambda, double mass)
+        : interactions{{new Standard_Interaction<Phi4SV>(-1.0)}}, quadratic(mass / beta), fourth(lambda / beta)
+    {
+        onsite.push_back(&quadratic);
+        onsite.push_back(&fourth);
+    }
+    ~UserPhi4() { delete interactions[0]; }
+};
+
+template <>
+class Initializer<UserPhi4>
+{
+public:
+    template <class RNG>
+    static Phi4SV newsv(const Phi4SV& osv, RNG& rng)
+    {
+        Phi4SV dir;
+        set_to_rnddir(rng, dir);
+        return osv + mult(0.5, dir);
+    }
+};
+
+static void run_phi4(int dim, int L, double beta, double lambda, double mass, int nsweeps, unsigned long long seed)
+{
+    RegularLattice lat(L, dim);
+    UserPhi4 ham(beta, lambda, mass);
+    RNGCache<std::mt19937_64> rng(seed);
+    std::vector<Phi4SV> s(lat.size());
+    for (std::size_t i = 0; i < lat.size(); ++i) set_to_rnddir(rng, s[i]);  // Core::init_hot, core.h:774-782
+    for (int t = 0; t < nsweeps; ++t)
+        for (std::size_t i = 0; i < lat.size(); ++i) {
+            const int rsite = rng.integer(lat.size());
+            MARQOV::Metropolis<UserPhi4, RegularLattice>::move(ham, lat, s, rng, beta, rsite);
+        }
+    for (std::size_t i = 0; i < lat.size(); ++i) std::printf("%.17g %.17g %.17g\n", s[i][0], s[i][1], s[i][2]);
+}
 
 template <class SV>
 static void run(int dim, int L, double beta, double J, int nsweeps, unsigned long long seed, bool ising)
@@ -121,6 +182,11 @@ int main(int argc, char** argv)
         const int nth = std::atoi(argv[10]) > 0 ? std::atoi(argv[10]) : 1;
         if (ising) bench<std::array<int, 1>>(dim, L, beta, J, nwarm, nsweeps, seed, nth, true);
         else bench<std::array<double, 3>>(dim, L, beta, J, nwarm, nsweeps, seed, nth, false);
+        return 0;
+    }
+    if (argc == 9 && std::strcmp(argv[1], "phi4") == 0) {
+        run_phi4(std::atoi(argv[2]), std::atoi(argv[3]), std::atof(argv[4]), std::atof(argv[5]), std::atof(argv[6]),
+                 std::atoi(argv[7]), std::strtoull(argv[8], nullptr, 10));
         return 0;
     }
     if (argc != 8) {
