@@ -18,6 +18,12 @@
  *
  * usage: ref_metropolis ising|heis DIM L BETA J NSWEEPS SEED
  *   prints the final state, one site per line, in site-index order.
+ * usage: ref_metropolis ea DIM L BETA J NSWEEPS SEED
+ *   the +-J Edwards-Anderson model: Ising spins on the reference's
+ *   RegularRandomBond<BimodalPDF> lattice (src/lattice/regular_random_bond.h,
+ *   bonds drawn and symmetrised there, from std::random_device), whose bnds the
+ *   Metropolis move weights the neighbour sum with (metropolishelpers.h:456-537);
+ *   prints the N x 2*DIM bonds in neighbour-slot order, then the final state.
  * usage: ref_metropolis phi4 DIM L BETA LAMBDA MASS NSWEEPS SEED
  *   the same for the Phi4 model (J = -1, Phi4.h:93), declared as a user model:
  *   the reference's Standard_Interaction and Onsite_Quadratic(mass/beta) terms,
@@ -42,10 +48,13 @@
 #include <thread>
 #include <vector>
 
+#include <algorithm>  // used, not included, by lattice/regular_random_bond.h
+
 #include "util/svmath.h"
 #include "metropolis.h"
 #include "hamiltonian/util/termcollection.h"
 #include "lattice/util/regular_lattice.h"
+#include "lattice/regular_random_bond.h"
 
 template <class SV>
 struct UserHam {
@@ -56,6 +65,27 @@ struct UserHam {
     explicit UserHam(double j) : J(j), interactions{{new Standard_Interaction<SV>(j)}} {}
     ~UserHam() { delete interactions[0]; }
 };
+
+static void run_ea(int dim, int L, double beta, double J, int nsweeps, unsigned long long seed)
+{
+    typedef std::array<int, 1> SV;
+    RegularRandomBond<BimodalPDF> lat(L, dim);
+    UserHam<SV> ham(J);
+    RNGCache<std::mt19937_64> rng(seed);
+    std::vector<SV> s(lat.size());
+    for (std::size_t i = 0; i < lat.size(); ++i) s[i][0] = (rng.real() > 0.5) ? 1 : -1;
+    for (int t = 0; t < nsweeps; ++t)
+        for (std::size_t i = 0; i < lat.size(); ++i) {
+            const int rsite = rng.integer(lat.size());
+            MARQOV::Metropolis<UserHam<SV>, RegularRandomBond<BimodalPDF>>::move(ham, lat, s, rng, beta, rsite);
+        }
+    for (std::size_t i = 0; i < lat.size(); ++i) {
+        const auto b = lat.bnds(1, (int)i);
+        for (std::size_t k = 0; k < b.size(); ++k) std::printf(k ? " %d" : "%d", b[k]);
+        std::printf("\n");
+    }
+    for (std::size_t i = 0; i < lat.size(); ++i) std::printf("%d\n", s[i][0]);
+}
 
 typedef std::array<double, 3> Phi4SV;
 
@@ -182,6 +212,11 @@ int main(int argc, char** argv)
         const int nth = std::atoi(argv[10]) > 0 ? std::atoi(argv[10]) : 1;
         if (ising) bench<std::array<int, 1>>(dim, L, beta, J, nwarm, nsweeps, seed, nth, true);
         else bench<std::array<double, 3>>(dim, L, beta, J, nwarm, nsweeps, seed, nth, false);
+        return 0;
+    }
+    if (argc == 8 && std::strcmp(argv[1], "ea") == 0) {
+        run_ea(std::atoi(argv[2]), std::atoi(argv[3]), std::atof(argv[4]), std::atof(argv[5]), std::atoi(argv[6]),
+               std::strtoull(argv[7], nullptr, 10));
         return 0;
     }
     if (argc == 9 && std::strcmp(argv[1], "phi4") == 0) {

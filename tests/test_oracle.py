@@ -76,6 +76,28 @@ def _ref_metropolis(model, dim, L, beta, J, nsweeps, seed):
 
 
 @pytest.mark.skipif(not os.path.exists(ob.REF_METRO), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("dim,L,beta,J,ns", [(3, 4, 0.8, -1.0, 20), (3, 6, 1.2, -1.0, 20), (2, 8, 0.9, 1.0, 20)])
+def test_ea_restatement_bit_exact_vs_reference_build(dim, L, beta, J, ns):
+    """+-J EA on the reference's own RegularRandomBond<BimodalPDF> lattice (bonds drawn and
+    symmetrised there): its bonds are symmetric in the (+d, -d) slot order the restatement
+    stores one bond per (site, +d) for, and the restatement run on those bonds equals the
+    reference's Metropolis move bit for bit."""
+    out = subprocess.run([ob.REF_METRO, "ea", str(dim), str(L), str(beta), str(J), str(ns), "12345"],
+                         capture_output=True, text=True, check=True).stdout
+    lines = out.strip().split("\n")
+    N = L ** dim
+    bonds = np.array([[int(x) for x in ln.split()] for ln in lines[:N]], np.int8)
+    spins = np.array([int(x) for x in lines[N:]])
+    nb = exact.neighbour_table(dim, L)
+    for d in range(dim):
+        assert np.array_equal(bonds[:, 2 * d + 1], bonds[nb[:, 2 * d + 1], 2 * d])
+    assert 0.3 < (bonds == 1).mean() < 0.7
+    mine, _, _ = ob.ref_run(ob.EA, dim, L, beta, J=J, nmetro=1, nwarm=ns, seed=12345,
+                            bonds=np.ascontiguousarray(bonds[:, 0::2]).reshape(-1))
+    assert np.array_equal(spins, mine)
+
+
+@pytest.mark.skipif(not os.path.exists(ob.REF_METRO), reason="oracle/_ref not built (reference tree absent)")
 @pytest.mark.parametrize("dim,L,beta,lam,mass,ns", [(3, 6, 0.69, 4.5, 1.0, 20), (2, 8, 1.0, 1.0, -0.5, 20),
                                                     (3, 4, 0.5, 0.3, 0.2, 30)])
 def test_phi4_restatement_bit_exact_vs_reference_build(dim, L, beta, lam, mass, ns):
