@@ -242,9 +242,12 @@ def leg_c5(gpu, ctx):
     """2D random geometric graph N=2^20, mean degree 6, Ising and Potts-3, 64 replicas vs 1."""
     from marqov_amd import graphs
     n = 1 << 20
-    rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n), 5)
+    t0 = time.perf_counter()
+    rp, ci = ctx.rgg(2, n, graphs.rgg_radius(n), 5)  # built on the device (mq_rgg_build)
+    t_build = time.perf_counter() - t0
     lat = gpu.CsrLattice(ctx, rp, ci)
-    res = {"N": n, "edges": int(rp[-1] // 2), "colours": int(len(lat.colors()))}
+    res = {"N": n, "edges": int(rp[-1] // 2), "colours": int(len(lat.colors())),
+           "graph_build_ms": round(1e3 * t_build, 1)}
     for model, beta in (("ising", 0.55), ("potts", 1.2)):
         for name, nrep, nstep in (("batched", 64, 10), ("plain", 1, 10)):
             st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, q=3)

@@ -93,6 +93,16 @@ MQ_API int mq_lattice_regular(mq_ctx* ctx, int dim, int L, mq_lattice** out);
  * random_geometric_graph.h:258-290).  The library colours it greedily. */
 MQ_API int mq_lattice_csr(mq_ctx* ctx, int64_t nsites, const int32_t* row_ptr, const int32_t* col_idx,
                    mq_lattice** out);
+/* Random geometric graph built on the device, as CSR for mq_lattice_csr: n
+ * points uniform in the periodic unit square (dim 2) or cube (dim 3), joined
+ * below minimum-image distance radius; replaces RandomGeometricGraph's PCL
+ * kd-tree construction (src/lattice/random_geometric_graph.h:29-168), in 2D as
+ * well as 3D.  Point i's coordinate d is the 53-bit Philox uniform of counter
+ * (i, d >> 1, 0, 8), key = seed; rows are sorted.  Writes *nnz and, when
+ * non-NULL, row_ptr[n+1], col_idx[col_cap >= nnz] and coords[n*dim]; call
+ * with col_idx = NULL to learn nnz first.  2 <= n <= 2^30, 0 < radius < 1/2. */
+MQ_API int mq_rgg_build(mq_ctx* ctx, int dim, int64_t n, double radius, uint64_t seed, int64_t* nnz,
+                        int32_t* row_ptr, int32_t* col_idx, int64_t col_cap, double* coords);
 /* number of colour classes and their sizes (ncolors <= 64) */
 MQ_API int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes64);
 MQ_API int64_t mq_lattice_nsites(const mq_lattice* lat);

@@ -86,6 +86,23 @@ private:
     mq_lattice* h_ = nullptr;
 };
 
+// Random geometric graph built on the device (mq_rgg_build), as CSR for GraphLattice.
+struct CsrGraph {
+    std::vector<int32_t> row_ptr, col_idx;
+};
+inline CsrGraph random_geometric_graph(Context& ctx, int dim, int64_t n, double radius, uint64_t seed)
+{
+    CsrGraph g;
+    g.row_ptr.resize((size_t)n + 1);
+    int64_t nnz = 0;
+    ctx.check(mq_rgg_build(ctx.get(), dim, n, radius, seed, &nnz, g.row_ptr.data(), nullptr, 0, nullptr),
+              "mq_rgg_build");
+    g.col_idx.resize((size_t)nnz);
+    ctx.check(mq_rgg_build(ctx.get(), dim, n, radius, seed, &nnz, g.row_ptr.data(), g.col_idx.data(), nnz, nullptr),
+              "mq_rgg_build");
+    return g;
+}
+
 // Model tags with the reference's parameters.
 struct Ising { double J = -1.0; int spin_bits = 8; };
 struct EdwardsAndersonIsing { double J = -1.0; };

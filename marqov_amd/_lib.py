@@ -51,6 +51,7 @@ SIGNATURES = {
     "mq_timer_end": (_I, [_P, ctypes.POINTER(_D)]),
     "mq_lattice_regular": (_I, [_P, _I, _I, ctypes.POINTER(_P)]),
     "mq_lattice_csr": (_I, [_P, _I64, _P, _P, ctypes.POINTER(_P)]),
+    "mq_rgg_build": (_I, [_P, _I, _I64, _D, _U64, _P, _P, _P, _I64, _P]),
     "mq_lattice_colors": (_I, [_P, ctypes.POINTER(_I), _P]),
     "mq_lattice_nsites": (_I64, [_P]),
     "mq_lattice_color_of": (_I, [_P, _P]),

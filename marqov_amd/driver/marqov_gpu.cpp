@@ -476,8 +476,9 @@ int main(int argc, char** argv)
                 int64_t n = 1;
                 for (int d = 0; d < pl.dim; ++d) n *= pl.L;
                 const double r = rgg_radius(pl.dim, n, 6.0);
-                for (int rep = 0; rep < pl.nrep; ++rep) {
-                    const Csr g = rgg_periodic(pl.dim, n, r, mix(mix(o.seed, 0x5267u), (uint64_t)pl.L * 1000003u + rep));
+                for (int rep = 0; rep < pl.nrep; ++rep) {  // one graph per replica, built on the device
+                    const CsrGraph g = random_geometric_graph(
+                        ctx, pl.dim, n, r, mix(mix(o.seed, 0x5267u), (uint64_t)pl.L * 1000003u + rep));
                     graphs.emplace_back(new GraphLattice(ctx, g.row_ptr, g.col_idx));
                 }
             }

@@ -55,6 +55,21 @@ class Context:
         self.check(self.lib.mq_timer_end(self.h, ctypes.byref(ms)), "mq_timer_end")
         return ms.value
 
+    def rgg(self, dim, n, radius, seed, coords=False):
+        """Random geometric graph built on the device (mq_rgg_build): returns
+        (row_ptr int32[n+1], col_idx int32[nnz]) and, with coords=True, the points
+        float64[n, dim]."""
+        nnz = ctypes.c_int64()
+        rp = np.zeros(n + 1, np.int32)
+        self.check(self.lib.mq_rgg_build(self.h, dim, n, float(radius), seed, ctypes.byref(nnz), _ptr(rp), None, 0,
+                                         None), "mq_rgg_build")
+        ci = np.zeros(max(nnz.value, 1), np.int32)
+        xyz = np.zeros((n, dim), np.float64) if coords else None
+        self.check(self.lib.mq_rgg_build(self.h, dim, n, float(radius), seed, ctypes.byref(nnz), _ptr(rp), _ptr(ci),
+                                         ci.size, _ptr(xyz) if coords else None), "mq_rgg_build")
+        ci = ci[:nnz.value]
+        return (rp, ci, xyz) if coords else (rp, ci)
+
     def close(self):
         if self.h:
             self.lib.mq_ctx_destroy(self.h)

@@ -2,7 +2,8 @@
 
 * rgg2d: random geometric graph on the periodic unit square -- N points uniform
   from a seed, an edge between points closer than `radius` (minimum-image
-  distance), built with a cell list.  The reference's RandomGeometricGraph is
+  distance), built with a cell list on the host (numpy).  The device builder,
+  2D and 3D, is `gpu.Context.rgg` (mq_rgg_build).  The reference's RandomGeometricGraph is
   3D only and uses a PCL kd-tree with a random_device seed
   (src/lattice/random_geometric_graph.h:29-168); this one is 2D, seeded and
   library-free.  rgg_radius(N, k) gives the radius for mean degree k:
@@ -17,7 +18,10 @@
 import numpy as np
 
 
-def rgg_radius(n, mean_degree=6.0):
+def rgg_radius(n, mean_degree=6.0, dim=2):
+    """Radius for mean degree k: pi r^2 n = k (2D), 4/3 pi r^3 n = k (3D)."""
+    if dim == 3:
+        return float(np.cbrt(3.0 / (4.0 * np.pi) * mean_degree / n))
     return float(np.sqrt(mean_degree / (np.pi * n)))
 
 

@@ -142,6 +142,20 @@ EXPORT void orc_init_potts(int64_t N, int8_t* s, int Q, uint64_t seed, uint32_t 
     }
 }
 
+/* Points of the device random geometric graph builder (mq_rgg_build): coordinate
+ * d of point i is the 53-bit uniform of counter (i, d >> 1, 0, 8), words 2(d&1)
+ * (high) and 2(d&1)+1 (low); out[i*dim + d]. */
+EXPORT void orc_rgg_points(int dim, int64_t n, uint64_t seed, double* out)
+{
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < dim; ++d) {
+            uint32_t w[4];
+            orc_philox_u64key((uint32_t)i, (uint32_t)(d >> 1), 0u, 8u, seed, w);
+            const uint64_t u = ((uint64_t)w[2 * (d & 1)] << 32) | w[2 * (d & 1) + 1];
+            out[i * dim + d] = (double)(u >> 11) * 0x1.0p-53;
+        }
+}
+
 /* EA +-J couplings with P(+1) = p_plus: bonds[i*dim+d] couples i and i+e_d. */
 EXPORT void orc_init_bonds(int dim, int64_t N, int8_t* bonds, double p_plus, uint64_t seed, uint32_t sample)
 {

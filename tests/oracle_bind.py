@@ -27,6 +27,7 @@ _SIG = {
     "orc_init_ising": (None, [_I64, _P, _U64, _U32]),
     "orc_init_potts": (None, [_I64, _P, _I, _U64, _U32]),
     "orc_init_bonds": (None, [_I, _I64, _P, _D, _U64, _U32]),
+    "orc_rgg_points": (None, [_I, _I64, _U64, _P]),
     "orc_ising_sweeps": (None, [_I, _I, _P, _P, _P, _P, _I, _U64, _U32, _U32]),
     "orc_potts_sweeps": (None, [_I, _I, _P, _I, _P, _P, _I, _U64, _U32, _U32]),
     "orc_wolff_ising": (_I, [_I, _I, _P, _U32, _I, _U64, _U32, _U32, _P]),
@@ -113,6 +114,32 @@ def init_potts(N, Q, seed, rep):
     s = np.zeros(N, np.int8)
     lib().orc_init_potts(N, p(s), Q, seed, rep)
     return s
+
+
+def rgg_points(dim, n, seed):
+    out = np.zeros((n, dim), np.float64)
+    lib().orc_rgg_points(dim, n, seed, p(out))
+    return out
+
+
+def rgg_brute(pts, radius):
+    """All-pairs minimum-image neighbour lists (the definition of the graph), as CSR:
+    dx = |x_i - x_j|, 1 - dx above 1/2, s = dx_0^2 + dx_1^2 (+ dx_2^2) rounded in that
+    order, edge when s < r^2."""
+    n, dim = pts.shape
+    r2 = radius * radius
+    rows = []
+    for i in range(n):
+        dx = np.abs(pts[i] - pts)
+        dx = np.where(dx > 0.5, 1.0 - dx, dx)
+        s = dx[:, 0] * dx[:, 0]
+        for d in range(1, dim):
+            s = s + dx[:, d] * dx[:, d]
+        nb = np.nonzero(s < r2)[0]
+        rows.append(nb[nb != i])
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    return rp.astype(np.int32), np.concatenate(rows).astype(np.int32)
 
 
 def init_bonds(dim, N, p_plus, seed, sample):

@@ -107,20 +107,6 @@ def test_hypercube_csr_reference_order(dim, L):
         assert list(ci[rp[i]:rp[i + 1]]) == want, i
 
 
-@pytest.mark.parametrize("dim,n,r", [(2, 400, 0.09), (3, 500, 0.16), (2, 60, 0.3), (3, 40, 0.45)])
-def test_rgg_matches_brute_force(dim, n, r):
-    out = subprocess.run([LCHK, "rgg", str(dim), str(n), repr(r), "7"], capture_output=True, text=True,
-                         check=True).stdout
-    lines = out.splitlines()
-    pts = np.array(lines[1].split(), np.float64).reshape(n, dim)
-    rp, ci = _read_csr(lines, 2)
-    d = np.abs(pts[:, None, :] - pts[None, :, :])
-    d = np.minimum(d, 1.0 - d)
-    adj = ((d * d).sum(-1) < r * r) & ~np.eye(n, dtype=bool)
-    for i in range(n):
-        assert list(ci[rp[i]:rp[i + 1]]) == list(np.nonzero(adj[i])[0]), i
-
-
 def test_csv_graph_reader_matches_python(tmp_path):
     """GraphFromCSV format: the C++ reader of the front-end and the Python reader agree,
     with 1-based neighbour indices shifted back by idxoffs."""
