@@ -166,7 +166,7 @@ def leg_c3(gpu, ctx):
         st.init(seed=3)
         st.metropolis(5, 3, 0)
         ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
-        w_rate, w_mean = wolff_rate(ctx, st, 30, 10, 10, 3, 4, sweep0=5 + nstep)
+        w_rate, w_mean = wolff_rate(ctx, st, 30, 10, 10, 3, 10, sweep0=5 + nstep)
         res[name] = {"replicas": nrep, "attempts_per_ns": round(nrep * N * nstep / (ms * 1e6), 3),
                      "wolff_sites_per_ns": round(w_rate, 4),
                      "mean_cluster": round(w_mean, 1),
@@ -176,7 +176,7 @@ def leg_c3(gpu, ctx):
             res[name]["e"] = float(obs[:, 0].mean())
         del st
     res["batched_over_plain"] = round(res["batched"]["attempts_per_ns"] / res["plain"]["attempts_per_ns"], 2)
-    res["wolff"] = "after 30 x (10 sweeps + 10 clusters); 4 timed calls of 10 clusters per replica"
+    res["wolff"] = "after 30 x (10 sweeps + 10 clusters); 10 timed calls of 10 clusters per replica"
     # finite-size-scaling shape: as many FP32 replicas as fill 2 GiB (8x the 256 MB last-level
     # cache) in one launch vs one per launch, L = 16, 32, 64 (a single L=128 replica already
     # saturates HBM)
@@ -257,13 +257,13 @@ def leg_c5(gpu, ctx):
             ms, _ = timed(ctx, lambda: st.metropolis(nstep, 5, sweep0=2))
             res[f"{model}_{name}_attempts_per_ns"] = round(nrep * n * nstep / (ms * 1e6), 3)
             if name == "batched":
-                w_rate, w_mean = wolff_rate(ctx, st, 20, 4, 4, 5, 4, sweep0=2 + nstep)
+                w_rate, w_mean = wolff_rate(ctx, st, 20, 4, 4, 5, 8, sweep0=2 + nstep)
                 res[f"{model}_wolff_sites_per_ns"] = round(w_rate, 4)
                 res[f"{model}_wolff_mean_cluster"] = round(w_mean, 1)
             del st
         res[f"{model}_batched_over_plain"] = round(res[f"{model}_batched_attempts_per_ns"] /
                                                    res[f"{model}_plain_attempts_per_ns"], 2)
-    res["wolff"] = "64 replicas after 20 x (4 sweeps + 4 clusters); 4 timed calls of 4 clusters per replica"
+    res["wolff"] = "64 replicas after 20 x (4 sweeps + 4 clusters); 8 timed calls of 4 clusters per replica"
     return res
 
 
@@ -318,7 +318,7 @@ def main():
     wolff_b = None
     if not a.quick:
         log("headline state: Wolff clusters")
-        wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 3, sweep0=a.warmup + a.steps)
+        wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps)
     del st
 
     line = {
@@ -359,13 +359,13 @@ def main():
         ms_p, _ = timed(ctx, lambda: sp.metropolis(k, seed=2, sweep0=4))
         line["plain"] = {"attempts_per_ns": round(k * N / (ms_p * 1e6), 3), "replicas": 1, "steps": k}
         line["batched_over_plain"] = round(value / line["plain"]["attempts_per_ns"], 3)
-        w_rate, w_mean = wolff_rate(ctx, sp, 20, 4, 10, 2, 3, sweep0=4 + k)
+        w_rate, w_mean = wolff_rate(ctx, sp, 20, 4, 10, 2, 8, sweep0=4 + k)
         line["plain"]["wolff_sites_per_ns"] = round(w_rate, 4)
         line["plain"]["wolff_mean_cluster"] = round(w_mean, 1)
         line["wolff"] = {"sites_per_ns": round(wolff_b[0], 4), "mean_cluster": round(wolff_b[1], 1),
                          "replicas": R, "plain_sites_per_ns": round(w_rate, 4),
                          "batched_over_plain": round(wolff_b[0] / w_rate, 2),
-                         "schedule": "after 20 x (4 sweeps + 10 clusters); 3 timed calls of 10 clusters per replica"}
+                         "schedule": "after 20 x (4 sweeps + 10 clusters); 8 timed calls of 10 clusters per replica"}
         del sp
         for name, fn in (("c2_msc", leg_c2_msc), ("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3),
                          ("c4_ea_batch", leg_c4), ("c5_rgg", leg_c5)):

@@ -77,3 +77,19 @@ def test_rgg_rejects_bad_arguments(ctx, dim, n, radius):
     from marqov_amd import gpu
     with pytest.raises(gpu.MarqovError):
         ctx.rgg(dim, n, radius, 1)
+
+
+def test_rgg_refuses_short_col_idx(ctx):
+    """col_idx smaller than nnz is refused with the count still reported."""
+    import ctypes
+    from marqov_amd import gpu
+    n, r = 2000, 0.03
+    rp_ref, ci_ref = ctx.rgg(2, n, r, 3)
+    nnz = ctypes.c_int64()
+    rp = np.zeros(n + 1, np.int32)
+    short = np.zeros(len(ci_ref) - 1, np.int32)
+    rc = ctx.lib.mq_rgg_build(ctx.h, 2, n, r, 3, ctypes.byref(nnz), gpu._ptr(rp), gpu._ptr(short), short.size, None)
+    assert rc != 0
+    assert nnz.value == len(ci_ref)
+    assert b"fewer entries" in ctx.lib.mq_last_error(ctx.h)
+
