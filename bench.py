@@ -417,6 +417,14 @@ def main():
             "value": round(float(ow[0]), 5), "unit": "sites/ns", "cores": nth, "per_core": round(float(ow[1]), 5),
             "sample": f"{nth} threads x (500 warm-up + 20000 timed) stack Wolff clusters of 2D Ising L=64 at "
                       f"beta_c, one replica per thread, {ow[2]:.1f} s timed"}
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
+        line["cpu_baseline"]["host"] = {"cpu_model": cpu_model, "cpus_usable": ncpu,
+                                        "cpus_online": os.cpu_count(), "threads_used": nth}
     print(json.dumps(line), flush=True)
 
 
