@@ -179,11 +179,16 @@ extern "C" int mq_ctx_create(int device, mq_ctx** out)
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return MQ_ERR_HIP;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->num_cus = prop.multiProcessorCount;
+        if (prop.maxGridSize[1] > 0) c->max_grid_y = prop.maxGridSize[1];
+    }
     *out = c;
     return MQ_OK;
 }
@@ -354,6 +359,12 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         for (int k = r; k < e; ++k) {
             st->rep_off[k] = g.off + (eabits ? (int64_t)((k - r) / 32) : (int64_t)(k - r)) * g.stride;
             st->rep_pos_stride[k] = g.pos_stride;
+        }
+        if ((eabits ? g.nrep / 32 : g.nrep) > ctx->max_grid_y) {  // replicas run along the grid's y dimension
+            const std::string msg = "at most " + std::to_string(ctx->max_grid_y) +
+                                    (eabits ? " words of 32 replicas" : " replicas") + " per lattice in one state";
+            delete st;
+            return mq_fail(ctx, MQ_ERR_UNSUPPORTED, msg);
         }
         st->groups.push_back(g);
         r = e;

@@ -183,6 +183,7 @@ struct mq_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     int num_cus = 256;
+    int max_grid_y = 65535;  // workgroups along y, the replica dimension of most launches
 };
 
 struct mq_lattice {
