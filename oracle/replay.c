@@ -142,6 +142,73 @@ EXPORT void orc_init_potts(int64_t N, int8_t* s, int Q, uint64_t seed, uint32_t 
     }
 }
 
+/* One colour half-sweep of the FP32 O(3)/Phi4 Metropolis kernel restated in
+ * FP64 (the GPU schedule; the reference's move is metropolis.h:86-126 with the
+ * proposals of initializers.h:61-81 / Phi4.h:20-34 and the on-site terms of
+ * Phi4.h:69-74,110-124).  For each site i of the colour: counter
+ * (i >> 1, phase, rep, STREAM_METRO) gives words w0..w2; the direction is
+ * (1 - a0, rho sin(pi a1), rho cos(pi a1)) with a_k = 2 (w_k >> 8) 2^-24 and
+ * rho = sqrt(a0 (2 - a0)) (randomdir.h:45-65); the proposal is that direction
+ * (fresh), s + delta r normalised (small angle) or s + 0.5 r (Phi4); dE =
+ * J (s_new - s_old) . sum of neighbours plus, for Phi4, (mass/beta) d|s|^2 +
+ * (lambda/beta) d(|s|^2 - 1)^2; accept iff dE <= 0 or u < exp(-beta dE) with
+ * u = (w2 >> 8) 2^-24.  margin[i] = p - u (p = 1 when dE <= 0), so a GPU
+ * decision made in FP32 can differ only where |margin| is tiny; other sites'
+ * margin is 0.  s is float[3] per site in natural site order. */
+EXPORT void orc_o3_halfsweep(int dim, int L, const double* s_in, double* s_out, double* margin, int phi4,
+                             int small_angle, double J, double delta, double lambda, double mass, double beta,
+                             int colour, uint32_t phase, uint32_t rep, uint64_t seed)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    memcpy(s_out, s_in, sizeof(double) * 3 * (size_t)N);
+    for (int64_t i = 0; i < N; ++i) {
+        margin[i] = 0.0;
+        int x[3] = {0, 0, 0};
+        coords(dim, L, i, x);
+        if (((x[0] + x[1] + x[2]) & 1) != colour) continue;
+        double nx = 0, ny = 0, nz = 0;
+        for (int d = 0; d < dim; ++d)
+            for (int sg = 1; sg >= -1; sg -= 2) {
+                const double* v = s_in + 3 * nbr(dim, L, i, d, sg);
+                nx += v[0];
+                ny += v[1];
+                nz += v[2];
+            }
+        uint32_t w[4];
+        orc_philox_u64key((uint32_t)(i >> 1), phase, rep, ST_METRO, seed, w);
+        const double a0 = 2.0 * (double)(w[0] >> 8) * 0x1.0p-24;
+        const double a1 = 2.0 * (double)(w[1] >> 8) * 0x1.0p-24;
+        const double rho = sqrt(fmax(a0 * (2.0 - a0), 0.0));
+        const double rd[3] = {1.0 - a0, rho * sin(3.14159265358979323846 * a1), rho * cos(3.14159265358979323846 * a1)};
+        const double* so = s_in + 3 * i;
+        double sn[3];
+        if (phi4) {
+            for (int k = 0; k < 3; ++k) sn[k] = so[k] + 0.5 * rd[k];
+        } else if (small_angle) {
+            double t[3], n2 = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                t[k] = so[k] + delta * rd[k];
+                n2 += t[k] * t[k];
+            }
+            for (int k = 0; k < 3; ++k) sn[k] = t[k] / sqrt(n2);
+        } else {
+            for (int k = 0; k < 3; ++k) sn[k] = rd[k];
+        }
+        double dE = J * ((sn[0] - so[0]) * nx + (sn[1] - so[1]) * ny + (sn[2] - so[2]) * nz);
+        if (phi4) {
+            const double n2 = sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2];
+            const double o2 = so[0] * so[0] + so[1] * so[1] + so[2] * so[2];
+            dE += (mass / beta) * (n2 - o2) + (lambda / beta) * ((n2 - 1.0) * (n2 - 1.0) - (o2 - 1.0) * (o2 - 1.0));
+        }
+        const double u = (double)(w[2] >> 8) * 0x1.0p-24;
+        const double p = dE <= 0.0 ? 1.0 : exp(-beta * dE);
+        margin[i] = p - u;
+        if (u < p)
+            for (int k = 0; k < 3; ++k) s_out[3 * i + k] = sn[k];
+    }
+}
+
 /* Points of the device random geometric graph builder (mq_rgg_build): coordinate
  * d of point i is the 53-bit uniform of counter (i, d >> 1, 0, 8), words 2(d&1)
  * (high) and 2(d&1)+1 (low); out[i*dim + d]. */

@@ -28,6 +28,7 @@ _SIG = {
     "orc_init_potts": (None, [_I64, _P, _I, _U64, _U32]),
     "orc_init_bonds": (None, [_I, _I64, _P, _D, _U64, _U32]),
     "orc_rgg_points": (None, [_I, _I64, _U64, _P]),
+    "orc_o3_halfsweep": (None, [_I, _I, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _U32, _U32, _U64]),
     "orc_ising_sweeps": (None, [_I, _I, _P, _P, _P, _P, _I, _U64, _U32, _U32]),
     "orc_potts_sweeps": (None, [_I, _I, _P, _I, _P, _P, _I, _U64, _U32, _U32]),
     "orc_wolff_ising": (_I, [_I, _I, _P, _U32, _I, _U64, _U32, _U32, _P]),
@@ -114,6 +115,18 @@ def init_potts(N, Q, seed, rep):
     s = np.zeros(N, np.int8)
     lib().orc_init_potts(N, p(s), Q, seed, rep)
     return s
+
+
+def o3_halfsweep(dim, L, s, colour, phase, rep, seed, beta, J=-1.0, phi4=False, small_angle=False, delta=0.5,
+                 lam=4.5, mass=1.0):
+    """FP64 restatement of one colour half-sweep of the FP32 O(3)/Phi4 kernel:
+    returns (new spins float64[N,3], margin p - u per site, 0 off the colour)."""
+    s_in = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    out = np.zeros_like(s_in)
+    margin = np.zeros(len(s_in), np.float64)
+    lib().orc_o3_halfsweep(dim, L, p(s_in), p(out), p(margin), int(phi4), int(small_angle), J, delta, lam, mass,
+                           beta, colour, phase, rep, seed)
+    return out, margin
 
 
 def rgg_points(dim, n, seed):

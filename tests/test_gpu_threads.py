@@ -1,7 +1,8 @@
 """The C ABI is re-entrant per handle: every context owns its own HIP stream, and
 nothing else is shared, so simulations driven from several host threads at once
 (as MARQOV's scheduler runs one Core per pool thread, marqovqueue.h:169-226) give
-exactly the spins and observables they give one after another."""
+exactly the spins and cluster sizes they give one after another, and the same
+observables (to the last-bit rounding of FP64 atomics for the FP32 models)."""
 import threading
 
 import numpy as np
@@ -61,5 +62,7 @@ def test_contexts_in_threads_match_sequential_runs(models):
     for i in range(len(models)):
         for a, b in zip(seq[i][0], par[i][0]):
             assert np.array_equal(a, b), i
-        assert np.array_equal(seq[i][1], par[i][1]), i
+        # FP64 observables of FP32 models are sums of per-block atomics, whose order
+        # (and last-bit rounding) follows the block schedule; integer models' are exact
+        assert np.allclose(seq[i][1], par[i][1], rtol=1e-12, atol=1e-15), i
         assert np.array_equal(seq[i][2], par[i][2]), i
