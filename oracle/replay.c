@@ -209,6 +209,93 @@ EXPORT void orc_o3_halfsweep(int dim, int L, const double* s_in, double* s_out, 
     }
 }
 
+/* One O(3)/Phi4 Wolff cluster (Heisenberg.h:191-243 with a Cartesian axis;
+ * Phi4.h:182-244 / the random-direction embedding: s -= 2 (s.r) r) restated in
+ * FP64 on the GPU's random numbers: seed and embedding from counter
+ * (0, ncl, rep, STREAM_WSEED) (seed = (w0 N) >> 32; axis = (w1 3) >> 32 or the
+ * direction of words w1, w2 as in orc_o3_halfsweep); bond b = owner*dim + d
+ * (owner = the site on the + side's lower end, as in orc_wolff_ising) is active
+ * from in-cluster a to b's other end j iff cpl = J p_a' p_j > 0 and
+ * u_b < 1 - exp(-2 beta cpl), u_b = (word >> 8) 2^-24 of counter
+ * (b >> 2, ncl, rep, STREAM_WBOND), p_a' the projection of a after its flip and
+ * p_j that of j before the cluster.  The cluster is the component of active
+ * bonds; *min_margin receives min |p - u| over every (member, neighbour) test,
+ * so a cluster grown in FP32 in any order can differ only if it is tiny.
+ * Returns the cluster size; s_out = s_in with the cluster flipped. */
+EXPORT int64_t orc_o3_wolff(int dim, int L, const double* s_in, double* s_out, int random_dir, double J, double beta,
+                            uint32_t ncl, uint32_t rep, uint64_t seed, double* min_margin)
+{
+    int64_t N = 1;
+    for (int d = 0; d < dim; ++d) N *= L;
+    memcpy(s_out, s_in, sizeof(double) * 3 * (size_t)N);
+    uint32_t sw[4];
+    orc_philox_u64key(0u, ncl, rep, ST_WSEED, seed, sw);
+    const int64_t s0 = (int64_t)(((uint64_t)sw[0] * (uint64_t)N) >> 32);
+    const int axis = (int)(((uint64_t)sw[1] * 3ull) >> 32);
+    double rd[3] = {0, 0, 0};
+    if (random_dir) {
+        const double a0 = 2.0 * (double)(sw[1] >> 8) * 0x1.0p-24;
+        const double a1 = 2.0 * (double)(sw[2] >> 8) * 0x1.0p-24;
+        const double rho = sqrt(fmax(a0 * (2.0 - a0), 0.0));
+        rd[0] = 1.0 - a0;
+        rd[1] = rho * sin(3.14159265358979323846 * a1);
+        rd[2] = rho * cos(3.14159265358979323846 * a1);
+    }
+    double* proj0 = (double*)malloc(sizeof(double) * (size_t)N);  /* projections before the cluster */
+    char* in = (char*)calloc((size_t)N, 1);
+    int64_t* stack = (int64_t*)malloc(sizeof(int64_t) * (size_t)N);
+    for (int64_t i = 0; i < N; ++i) {
+        const double* v = s_in + 3 * i;
+        proj0[i] = random_dir ? v[0] * rd[0] + v[1] * rd[1] + v[2] * rd[2] : v[axis];
+    }
+    /* flip: Cartesian negates the axis component; random direction reflects */
+#define ORC_FLIP(i)                                                                            \
+    do {                                                                                       \
+        double* v_ = s_out + 3 * (i);                                                          \
+        if (random_dir) {                                                                      \
+            const double d_ = v_[0] * rd[0] + v_[1] * rd[1] + v_[2] * rd[2];                   \
+            for (int k_ = 0; k_ < 3; ++k_) v_[k_] -= 2.0 * d_ * rd[k_];                        \
+        } else {                                                                               \
+            v_[axis] = -v_[axis];                                                              \
+        }                                                                                      \
+    } while (0)
+    double mm = INFINITY;
+    int64_t top = 0, size = 1;
+    in[s0] = 1;
+    ORC_FLIP(s0);
+    stack[top++] = s0;
+    while (top > 0) {
+        const int64_t a = stack[--top];
+        const double* va = s_out + 3 * a;
+        const double pa = random_dir ? va[0] * rd[0] + va[1] * rd[1] + va[2] * rd[2] : va[axis];
+        for (int d = 0; d < dim; ++d)
+            for (int sg = 1; sg >= -1; sg -= 2) {
+                const int64_t j = nbr(dim, L, a, d, sg);
+                const int64_t owner = sg > 0 ? a : j;
+                const uint64_t b = (uint64_t)owner * (uint64_t)dim + (uint64_t)d;
+                uint32_t w[4];
+                orc_philox_u64key((uint32_t)(b >> 2), ncl, rep, ST_WBOND, seed, w);
+                const double u = (double)(w[b & 3] >> 8) * 0x1.0p-24;
+                const double cpl = J * pa * proj0[j];
+                const double p = cpl > 0.0 ? -expm1(-2.0 * beta * cpl) : 0.0;
+                const double mg = fabs(p - u);
+                if (mg < mm) mm = mg;
+                if (!in[j] && u < p) {
+                    in[j] = 1;
+                    ORC_FLIP(j);
+                    stack[top++] = j;
+                    ++size;
+                }
+            }
+    }
+#undef ORC_FLIP
+    *min_margin = mm;
+    free(proj0);
+    free(in);
+    free(stack);
+    return size;
+}
+
 /* Points of the device random geometric graph builder (mq_rgg_build): coordinate
  * d of point i is the 53-bit uniform of counter (i, d >> 1, 0, 8), words 2(d&1)
  * (high) and 2(d&1)+1 (low); out[i*dim + d]. */
@@ -573,6 +660,79 @@ EXPORT int orc_csr_wolff(int64_t N, const int32_t* rp, const int32_t* ci, int8_t
         if (sizes) sizes[c] = size;
     }
     return 0;
+}
+
+/* One O(3)/Phi4 Wolff cluster on a CSR graph, FP64, as orc_o3_wolff: seed
+ * vertex (w0 N) >> 32, bond uniforms keyed by the canonical edge id (the slot of
+ * the edge in its lower endpoint's row), as orc_csr_wolff. */
+EXPORT int64_t orc_csr_o3_wolff(int64_t N, const int32_t* rp, const int32_t* ci, const double* s_in, double* s_out,
+                                int random_dir, double J, double beta, uint32_t ncl, uint32_t rep, uint64_t seed,
+                                double* min_margin)
+{
+    memcpy(s_out, s_in, sizeof(double) * 3 * (size_t)N);
+    uint32_t sw[4];
+    orc_philox_u64key(0u, ncl, rep, ST_WSEED, seed, sw);
+    const int64_t s0 = (int64_t)(((uint64_t)sw[0] * (uint64_t)N) >> 32);
+    const int axis = (int)(((uint64_t)sw[1] * 3ull) >> 32);
+    double rd[3] = {0, 0, 0};
+    if (random_dir) {
+        const double a0 = 2.0 * (double)(sw[1] >> 8) * 0x1.0p-24;
+        const double a1 = 2.0 * (double)(sw[2] >> 8) * 0x1.0p-24;
+        const double rho = sqrt(fmax(a0 * (2.0 - a0), 0.0));
+        rd[0] = 1.0 - a0;
+        rd[1] = rho * sin(3.14159265358979323846 * a1);
+        rd[2] = rho * cos(3.14159265358979323846 * a1);
+    }
+    double* proj0 = (double*)malloc(sizeof(double) * (size_t)N);
+    char* in = (char*)calloc((size_t)N, 1);
+    int64_t* stack = (int64_t*)malloc(sizeof(int64_t) * (size_t)N);
+    for (int64_t i = 0; i < N; ++i) {
+        const double* v = s_in + 3 * i;
+        proj0[i] = random_dir ? v[0] * rd[0] + v[1] * rd[1] + v[2] * rd[2] : v[axis];
+    }
+#define ORC_FLIP(i)                                                                            \
+    do {                                                                                       \
+        double* v_ = s_out + 3 * (i);                                                          \
+        if (random_dir) {                                                                      \
+            const double d_ = v_[0] * rd[0] + v_[1] * rd[1] + v_[2] * rd[2];                   \
+            for (int k_ = 0; k_ < 3; ++k_) v_[k_] -= 2.0 * d_ * rd[k_];                        \
+        } else {                                                                               \
+            v_[axis] = -v_[axis];                                                              \
+        }                                                                                      \
+    } while (0)
+    double mm = INFINITY;
+    int64_t top = 0, size = 1;
+    in[s0] = 1;
+    ORC_FLIP(s0);
+    stack[top++] = s0;
+    while (top > 0) {
+        const int64_t a = stack[--top];
+        const double* va = s_out + 3 * a;
+        const double pa = random_dir ? va[0] * rd[0] + va[1] * rd[1] + va[2] * rd[2] : va[axis];
+        for (int64_t k = rp[a]; k < rp[a + 1]; ++k) {
+            const int64_t j = ci[k];
+            const int64_t id = (a < j) ? k : csr_mirror(rp, ci, a, k);
+            uint32_t w[4];
+            orc_philox_u64key((uint32_t)(id >> 2), ncl, rep, ST_WBOND, seed, w);
+            const double u = (double)(w[id & 3] >> 8) * 0x1.0p-24;
+            const double cpl = J * pa * proj0[j];
+            const double p = cpl > 0.0 ? -expm1(-2.0 * beta * cpl) : 0.0;
+            const double mg = fabs(p - u);
+            if (mg < mm) mm = mg;
+            if (!in[j] && u < p) {
+                in[j] = 1;
+                ORC_FLIP(j);
+                stack[top++] = j;
+                ++size;
+            }
+        }
+    }
+#undef ORC_FLIP
+    *min_margin = mm;
+    free(proj0);
+    free(in);
+    free(stack);
+    return size;
 }
 
 /* ---------------------------------------------- multi-spin-coded 3D Ising */

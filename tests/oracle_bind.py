@@ -29,6 +29,8 @@ _SIG = {
     "orc_init_bonds": (None, [_I, _I64, _P, _D, _U64, _U32]),
     "orc_rgg_points": (None, [_I, _I64, _U64, _P]),
     "orc_o3_halfsweep": (None, [_I, _I, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _U32, _U32, _U64]),
+    "orc_o3_wolff": (_I64, [_I, _I, _P, _P, _I, _D, _D, _U32, _U32, _U64, _P]),
+    "orc_csr_o3_wolff": (_I64, [_I64, _P, _P, _P, _P, _I, _D, _D, _U32, _U32, _U64, _P]),
     "orc_ising_sweeps": (None, [_I, _I, _P, _P, _P, _P, _I, _U64, _U32, _U32]),
     "orc_potts_sweeps": (None, [_I, _I, _P, _I, _P, _P, _I, _U64, _U32, _U32]),
     "orc_wolff_ising": (_I, [_I, _I, _P, _U32, _I, _U64, _U32, _U32, _P]),
@@ -127,6 +129,28 @@ def o3_halfsweep(dim, L, s, colour, phase, rep, seed, beta, J=-1.0, phi4=False, 
     lib().orc_o3_halfsweep(dim, L, p(s_in), p(out), p(margin), int(phi4), int(small_angle), J, delta, lam, mass,
                            beta, colour, phase, rep, seed)
     return out, margin
+
+
+def o3_wolff(dim, L, s, random_dir, J, beta, ncl, rep, seed):
+    """FP64 restatement of one O(3)/Phi4 Wolff cluster on the GPU's random numbers:
+    returns (spins after the flip, cluster size, smallest decision margin)."""
+    s_in = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    out = np.zeros_like(s_in)
+    mm = np.zeros(1, np.float64)
+    size = lib().orc_o3_wolff(dim, L, p(s_in), p(out), int(random_dir), J, beta, ncl, rep, seed, p(mm))
+    return out, int(size), float(mm[0])
+
+
+def csr_o3_wolff(rp, ci, s, random_dir, J, beta, ncl, rep, seed):
+    """orc_csr_o3_wolff: one O(3)/Phi4 cluster on a CSR graph, FP64."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    s_in = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    out = np.zeros_like(s_in)
+    mm = np.zeros(1, np.float64)
+    size = lib().orc_csr_o3_wolff(len(rp) - 1, p(rp), p(ci), p(s_in), p(out), int(random_dir), J, beta, ncl, rep,
+                                  seed, p(mm))
+    return out, int(size), float(mm[0])
 
 
 def rgg_points(dim, n, seed):

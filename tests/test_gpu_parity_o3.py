@@ -14,6 +14,9 @@ import oracle_bind as ob
 pytestmark = pytest.mark.gpu
 
 TOL_MARGIN = 1e-4
+# bond tests: FP32 coupling products and expm1f err by ~1e-7 relative; uniforms are
+# multiples of 2^-24
+TOL_WOLFF = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -63,3 +66,79 @@ def test_o3_sweep_decisions_match_fp64_restatement(ctx, model, proposal, dim, L,
             old = np.where(sites[:, None], s1[r], old)
     assert n_near <= 3, n_near
     assert 0.1 * n_checked < n_acc < 0.95 * n_checked, (n_acc, n_checked)
+
+
+@pytest.mark.parametrize("model,embedding,dim,L,beta", [
+    ("heisenberg", "cartesian", 3, 8, 0.8), ("heisenberg", "random", 3, 8, 0.8), ("heisenberg", "cartesian", 2, 16, 1.3),
+    ("phi4", "random", 3, 8, 0.69), ("phi4", "random", 2, 16, 1.0)])
+def test_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, dim, L, beta):
+    """Each O(3)/Phi4 cluster, grown by the GPU from a downloaded FP32 state, equals the
+    FP64 restatement's component of active bonds (same seed site, embedding and bond
+    uniforms): equal size, the same sites flipped, flipped spins to FP32 rounding.  A
+    cluster is compared when no bond test of it lies within 1e-5 of its threshold."""
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, dim, L)
+    nrep, seed = 2, 13
+    st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, proposal="small", delta=0.6, embedding=embedding,
+                   lam=4.5, mass=1.0)
+    st.set_beta(beta)
+    st.init(seed=7)
+    st.metropolis(5, seed=seed, sweep0=0)
+    compared = skipped = 0
+    sizes_seen = []
+    for k in range(8):
+        pre = [st.download(r).astype(np.float64) for r in range(nrep)]
+        sizes = st.wolff(1, seed=seed, cluster0=k)
+        post = [st.download(r).astype(np.float64) for r in range(nrep)]
+        for r in range(nrep):
+            cpu, size, margin = ob.o3_wolff(dim, L, pre[r], embedding == "random" or model == "phi4", -1.0, beta,
+                                            k, r, seed)
+            if margin <= TOL_WOLFF:
+                skipped += 1
+                continue
+            compared += 1
+            sizes_seen.append(size)
+            assert int(sizes[r, 0]) == size, (k, r)
+            moved_gpu = np.any(post[r] != pre[r], axis=1)
+            moved_cpu = np.any(cpu != pre[r], axis=1)
+            assert np.array_equal(moved_gpu, moved_cpu), (k, r)
+            assert np.abs(post[r] - cpu).max() < 2e-5, (k, r)
+    assert compared >= 12 and skipped <= 4, (compared, skipped)
+    assert max(sizes_seen) > 10, sizes_seen
+
+
+@pytest.mark.parametrize("model,embedding,beta", [("heisenberg", "cartesian", 1.2), ("heisenberg", "random", 1.2),
+                                                  ("phi4", "random", 1.0)])
+def test_csr_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, beta):
+    """The graph kernel's O(3)/Phi4 clusters (bond uniforms keyed by the canonical edge
+    id) against the FP64 restatement on a 2D random geometric graph of mean degree 8."""
+    from marqov_amd import graphs, gpu
+    n = 3000
+    rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n, 8.0), 17)
+    lat = gpu.CsrLattice(ctx, rp, ci)
+    nrep, seed = 2, 19
+    st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, proposal="small", delta=0.6, embedding=embedding,
+                   lam=4.5, mass=1.0)
+    st.set_beta(beta)
+    st.init(seed=8)
+    st.metropolis(5, seed=seed, sweep0=0)
+    compared = skipped = 0
+    sizes_seen = []
+    for k in range(8):
+        pre = [st.download(r).astype(np.float64) for r in range(nrep)]
+        sizes = st.wolff(1, seed=seed, cluster0=k)
+        post = [st.download(r).astype(np.float64) for r in range(nrep)]
+        for r in range(nrep):
+            cpu, size, margin = ob.csr_o3_wolff(rp, ci, pre[r], embedding == "random" or model == "phi4", -1.0, beta,
+                                                k, r, seed)
+            if margin <= TOL_WOLFF:
+                skipped += 1
+                continue
+            compared += 1
+            sizes_seen.append(size)
+            assert int(sizes[r, 0]) == size, (k, r)
+            assert np.array_equal(np.any(post[r] != pre[r], axis=1), np.any(cpu != pre[r], axis=1)), (k, r)
+            assert np.abs(post[r] - cpu).max() < 2e-5, (k, r)
+    assert compared >= 12 and skipped <= 4, (compared, skipped)
+    assert max(sizes_seen) > 10, sizes_seen
+
