@@ -155,6 +155,38 @@ EXPORT void orc_init_potts(int64_t N, int8_t* s, int Q, uint64_t seed, uint32_t 
  * u = (w2 >> 8) 2^-24.  margin[i] = p - u (p = 1 when dE <= 0), so a GPU
  * decision made in FP32 can differ only where |margin| is tiny; other sites'
  * margin is 0.  s is float[3] per site in natural site order. */
+/* proposal, energy difference and decision of one O(3)/Phi4 site: so = old spin,
+ * nsum = neighbour sum, w = its Philox words; writes sn, returns p - u */
+static double o3_site(const double* so, const double nsum[3], const uint32_t w[4], int phi4, int small_angle,
+                      double J, double delta, double lambda, double mass, double beta, double sn[3])
+{
+    const double a0 = 2.0 * (double)(w[0] >> 8) * 0x1.0p-24;
+    const double a1 = 2.0 * (double)(w[1] >> 8) * 0x1.0p-24;
+    const double rho = sqrt(fmax(a0 * (2.0 - a0), 0.0));
+    const double rd[3] = {1.0 - a0, rho * sin(3.14159265358979323846 * a1), rho * cos(3.14159265358979323846 * a1)};
+    if (phi4) {
+        for (int k = 0; k < 3; ++k) sn[k] = so[k] + 0.5 * rd[k];
+    } else if (small_angle) {
+        double t[3], n2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            t[k] = so[k] + delta * rd[k];
+            n2 += t[k] * t[k];
+        }
+        for (int k = 0; k < 3; ++k) sn[k] = t[k] / sqrt(n2);
+    } else {
+        for (int k = 0; k < 3; ++k) sn[k] = rd[k];
+    }
+    double dE = J * ((sn[0] - so[0]) * nsum[0] + (sn[1] - so[1]) * nsum[1] + (sn[2] - so[2]) * nsum[2]);
+    if (phi4) {
+        const double n2 = sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2];
+        const double o2 = so[0] * so[0] + so[1] * so[1] + so[2] * so[2];
+        dE += (mass / beta) * (n2 - o2) + (lambda / beta) * ((n2 - 1.0) * (n2 - 1.0) - (o2 - 1.0) * (o2 - 1.0));
+    }
+    const double u = (double)(w[2] >> 8) * 0x1.0p-24;
+    const double p = dE <= 0.0 ? 1.0 : exp(-beta * dE);
+    return p - u;
+}
+
 EXPORT void orc_o3_halfsweep(int dim, int L, const double* s_in, double* s_out, double* margin, int phi4,
                              int small_angle, double J, double delta, double lambda, double mass, double beta,
                              int colour, uint32_t phase, uint32_t rep, uint64_t seed)
@@ -167,45 +199,42 @@ EXPORT void orc_o3_halfsweep(int dim, int L, const double* s_in, double* s_out, 
         int x[3] = {0, 0, 0};
         coords(dim, L, i, x);
         if (((x[0] + x[1] + x[2]) & 1) != colour) continue;
-        double nx = 0, ny = 0, nz = 0;
+        double ns[3] = {0, 0, 0};
         for (int d = 0; d < dim; ++d)
             for (int sg = 1; sg >= -1; sg -= 2) {
                 const double* v = s_in + 3 * nbr(dim, L, i, d, sg);
-                nx += v[0];
-                ny += v[1];
-                nz += v[2];
+                for (int k = 0; k < 3; ++k) ns[k] += v[k];
             }
         uint32_t w[4];
         orc_philox_u64key((uint32_t)(i >> 1), phase, rep, ST_METRO, seed, w);
-        const double a0 = 2.0 * (double)(w[0] >> 8) * 0x1.0p-24;
-        const double a1 = 2.0 * (double)(w[1] >> 8) * 0x1.0p-24;
-        const double rho = sqrt(fmax(a0 * (2.0 - a0), 0.0));
-        const double rd[3] = {1.0 - a0, rho * sin(3.14159265358979323846 * a1), rho * cos(3.14159265358979323846 * a1)};
-        const double* so = s_in + 3 * i;
         double sn[3];
-        if (phi4) {
-            for (int k = 0; k < 3; ++k) sn[k] = so[k] + 0.5 * rd[k];
-        } else if (small_angle) {
-            double t[3], n2 = 0.0;
-            for (int k = 0; k < 3; ++k) {
-                t[k] = so[k] + delta * rd[k];
-                n2 += t[k] * t[k];
-            }
-            for (int k = 0; k < 3; ++k) sn[k] = t[k] / sqrt(n2);
-        } else {
-            for (int k = 0; k < 3; ++k) sn[k] = rd[k];
-        }
-        double dE = J * ((sn[0] - so[0]) * nx + (sn[1] - so[1]) * ny + (sn[2] - so[2]) * nz);
-        if (phi4) {
-            const double n2 = sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2];
-            const double o2 = so[0] * so[0] + so[1] * so[1] + so[2] * so[2];
-            dE += (mass / beta) * (n2 - o2) + (lambda / beta) * ((n2 - 1.0) * (n2 - 1.0) - (o2 - 1.0) * (o2 - 1.0));
-        }
-        const double u = (double)(w[2] >> 8) * 0x1.0p-24;
-        const double p = dE <= 0.0 ? 1.0 : exp(-beta * dE);
-        margin[i] = p - u;
-        if (u < p)
+        margin[i] = o3_site(s_in + 3 * i, ns, w, phi4, small_angle, J, delta, lambda, mass, beta, sn);
+        if (margin[i] > 0.0)
             for (int k = 0; k < 3; ++k) s_out[3 * i + k] = sn[k];
+    }
+}
+
+/* The same for one colour class of a CSR graph: counter (vertex, phase, rep,
+ * STREAM_METRO) with phase = 64 * sweep + colour; colour_of[] from
+ * mq_lattice_color_of. */
+EXPORT void orc_csr_o3_colour(int64_t N, const int32_t* rp, const int32_t* ci, const int32_t* colour_of,
+                              const double* s_in, double* s_out, double* margin, int phi4, int small_angle, double J,
+                              double delta, double lambda, double mass, double beta, int colour, uint32_t phase,
+                              uint32_t rep, uint64_t seed)
+{
+    memcpy(s_out, s_in, sizeof(double) * 3 * (size_t)N);
+    for (int64_t v = 0; v < N; ++v) {
+        margin[v] = 0.0;
+        if (colour_of[v] != colour) continue;
+        double ns[3] = {0, 0, 0};
+        for (int64_t k = rp[v]; k < rp[v + 1]; ++k)
+            for (int c = 0; c < 3; ++c) ns[c] += s_in[3 * (int64_t)ci[k] + c];
+        uint32_t w[4];
+        orc_philox_u64key((uint32_t)v, phase, rep, ST_METRO, seed, w);
+        double sn[3];
+        margin[v] = o3_site(s_in + 3 * v, ns, w, phi4, small_angle, J, delta, lambda, mass, beta, sn);
+        if (margin[v] > 0.0)
+            for (int c = 0; c < 3; ++c) s_out[3 * v + c] = sn[c];
     }
 }
 

@@ -30,6 +30,7 @@ _SIG = {
     "orc_rgg_points": (None, [_I, _I64, _U64, _P]),
     "orc_o3_halfsweep": (None, [_I, _I, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _U32, _U32, _U64]),
     "orc_o3_wolff": (_I64, [_I, _I, _P, _P, _I, _D, _D, _U32, _U32, _U64, _P]),
+    "orc_csr_o3_colour": (None, [_I64, _P, _P, _P, _P, _P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _U32, _U32, _U64]),
     "orc_csr_o3_wolff": (_I64, [_I64, _P, _P, _P, _P, _I, _D, _D, _U32, _U32, _U64, _P]),
     "orc_ising_sweeps": (None, [_I, _I, _P, _P, _P, _P, _I, _U64, _U32, _U32]),
     "orc_potts_sweeps": (None, [_I, _I, _P, _I, _P, _P, _I, _U64, _U32, _U32]),
@@ -139,6 +140,20 @@ def o3_wolff(dim, L, s, random_dir, J, beta, ncl, rep, seed):
     mm = np.zeros(1, np.float64)
     size = lib().orc_o3_wolff(dim, L, p(s_in), p(out), int(random_dir), J, beta, ncl, rep, seed, p(mm))
     return out, int(size), float(mm[0])
+
+
+def csr_o3_colour(rp, ci, colour_of, s, colour, phase, rep, seed, beta, J=-1.0, phi4=False, small_angle=False,
+                  delta=0.5, lam=4.5, mass=1.0):
+    """orc_csr_o3_colour: one colour class of the FP32 O(3)/Phi4 graph sweep in FP64."""
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    col = np.ascontiguousarray(colour_of, np.int32)
+    s_in = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    out = np.zeros_like(s_in)
+    margin = np.zeros(len(s_in), np.float64)
+    lib().orc_csr_o3_colour(len(rp) - 1, p(rp), p(ci), p(col), p(s_in), p(out), p(margin), int(phi4),
+                            int(small_angle), J, delta, lam, mass, beta, colour, phase, rep, seed)
+    return out, margin
 
 
 def csr_o3_wolff(rp, ci, s, random_dir, J, beta, ncl, rep, seed):

@@ -142,3 +142,45 @@ def test_csr_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, bet
     assert compared >= 12 and skipped <= 4, (compared, skipped)
     assert max(sizes_seen) > 10, sizes_seen
 
+
+@pytest.mark.parametrize("model,proposal,beta", [("heisenberg", "fresh", 1.2), ("heisenberg", "small", 1.2),
+                                                 ("phi4", "fresh", 1.0)])
+def test_csr_o3_sweep_decisions_match_fp64_restatement(ctx, model, proposal, beta):
+    """The graph kernel's FP32 O(3)/Phi4 sweep, colour class by colour class (counter
+    phase 64 * sweep + colour), against the FP64 restatement on a 2D random geometric
+    graph; each class is replayed from the GPU's result for the classes before it."""
+    import ctypes
+    from marqov_amd import graphs, gpu
+    n = 2500
+    rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n, 6.0), 23)
+    lat = gpu.CsrLattice(ctx, rp, ci)
+    colour = np.zeros(n, np.int32)
+    ctx.check(ctx.lib.mq_lattice_color_of(lat.h, colour.ctypes.data_as(ctypes.c_void_p)), "mq_lattice_color_of")
+    ncol = int(colour.max()) + 1
+    nrep, seed, sweep0 = 3, 29, 2
+    st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, proposal=proposal, delta=0.6, lam=4.5, mass=1.0)
+    st.set_beta(beta)
+    st.init(seed=4)
+    st.metropolis(sweep0, seed=seed, sweep0=0)
+    s0 = [st.download(r).astype(np.float64) for r in range(nrep)]
+    st.metropolis(1, seed=seed, sweep0=sweep0)
+    s1 = [st.download(r).astype(np.float64) for r in range(nrep)]
+    kw = dict(phi4=(model == "phi4"), small_angle=(proposal == "small"), delta=0.6, lam=4.5, mass=1.0)
+    n_checked = n_acc = n_near = 0
+    for r in range(nrep):
+        old = s0[r]
+        for c in range(ncol):
+            cpu, m = ob.csr_o3_colour(rp, ci, colour, old, c, 64 * sweep0 + c, r, seed, beta, J=-1.0, **kw)
+            sites = colour == c
+            moved = np.any(s1[r] != old, axis=1)
+            sure = sites & (np.abs(m) > TOL_MARGIN)
+            n_near += int((sites & ~sure).sum())
+            assert np.array_equal(moved[sure], (m > 0)[sure]), (r, c)
+            acc = sure & (m > 0)
+            assert np.abs(s1[r][acc] - cpu[acc]).max(initial=0.0) < 2e-5, (r, c)
+            n_checked += int(sure.sum())
+            n_acc += int(acc.sum())
+            old = np.where(sites[:, None], s1[r], old)
+    assert n_near <= 3, n_near
+    assert 0.1 * n_checked < n_acc < 0.95 * n_checked, (n_acc, n_checked)
+
