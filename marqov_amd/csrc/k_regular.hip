@@ -214,7 +214,10 @@ __global__ void __launch_bounds__(512) k_sweep_ising3_row16(int8_t* __restrict__
     __shared__ uint32_t s_tlo[13];
     const uint32_t B = gridDim.x;
     const uint32_t b = blockIdx.x;
-    const uint32_t lb = (xcd && (B & 7u) == 0u) ? (b & 7u) * (B >> 3) + (b >> 3) : b;
+    const uint32_t lb0 = (xcd && (B & 7u) == 0u) ? (b & 7u) * (B >> 3) + (b >> 3) : b;
+    // colour 1 walks the blocks backwards, so every pass starts where the previous
+    // one ended: those lines are still in the 256 MB last-level cache
+    const uint32_t lb = colour ? B - 1u - lb0 : lb0;
     const uint32_t r = lb >> lgbpr;
     const uint32_t grep = (uint32_t)rep0 + r;
     if (threadIdx.x < 13) {
@@ -504,6 +507,11 @@ __global__ void __launch_bounds__(512) k_sweep_o3(float4* __restrict__ base, int
         const uint32_t lb = (lin & 7u) * (nblk >> 3) + (lin >> 3);
         by = lb / gridDim.x;
         bx = lb - by * gridDim.x;
+    }
+    if (colour) {  // backwards, so the pass starts on what the previous one left in the last-level cache
+        const uint32_t lin = nblk - 1u - (bx + by * gridDim.x);
+        by = lin / gridDim.x;
+        bx = lin - by * gridDim.x;
     }
     const int r = (int)by;
     const int grep = rep0 + r;
