@@ -417,6 +417,22 @@ def main():
             "value": round(float(ow[0]), 5), "unit": "sites/ns", "cores": nth, "per_core": round(float(ow[1]), 5),
             "sample": f"{nth} threads x (500 warm-up + 20000 timed) stack Wolff clusters of 2D Ising L=64 at "
                       f"beta_c, one replica per thread, {ow[2]:.1f} s timed"}
+        if os.path.exists(ref_bin):
+            # the reference's move for the FP64 Heisenberg model, fresh-direction proposal, 3D L=128
+            import subprocess
+            try:
+                r = subprocess.run([ref_bin, "bench", "heis", "3", "128", "0.693", "-1.0", "1", "2", "3", str(nth)],
+                                   capture_output=True, text=True, timeout=300)
+                if r.returncode == 0:
+                    hz = json.loads(r.stdout.strip().splitlines()[-1])
+                    line["cpu_baseline"]["heisenberg_3d_L128"] = {
+                        "value": round(float(hz["attempts_per_ns"]), 5), "unit": "attempts/ns", "cores": nth,
+                        "kind": "reference", "per_core": round(float(hz["per_thread"]), 5),
+                        "sample": f"{nth} threads x (1 warm-up + 2 timed) sweeps of the reference's "
+                                  f"Metropolis<Heisenberg, RegularLattice>::move (FP64, fresh direction) on 3D L=128, "
+                                  f"one simulation per thread, {hz['seconds']:.1f} s timed"}
+            except (OSError, ValueError, subprocess.TimeoutExpired):
+                pass
         cpu_model = ""
         try:
             with open("/proc/cpuinfo") as f:
