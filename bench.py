@@ -301,7 +301,8 @@ def main():
         st.set_tuning(**tuning)
     elif not a.no_tune:
         # launch shape from the project's autotuner (its timing sweeps use counters far from
-        # the run's, so they only add warm-up to this chain); cached per box in a scratch file
+        # the run's, so they only add warm-up to this chain); retuned on every run, the choice
+        # also written to a scratch file
         from marqov_amd import autotune
         shape, _ = autotune.tune(ctx, st, cache_path=os.path.join("/tmp", f"marqov_amd_tuning_{os.getuid()}.json"),
                                  force=True, sweeps=6)
