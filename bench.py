@@ -150,6 +150,10 @@ def leg_c1(gpu, ctx):
 
 
 C3_REPLICAS = int(os.environ.get("MARQOV_BENCH_C3_REPLICAS", "256"))
+# small-angle step s' = (s + delta r) / |s + delta r|: at beta = 0.693 the FP64 restatement
+# accepts 0.80 / 0.59 / 0.52 / 0.49 of the moves at delta = 0.6 / 1.5 / 3 / 5 (3D L=16, after
+# 30 sweeps); delta = 4 gives the ~50% the configuration asks for
+C3_DELTA = 4.0
 
 
 def leg_c3(gpu, ctx):
@@ -161,13 +165,19 @@ def leg_c3(gpu, ctx):
     N = L ** 3
     res = {}
     for name, nrep, nstep in (("batched", C3_REPLICAS, 20), ("plain", 1, 20)):
-        st = gpu.State(ctx, "heisenberg", [lat] * nrep, J=-1.0, proposal="small", delta=0.6, embedding="cartesian")
+        st = gpu.State(ctx, "heisenberg", [lat] * nrep, J=-1.0, proposal="small", delta=C3_DELTA,
+                       embedding="cartesian")
         st.set_beta(beta)
         st.init(seed=3)
         st.metropolis(5, 3, 0)
         ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
         w_rate, w_mean = wolff_rate(ctx, st, 30, 10, 10, 3, 10, sweep0=5 + nstep)
+        # acceptance of one sweep of replica 0 on the thermalised state
+        before = st.download(0)
+        st.metropolis(1, 3, sweep0=10 ** 6)
+        acceptance = float(np.any(st.download(0) != before, axis=1).mean())
         res[name] = {"replicas": nrep, "attempts_per_ns": round(nrep * N * nstep / (ms * 1e6), 3),
+                     "delta": C3_DELTA, "acceptance": round(acceptance, 3),
                      "wolff_sites_per_ns": round(w_rate, 4),
                      "mean_cluster": round(w_mean, 1),
                      "roofline_frac": round(48.0 * nrep * N * nstep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -184,7 +194,7 @@ def leg_c3(gpu, ctx):
         latS = gpu.RegularLattice(ctx, 3, Ls)
         small = {}
         for name, nr, nstep in (("batched", nrep, 20), ("plain", 1, 20)):
-            st = gpu.State(ctx, "heisenberg", [latS] * nr, J=-1.0, proposal="small", delta=0.6)
+            st = gpu.State(ctx, "heisenberg", [latS] * nr, J=-1.0, proposal="small", delta=C3_DELTA)
             st.set_beta(beta)
             st.init(seed=3)
             st.metropolis(5, 3, 0)
