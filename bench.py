@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--legs", default="", help="comma-separated subset of the extra legs to run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM traffic)")
+    ap.add_argument("--locate-c5-beta", action="store_true",
+                    help="only locate the pseudo-critical betas of the 2D RGG legs by Binder-cumulant crossings")
     return ap.parse_args()
 
 
@@ -258,7 +260,9 @@ def leg_c5(gpu, ctx):
     lat = gpu.CsrLattice(ctx, rp, ci)
     res = {"N": n, "edges": int(rp[-1] // 2), "colours": int(len(lat.colors())),
            "graph_build_ms": round(1e3 * t_build, 1)}
-    for model, beta in (("ising", 0.55), ("potts", 1.2)):
+    # pseudo-critical points of these graphs: Binder-cumulant crossings of N = 16,384 and 65,536
+    # (`bench.py --locate-c5-beta`): Ising 0.494, Potts-3 1.088
+    for model, beta in (("ising", 0.494), ("potts", 1.088)):
         for name, nrep, nstep in (("batched", 64, 10), ("plain", 1, 10)):
             st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, q=3)
             st.set_beta(beta)
@@ -275,6 +279,50 @@ def leg_c5(gpu, ctx):
                                                    res[f"{model}_plain_attempts_per_ns"], 2)
     res["wolff"] = "64 replicas after 20 x (4 sweeps + 4 clusters); 8 timed calls of 4 clusters per replica"
     return res
+
+
+def locate_c5_beta(gpu, ctx, sizes=(4096, 16384, 65536), R=32, nwarm=300, nmeas=1500):
+    """Pseudo-critical betas of Ising and Potts-3 on 2D random geometric graphs of mean degree 6
+    (the RGG legs' graphs at smaller N, built on the device with the legs' seed): for each size,
+    R replicas at each of 13 betas, EMCS = 5 Wolff clusters + 1 sweep, nwarm + nmeas EMCS with a
+    measurement after each; U = 1 - <m^4> / (3 <m^2>^2) per beta, and the beta where the U(beta)
+    curves of consecutive sizes cross (linear interpolation between grid points)."""
+    from marqov_amd import graphs
+    grids = {"ising": np.linspace(0.46, 0.54, 9), "potts": np.linspace(1.04, 1.20, 9)}
+    out = {}
+    for model, betas in grids.items():
+        curves = {}
+        for n in sizes:
+            rp, ci = ctx.rgg(2, n, graphs.rgg_radius(n), 5)
+            lat = gpu.CsrLattice(ctx, rp, ci)
+            nb = len(betas)
+            st = gpu.State(ctx, model, [lat] * (nb * R), J=-1.0, q=3)
+            st.set_beta(np.repeat(betas, R))
+            st.init(seed=11)
+            for t in range(nwarm):
+                st.emcs(5, 1, 11, t)
+            m2 = np.zeros(nb * R)
+            m4 = np.zeros(nb * R)
+            for t in range(nmeas):
+                st.emcs(5, 1, 11, nwarm + t)
+                o = st.measure()
+                m2 += o[:, 2]
+                m4 += o[:, 3]
+            m2 = (m2 / nmeas).reshape(nb, R).mean(1)
+            m4 = (m4 / nmeas).reshape(nb, R).mean(1)
+            curves[n] = 1.0 - m4 / (3.0 * m2 * m2)
+            log(f"{model} N={n}: U = {np.round(curves[n], 3).tolist()}")
+            del st, lat
+        cross = {}
+        for n1, n2 in zip(sizes[:-1], sizes[1:]):
+            d = curves[n2] - curves[n1]
+            idx = [i for i in range(len(betas) - 1) if d[i] * d[i + 1] < 0]
+            cross[f"{n1}-{n2}"] = [round(float(betas[i] - d[i] * (betas[i + 1] - betas[i]) / (d[i + 1] - d[i])), 4)
+                                   for i in idx]
+        out[model] = {"betas": [round(float(b), 4) for b in betas],
+                      "binder": {str(n): [round(float(u), 4) for u in curves[n]] for n in sizes},
+                      "crossings": cross}
+    return out
 
 
 def main():
@@ -295,6 +343,9 @@ def main():
     from marqov_amd import gpu
 
     ctx = gpu.Context(0)
+    if a.locate_c5_beta:
+        print(json.dumps(locate_c5_beta(gpu, ctx)), flush=True)
+        return
     L, R, beta = a.L, a.replicas, 0.2216546
     N = L ** 3
     lat = gpu.RegularLattice(ctx, 3, L)
