@@ -60,6 +60,20 @@ def test_heisenberg_vs_fp64_restatement(ctx, proposal, embedding):
     assert ok_m, im
 
 
+def test_heisenberg_c3_schedule_L12_vs_fp64_restatement(ctx):
+    """The C3 configuration's update (small-angle step delta = 4, about 50% acceptance, and
+    Cartesian-embedding Wolff) at beta = 0.693 on 3D L = 12: FP32 GPU replicas against the
+    FP64 restatement of the reference's fresh-direction Metropolis and Cartesian Wolff."""
+    beta = 0.693
+    obs, _ = gpu_series(ctx, "heisenberg", 3, 12, beta, 64, 100, 1500, J=-1.0, proposal="small", delta=4.0,
+                        embedding="cartesian")
+    cpu = cpu_series(ob.HEIS, 3, 12, beta, 16, 100, 1500, J=-1.0)
+    ok_e, ie = agree(obs[:, :, 0].mean(0), cpu[:, :, 0].mean(1), label="heisenberg C3 schedule L=12 E/N")
+    ok_m, im = agree(obs[:, :, 1].mean(0), cpu[:, :, 1].mean(1), label="heisenberg C3 schedule L=12 |m|")
+    assert ok_e, ie
+    assert ok_m, im
+
+
 def test_phi4_vs_fp64_restatement(ctx):
     beta, lam, mass = 0.69, 4.5, 1.0  # config/Phi4.ini
     obs, _ = gpu_series(ctx, "phi4", 3, 6, beta, 64, 100, 400, J=-1.0, lam=lam, mass=mass)
