@@ -107,6 +107,34 @@ def test_ising_c1_energy_vs_kaufman(ctx):
     assert 0.55 < binder < 0.66, binder  # U* = 0.6107 (Salas & Sokal 2000) at the critical point
 
 
+def test_potts3_critical_energy_extrapolates_to_exact(ctx):
+    """2D Potts-3 at its exact critical point beta_c = ln(1 + sqrt 3): the energy per site of
+    L = 32, 64, 128 tori, fitted as E(L) = E_inf + a L^((alpha - 1)/nu) with the q = 3
+    exponents alpha = 1/3, nu = 5/6 (so L^-0.8), must extrapolate to the exact critical energy
+    -(1 + 1/sqrt 3) (Baxter; two bonds per site, E = J sum delta(s_i, s_j), J = -1) within 4
+    standard errors, and the three sizes must follow the fitted form."""
+    bc = np.log(1.0 + np.sqrt(3.0))
+    sizes = np.array([32.0, 64.0, 128.0])
+    means, errs = [], []
+    for L in sizes.astype(int):
+        obs, _ = gpu_series(ctx, "potts", 2, int(L), bc, 256, 300, 1000, ncluster=4, nmetro=1, seed=4, J=-1.0, q=3)
+        e = obs[:, :, 0].mean(0)
+        means.append(e.mean())
+        errs.append(e.std(ddof=1) / np.sqrt(len(e)))
+    E, sd = np.array(means), np.array(errs)
+    A = np.vstack([np.ones(3), sizes ** -0.8]).T
+    W = np.diag(1.0 / sd ** 2)
+    cov = np.linalg.inv(A.T @ W @ A)
+    p = cov @ A.T @ W @ E
+    chi2 = float((((E - A @ p) / sd) ** 2).sum())
+    exact_e = -(1.0 + 1.0 / np.sqrt(3.0))
+    z = (p[0] - exact_e) / np.sqrt(cov[0, 0])
+    print(f"[potts3 critical] E(L) = {np.round(E, 5).tolist()}  E_inf = {p[0]:.5f} +- {np.sqrt(cov[0, 0]):.5f}  "
+          f"exact {exact_e:.5f}  ({z:.2f} sigma)  chi2 = {chi2:.2f}", flush=True)
+    assert abs(z) < 4.0, (p, cov, E, sd)
+    assert chi2 < 10.0, (chi2, E, sd)
+
+
 def test_heisenberg_renormalize_keeps_unit_length(ctx):
     _, st = gpu_series(ctx, "heisenberg", 3, 8, 0.693, 2, 0, 50, J=-1.0, proposal="small", delta=0.5)
     s = st.download(0).astype(np.float64)
