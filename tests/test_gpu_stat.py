@@ -135,6 +135,32 @@ def test_potts3_critical_energy_extrapolates_to_exact(ctx):
     assert chi2 < 10.0, (chi2, E, sd)
 
 
+def test_ising_graph_path_torus_vs_kaufman(ctx):
+    """The graph path (greedy-coloured CSR sweeps and CSR Wolff) on the 2D 32 x 32 torus given
+    as a graph, at beta_c: the energy equals Kaufman's exact finite-torus value within 4
+    standard errors."""
+    from marqov_amd import gpu, graphs
+    L, beta = 32, 0.4406868
+    rp, ci = graphs.regular_csr(2, L)
+    lat = gpu.CsrLattice(ctx, rp, ci)
+    R = 256
+    st = gpu.State(ctx, "ising", [lat] * R, J=-1.0)
+    st.set_beta(beta)
+    st.init(seed=6)
+    for t in range(200):
+        st.emcs(4, 1, 6, t)
+    e = np.zeros(R)
+    n = 800
+    for t in range(n):
+        st.emcs(4, 1, 6, 200 + t)
+        e += st.measure()[:, 0]
+    e /= n
+    se = e.std(ddof=1) / np.sqrt(R)
+    ka = exact.kaufman_energy(beta, L, L)
+    print(f"[graph torus] E/N = {e.mean():.5f} +- {se:.5f}  Kaufman {ka:.5f}  ({(e.mean() - ka) / se:.2f} sigma)", flush=True)
+    assert abs(e.mean() - ka) < 4 * se, (e.mean(), se, ka)
+
+
 def test_heisenberg_renormalize_keeps_unit_length(ctx):
     _, st = gpu_series(ctx, "heisenberg", 3, 8, 0.693, 2, 0, 50, J=-1.0, proposal="small", delta=0.5)
     s = st.download(0).astype(np.float64)
