@@ -171,6 +171,9 @@ def leg_c3(gpu, ctx):
                        embedding="cartesian")
         st.set_beta(beta)
         st.init(seed=3)
+        from marqov_amd import autotune
+        shape, _ = autotune.tune(ctx, st, cache_path=os.path.join("/tmp", f"marqov_amd_tuning_{os.getuid()}.json"),
+                                 force=True, sweeps=3)
         st.metropolis(5, 3, 0)
         ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
         w_rate, w_mean = wolff_rate(ctx, st, 30, 10, 10, 3, 10, sweep0=5 + nstep)
@@ -179,7 +182,7 @@ def leg_c3(gpu, ctx):
         st.metropolis(1, 3, sweep0=10 ** 6)
         acceptance = float(np.any(st.download(0) != before, axis=1).mean())
         res[name] = {"replicas": nrep, "attempts_per_ns": round(nrep * N * nstep / (ms * 1e6), 3),
-                     "delta": C3_DELTA, "acceptance": round(acceptance, 3),
+                     "delta": C3_DELTA, "acceptance": round(acceptance, 3), "launch_shape": shape,
                      "wolff_sites_per_ns": round(w_rate, 4),
                      "mean_cluster": round(w_mean, 1),
                      "roofline_frac": round(48.0 * nrep * N * nstep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
