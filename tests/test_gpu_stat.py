@@ -161,6 +161,38 @@ def test_ising_graph_path_torus_vs_kaufman(ctx):
     assert abs(e.mean() - ka) < 4 * se, (e.mean(), se, ka)
 
 
+def test_ising3d_binder_at_critical_point(ctx):
+    """3D Ising at beta_c = 0.2216546 (Ferrenberg-Xu-Landau 2018), L = 32 and 64 (the row-vectorised
+    headline kernel and Wolff): the Binder cumulant U = 1 - <m^4>/(3 <m^2>^2) lies at the universal
+    value U* = 0.46548 (Hasenbusch 2010) within 4 jackknife errors plus 0.01 for corrections to
+    scaling at these sizes (a sanity band: there is no closed form in 3D)."""
+    from marqov_amd import gpu
+    ustar = 0.46548
+    for L in (32, 64):
+        lat = gpu.RegularLattice(ctx, 3, L)
+        R, n = 128, 1000
+        st = gpu.State(ctx, "ising", [lat] * R, J=-1.0)
+        st.set_beta(0.2216546)
+        st.init(seed=8)
+        for t in range(200):
+            st.emcs(2, 1, 8, t)
+        m2 = np.zeros(R)
+        m4 = np.zeros(R)
+        for t in range(n):
+            st.emcs(2, 1, 8, 200 + t)
+            o = st.measure()
+            m2 += o[:, 2]
+            m4 += o[:, 3]
+        m2 /= n
+        m4 /= n
+        u = 1 - m4.mean() / (3 * m2.mean() ** 2)
+        jk = np.array([1 - np.delete(m4, i).mean() / (3 * np.delete(m2, i).mean() ** 2) for i in range(R)])
+        err = np.sqrt((R - 1) / R * ((jk - jk.mean()) ** 2).sum())
+        print(f"[3d ising binder] L={L}: U = {u:.5f} +- {err:.5f} (U* = {ustar})", flush=True)
+        assert abs(u - ustar) < 4 * err + 0.01, (L, u, err)
+        del st
+
+
 def test_heisenberg_renormalize_keeps_unit_length(ctx):
     _, st = gpu_series(ctx, "heisenberg", 3, 8, 0.693, 2, 0, 50, J=-1.0, proposal="small", delta=0.5)
     s = st.download(0).astype(np.float64)
