@@ -149,7 +149,9 @@ MQ_API int mq_metropolis(mq_state* st, int nsweeps, uint64_t seed, uint64_t swee
  * nrep*nclusters cluster sizes, replica-major. */
 MQ_API int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t cluster0, int64_t* sizes);
 /* one EMCS (emcs.h:69-109): ncluster Wolff moves then nmetro sweeps; writes
- * the per-replica mean cluster size (NaN when ncluster = 0, as emcs.h:108). */
+ * the per-replica mean cluster size (NaN when ncluster = 0, as emcs.h:108).
+ * Equal to mq_wolff(ncluster, cluster0 = step*ncluster) followed by
+ * mq_metropolis(nmetro, sweep0 = step*nmetro); negative counts are refused. */
 MQ_API int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, uint64_t step, double* mean_cluster);
 /* observables, FP64 (observables.h:32-66,173-352; Potts.h:40-65): out[r*MQ_NOBS+k] */
 MQ_API int mq_measure(mq_state* st, double* out);

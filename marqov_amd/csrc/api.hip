@@ -815,6 +815,7 @@ extern "C" int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, ui
 {
     int rc = check_state(st);
     if (rc) return rc;
+    if (ncluster < 0 || nmetro < 0) return mq_fail(st->ctx, MQ_ERR_ARG, "ncluster and nmetro must be >= 0");
     std::vector<int64_t> sizes;
     if (ncluster > 0) {
         sizes.resize((size_t)st->nrep * ncluster);

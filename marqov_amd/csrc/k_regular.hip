@@ -343,14 +343,16 @@ struct LdsGroup {
     int64_t off;     // byte offset of the group's first replica
     int64_t stride;  // bytes per replica
     int L;
-    uint32_t magic_h, magic_l;  // ceil(2^32 / (L/2)), ceil(2^32 / L): exact division for indices < 2^16
+    // ceil(2^32 / (L/2)), ceil(2^32 / L): exact division for indices < 2^16;
+    // L = 2 has no 32-bit magic for L/2 = 1 (magic_h = 0 there, row = h)
+    uint32_t magic_h, magic_l;
     int rep0, nrep, block0;
 };
 
 template <int DIM>
 __device__ __forceinline__ void lds_nbrs(int L, int H, uint32_t mh, uint32_t ml, int colour, int h, int nb[2 * DIM])
 {
-    const int row = (int)__umulhi((uint32_t)h, mh);
+    const int row = (H == 1) ? h : (int)__umulhi((uint32_t)h, mh);
     const int k = h - row * H;
     int x1 = row, x2 = 0;
     if (DIM == 3) {
@@ -974,7 +976,7 @@ hipError_t prepare_lds(mq_state* st)
         d.off = g.off;
         d.stride = g.stride;
         d.L = g.lat->L;
-        d.magic_h = (uint32_t)((((uint64_t)1 << 32) + (uint64_t)(g.lat->L / 2) - 1) / (uint64_t)(g.lat->L / 2));
+        d.magic_h = g.lat->L == 2 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)(g.lat->L / 2) - 1) / (uint64_t)(g.lat->L / 2));
         d.magic_l = (uint32_t)((((uint64_t)1 << 32) + (uint64_t)g.lat->L - 1) / (uint64_t)g.lat->L);
         d.rep0 = g.rep0;
         d.nrep = g.nrep;

@@ -1,0 +1,205 @@
+"""Edge cases of the update path on the GPU against the oracle.
+
+* The smallest lattice, L = 2: both x0 neighbours of a site are the same site, so
+  its bond is listed twice, as the reference's neighbour arithmetic lists it
+  (regular_hypercubic.h:31-66).  Sweeps on the LDS and the streaming kernels, Wolff
+  on the small-lattice and the workgroup kernels, and the observables.
+* Graphs with isolated vertices (degree 0), a hub of degree 127 (the largest the
+  8-replicas-per-lane kernels take) and of degree 128 (the per-replica kernels then
+  run), a single edge, and the complete graph K_64 (64 colour classes of one vertex,
+  the most a sweep's counter phase holds); K_65 is refused.
+* Zero counts are no-ops, negative counts are refused, and one mq_emcs equals
+  mq_wolff followed by mq_metropolis with the counters it documents
+  (emcs.h:69-109; mean cluster size NaN when ncluster = 0, emcs.h:108).
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from marqov_amd import gpu
+    return gpu.Context(0)
+
+
+@pytest.mark.parametrize("kernels", ["0", "1"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_L2_sweeps_bit_exact(ctx, dim, kernels, monkeypatch):
+    from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
+    L, N, R = 2, 2 ** dim, 4
+    lat = gpu.RegularLattice(ctx, dim, L)
+    betas = [0.2, 0.5, 1.0, 2.0]
+    for model in ("ising", "ea", "potts"):
+        st = gpu.State(ctx, model, [lat] * R, J=-1.0, q=3)
+        st.set_beta(betas)
+        if model == "ea":
+            st.random_bonds(rep_sample=np.arange(R, dtype=np.int32), p_plus=0.5, seed=4)
+        st.init(seed=7)
+        st.metropolis(6, seed=8, sweep0=1)
+        for r in range(R):
+            if model == "potts":
+                exp = ob.potts_sweeps(dim, L, ob.init_potts(N, 3, 7, r), 3, betas[r], -1.0, 6, 8, 1, r)
+            else:
+                b = ob.init_bonds(dim, N, 0.5, 4, r) if model == "ea" else None
+                exp = ob.ising_sweeps(dim, L, ob.init_ising(N, 7, r), betas[r], -1.0, 6, 8, 1, r, bonds=b)
+            assert np.array_equal(st.download(r), exp), (model, kernels, r)
+
+
+@pytest.mark.parametrize("kernels", ["0", "1"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_L2_wolff_and_measure(ctx, dim, kernels, monkeypatch):
+    from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
+    L, N = 2, 2 ** dim
+    lat = gpu.RegularLattice(ctx, dim, L)
+    for model in ("ising", "potts"):
+        st = gpu.State(ctx, model, [lat] * 3, J=-1.0, q=3)
+        st.set_beta(0.7)
+        st.init(seed=9)
+        sizes = st.wolff(8, seed=10, cluster0=2)
+        obs = st.measure()
+        for r in range(3):
+            if model == "ising":
+                exp, esz = ob.wolff_ising(dim, L, ob.init_ising(N, 9, r), 0.7, -1.0, 8, 10, 2, r)
+                e = -1.0 * ob.lib().orc_bond_sum_ising(dim, L, ob.p(exp), None) / N
+                m = abs(int(exp.astype(np.int64).sum())) / N
+            else:
+                exp, esz = ob.wolff_potts(dim, L, ob.init_potts(N, 3, 9, r), 3, 0.7, -1.0, 8, 10, 2, r)
+                e = -1.0 * ob.lib().orc_bond_sum_potts(dim, L, ob.p(exp)) / N
+                cnt = np.bincount(exp.astype(np.int64), minlength=3)
+                m = (3 * cnt.max() - N) / (N * 2)
+            assert np.array_equal(sizes[r], esz), (model, r, sizes[r], esz)
+            assert np.array_equal(st.download(r), exp), (model, r)
+            assert obs[r, 0] == pytest.approx(e, abs=1e-12)
+            assert obs[r, 1] == pytest.approx(m, abs=1e-12)
+
+
+def _csr(n, edges):
+    """Sorted symmetric CSR of an undirected simple graph on n vertices."""
+    e = np.asarray(edges, np.int64).reshape(-1, 2)
+    src = np.concatenate([e[:, 0], e[:, 1]])
+    dst = np.concatenate([e[:, 1], e[:, 0]])
+    order = np.lexsort((dst, src))
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, src + 1, 1)
+    return np.cumsum(rp).astype(np.int32), dst[order].astype(np.int32)
+
+
+def _odd_graph(hub_deg):
+    """A hub with hub_deg leaves, a 20-cycle, one lone edge and 5 isolated vertices,
+    with the vertex ids shuffled."""
+    edges = [(0, 1 + k) for k in range(hub_deg)]
+    c0 = hub_deg + 1
+    edges += [(c0 + k, c0 + (k + 1) % 20) for k in range(20)]
+    e0 = c0 + 20
+    edges.append((e0, e0 + 1))
+    n = e0 + 2 + 5
+    perm = np.random.default_rng(hub_deg).permutation(n)
+    return _csr(n, [(perm[a], perm[b]) for a, b in edges])
+
+
+def _colours(ctx, lat):
+    c = np.zeros(lat.N, np.int32)
+    ctx.check(ctx.lib.mq_lattice_color_of(lat.h, c.ctypes.data), "mq_lattice_color_of")
+    return c
+
+
+@pytest.mark.parametrize("hub_deg", [127, 128])
+@pytest.mark.parametrize("nrep", [1, 16])
+def test_odd_graph_sweeps_and_wolff_bit_exact(ctx, hub_deg, nrep):
+    """nrep = 16 takes the 8-replicas-per-lane sweeps when the hub has 127 leaves and
+    the per-replica sweeps at 128; nrep = 1 always takes the per-replica sweeps."""
+    from marqov_amd import gpu
+    rp, ci = _odd_graph(hub_deg)
+    N = len(rp) - 1
+    lat = gpu.CsrLattice(ctx, rp, ci)
+    col = _colours(ctx, lat)
+    betas = np.linspace(0.2, 1.6, nrep)
+    for model in ("ising", "potts"):
+        potts = model == "potts"
+        st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, q=3)
+        st.set_beta(betas)
+        st.init(seed=3)
+        st.metropolis(5, seed=4, sweep0=0)
+        starts = [st.download(r) for r in range(nrep)]
+        for r in range(nrep):
+            s0 = ob.init_potts(N, 3, 3, r) if potts else ob.init_ising(N, 3, r)
+            exp = ob.csr_sweeps(rp, ci, col, s0, betas[r], -1.0, 5, 4, 0, r, potts=potts)
+            assert np.array_equal(starts[r], exp), (model, hub_deg, nrep, r)
+        sizes = st.wolff(12, seed=5, cluster0=0)
+        for r in range(nrep):
+            exp, esz = ob.csr_wolff(rp, ci, starts[r], betas[r], -1.0, 12, 5, 0, r, potts=potts)
+            assert np.array_equal(sizes[r], esz), (model, hub_deg, nrep, r)
+            assert np.array_equal(st.download(r), exp), (model, hub_deg, nrep, r)
+        assert sizes.min() >= 1
+
+
+def test_complete_graph_64_colours_and_65_refused(ctx):
+    from marqov_amd import gpu
+    n = 64
+    rp, ci = _csr(n, [(a, b) for a in range(n) for b in range(a + 1, n)])
+    lat = gpu.CsrLattice(ctx, rp, ci)
+    assert len(lat.colors()) == 64
+    col = _colours(ctx, lat)
+    st = gpu.State(ctx, "ising", [lat] * 2, J=-1.0)
+    st.set_beta([0.01, 0.05])
+    st.init(seed=1)
+    st.metropolis(3, seed=2, sweep0=5)
+    for r in range(2):
+        exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(n, 1, r), [0.01, 0.05][r], -1.0, 3, 2, 5, r)
+        assert np.array_equal(st.download(r), exp), r
+    rp65, ci65 = _csr(65, [(a, b) for a in range(65) for b in range(a + 1, 65)])
+    with pytest.raises(gpu.MarqovError, match="more than 64 colours"):
+        gpu.CsrLattice(ctx, rp65, ci65)
+
+
+def test_zero_and_negative_counts(ctx):
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, 2, 16)
+    st = gpu.State(ctx, "ising", [lat] * 2, J=-1.0)
+    st.set_beta(0.44)
+    st.init(seed=3)
+    s0 = [st.download(r) for r in range(2)]
+    st.metropolis(0, seed=1)
+    assert st.wolff(0, seed=1).shape == (2, 0)
+    mc = st.emcs(0, 0, seed=1, step=0)
+    assert np.all(np.isnan(mc))
+    for r in range(2):
+        assert np.array_equal(st.download(r), s0[r])
+    for bad in (lambda: st.metropolis(-1, seed=1), lambda: st.wolff(-1, seed=1, sizes=False),
+                lambda: st.emcs(-1, 1, seed=1, step=0), lambda: st.emcs(1, -1, seed=1, step=0)):
+        with pytest.raises(gpu.MarqovError):
+            bad()
+    for r in range(2):
+        assert np.array_equal(st.download(r), s0[r])
+
+
+@pytest.mark.parametrize("model", ["ising", "heisenberg"])
+def test_emcs_is_wolff_then_metropolis(ctx, model):
+    """Two states from the same start: mq_emcs(nc, nm, step) on one, mq_wolff(nc,
+    cluster0 = step*nc) then mq_metropolis(nm, sweep0 = step*nm) on the other: the
+    states are identical and the mean cluster size is the mean of the sizes."""
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, 3, 8)
+    kw = dict(J=-1.0) if model == "ising" else dict(J=-1.0, proposal="small", delta=1.0)
+    a = gpu.State(ctx, model, [lat] * 3, **kw)
+    b = gpu.State(ctx, model, [lat] * 3, **kw)
+    for st in (a, b):
+        st.set_beta([0.2, 0.22, 0.7] if model == "ising" else [0.5, 0.69, 0.9])
+        st.init(seed=6)
+    nc, nm = 3, 2
+    means = []
+    sizes = []
+    for step in range(4):
+        means.append(a.emcs(nc, nm, seed=7, step=step))
+        sizes.append(b.wolff(nc, seed=7, cluster0=step * nc))
+        b.metropolis(nm, seed=7, sweep0=step * nm)
+    for step in range(4):
+        assert np.array_equal(means[step], sizes[step].mean(axis=1)), step
+    for r in range(3):
+        assert np.array_equal(a.download(r), b.download(r)), r
