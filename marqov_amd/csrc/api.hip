@@ -258,7 +258,8 @@ extern "C" int64_t mq_lattice_nsites(const mq_lattice* lat) { return lat ? lat->
 
 extern "C" int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes64)
 {
-    if (!lat || !ncolors) return MQ_ERR_ARG;
+    if (!lat) return MQ_ERR_ARG;
+    if (!ncolors) return mq_fail(lat->ctx, MQ_ERR_ARG, "mq_lattice_colors: ncolors is NULL");
     *ncolors = lat->ncolors;
     if (sizes64)
         for (int c = 0; c < lat->ncolors; ++c) sizes64[c] = lat->color_off[c + 1] - lat->color_off[c];
@@ -284,8 +285,8 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     *out = nullptr;
     const int model = params->model;
     if (model < MQ_ISING || model > MQ_PHI4) return mq_fail(ctx, MQ_ERR_ARG, "unknown model");
-    if (model == MQ_POTTS && (params->q < 2 || params->q > 7))
-        return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "Potts: q must be in [2,7]");
+    if (model == MQ_POTTS && (params->q < 2 || params->q > 8))
+        return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "Potts: q must be in [2,8]");
     const int kind0 = lattices[0] ? lattices[0]->kind : -1;
     const int dim0 = lattices[0] ? lattices[0]->dim : -1;
     for (int r = 0; r < nrep; ++r) {
@@ -487,7 +488,8 @@ static int upload_tables(mq_state* st)
 
 extern "C" int mq_state_set_beta(mq_state* st, const double* beta)
 {
-    if (!st || !beta) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!beta) return mq_fail(st->ctx, MQ_ERR_ARG, "beta is NULL");
     for (int r = 0; r < st->nrep; ++r)
         if (!(beta[r] >= 0.0) || !std::isfinite(beta[r])) return mq_fail(st->ctx, MQ_ERR_ARG, "beta must be finite and >= 0");
     st->beta.assign(beta, beta + st->nrep);
@@ -497,7 +499,9 @@ extern "C" int mq_state_set_beta(mq_state* st, const double* beta)
 
 extern "C" int mq_state_get_beta(const mq_state* st, double* beta)
 {
-    if (!st || !beta || st->beta.empty()) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!beta) return mq_fail(st->ctx, MQ_ERR_ARG, "beta is NULL");
+    if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set (call mq_state_set_beta)");
     // current temperature of every replica (parallel tempering moves them)
     MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
     MQ_HIP(st->ctx, hipMemcpy(beta, st->d_betad, sizeof(double) * st->nrep, hipMemcpyDeviceToHost));
@@ -529,7 +533,8 @@ static int set_samples(mq_state* st, const int32_t* rep_sample)
 
 extern "C" int mq_state_set_samples(mq_state* st, const int32_t* rep_sample)
 {
-    if (!st || !rep_sample) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!rep_sample) return mq_fail(st->ctx, MQ_ERR_ARG, "rep_sample is NULL");
     int rc = set_samples(st, rep_sample);
     if (rc) return rc;
     return ensure_bond_storage(st);
@@ -561,7 +566,9 @@ extern "C" int mq_state_random_bonds(mq_state* st, const int32_t* rep_sample, do
 
 extern "C" int mq_state_set_bonds(mq_state* st, int sample, const int8_t* bonds)
 {
-    if (!st || !bonds || sample < 0 || sample >= st->nsamples) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!bonds) return mq_fail(st->ctx, MQ_ERR_ARG, "bonds is NULL");
+    if (sample < 0 || sample >= st->nsamples) return mq_fail(st->ctx, MQ_ERR_ARG, "sample out of range");
     int rc = ensure_bond_storage(st);
     if (rc) return rc;
     int r0 = -1;
@@ -645,7 +652,9 @@ static int copy_replica(mq_state* st, int rep, void* host, size_t elem, bool to_
 
 extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t bytes)
 {
-    if (!st || !host || rep < 0 || rep >= st->nrep) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!host) return mq_fail(st->ctx, MQ_ERR_ARG, "host buffer is NULL");
+    if (rep < 0 || rep >= st->nrep) return mq_fail(st->ctx, MQ_ERR_ARG, "replica index out of range");
     const mq_lattice* l = st->lat[rep];
     const int64_t N = l->N;
     if (is_vec(st->p.model)) {
@@ -701,7 +710,9 @@ extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t 
 
 extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t bytes)
 {
-    if (!st || !host || rep < 0 || rep >= st->nrep) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!host) return mq_fail(st->ctx, MQ_ERR_ARG, "host buffer is NULL");
+    if (rep < 0 || rep >= st->nrep) return mq_fail(st->ctx, MQ_ERR_ARG, "replica index out of range");
     const mq_lattice* l = st->lat[rep];
     const int64_t N = l->N;
     MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
@@ -865,7 +876,8 @@ static int measure_sums(mq_state* st)
 
 extern "C" int mq_measure(mq_state* st, double* out)
 {
-    if (!st || !out) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!out) return mq_fail(st->ctx, MQ_ERR_ARG, "out is NULL");
     if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
     int rc = measure_sums(st);
     if (rc) return rc;
@@ -881,8 +893,13 @@ extern "C" int mq_measure(mq_state* st, double* out)
         double e = 0, m = 0;
         if (st->p.model == MQ_POTTS) {
             e = st->p.J * (double)I[0] / N;
-            long long mx = 0;
-            for (int q = 0; q < st->p.q; ++q) mx = std::max(mx, I[1 + q]);
+            // the kernels count colours 0..6; colour 7 (q = 8) is what they leave
+            long long mx = 0, counted = 0;
+            for (int q = 0; q < std::min(st->p.q, 7); ++q) {
+                mx = std::max(mx, I[1 + q]);
+                counted += I[1 + q];
+            }
+            if (st->p.q == 8) mx = std::max(mx, (long long)st->lat[r]->N - counted);
             m = ((double)st->p.q * (double)mx - N) / (N * (st->p.q - 1));  // Potts.h:40-65
         } else if (is_int(st->p.model)) {
             e = st->p.J * (double)I[0] / N;
@@ -905,7 +922,8 @@ extern "C" int mq_measure(mq_state* st, double* out)
 
 extern "C" int mq_energy(mq_state* st, double* out)
 {
-    if (!st || !out) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!out) return mq_fail(st->ctx, MQ_ERR_ARG, "out is NULL");
     std::vector<double> obs((size_t)st->nrep * MQ_NOBS);
     int rc = mq_measure(st, obs.data());
     if (rc) return rc;
@@ -915,7 +933,8 @@ extern "C" int mq_energy(mq_state* st, double* out)
 
 extern "C" int mq_state_set_tuning(mq_state* st, const mq_tuning* t)
 {
-    if (!st || !t) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!t) return mq_fail(st->ctx, MQ_ERR_ARG, "tuning is NULL");
     auto pow2 = [](int v) { return v >= 64 && v <= 512 && (v & (v - 1)) == 0; };
     if (t->kernels < 0 || t->kernels > 2 || !pow2(t->block) || !pow2(t->lds_block) || (t->xcd_remap != 0 && t->xcd_remap != 1))
         return mq_fail(st->ctx, MQ_ERR_ARG, "tuning: kernels in 0..2, block and lds_block powers of two in 64..512");
@@ -928,7 +947,8 @@ extern "C" int mq_state_set_tuning(mq_state* st, const mq_tuning* t)
 
 extern "C" int mq_state_get_tuning(const mq_state* st, mq_tuning* t)
 {
-    if (!st || !t) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!t) return mq_fail(st->ctx, MQ_ERR_ARG, "tuning is NULL");
     t->kernels = st->kernel_choice;
     t->block = st->tune_block;
     t->xcd_remap = st->tune_xcd;

@@ -712,7 +712,9 @@ inline dim3 grid_for(int64_t work, int nrep) { return dim3((unsigned)((work + 25
 // ------------------------------------------------------------------ host
 extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, const int32_t* col_idx, mq_lattice** out)
 {
-    if (!ctx || !out || !row_ptr || !col_idx || N <= 0) return MQ_ERR_ARG;
+    if (!ctx) return MQ_ERR_ARG;
+    if (!out || !row_ptr || !col_idx) return mq_fail(ctx, MQ_ERR_ARG, "mq_lattice_csr: NULL argument");
+    if (N <= 0) return mq_fail(ctx, MQ_ERR_ARG, "mq_lattice_csr: N must be > 0");
     *out = nullptr;
     if (N >= ((int64_t)1 << 31)) return mq_fail(ctx, MQ_ERR_ARG, "graph too large");
     if (row_ptr[0] != 0) return mq_fail(ctx, MQ_ERR_ARG, "row_ptr[0] must be 0");
@@ -850,7 +852,8 @@ extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, co
 
 extern "C" int mq_lattice_color_of(const mq_lattice* lat, int32_t* colour)
 {
-    if (!lat || !colour) return MQ_ERR_ARG;
+    if (!lat) return MQ_ERR_ARG;
+    if (!colour) return mq_fail(lat->ctx, MQ_ERR_ARG, "mq_lattice_color_of: colour is NULL");
     if (lat->kind == LAT_CSR) {
         std::copy(lat->colour.begin(), lat->colour.end(), colour);
     } else {

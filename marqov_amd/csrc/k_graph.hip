@@ -160,7 +160,8 @@ struct DevBuf {
 extern "C" int mq_rgg_build(mq_ctx* ctx, int dim, int64_t n, double radius, uint64_t seed, int64_t* nnz,
                             int32_t* row_ptr, int32_t* col_idx, int64_t col_cap, double* coords)
 {
-    if (!ctx || !nnz) return MQ_ERR_ARG;
+    if (!ctx) return MQ_ERR_ARG;
+    if (!nnz) return mq_fail(ctx, MQ_ERR_ARG, "random geometric graph: nnz is NULL");
     if (dim != 2 && dim != 3) return mq_fail(ctx, MQ_ERR_ARG, "random geometric graph: dim must be 2 or 3");
     if (n < 2 || n > ((int64_t)1 << 30)) return mq_fail(ctx, MQ_ERR_ARG, "random geometric graph: 2 <= n <= 2^30");
     if (!(radius > 0.0 && radius < 0.5))

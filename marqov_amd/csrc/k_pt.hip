@@ -237,7 +237,8 @@ extern "C" int mq_pt_swap(mq_state* st, int ntemp, int parity, uint64_t seed, ui
 extern "C" int mq_pt_swap_pairs(mq_state* st, int ntemp, int npairs, const int32_t* pairs, uint64_t seed, uint64_t step,
                                 int32_t* accepted)
 {
-    if (!st || npairs < 0 || (npairs > 0 && !pairs)) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (npairs < 0 || (npairs > 0 && !pairs)) return mq_fail(st->ctx, MQ_ERR_ARG, "pairs: npairs < 0 or pairs is NULL");
     if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
     int rc = ensure_pt(st, ntemp);
     if (rc) return rc;
@@ -290,7 +291,8 @@ extern "C" int mq_pt_swap_pairs(mq_state* st, int ntemp, int npairs, const int32
 
 extern "C" int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep)
 {
-    if (!st || !temp_of_rep) return MQ_ERR_ARG;
+    if (!st) return MQ_ERR_ARG;
+    if (!temp_of_rep) return mq_fail(st->ctx, MQ_ERR_ARG, "temp_of_rep is NULL");
     if (!st->d_temp_of_rep) {
         for (int r = 0; r < st->nrep; ++r) temp_of_rep[r] = -1;
         return MQ_OK;

@@ -599,6 +599,7 @@ __device__ __forceinline__ void block_sum_atomic(T (&v)[K], T* out)
 
 // isum[r*8 + 0] = sum over bonds of b s_i s_j (Ising/EA) or #equal bonds (Potts)
 // isum[r*8 + 1] = sum s (Ising/EA); Potts: isum[r*8 + 1 + q] = count of colour q
+// for q < 7 (with q = 8 colours, colour 7 is N minus the others)
 template <int DIM, bool BONDS>
 __global__ void __launch_bounds__(256) k_measure_int8(const int8_t* __restrict__ base, int64_t stride,
                                                       const int8_t* __restrict__ bonds,

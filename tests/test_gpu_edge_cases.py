@@ -8,6 +8,8 @@
   8-replicas-per-lane kernels take) and of degree 128 (the per-replica kernels then
   run), a single edge, and the complete graph K_64 (64 colour classes of one vertex,
   the most a sweep's counter phase holds); K_65 is refused.
+* Potts with q = 8, the largest Potts<Q> the reference instantiates, on lattices
+  and graphs (sweeps, Wolff, observables).
 * Zero counts are no-ops, negative counts are refused, and one mq_emcs equals
   mq_wolff followed by mq_metropolis with the counters it documents
   (emcs.h:69-109; mean cluster size NaN when ncluster = 0, emcs.h:108).
@@ -203,3 +205,61 @@ def test_emcs_is_wolff_then_metropolis(ctx, model):
         assert np.array_equal(means[step], sizes[step].mean(axis=1)), step
     for r in range(3):
         assert np.array_equal(a.download(r), b.download(r)), r
+
+
+def _potts_obs(s, q, bond_equal, N):
+    cnt = np.bincount(s.astype(np.int64), minlength=q)
+    return -1.0 * bond_equal / N, (q * cnt.max() - N) / (N * (q - 1))
+
+
+@pytest.mark.parametrize("dim,L", [(2, 8), (3, 4)])
+def test_potts8_regular_bit_exact(ctx, dim, L):
+    """q = 8, the largest Potts<Q> the reference instantiates (demo/Potts.cpp:88): the
+    kernels count colours 0..6 and the eighth count is what they leave."""
+    from marqov_amd import gpu
+    N, q, betas = L ** dim, 8, [1.2, 2.5]
+    lat = gpu.RegularLattice(ctx, dim, L)
+    st = gpu.State(ctx, "potts", [lat] * 2, J=-1.0, q=q)
+    st.set_beta(betas)
+    st.init(seed=5)
+    st.metropolis(4, seed=6, sweep0=0)
+    sizes = st.wolff(5, seed=7, cluster0=0)
+    obs = st.measure()
+    for r in range(2):
+        s1 = ob.potts_sweeps(dim, L, ob.init_potts(N, q, 5, r), q, betas[r], -1.0, 4, 6, 0, r)
+        exp, esz = ob.wolff_potts(dim, L, s1, q, betas[r], -1.0, 5, 7, 0, r)
+        assert np.array_equal(sizes[r], esz), r
+        got = st.download(r)
+        assert np.array_equal(got, exp), r
+        assert got.min() >= 0 and got.max() <= 7
+        e, m = _potts_obs(exp, q, ob.lib().orc_bond_sum_potts(dim, L, ob.p(exp)), N)
+        assert obs[r, 0] == pytest.approx(e, abs=1e-12)
+        assert obs[r, 1] == pytest.approx(m, abs=1e-12)
+
+
+@pytest.mark.parametrize("nrep", [2, 16])
+def test_potts8_graph_bit_exact(ctx, nrep):
+    """q = 8 on a graph with the per-replica (2 replicas) and the 8-replicas-per-lane
+    (16 replicas) sweeps, Wolff, and the observables."""
+    from marqov_amd import gpu, graphs
+    rp, ci, _ = graphs.rgg2d(1500, graphs.rgg_radius(1500), 12)
+    N, q = len(rp) - 1, 8
+    lat = gpu.CsrLattice(ctx, rp, ci)
+    col = _colours(ctx, lat)
+    betas = np.linspace(0.8, 2.4, nrep)
+    st = gpu.State(ctx, "potts", [lat] * nrep, J=-1.0, q=q)
+    st.set_beta(betas)
+    st.init(seed=2)
+    st.metropolis(3, seed=3, sweep0=1)
+    sizes = st.wolff(4, seed=4, cluster0=0)
+    obs = st.measure()
+    src = np.repeat(np.arange(N), np.diff(rp))
+    for r in range(nrep):
+        s1 = ob.csr_sweeps(rp, ci, col, ob.init_potts(N, q, 2, r), betas[r], -1.0, 3, 3, 1, r, potts=True, Q=q)
+        exp, esz = ob.csr_wolff(rp, ci, s1, betas[r], -1.0, 4, 4, 0, r, potts=True, Q=q)
+        assert np.array_equal(sizes[r], esz), r
+        assert np.array_equal(st.download(r), exp), r
+        eq = int((exp[src] == exp[ci]).sum()) // 2
+        e, m = _potts_obs(exp, q, eq, N)
+        assert obs[r, 0] == pytest.approx(e, abs=1e-12)
+        assert obs[r, 1] == pytest.approx(m, abs=1e-12)

@@ -109,3 +109,26 @@ def test_calls_in_the_wrong_state_refused(ctx):
     st.metropolis(2, seed=1)
     ea.metropolis(2, seed=1)
     assert np.all(np.isfinite(st.measure())) and np.all(np.isfinite(ea.measure()))
+
+
+def test_refusals_carry_their_own_message(ctx):
+    """An argument refused on a valid handle leaves its own message, not the previous one."""
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, 2, 8)
+    st = gpu.State(ctx, "ising", [lat] * 2)
+    buf = np.zeros(64, np.int8)
+    b = np.zeros(2, np.float64)
+    lib = ctx.lib
+    assert lib.mq_state_get_beta(st.h, gpu._ptr(b)) == -3
+    assert b"beta not set" in lib.mq_last_error(ctx.h)
+    st.set_beta(0.4)
+    assert lib.mq_state_download(st.h, 5, gpu._ptr(buf), buf.nbytes) != 0
+    assert b"replica index out of range" in lib.mq_last_error(ctx.h)
+    assert lib.mq_state_upload(st.h, 0, None, buf.nbytes) != 0
+    assert b"host buffer is NULL" in lib.mq_last_error(ctx.h)
+    acc = np.zeros(1, np.int32)
+    assert lib.mq_pt_swap_pairs(st.h, 2, -1, None, 1, 0, gpu._ptr(acc)) != 0
+    assert b"npairs < 0" in lib.mq_last_error(ctx.h)
+    assert lib.mq_lattice_csr(ctx.h, 0, gpu._ptr(np.zeros(1, np.int32)), gpu._ptr(np.zeros(1, np.int32)),
+                              None) != 0
+    assert b"NULL argument" in lib.mq_last_error(ctx.h)
