@@ -155,6 +155,13 @@ MQ_API int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t cluster
 MQ_API int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, uint64_t step, double* mean_cluster);
 /* observables, FP64 (observables.h:32-66,173-352; Potts.h:40-65): out[r*MQ_NOBS+k] */
 MQ_API int mq_measure(mq_state* st, double* out);
+/* Fourier component of the magnetisation at the smallest wavevector along each
+ * axis, MagFTComp (observables.h:104-170; the reference measures it for Ising,
+ * axes 0 and 1, Ising.h:82-94, and for Phi4, axes 0-2, Phi4.h:116-140):
+ * out[r*3 + d] = sum_j |(1/N) sum_i s_i[j] exp(2 pi i x_d)|^2 with the site
+ * coordinate x_d = (index_d + 1/2)/L (regular_hypercubic.h:73-85), NaN for
+ * d >= dim.  Ising / EA / Heisenberg / Phi4 on regular lattices, FP64. */
+MQ_API int mq_measure_ft(mq_state* st, double* out);
 /* total energy per replica (sum over bonds, FP64), as Core::calcAction/beta (core.h:1145-1151) */
 MQ_API int mq_energy(mq_state* st, double* out);
 /* FP64 renormalisation of O(3) spins to unit length */

@@ -434,6 +434,7 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_rep_site_off);
     dfree(st->d_isum);
     dfree(st->d_fsum);
+    dfree(st->d_ft);
     dfree(st->d_temp_of_rep);
     dfree(st->d_rep_of_temp);
     dfree(st->d_pt_acc);
@@ -916,6 +917,48 @@ extern "C" int mq_measure(mq_state* st, double* out)
         out[(size_t)r * MQ_NOBS + 1] = m;
         out[(size_t)r * MQ_NOBS + 2] = m * m;
         out[(size_t)r * MQ_NOBS + 3] = m * m * m * m;
+    }
+    return MQ_OK;
+}
+
+extern "C" int mq_measure_ft(mq_state* st, double* out)
+{
+    if (!st) return MQ_ERR_ARG;
+    if (!out) return mq_fail(st->ctx, MQ_ERR_ARG, "out is NULL");
+    if (st->p.model == MQ_POTTS)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED,
+                       "MagFTComp: Ising, EA, Heisenberg and Phi4 spins only (the reference measures none for Potts)");
+    if (st->msc || st->eabits)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "MagFTComp: int8 or float4 storage only (spin_bits = 8)");
+    for (const auto* l : st->lat)
+        if (l->kind != LAT_REGULAR)
+            return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "MagFTComp needs site coordinates (regular lattices)");
+    hipStream_t s = st->ctx->stream;
+    const int64_t n = (int64_t)st->nrep * 18;
+    if (!st->d_ft) MQ_HIP(st->ctx, dalloc(&st->d_ft, (size_t)n));
+    MQ_HIP(st->ctx, zero64(st->d_ft, n, s));
+    for (const auto& g : st->groups) {
+        hipError_t e = launch_measure_ft_regular(st, g);
+        if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
+    }
+    std::vector<double> h((size_t)n);
+    MQ_HIP(st->ctx, hipMemcpyAsync(h.data(), st->d_ft, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+    MQ_HIP(st->ctx, hipStreamSynchronize(s));
+    for (int r = 0; r < st->nrep; ++r) {
+        const double N = (double)st->lat[r]->N;
+        for (int d = 0; d < 3; ++d) {
+            double v = std::numeric_limits<double>::quiet_NaN();
+            if (d < st->lat[r]->dim) {
+                // |(1/N) sum_i s_i[j] e^{2 pi i x_d}|^2 summed over components (observables.h:127-140)
+                v = 0.0;
+                for (int j = 0; j < 3; ++j) {
+                    const double re = h[(size_t)r * 18 + (size_t)(d * 3 + j) * 2] / N;
+                    const double im = h[(size_t)r * 18 + (size_t)(d * 3 + j) * 2 + 1] / N;
+                    v += re * re + im * im;
+                }
+            }
+            out[(size_t)r * 3 + d] = v;
+        }
     }
     return MQ_OK;
 }

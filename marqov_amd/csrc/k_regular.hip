@@ -766,6 +766,55 @@ __global__ void __launch_bounds__(256) k_measure_f4(const float4* __restrict__ b
     block_sum_atomic<double, 6>(acc, fsum + (int64_t)grep * 8);
 }
 
+// MagFTComp sums (observables.h:104-170, with the reference's site coordinates
+// (x + 1/2)/L, regular_hypercubic.h:73-85): ft[r*18 + (d*3 + j)*2 + {0, 1}] =
+// sum_i s_i[j] * {cos, sin}(2 pi (x_d + 1/2) / L) for axes d < DIM and spin
+// components j < NC (1 for int8 spins, 3 for float4), over both colours, in FP64.
+template <int DIM, bool VEC>
+__global__ void __launch_bounds__(256) k_measure_ft(const char* __restrict__ base, int64_t stride, int L, int rep0,
+                                                    double* __restrict__ ft)
+{
+    constexpr int NC = VEC ? 3 : 1;
+    const int r = blockIdx.y;
+    const int grep = rep0 + r;
+    const RegGeom G = reg_geom(DIM, L);
+    double acc[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) acc[k] = 0.0;
+    const double twoL = 2.0 / (double)L;
+    const char* rb = base + r * stride;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * G.half;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int c = t >= G.half ? 1 : 0;
+        const int64_t h = t - c * G.half;
+        const int64_t row = h / G.H;
+        const int k = (int)(h - row * G.H);
+        const int x1 = (int)(row % G.L);
+        const int x2 = (DIM == 3) ? (int)(row / G.L) : 0;
+        const int x[3] = {2 * k + ((c + x1 + x2) & 1), x1, x2};
+        double v[3] = {0.0, 0.0, 0.0};
+        if (VEC) {
+            const float4 s = reinterpret_cast<const float4*>(rb)[t];
+            v[0] = s.x;
+            v[1] = s.y;
+            v[2] = s.z;
+        } else {
+            v[0] = (double)reinterpret_cast<const int8_t*>(rb)[t];
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            double sn, cs;
+            sincospi(((double)x[d] + 0.5) * twoL, &sn, &cs);
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                acc[(d * 3 + j) * 2] += v[j] * cs;
+                acc[(d * 3 + j) * 2 + 1] += v[j] * sn;
+            }
+        }
+    }
+    block_sum_atomic<double, 18>(acc, ft + (int64_t)grep * 18);
+}
+
 // EA couplings for one sample: natural bond b = i*dim + d drawn from
 // (seed, sample, ST_DISORDER), +1 with probability T/2^32.
 template <int DIM>
@@ -892,6 +941,26 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
                                    st->d_rep_bond_off, st->d_betaf, L, model, st->p.proposal, J, delta, lam, mass,
                                    colour, phase, st->cur_dyn, g.rep0, st->tune_xcd, key);
         }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_measure_ft_regular(mq_state* st, const Group& g)
+{
+    const int L = g.lat->L, dim = g.lat->dim;
+    const int64_t N = g.lat->N;
+    const int64_t per_rep = std::max<int64_t>(1, std::min<int64_t>((N + 255) / 256,
+                                                                   (8ll * st->ctx->num_cus + g.nrep - 1) / g.nrep));
+    const dim3 grid((unsigned)per_rep, (unsigned)g.nrep, 1);
+    const char* b = st->d_spins + g.off;
+    hipStream_t s = st->ctx->stream;
+    const bool vec = st->site_bytes == 16;
+    if (dim == 3) {
+        if (vec) hipLaunchKernelGGL((k_measure_ft<3, true>), grid, dim3(256), 0, s, b, g.stride, L, g.rep0, st->d_ft);
+        else hipLaunchKernelGGL((k_measure_ft<3, false>), grid, dim3(256), 0, s, b, g.stride, L, g.rep0, st->d_ft);
+    } else {
+        if (vec) hipLaunchKernelGGL((k_measure_ft<2, true>), grid, dim3(256), 0, s, b, g.stride, L, g.rep0, st->d_ft);
+        else hipLaunchKernelGGL((k_measure_ft<2, false>), grid, dim3(256), 0, s, b, g.stride, L, g.rep0, st->d_ft);
     }
     return hipGetLastError();
 }

@@ -253,6 +253,7 @@ struct mq_state {
     // measurement
     long long* d_isum = nullptr;  // [nrep][8]
     double* d_fsum = nullptr;     // [nrep][8]
+    double* d_ft = nullptr;       // [nrep][3 axes][3 components][re, im] MagFTComp sums
     // parallel tempering
     int32_t* d_temp_of_rep = nullptr;  // [nrep]
     int32_t* d_rep_of_temp = nullptr;  // [nrep] (grouped by ntemp)
@@ -303,6 +304,7 @@ int mq_fail(mq_ctx* ctx, int code, const std::string& msg);
 hipError_t launch_init_regular(mq_state* st, const Group& g, Key2 key);
 hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
 hipError_t launch_measure_regular(mq_state* st, const Group& g);
+hipError_t launch_measure_ft_regular(mq_state* st, const Group& g);
 hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
 hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key);
 hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key);

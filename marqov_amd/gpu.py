@@ -201,6 +201,12 @@ class State:
         self.ctx.check(self.ctx.lib.mq_measure(self.h, _ptr(out)), "mq_measure")
         return out
 
+    def measure_ft(self):
+        """MagFTComp per replica and axis (observables.h:104-170): (nrep, 3), NaN for axes >= dim."""
+        out = np.zeros((self.nrep, 3), np.float64)
+        self.ctx.check(self.ctx.lib.mq_measure_ft(self.h, _ptr(out)), "mq_measure_ft")
+        return out
+
     def energy(self):
         out = np.zeros(self.nrep, np.float64)
         self.ctx.check(self.ctx.lib.mq_energy(self.h, _ptr(out)), "mq_energy")

@@ -362,3 +362,26 @@ def test_graph_replay_planted_error_is_caught():
     e, _ = _graph_chain(rp, ci, col, 1.0, +1.0, 4000, 11, 6, True)
     me, se = _mean_err(-e[200:])  # energies of the sign-flipped chain, measured with J = -1
     assert abs(me - ex["e"]) > 10 * se, (me, se, ex["e"])
+
+
+def test_magft_restatement_vs_fft_and_stripe():
+    """oracle/observables.py's MagFTComp against numpy's FFT of the axis profile (an
+    independent algorithm) and the half-and-half stripe's closed form."""
+    import observables
+    rng = np.random.default_rng(3)
+    for dim, L in [(2, 6), (3, 4), (2, 16)]:
+        N = L ** dim
+        shape = (L,) * dim  # numpy order: axis 0 slowest, so natural axis d is numpy axis dim-1-d
+        for ncomp in (1, 3):
+            s = rng.normal(size=(N, ncomp)) if ncomp == 3 else np.where(rng.random((N, 1)) < 0.5, 1.0, -1.0)
+            for d in range(dim):
+                prof = s.reshape(shape + (ncomp,)).sum(axis=tuple(a for a in range(dim) if a != dim - 1 - d))
+                f = np.fft.fft(prof, axis=0)[1] / N
+                assert observables.magft(s, dim, L, d) == pytest.approx(float(np.sum(np.abs(f) ** 2)), rel=1e-12, abs=1e-15)
+    for dim, L in [(2, 8), (3, 16)]:
+        N = L ** dim
+        x0 = np.arange(N) % L
+        stripe = np.where(x0 < L // 2, 1.0, -1.0)
+        assert observables.magft(stripe, dim, L, 0) == pytest.approx((2.0 / (L * np.sin(np.pi / L))) ** 2, rel=1e-12)
+        assert observables.magft(stripe, dim, L, 1) == pytest.approx(0.0, abs=1e-28)
+        assert observables.magft(np.ones(N), dim, L, dim - 1) == pytest.approx(0.0, abs=1e-28)
