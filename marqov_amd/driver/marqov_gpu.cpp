@@ -49,7 +49,9 @@
 // <out>/<L>/beta<beta>_<repid>.txt (the reference's outpath and defaultfilter
 // name, util/filters.h:28-37, with .txt for .h5), holding a commented header
 // and one row per measurement: step, E/N, |m|, m^2, m^4, mean cluster size of
-// that EMCS.  <out>/<L>/summary.txt holds per-simulation means.
+// that EMCS, and for Ising / Phi4 on regular lattices the MagFTComp columns of
+// their reference observable tuples (mq_measure_ft).  <out>/<L>/summary.txt
+// holds per-simulation means.
 #include <sys/stat.h>
 
 #include <cerrno>
@@ -288,18 +290,33 @@ void print_usage()
                  "               Potts<q>, EdwardsAnderson-Ising\n");
 }
 
-void write_sim(const Plan& pl, const Spec& s, const Sim& sim, uint64_t seed, const std::vector<double>& obs,
-               const std::vector<double>& cl, int nrep, int r, const std::string& path, int block, uint64_t step0)
+// MagFTComp columns of the reference's observable tuples: Ising axes 0 and 1
+// (Ising.h:82-94), Phi4 every axis (Phi4.h:116-140); regular lattices only
+int ft_columns(const Plan& pl, const Spec& s)
 {
+    if (pl.lattice != "regular") return 0;
+    return s.model == MQ_ISING ? 2 : (s.model == MQ_PHI4 ? pl.dim : 0);
+}
+
+void write_row(FILE* fp, unsigned long long step, const double* o, double cl, const double* ft, int nft)
+{
+    std::fprintf(fp, "%llu %.17g %.17g %.17g %.17g %.17g", step, o[0], o[1], o[2], o[3], cl);
+    for (int d = 0; d < nft; ++d) std::fprintf(fp, " %.17g", ft[d]);
+    std::fprintf(fp, "\n");
+}
+
+void write_sim(const Plan& pl, const Spec& s, const Sim& sim, uint64_t seed, const std::vector<double>& obs,
+               const std::vector<double>& cl, const std::vector<double>& ft, int nrep, int r, const std::string& path,
+               int block, uint64_t step0)
+{
+    const int nft = ft_columns(pl, s);
     FILE* fp = std::fopen(path.c_str(), block > 0 ? "a" : "w");
     if (!fp) throw std::runtime_error("cannot write " + path);
     if (block > 0) {  // a resumed run appends its game loop as the next block
         std::fprintf(fp, "# block %d (resumed after %llu EMCS)\n", block, (unsigned long long)step0);
-        for (int t = 0; t < pl.nmeas; ++t) {
-            const double* o = &obs[((size_t)t * nrep + r) * MQ_NOBS];
-            std::fprintf(fp, "%llu %.17g %.17g %.17g %.17g %.17g\n", (unsigned long long)(step0 + t), o[0], o[1],
-                         o[2], o[3], cl[(size_t)t * nrep + r]);
-        }
+        for (int t = 0; t < pl.nmeas; ++t)
+            write_row(fp, (unsigned long long)(step0 + t), &obs[((size_t)t * nrep + r) * MQ_NOBS], cl[(size_t)t * nrep + r],
+                      &ft[((size_t)t * nrep + r) * 3], nft);
         std::fclose(fp);
         return;
     }
@@ -310,12 +327,12 @@ void write_sim(const Plan& pl, const Spec& s, const Sim& sim, uint64_t seed, con
     if (s.model == MQ_POTTS) std::fprintf(fp, " q=%d", s.q);
     std::fprintf(fp, " repid=%d id=%d seed=%llu\n", sim.repid, sim.id, (unsigned long long)seed);
     std::fprintf(fp, "# ncluster=%d nmetro=%d warmup=%d measure=%d\n", pl.ncluster, pl.nmetro, pl.nwarm, pl.nmeas);
-    std::fprintf(fp, "# step E M M2 M4 clustersize\n");
-    for (int t = 0; t < pl.nmeas; ++t) {
-        const double* o = &obs[((size_t)t * nrep + r) * MQ_NOBS];
-        std::fprintf(fp, "%d %.17g %.17g %.17g %.17g %.17g\n", pl.nwarm + t, o[0], o[1], o[2], o[3],
-                     cl[(size_t)t * nrep + r]);
-    }
+    std::fprintf(fp, "# step E M M2 M4 clustersize");
+    for (int d = 0; d < nft; ++d) std::fprintf(fp, " magft%d", d);
+    std::fprintf(fp, "\n");
+    for (int t = 0; t < pl.nmeas; ++t)
+        write_row(fp, (unsigned long long)(pl.nwarm + t), &obs[((size_t)t * nrep + r) * MQ_NOBS], cl[(size_t)t * nrep + r],
+                  &ft[((size_t)t * nrep + r) * 3], nft);
     std::fclose(fp);
 }
 
@@ -563,10 +580,13 @@ int main(int argc, char** argv)
                 }
                 const uint64_t meas0 = emcs.steps();
                 std::vector<double> obs((size_t)pl.nmeas * nr * MQ_NOBS), cl((size_t)pl.nmeas * nr);
+                std::vector<double> ft((size_t)pl.nmeas * nr * 3, 0.0);
+                const bool want_ft = ft_columns(pl, s) > 0;
                 for (int k = 0, t = 0; k < 10; ++k) {
                     for (int i = 0; i < pl.nmeas / 10; ++i, ++t) {
                         const std::vector<double> mc = emcs.move(pl.ncluster, pl.nmetro);
                         ctx.check(mq_measure(b->get(), &obs[(size_t)t * nr * MQ_NOBS]), "mq_measure");
+                        if (want_ft) ctx.check(mq_measure_ft(b->get(), &ft[(size_t)t * nr * 3]), "mq_measure_ft");
                         for (int r = 0; r < nr; ++r) cl[(size_t)t * nr + r] = mc[r];
                     }
                     std::printf(".");
@@ -588,7 +608,7 @@ int main(int argc, char** argv)
                 for (int r = 0; r < nr; ++r) {
                     const Sim& x = sims[r];
                     const std::string name = sim_name(x.p.beta, x.repid, (int)g, (int)pl.groups.size());
-                    write_sim(pl, s, x, seed, obs, cl, nr, r, pl.outpath + name + ".txt", block, meas0);
+                    write_sim(pl, s, x, seed, obs, cl, ft, nr, r, pl.outpath + name + ".txt", block, meas0);
                     double se = 0, se2 = 0, sm = 0, sm2 = 0, sm4 = 0, sc = 0;
                     for (int t = 0; t < pl.nmeas; ++t) {
                         const double* v = &obs[((size_t)t * nr + r) * MQ_NOBS];

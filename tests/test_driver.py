@@ -161,6 +161,22 @@ def _hypercube_graph(dim, L):
     return rp.astype(np.int32), ci.astype(np.int32)
 
 
+def _exact_magft_4x4(beta):
+    """Boltzmann averages of MagFTComp along both axes of the 2D 4x4 Ising torus
+    (J = -1) over all 2^16 states, with the reference's site coordinates (x + 1/2)/L."""
+    L, N = 4, 16
+    nb = exact.neighbour_table(2, L)
+    S = (((np.arange(1 << N)[:, None] >> np.arange(N)) & 1) * 2 - 1).astype(np.float64)
+    E = -((S * S[:, nb[:, 0]]).sum(1) + (S * S[:, nb[:, 2]]).sum(1))
+    w = np.exp(-beta * (E - E.min()))
+    i = np.arange(N)
+    out = []
+    for x in (i % L, i // L):
+        ph = np.exp(2j * np.pi * (x + 0.5) / L)
+        out.append(float((w * np.abs(S @ ph / N) ** 2).sum() / w.sum()))
+    return out
+
+
 @pytest.mark.gpu
 def test_gpu_ising_ini_vs_exact(tmp_path):
     """The whole front-end on the GPU: Ising.ini runs 2D Ising on a 4x4 torus
@@ -172,17 +188,24 @@ def test_gpu_ising_ini_vs_exact(tmp_path):
     for L in (4, 3):
         rp, ci = _hypercube_graph(2, L)
         for beta in (0.3, 0.4406868, 0.7):
-            e_rep, m2_rep = [], []
+            e_rep, m2_rep, ft_rep = [], [], []
             for rep in range(8):
                 hdr, d = _series(tmp_path / str(L) / f"beta{beta:.6f}_{rep}.txt")
-                assert d.shape == (4000, 6), d.shape
+                # the 4x4 torus runs on the checkerboard kernels and carries the MagFTComp columns
+                assert d.shape == (4000, 8 if L == 4 else 6), d.shape
                 assert np.allclose(d[:, 0], np.arange(200, 4200))
                 assert np.all(d[:, 5] >= 1.0)
                 e_rep.append(d[:, 1].mean())
                 m2_rep.append(d[:, 3].mean())
+                if L == 4:
+                    ft_rep.append(d[:, 6:8].mean(axis=0))
             ex = exact.enumerate_ising(2, 4, beta, J=-1.0) if L == 4 else exact.enumerate_graph(rp, ci, beta, J=-1.0)
             ee, m2e = ex["e"], ex["m2"]
-            for got, want, what in ((e_rep, ee, "E"), (m2_rep, m2e, "m2")):
+            checks = [(e_rep, ee, "E"), (m2_rep, m2e, "m2")]
+            if L == 4:
+                fte = _exact_magft_4x4(beta)
+                checks += [([f[0] for f in ft_rep], fte[0], "magft0"), ([f[1] for f in ft_rep], fte[1], "magft1")]
+            for got, want, what in checks:
                 g = np.asarray(got)
                 se = g.std(ddof=1) / np.sqrt(len(g))
                 print(f"L={L} beta={beta} {what}: {g.mean():.5f} +- {se:.5f} exact {want:.5f}", flush=True)
@@ -203,9 +226,13 @@ def test_gpu_ini_models_run(tmp_path, ham, dirs):
     for x, dname in zip(p, dirs):
         for g, grp in enumerate(x["groups"]):
             for s in grp:
-                _, d = _series(tmp_path / dname / f"{s['name']}.txt")
-                assert d.shape == (x["measure"], 6)
+                hdr, d = _series(tmp_path / dname / f"{s['name']}.txt")
+                # MagFTComp columns of the reference's Ising / Phi4 observable tuples
+                nft = {"Phi4": x["dim"]}.get(ham, 0) if x["lattice"] == "regular" else 0
+                assert d.shape == (x["measure"], 6 + nft)
                 assert np.all(np.isfinite(d[:, 1:5]))
+                assert np.all((d[:, 6:] >= 0) & (d[:, 6:] <= 1))
+                assert hdr[-1].split()[7:] == [f"magft{k}" for k in range(nft)]
                 if x["ncluster"] == 0:
                     assert np.all(np.isnan(d[:, 5]))
         lines = [line.split() for line in (tmp_path / dname / "summary.txt").read_text().splitlines()[1:]]
