@@ -180,6 +180,13 @@ public:
         for (int r = 0; r < nrep_; ++r) o[r] = {raw[4 * r], raw[4 * r + 1], raw[4 * r + 2], raw[4 * r + 3]};
         return o;
     }
+    // MagFTComp per replica and axis (observables.h:104-170): [r*3 + d], NaN for d >= dim
+    std::vector<double> measure_ft()
+    {
+        std::vector<double> ft((size_t)nrep_ * 3);
+        ctx_.check(mq_measure_ft(h_, ft.data()), "mq_measure_ft");
+        return ft;
+    }
     std::vector<int32_t> pt_swap(int ntemp, int parity, uint64_t seed, uint64_t step)
     {
         std::vector<int32_t> acc(nrep_ / ntemp);
