@@ -432,7 +432,7 @@ int main(int argc, char** argv)
                 pl.L = nL[li];
                 pl.nsites = (int64_t)std::llround(std::pow((double)pl.L, (double)dim));
                 pl.ncluster = (int)(amp * std::pow((double)pl.L, (double)cexp));
-                pl.lattice = s.rgg ? "rgg" : ((dim == 2 || dim == 3) && pl.L % 2 == 0 && pl.L >= 4) ? "regular" : "csr-hypercube";
+                pl.lattice = s.rgg ? "rgg" : ((dim == 2 || dim == 3) && pl.L % 2 == 0 && pl.L >= 2) ? "regular" : "csr-hypercube";
                 pl.outpath = outdir + "/" + std::to_string(pl.L) + "/";
             } else {
                 pl.L = 0;
