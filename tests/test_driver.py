@@ -317,3 +317,33 @@ def test_gpu_ini_csv_graph_vs_exact(tmp_path):
             g = np.asarray(got)
             se = g.std(ddof=1) / np.sqrt(len(g))
             assert abs(g.mean() - want) < 4 * se + 1e-9, (beta, g.mean(), se, want)
+
+
+@pytest.mark.gpu
+def test_gpu_ini_L2_vs_exact(tmp_path):
+    """L = 2, where both x0 neighbours of a site are the same site and the reference's
+    neighbour lists count that bond twice (regular_hypercubic.h:31-66), runs on the
+    checkerboard kernels; <E/N> and <m^2> agree with exact enumeration of the 2x2 torus
+    with doubled bonds within 4 standard errors over 8 replicas."""
+    conf = tmp_path / "config"
+    conf.mkdir()
+    text = open(os.path.join(CONF, "Ising.ini")).read()
+    text = text.replace("L   = 4, 3", "L   = 2").replace("beta = 0.3, 0.4406868, 0.7", "beta = 0.3, 0.7")
+    (conf / "Ising.ini").write_text(text)
+    out = tmp_path / "out"
+    r = subprocess.run([BIN, "Ising", "--config", str(conf), "--out", str(out), "--seed", "3"], capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 0, r.stderr
+    for beta in (0.3, 0.7):
+        e_rep, m2_rep = [], []
+        for rep in range(8):
+            hdr, d = _series(out / "2" / f"beta{beta:.6f}_{rep}.txt")
+            assert "lattice=regular" in hdr[0] and d.shape == (4000, 8), (hdr[0], d.shape)
+            e_rep.append(d[:, 1].mean())
+            m2_rep.append(d[:, 3].mean())
+        ex = exact.enumerate_ising(2, 2, beta, J=-1.0)
+        for got, want, what in ((e_rep, ex["e"], "E"), (m2_rep, ex["m2"], "m2")):
+            g = np.asarray(got)
+            se = g.std(ddof=1) / np.sqrt(len(g))
+            print(f"L=2 beta={beta} {what}: {g.mean():.5f} +- {se:.5f} exact {want:.5f}", flush=True)
+            assert abs(g.mean() - want) < 4 * se + 1e-9, (beta, what, g.mean(), se, want)
