@@ -50,6 +50,11 @@
 
 namespace {
 
+/* Planted mistakes for the tests' power checks (0 = none, the faithful
+ * restatement; 1 = Phi4 Wolff renormalises phi to unit length after each
+ * reflection, which Phi4.h:241-242 does not do). */
+int g_plant = 0;
+
 enum Model { ISING = 1, EA = 2, POTTS = 3, HEIS = 4, PHI4 = 5 };
 
 struct Cache {
@@ -299,6 +304,10 @@ struct Sim {
                 double* v = ds + 3 * i;
                 double d = v[0] * rd[0] + v[1] * rd[1] + v[2] * rd[2];
                 for (int k = 0; k < 3; ++k) v[k] -= 2 * d * rd[k];
+                if (g_plant == 1) {
+                    const double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                    for (int k = 0; k < 3; ++k) v[k] /= n;
+                }
             };
             auto proj = [&](int64_t i) {
                 const double* v = ds + 3 * i;
@@ -334,9 +343,19 @@ struct Sim {
         return avg / ncluster;
     }
 
-    /* obs[0] = E/N, obs[1] = |m|; Potts: obs[1] = Potts magnetisation */
+    /* obs[0] = E/N, obs[1] = |m| (Potts: the Potts magnetisation), obs[2] = m^2,
+     * obs[3] = m^4 (the moments Binder's cumulant 1 - <m^4>/(3<m^2>^2) is formed
+     * from), obs[4] = mean |phi|^2 per site (1 for unit spins) */
     void measure(double beta, double* obs) const
     {
+        moments(beta, obs);
+        obs[2] = obs[1] * obs[1];
+        obs[3] = obs[2] * obs[2];
+    }
+
+    void moments(double beta, double* obs) const
+    {
+        obs[4] = 1.0;
         const int64_t N = lat.N;
         double e = 0;
         if (p.model == POTTS) {
@@ -362,10 +381,11 @@ struct Sim {
             return;
         }
         double M[3] = {0, 0, 0};
-        double on = 0;
+        double on = 0, phi2 = 0;
         for (int64_t i = 0; i < N; ++i) {
             const double* v = ds + 3 * i;
             for (int k = 0; k < 3; ++k) M[k] += v[k];
+            phi2 += v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
             for (int s = 0; s < lat.z(i); ++s) {
                 const double* w = ds + 3 * lat.nb(i, s);
                 e += bond(lat, bonds, i, s) * (v[0] * w[0] + v[1] * w[1] + v[2] * w[2]);
@@ -377,6 +397,7 @@ struct Sim {
         }
         obs[0] = p.J * e / double(2 * N) + on / double(N);
         obs[1] = std::sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]) / double(N);
+        obs[4] = phi2 / double(N);
     }
 };
 
@@ -391,7 +412,7 @@ int run_impl(const Lat& lat, Params p, const int8_t* bonds, double beta, int ncl
     for (int t = 0; t < nmeas; ++t) {
         double c = sim.emcs(beta, ncluster, nmetro, nacc);
         if (clusters) clusters[t] = c;
-        if (obs) sim.measure(beta, obs + 2 * t);
+        if (obs) sim.measure(beta, obs + 5 * t);
     }
     return 0;
 }
@@ -400,7 +421,7 @@ int run_impl(const Lat& lat, Params p, const int8_t* bonds, double beta, int ncl
 
 /*
  * Run one simulation: optional init, nwarm + nmeas EMCS, one measurement per
- * measured EMCS (obs[2*t] = E/N, obs[2*t+1] = |m|).  Integer models use is[N],
+ * measured EMCS (obs[5*t .. 5*t+4] = E/N, |m|, m^2, m^4, mean |phi|^2).  Integer models use is[N],
  * O(3)/Phi4 use ds[3N].  bonds may be NULL (uniform coupling).
  */
 EXPORT int ref_run_regular(int model, int dim, int L, double J, int Q, double lambda, double mass,
@@ -422,6 +443,9 @@ EXPORT int ref_run_csr(int model, int64_t N, const int* rp, const int* ci, doubl
     Params p{model, J, Q, lambda, mass};
     return run_impl(lat, p, bonds, beta, ncluster, nmetro, nwarm, nmeas, seed, init, is, ds, obs, clusters, nacc);
 }
+
+/* test hook: plant a known mistake (see g_plant) to show a check can fail */
+EXPORT void ref_set_plant(int which) { g_plant = which; }
 
 EXPORT uint64_t ref_mt19937_64_nth(uint64_t seed, int64_t n)
 {

@@ -46,6 +46,7 @@ _SIG = {
     "ref_run_regular": (_I, [_I, _I, _I, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_run_csr": (_I, [_I, _I64, _P, _P, _D, _I, _D, _D, _P, _D, _I, _I, _I, _I, _U64, _I, _P, _P, _P, _P, _P]),
     "ref_mt19937_64_nth": (_U64, [_U64, _I64]),
+    "ref_set_plant": (None, [_I]),
     "ref_rngcache_draws": (None, [_U64, _I, _U64, _I64, _P]),
     "ref_bench_regular": (_I, [_I, _I, _I, _D, _I, _D, _I, _I, _I, _U64, _I, _P]),
 }
@@ -233,7 +234,8 @@ def wolff_potts(dim, L, s, Q, beta, k, nclusters, seed, cl0, rep):
 
 def ref_run(model, dim, L, beta, J=-1.0, Q=3, lam=4.5, mass=1.0, ncluster=0, nmetro=1, nwarm=0, nmeas=0,
             seed=1, init=True, spins=None, bonds=None):
-    """Reference-faithful CPU restatement: returns (spins, obs[nmeas,2], mean_cluster[nmeas])."""
+    """Reference-faithful CPU restatement: returns (spins, obs[nmeas, 5], mean_cluster[nmeas]);
+    obs columns are E/N, |m| (Potts: its order parameter), m^2, m^4 and the mean |phi|^2."""
     N = L ** dim
     vec = model in (HEIS, PHI4)
     is_ = np.zeros(N, np.int32)
@@ -243,12 +245,21 @@ def ref_run(model, dim, L, beta, J=-1.0, Q=3, lam=4.5, mass=1.0, ncluster=0, nme
             ds[:] = np.asarray(spins, np.float64).reshape(-1)
         else:
             is_[:] = np.asarray(spins).reshape(-1)
-    obs = np.zeros((max(nmeas, 1), 2))
+    obs = np.zeros((max(nmeas, 1), 5))
     cl = np.zeros(max(nmeas, 1))
     b = None if bonds is None else np.ascontiguousarray(bonds, np.int8)
     lib().ref_run_regular(model, dim, L, J, Q, lam, mass, None if b is None else p(b), beta, ncluster, nmetro, nwarm,
                           nmeas, seed, int(init), p(is_), p(ds), p(obs), p(cl), None)
     return (ds.reshape(N, 3) if vec else is_), obs[:nmeas], cl[:nmeas]
+
+
+def ref_chains(model, dim, L, beta, nseeds, seed0=100, threads=8, **kw):
+    """nseeds independent chains of ref_run (seeds seed0, seed0+1, ...) on a thread pool (the
+    library call releases the GIL); returns obs[nseeds, nmeas, 5]."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, min(threads, nseeds))) as ex:
+        res = list(ex.map(lambda s: ref_run(model, dim, L, beta, seed=seed0 + s, **kw)[1], range(nseeds)))
+    return np.array(res)
 
 
 def csr_sweeps(rp, ci, colour, s, beta, J, nsweeps, seed, sweep0, rep, potts=False, Q=3, bonds=None):

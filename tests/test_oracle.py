@@ -385,3 +385,78 @@ def test_magft_restatement_vs_fft_and_stripe():
         assert observables.magft(stripe, dim, L, 0) == pytest.approx((2.0 / (L * np.sin(np.pi / L))) ** 2, rel=1e-12)
         assert observables.magft(stripe, dim, L, 1) == pytest.approx(0.0, abs=1e-28)
         assert observables.magft(np.ones(N), dim, L, dim - 1) == pytest.approx(0.0, abs=1e-28)
+
+
+# ---------------------------------------------------------------------------
+# Wolff updates of continuous spins pinned by the Boltzmann distribution.
+# No closed form exists for Heisenberg or Phi4, and the reference's generic Wolff
+# move and Embedder<Phi4> (wolff.h:48-125, Phi4.h:182-244) cannot be built here
+# (they need core.h, which needs HDF5).  What fixes them is that a Wolff move
+# must leave the Boltzmann distribution invariant: chains that add Wolff clusters
+# to Metropolis sweeps must reproduce the averages of Metropolis-only chains,
+# whose move equals the reference build's bit for bit
+# (test_phi4_restatement_bit_exact_vs_reference_build).
+
+PHI4_INI = dict(beta=0.69, lam=4.5, mass=1.0)  # config/Phi4.ini (beta from its list)
+
+
+def _phi4_chains(ncluster, seed0, nmeas=3000):
+    return ob.ref_chains(ob.PHI4, 3, 6, PHI4_INI["beta"], 16, seed0=seed0, ncluster=ncluster, nmetro=5, nwarm=200,
+                         nmeas=nmeas, J=-1.0, lam=PHI4_INI["lam"], mass=PHI4_INI["mass"])
+
+
+def _zscores(a, b, cols):
+    import stat_util as su
+    out = {}
+    for k, nm in cols:
+        g, sg = su.mean_err(su.per_chain(a[:, :, k]))
+        c, sc = su.mean_err(su.per_chain(b[:, :, k]))
+        out[nm] = (g - c) / np.hypot(sg, sc)
+    return out
+
+
+def test_phi4_wolff_restatement_vs_metropolis_only():
+    """3D L = 6 Phi4 at the config/Phi4.ini parameters: 16 chains of EMCS = 3 random-direction
+    Wolff clusters + 5 sweeps against 16 Metropolis-only chains; <E/N>, <|m|> and <|phi|^2>
+    agree within 4 standard errors.  With the planted mistake (phi renormalised to unit length
+    after each reflection, which Phi4.h:241-242 does not do) <E/N> and <|phi|^2> are off by
+    more than 8."""
+    cols = [(0, "E/N"), (1, "|m|"), (4, "|phi|^2")]
+    metro = _phi4_chains(0, 300)
+    ob.lib().ref_set_plant(0)
+    z = _zscores(_phi4_chains(3, 200), metro, cols)
+    print("[phi4 wolff vs metropolis-only]", {k: round(v, 2) for k, v in z.items()}, flush=True)
+    assert all(abs(v) < 4.0 for v in z.values()), z
+    ob.lib().ref_set_plant(1)
+    try:
+        zp = _zscores(_phi4_chains(3, 200), metro, cols)
+    finally:
+        ob.lib().ref_set_plant(0)
+    print("[phi4 wolff vs metropolis-only, planted renormalisation]", {k: round(v, 2) for k, v in zp.items()},
+          flush=True)
+    assert abs(zp["E/N"]) > 8.0 and abs(zp["|phi|^2"]) > 8.0, zp
+
+
+@pytest.mark.parametrize("model,small,delta,rdir", [("heisenberg", True, 0.6, True),
+                                                    ("heisenberg", False, 0.5, False),
+                                                    ("phi4", False, 0.5, True)])
+def test_csr_o3_oracle_chains_vs_restatement(model, small, delta, rdir):
+    """The graph path's FP64 restatements (orc_csr_o3_colour, orc_csr_o3_wolff), driven as
+    Markov chains on the 3D L = 4 torus given as a CSR graph, against Metropolis-only chains of
+    the reference-faithful restatement on the regular lattice (bit-exact to the reference build):
+    <E/N>, <|m|> (and <|phi|^2> for Phi4) within 4 standard errors.  Heisenberg at beta = 0.693
+    with small-angle or fresh-direction proposals and random-direction or Cartesian clusters;
+    Phi4 at the config/Phi4.ini parameters."""
+    import o3_chains
+    from marqov_amd import graphs
+    phi4 = model == "phi4"
+    beta = PHI4_INI["beta"] if phi4 else 0.693
+    rp, ci = graphs.regular_csr(3, 4)
+    orc = o3_chains.oracle_chains(rp, ci, phi4, beta, 16, 17, 100, 1200, 2, 3, small=small, delta=delta, rdir=rdir)
+    ref = ob.ref_chains(ob.PHI4 if phi4 else ob.HEIS, 3, 4, beta, 16, seed0=400, ncluster=0, nmetro=3, nwarm=300,
+                        nmeas=4000, J=-1.0, lam=PHI4_INI["lam"], mass=PHI4_INI["mass"])
+    cols = [(0, "E/N"), (1, "|m|")] + ([(4, "|phi|^2")] if phi4 else [])
+    z = _zscores(orc, ref, cols)
+    print(f"[csr o3 oracle chains] {model} small={small} rdir={rdir}:", {k: round(v, 2) for k, v in z.items()},
+          flush=True)
+    assert all(abs(v) < 4.0 for v in z.values()), z
