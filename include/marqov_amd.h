@@ -141,7 +141,10 @@ MQ_API int mq_state_download(mq_state* st, int rep, void* host, int64_t bytes);
 /* ---- updates ---------------------------------------------------------- */
 /* nsweeps checkerboard / colour-class Metropolis sweeps of every replica;
  * replaces EMCS's Metropolis loop (emcs.h:93-106) with Metropolis::move
- * (metropolis.h:86-126).  Sweep t uses Philox counter phase 2*(sweep0+t)+colour. */
+ * (metropolis.h:86-126).  Sweep t uses the 32-bit Philox counter phase
+ * 2*(sweep0+t) + colour on regular lattices and 64*(sweep0+t) + colour on
+ * graphs, so every sweep number must stay below 2^31 (regular) or 2^26 (a
+ * state with any graph lattice); calls that would pass it are refused. */
 MQ_API int mq_metropolis(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0);
 /* nclusters single-cluster Wolff moves of every replica; replaces EMCS's
  * cluster loop (emcs.h:83-89) with Wolff::move (wolff.h:48-125, Ising.h:122-169,

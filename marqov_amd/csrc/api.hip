@@ -760,14 +760,15 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
 int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0)
 {
     const Key2 key = key_of(seed);
-    if (nsweeps > 0 && st->kernel_choice == 0 && !st->msc && !st->eabits && sweep0 + (uint64_t)nsweeps < (1ull << 31)) {
+    if (nsweeps > 0 && (sweep0 >= sweep_limit(st) || (uint64_t)nsweeps > sweep_limit(st) - sweep0))
+        return mq_fail(st->ctx, MQ_ERR_ARG, sweep_limit_msg(st));
+    if (nsweeps > 0 && st->kernel_choice == 0 && !st->msc && !st->eabits) {
         hipError_t e = launch_sweeps_lds(st, nsweeps, (uint32_t)sweep0, key);
         if (e == hipSuccess) return MQ_OK;
         if (e != hipErrorNotSupported) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("lds sweep: ") + hipGetErrorString(e));
     }
     for (int t = 0; t < nsweeps; ++t) {
         const uint64_t sw = sweep0 + (uint64_t)t;
-        if (sw >= (1ull << 31)) return mq_fail(st->ctx, MQ_ERR_ARG, "sweep counter exceeds 2^31");
         if (st->eabits) {  // one fused launch per colour over every lattice size
             for (int c = 0; c < 2; ++c) {
                 hipError_t e = launch_sweep_eabits_all(st, c, (uint32_t)(2 * sw + c), key);
@@ -808,6 +809,9 @@ extern "C" int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t clu
         return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff is not defined for the +-J spin glass (frustrated bonds)");
     if (st->msc) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff runs on int8 storage (spin_bits = 8)");
     if (cluster0 + (uint64_t)nclusters > (1ull << 32)) return mq_fail(st->ctx, MQ_ERR_ARG, "cluster counter exceeds 2^32");
+    for (const auto* l : st->lat)
+        if (l->kind == LAT_REGULAR && l->dim == 3 && l->L > 1024)
+            return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff on 3D lattices needs L <= 1024 (10-bit site coordinates)");
     rc = ensure_wolff_scratch(st, nclusters);
     if (rc) return rc;
     for (const auto& g : st->groups) {
@@ -858,11 +862,12 @@ static int measure_sums(mq_state* st)
     }
     MQ_HIP(st->ctx, zero64(st->d_isum, (int64_t)st->nrep * 8, s));
     MQ_HIP(st->ctx, zero64(st->d_fsum, (int64_t)st->nrep * 8, s));
-    if (st->eabits) MQ_HIP(st->ctx, launch_measure_eabits_all(st));
+    if (st->eabits) MQ_HIP(st->ctx, launch_measure_eabits_all(st, st->d_isum));
     for (const auto& g : st->groups) {
         if (st->eabits) break;
-        hipError_t e = st->msc ? launch_measure_msc(st, g)
-                       : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
+        hipError_t e = st->msc ? launch_measure_msc(st, g, st->d_isum)
+                       : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g, st->d_isum, st->d_fsum)
+                                                      : launch_measure_csr(st, g, st->d_isum, st->d_fsum);
         if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
     }
     const int64_t n = (int64_t)st->nrep * 17;

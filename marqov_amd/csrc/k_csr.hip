@@ -926,7 +926,7 @@ hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t p
     return hipGetLastError();
 }
 
-hipError_t launch_measure_csr(mq_state* st, const Group& g)
+hipError_t launch_measure_csr(mq_state* st, const Group& g, long long* isum, double* fsum)
 {
     const mq_lattice* l = g.lat;
     const bool vec = st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4;
@@ -938,8 +938,8 @@ hipError_t launch_measure_csr(mq_state* st, const Group& g)
 #define MQ_M(B, V)                                                                                                  \
     hipLaunchKernelGGL((k_csr_measure<B, V>), grid, dim3(256), 0, s, st->d_spins + g.off, (int64_t)1,             \
                        (int64_t)g.nrep, l->d_rp,                                                                 \
-                       l->d_ci, st->d_bonds, st->d_rep_bond_off, l->N, st->p.model, st->p.q, g.rep0, st->d_isum, \
-                       st->d_fsum)
+                       l->d_ci, st->d_bonds, st->d_rep_bond_off, l->N, st->p.model, st->p.q, g.rep0, isum,       \
+                       fsum)
     if (vec) {
         if (bonds) MQ_M(true, true);
         else MQ_M(false, true);

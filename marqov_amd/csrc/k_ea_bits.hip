@@ -426,11 +426,11 @@ hipError_t launch_sweep_eabits_all(mq_state* st, int colour, uint32_t phase, Key
 }
 
 // one fused launch over every lattice group
-hipError_t launch_measure_eabits_all(mq_state* st)
+hipError_t launch_measure_eabits_all(mq_state* st, long long* isum)
 {
     const BitsGroup* dg = reinterpret_cast<const BitsGroup*>(st->d_bits_groups);
     const dim3 grid((unsigned)((st->bits_mwaves + 3) / 4));
-    unsigned long long* is = reinterpret_cast<unsigned long long*>(st->d_isum);
+    unsigned long long* is = reinterpret_cast<unsigned long long*>(isum);
     if (st->lat[0]->dim == 3)
         hipLaunchKernelGGL(k_measure_eabits<3>, grid, dim3(256), 0, st->ctx->stream, st->d_spins, dg, st->bits_ngroups,
                            st->bits_mwaves, st->d_bonds, st->d_rep_bond_off, is);

@@ -258,12 +258,12 @@ hipError_t launch_init_msc(mq_state* st, const Group& g, Key2 key)
     return hipGetLastError();
 }
 
-hipError_t launch_measure_msc(mq_state* st, const Group& g)
+hipError_t launch_measure_msc(mq_state* st, const Group& g, long long* isum)
 {
     const int lgL = lg2(g.lat->L);
     const int64_t words = g.lat->N / 64;
     hipLaunchKernelGGL(k_measure_msc, dim3((unsigned)((words + 255) / 256), (unsigned)g.nrep), dim3(256), 0,
                        st->ctx->stream, reinterpret_cast<const uint32_t*>(st->d_spins + g.off), g.stride / 4, lgL,
-                       g.rep0, reinterpret_cast<unsigned long long*>(st->d_isum));
+                       g.rep0, reinterpret_cast<unsigned long long*>(isum));
     return hipGetLastError();
 }

@@ -159,12 +159,13 @@ hipError_t energy_async(mq_state* st)
     if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     if ((e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     if (st->eabits) {
-        if ((e = launch_measure_eabits_all(st)) != hipSuccess) return e;
+        if ((e = launch_measure_eabits_all(st, st->d_isum)) != hipSuccess) return e;
     }
     for (const auto& g : st->groups) {
         if (st->eabits) break;
-        e = st->msc ? launch_measure_msc(st, g)
-            : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g) : launch_measure_csr(st, g);
+        e = st->msc ? launch_measure_msc(st, g, st->d_isum)
+            : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g, st->d_isum, st->d_fsum)
+                                           : launch_measure_csr(st, g, st->d_isum, st->d_fsum);
         if (e != hipSuccess) return e;
     }
     const int vec = (st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4);
@@ -323,7 +324,7 @@ extern "C" int mq_run_graph(mq_state* st, int nsweeps, int ntemp, uint64_t seed,
     const bool regular = st->lat[0]->kind == LAT_REGULAR;
     const uint64_t per_sweep = regular ? 2 : 64;
     const uint64_t total = sweep0 + (uint64_t)nsweeps * steps * repeats;
-    if (total >= (1ull << 31)) return mq_fail(st->ctx, MQ_ERR_ARG, "sweep counter exceeds 2^31");
+    if (sweep0 >= sweep_limit(st) || total > sweep_limit(st)) return mq_fail(st->ctx, MQ_ERR_ARG, sweep_limit_msg(st));
     if (ntemp > 1) {
         int rc = ensure_pt(st, ntemp);
         if (rc) return rc;

@@ -965,7 +965,7 @@ hipError_t launch_measure_ft_regular(mq_state* st, const Group& g)
     return hipGetLastError();
 }
 
-hipError_t launch_measure_regular(mq_state* st, const Group& g)
+hipError_t launch_measure_regular(mq_state* st, const Group& g, long long* isum, double* fsum)
 {
     const int L = g.lat->L, dim = g.lat->dim;
     const int64_t half = g.lat->N / 2;
@@ -980,15 +980,15 @@ hipError_t launch_measure_regular(mq_state* st, const Group& g)
         const float4* b = reinterpret_cast<const float4*>(st->d_spins + g.off);
         const int64_t s4 = g.stride / 16;
         if (dim == 3) {
-            if (bonds) hipLaunchKernelGGL((k_measure_f4<3, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
-            else hipLaunchKernelGGL((k_measure_f4<3, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
+            if (bonds) hipLaunchKernelGGL((k_measure_f4<3, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, fsum);
+            else hipLaunchKernelGGL((k_measure_f4<3, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, fsum);
         } else {
-            if (bonds) hipLaunchKernelGGL((k_measure_f4<2, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
-            else hipLaunchKernelGGL((k_measure_f4<2, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, st->d_fsum);
+            if (bonds) hipLaunchKernelGGL((k_measure_f4<2, true>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, fsum);
+            else hipLaunchKernelGGL((k_measure_f4<2, false>), grid, dim3(256), 0, s, b, s4, st->d_bonds, st->d_rep_bond_off, L, g.rep0, fsum);
         }
     } else {
         const int8_t* b = reinterpret_cast<const int8_t*>(st->d_spins + g.off);
-        unsigned long long* is = reinterpret_cast<unsigned long long*>(st->d_isum);
+        unsigned long long* is = reinterpret_cast<unsigned long long*>(isum);
         const bool pow2 = (L & (L - 1)) == 0;
         if (model == MQ_ISING && dim == 3 && !bonds && pow2 && L >= 32 && L <= 1024 && st->kernel_choice != 2) {
             int lgL = 0;

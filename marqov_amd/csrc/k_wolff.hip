@@ -94,6 +94,26 @@ constexpr int WCAP = 4096;
 // site, or a bit per site when packed (16-byte multiples)
 __host__ __device__ inline uint32_t wave_spin_bytes(uint32_t N, bool pack) { return pack ? (N / 8 + 15) & ~15u : (N + 15) & ~15u; }
 
+// Sites are handled as packed coordinates: 10 bits per axis in 3D (L <= 1024),
+// 16 bits in 2D.  Frontier entries and LDS set keys are packed sites, so a
+// neighbour, its natural index and its colour-split address come from adds and
+// multiplies, with no integer division per bond.
+template <int DIM>
+struct SitePack {
+    static constexpr int SB = DIM == 3 ? 10 : 16;
+    static constexpr uint32_t CM = (1u << SB) - 1u;
+    __device__ static uint32_t pack(uint32_t x0, uint32_t x1, uint32_t x2)
+    {
+        return DIM == 3 ? x0 | (x1 << SB) | (x2 << (2 * (DIM == 3 ? SB : 0))) : x0 | (x1 << SB);
+    }
+    __device__ static uint32_t z(uint32_t p) { return DIM == 3 ? p >> (2 * (DIM == 3 ? SB : 0)) : 0u; }
+    // natural site index sum_j x_j L^j
+    __device__ static uint32_t natural(uint32_t p, uint32_t uL, uint32_t LL)
+    {
+        return (p & CM) + uL * ((p >> SB) & CM) + LL * z(p);
+    }
+};
+
 template <int DIM, int KIND>
 __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base, int64_t stride, int L, int nclusters,
                                                        uint32_t cl0, int rep0, int Q,
@@ -104,10 +124,13 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                                                        int64_t* __restrict__ csize, int vlim, Key2 key)
 {
     constexpr bool VEC = (KIND == WK_CART || KIND == WK_RDIR);
+    constexpr int SB = SitePack<DIM>::SB;
+    constexpr uint32_t CM = SitePack<DIM>::CM;
+    constexpr int NS = 2 * DIM;
     __shared__ int s_tail;
-    __shared__ int32_t s_ring[WCAP];
+    __shared__ uint32_t s_ring[WCAP];
     __shared__ float s_proj[VEC ? WCAP : 1];
-    __shared__ uint32_t s_vset[VEC ? MQ_VSET : 1];
+    __shared__ uint32_t s_vset[MQ_VSET];
     const int r = blockIdx.x;
     const int grep = rep0 + r;
     const uint32_t uL = (uint32_t)L;
@@ -120,24 +143,31 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
     uint32_t* vis = visit + (rep_site_off[grep] >> 5);
     const uint32_t tw = VEC ? 0u : wthr[grep];
     const float beta = VEC ? betaf[grep] : 0.f;
-    if (VEC)
+    if (vlim > 0)
         for (int i = threadIdx.x; i < MQ_VSET; i += blockDim.x) s_vset[i] = 0xFFFFFFFFu;
 
     for (int c = 0; c < nclusters; ++c) {
         const uint32_t ncl = cl0 + (uint32_t)c;
         const uint4 sw = philox4x32_10(0u, ncl, (uint32_t)grep, ST_WSEED, key);
         const uint32_t seed = (uint32_t)(((uint64_t)sw.x * (uint64_t)N) >> 32);
-        const uint32_t seed_a = (uint32_t)split_addr<DIM>(L, half, seed);
-        // O(3)/Phi4 membership: the LDS set while the cluster can stay within vlim
-        // sites, the global bitmap after that (block-uniform)
+        const uint32_t sx0 = seed % uL, srow = seed / uL;
+        const uint32_t sx1 = (DIM == 3) ? srow % uL : srow, sx2 = (DIM == 3) ? srow / uL : 0u;
+        const uint32_t seed_p = SitePack<DIM>::pack(sx0, sx1, sx2);
+        const uint32_t seed_a = ((sx0 + sx1 + sx2) & 1u) * half + (seed >> 1);
+        // Membership: while the cluster cannot outgrow vlim sites, an LDS hash set of
+        // packed sites (one LDS compare-and-swap per claim, and for int8 spins the
+        // alignment test is a plain load issued before the bond's Philox draw); after
+        // that, the spin byte itself (Ising, Potts: "still holds the old value" is "not
+        // in the cluster", claimed with one global atomic) or the global bitmap
+        // (O(3)/Phi4).  The choice is block-uniform.
         bool glob = vlim == 0;
         int val = 0, newq = 0, axis = 0;
         float3 rd = make_float3(0.f, 0.f, 0.f);
         if (KIND == WK_ISING || KIND == WK_POTTS) {
-            // atomic load: earlier clusters changed bytes with L2 atomics only
+            // atomic load: the previous cluster changed bytes with stores and atomics
             uint32_t sh;
             val = (int8_t)(peek_word(sp8 + seed_a, sh) >> sh);
-            if (KIND == WK_POTTS) newq = (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q;
+            newq = (KIND == WK_POTTS) ? (val + 1 + (int)(((uint64_t)sw.y * (uint64_t)(Q - 1)) >> 32)) % Q : -val;
         } else if (KIND == WK_CART) {
             axis = (int)(((uint64_t)sw.y * 3ull) >> 32);
         } else {
@@ -145,20 +175,27 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
         }
         __syncthreads();  // every lane has read the seed's old value
         if (threadIdx.x == 0) {
-            if (KIND == WK_ISING) {
-                claim_ising(sp8 + seed_a, val);
-            } else if (KIND == WK_POTTS) {
-                uint32_t sh;
-                claim_potts(sp8 + seed_a, peek_word(sp8 + seed_a, sh), val, newq);
+            if (KIND == WK_ISING || KIND == WK_POTTS) {
+                if (glob) {
+                    if (KIND == WK_ISING) {
+                        claim_ising(sp8 + seed_a, val);
+                    } else {
+                        uint32_t sh;
+                        claim_potts(sp8 + seed_a, peek_word(sp8 + seed_a, sh), val, newq);
+                    }
+                } else {
+                    sp8[seed_a] = (int8_t)newq;
+                    vset_insert(s_vset, seed_p);
+                }
             } else {
                 const float4 v = reflect<KIND>(sp4[seed_a], axis, rd);
                 sp4[seed_a] = v;
                 s_proj[0] = proj<KIND>(v, axis, rd);
                 if (glob) atomicOr(&vis[seed >> 5], 1u << (seed & 31));
-                else vset_insert(s_vset, seed);
-                q[0] = (int32_t)seed;
+                else vset_insert(s_vset, seed_p);
+                q[0] = (int32_t)seed_p;
             }
-            s_ring[0] = (int32_t)seed;
+            s_ring[0] = seed_p;
             s_tail = 1;
         }
         __syncthreads();
@@ -167,20 +204,20 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
             const int tail = s_tail;
             __syncthreads();
             if (head >= tail) break;
-            // One lane per (frontier site, bond slot): a narrow level then costs one
-            // Philox evaluation and one claim round trip per lane instead of 2*DIM
-            // Philox evaluations in a row; new entries are placed with one LDS
-            // atomic per wavefront.  (Two or four tasks per lane with their claims
-            // issued together measured 13% / 24% slower on 3D Ising L=256: the
-            // workgroup's wavefronts already overlap the round trips.)
-            constexpr int NS = 2 * DIM;
+            // One lane per (frontier site, bond slot): a level costs one Philox
+            // evaluation and one memory round trip per lane; new entries are placed
+            // with one LDS atomic per wavefront.
             const int ntask = (tail - head) * NS;
-            if (VEC && !glob && tail + ntask > vlim) {
-                // this level could take the cluster past vlim sites: move the members
-                // (all listed in q) to the global bitmap and claim there from now on
-                for (int idx = threadIdx.x; idx < tail; idx += blockDim.x) {
-                    const uint32_t m = (uint32_t)q[idx];
-                    atomicOr(&vis[m >> 5], 1u << (m & 31));
+            if (!glob && tail + ntask > vlim) {
+                // this level could take the cluster past vlim sites: from now on claim
+                // in global memory.  Ising/Potts members already hold the new value;
+                // O(3)/Phi4 members (all listed in q) move to the bitmap.
+                if (VEC) {
+                    for (int idx = threadIdx.x; idx < tail; idx += blockDim.x) {
+                        const uint32_t m = (uint32_t)q[idx];
+                        const uint32_t mn = SitePack<DIM>::natural(m, uL, LL);
+                        atomicOr(&vis[mn >> 5], 1u << (mn & 31));
+                    }
                 }
                 glob = true;
                 __syncthreads();
@@ -190,55 +227,56 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                 const int sl = t - e * NS;
                 const int idx = head + e;
                 const bool in_ring = idx - hprev < WCAP;
-                const uint32_t cur = (uint32_t)(in_ring ? s_ring[idx & (WCAP - 1)] : q[idx]);
-                // coordinates of cur; its neighbours all have the other colour
-                uint32_t x[3] = {0u, 0u, 0u};
-                {
-                    const uint32_t r1 = cur / uL;
-                    x[0] = cur - r1 * uL;
-                    if (DIM == 3) {
-                        x[2] = r1 / uL;
-                        x[1] = r1 - x[2] * uL;
-                    } else {
-                        x[1] = r1;
-                    }
-                }
-                const uint32_t nbase = ((x[0] + x[1] + x[2]) & 1u) ? 0u : half;
+                const uint32_t cur = in_ring ? s_ring[idx & (WCAP - 1)] : (uint32_t)q[idx];
+                const uint32_t x0 = cur & CM, x1 = (cur >> SB) & CM, x2 = SitePack<DIM>::z(cur);
                 const int d = sl >> 1;
-                const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
-                const uint32_t xd = d == 0 ? x[0] : (d == 1 ? x[1] : x[2]);
-                uint32_t j;
-                if ((sl & 1) == 0) j = (xd == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
-                else j = (xd == 0) ? cur + (uL - 1) * pw : cur - pw;
-                const uint32_t jan = nbase + (j >> 1);
-                const uint64_t b = (uint64_t)((sl & 1) ? j : cur) * DIM + (uint64_t)d;
+                const uint32_t xd = d == 0 ? x0 : (d == 1 ? x1 : x2);
+                uint32_t nx;
+                if ((sl & 1) == 0) nx = (xd == uL - 1) ? 0u : xd + 1u;
+                else nx = (xd == 0) ? uL - 1u : xd - 1u;
+                const uint32_t jp = (cur & ~(CM << (SB * d))) | (nx << (SB * d));
+                const uint32_t curn = x0 + uL * x1 + LL * x2;
+                const uint32_t jn = curn + (nx - xd) * (d == 0 ? 1u : (d == 1 ? uL : LL));  // mod 2^32
+                // the neighbours of a site all have the other colour
+                const uint32_t jan = (((x0 + x1 + x2) & 1u) ? 0u : half) + (jn >> 1);
+                const uint64_t b = (uint64_t)((sl & 1) ? jn : curn) * DIM + (uint64_t)d;
                 bool ok = false;
                 float pnew = 0.f;
-                if (KIND == WK_ISING) {
-                    // the claim itself tests alignment
-                    const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                    if (u4_get(bw, (int)(b & 3)) < tw) ok = claim_ising(sp8 + jan, val);
-                } else if (KIND == WK_POTTS) {
-                    uint32_t sh;
-                    const uint32_t seen = peek_word(sp8 + jan, sh);  // in flight during the Philox rounds
-                    const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                    if ((int8_t)(seen >> sh) == (int8_t)val && u4_get(bw, (int)(b & 3)) < tw) {
-                        uint32_t* w = spin_word(sp8 + jan, sh);
-                        const uint32_t prev = atomicCAS(w, seen, (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
-                        // the word changed under the first swap: retry from what it held
-                        ok = (prev == seen) || claim_potts(sp8 + jan, prev, val, newq);
+                if (KIND == WK_ISING || KIND == WK_POTTS) {
+                    if (!glob) {
+                        const int sv = sp8[jan];  // in flight during the Philox rounds
+                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                        if (sv == val && u4_get(bw, (int)(b & 3)) < tw) {
+                            ok = vset_insert(s_vset, jp);
+                            if (ok) sp8[jan] = (int8_t)newq;
+                        }
+                    } else if (KIND == WK_ISING) {
+                        // the claim itself tests alignment
+                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                        if (u4_get(bw, (int)(b & 3)) < tw) ok = claim_ising(sp8 + jan, val);
+                    } else {
+                        uint32_t sh;
+                        const uint32_t seen = peek_word(sp8 + jan, sh);  // in flight during the Philox rounds
+                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                        if ((int8_t)(seen >> sh) == (int8_t)val && u4_get(bw, (int)(b & 3)) < tw) {
+                            uint32_t* w = spin_word(sp8 + jan, sh);
+                            const uint32_t prev = atomicCAS(w, seen, (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
+                            // the word changed under the first swap: retry from what it held
+                            ok = (prev == seen) || claim_potts(sp8 + jan, prev, val, newq);
+                        }
                     }
                 } else {
-                    const float pc = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[split_addr<DIM>(L, half, cur)], axis, rd);
+                    const uint32_t cura = (((x0 + x1 + x2) & 1u) ? half : 0u) + (curn >> 1);
+                    const float pc = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[cura], axis, rd);
                     const float4 vj = sp4[jan];  // in flight during the Philox rounds
                     const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
                     const float cpl = J * pc * proj<KIND>(vj, axis, rd);
                     if (cpl > 0.f && u01f(u4_get(bw, (int)(b & 3))) < -expm1f(-2.0f * beta * cpl)) {
                         if (glob) {
-                            const uint32_t ob = atomicOr(&vis[j >> 5], 1u << (j & 31));
-                            ok = !(ob & (1u << (j & 31)));
+                            const uint32_t ob = atomicOr(&vis[jn >> 5], 1u << (jn & 31));
+                            ok = !(ob & (1u << (jn & 31)));
                         } else {
-                            ok = vset_insert(s_vset, j);
+                            ok = vset_insert(s_vset, jp);
                         }
                         if (ok) {
                             // the claim succeeded, so nobody changed vj since it was read
@@ -251,10 +289,10 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                 int pos = wave_reserve(&s_tail, ok ? 1u : 0u);
                 if (ok) {
                     if (pos - head < WCAP) {
-                        s_ring[pos & (WCAP - 1)] = (int32_t)j;
+                        s_ring[pos & (WCAP - 1)] = jp;
                         if (VEC) s_proj[pos & (WCAP - 1)] = pnew;
                     }
-                    if (VEC || pos - head >= WCAP) q[pos] = (int32_t)j;  // VEC: every member, for the bitmap
+                    if (VEC || pos - head >= WCAP) q[pos] = (int32_t)jp;  // VEC: every member, for the bitmap
                 }
             }
             __syncthreads();
@@ -262,12 +300,13 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
             head = tail;
         }
         const int size = s_tail;
-        if (VEC) {
-            if (glob)
-                for (int idx = threadIdx.x; idx < size; idx += blockDim.x) vis[q[idx] >> 5] = 0u;
-            if (vlim > 0)
-                for (int i = threadIdx.x; i < MQ_VSET; i += blockDim.x) s_vset[i] = 0xFFFFFFFFu;
-        }
+        if (VEC && glob)
+            for (int idx = threadIdx.x; idx < size; idx += blockDim.x) {
+                const uint32_t m = (uint32_t)q[idx];
+                vis[SitePack<DIM>::natural(m, uL, LL) >> 5] = 0u;
+            }
+        if (vlim > 0)
+            for (int i = threadIdx.x; i < MQ_VSET; i += blockDim.x) s_vset[i] = 0xFFFFFFFFu;
         __syncthreads();
         if (threadIdx.x == 0 && csize) csize[(int64_t)grep * nclusters + c] = size;
     }
@@ -483,6 +522,7 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
     else if (model == MQ_HEISENBERG || model == MQ_PHI4) kind = WK_RDIR;
     else return hipErrorInvalidValue;
     const dim3 grid(g.nrep), block(wolff_block(st, g.nrep));
+    if (dim == 3 && L > 1024) return hipErrorNotSupported;  // packed sites: 10 bits per axis
     hipStream_t s = st->ctx->stream;
     char* base = st->d_spins + g.off;
     const float J = (float)st->p.J;

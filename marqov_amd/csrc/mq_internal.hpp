@@ -147,10 +147,11 @@ __device__ __forceinline__ int wave_reserve(int* tail, uint32_t n)
 // hashing of the index.  Callers keep at most MQ_VSET/2 members, so a free slot
 // always exists and probe runs stay short.  Returns true when j was not a
 // member and is one now; concurrent inserts of one j let exactly one lane win.
-constexpr int MQ_VSET = 4096;
+constexpr int MQ_VSET = 8192;
+constexpr int MQ_VSET_LG = 13;
 __device__ __forceinline__ bool vset_insert(uint32_t* set, uint32_t j)
 {
-    uint32_t h = (j * 2654435769u) >> 20;
+    uint32_t h = (j * 2654435769u) >> (32 - MQ_VSET_LG);
     while (true) {
         const uint32_t old = atomicCAS(&set[h], 0xFFFFFFFFu, j);
         if (old == 0xFFFFFFFFu) return true;
@@ -286,7 +287,7 @@ struct mq_state {
     int kernel_choice = 0;              // 0 auto, 1 streaming kernels only, 2 generic streaming kernel only  // force the generic kernels (parity of the specialised ones)
     int wolff_ring = 512;               // LDS frontier ring entries of the small-lattice Wolff kernel (power of two, 64..2048)
     int wolff_block = 0;                // threads per replica of the global-memory Wolff kernels (0 = automatic)
-    int wolff_vset = 2048;              // O(3)/Phi4 Wolff: clusters up to this size keep their visited set in LDS (0..MQ_VSET/2)
+    int wolff_vset = MQ_VSET / 2;       // Wolff on large lattices: clusters up to this size keep their membership in LDS (0..MQ_VSET/2)
 };
 
 // --------------------------------------------------------------------------
@@ -303,7 +304,7 @@ int mq_fail(mq_ctx* ctx, int code, const std::string& msg);
 // kernel launchers (each returns hipError_t of the launch)
 hipError_t launch_init_regular(mq_state* st, const Group& g, Key2 key);
 hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
-hipError_t launch_measure_regular(mq_state* st, const Group& g);
+hipError_t launch_measure_regular(mq_state* st, const Group& g, long long* isum, double* fsum);
 hipError_t launch_measure_ft_regular(mq_state* st, const Group& g);
 hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
 hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key);
@@ -314,16 +315,16 @@ size_t wtab_bytes();
 hipError_t launch_init_eabits(mq_state* st, const Group& g, Key2 key);
 hipError_t prepare_eabits(mq_state* st);
 hipError_t launch_sweep_eabits_all(mq_state* st, int colour, uint32_t phase, Key2 key);
-hipError_t launch_measure_eabits_all(mq_state* st);
+hipError_t launch_measure_eabits_all(mq_state* st, long long* isum);
 bool msc_supported(const mq_lattice* lat);
 hipError_t msc_upload_classes(mq_state* st, const std::vector<uint32_t>& thr);
 hipError_t launch_sweep_msc(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
 hipError_t launch_init_msc(mq_state* st, const Group& g, Key2 key);
-hipError_t launch_measure_msc(mq_state* st, const Group& g);
+hipError_t launch_measure_msc(mq_state* st, const Group& g, long long* isum);
 
 hipError_t launch_init_csr(mq_state* st, const Group& g, Key2 key);
 hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
-hipError_t launch_measure_csr(mq_state* st, const Group& g);
+hipError_t launch_measure_csr(mq_state* st, const Group& g, long long* isum, double* fsum);
 hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
 
 // Threads of the workgroup that grows one replica's clusters in global memory.
@@ -337,6 +338,21 @@ inline int wolff_block(const mq_state* st, int nwg)
 {
     if (st->wolff_block > 0) return st->wolff_block;
     return 2 * nwg <= st->ctx->num_cus ? 1024 : 256;
+}
+
+// Sweep numbers a state accepts: the Philox phase of a sweep is one 32-bit
+// counter word, 2*sweep + colour on regular lattices and 64*sweep + colour on
+// graphs (up to 64 colour classes), so sweeps must stay below 2^31 or 2^26.
+inline uint64_t sweep_limit(const mq_state* st)
+{
+    for (const auto* l : st->lat)
+        if (l->kind != LAT_REGULAR) return 1ull << 26;
+    return 1ull << 31;
+}
+inline const char* sweep_limit_msg(const mq_state* st)
+{
+    return sweep_limit(st) == (1ull << 26) ? "sweep counter exceeds 2^26 (graph lattices: phase = 64*sweep + colour)"
+                                           : "sweep counter exceeds 2^31 (phase = 2*sweep + colour)";
 }
 
 hipError_t launch_energy_all(mq_state* st);

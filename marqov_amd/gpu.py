@@ -99,8 +99,10 @@ class RegularLattice:
         return sizes[: n.value].copy()
 
     def __del__(self):
+        # a closed context has released the device; its lattices must not be touched
         try:
-            self.ctx.lib.mq_lattice_destroy(self.h)
+            if self.ctx.h:
+                self.ctx.lib.mq_lattice_destroy(self.h)
         except Exception:
             pass
 
@@ -248,6 +250,7 @@ class State:
 
     def __del__(self):
         try:
-            self.ctx.lib.mq_state_destroy(self.h)
+            if self.ctx.h:
+                self.ctx.lib.mq_state_destroy(self.h)
         except Exception:
             pass

@@ -132,3 +132,33 @@ def test_refusals_carry_their_own_message(ctx):
     assert lib.mq_lattice_csr(ctx.h, 0, gpu._ptr(np.zeros(1, np.int32)), gpu._ptr(np.zeros(1, np.int32)),
                               None) != 0
     assert b"NULL argument" in lib.mq_last_error(ctx.h)
+
+
+def test_sweep_counter_limit_per_lattice_kind(ctx):
+    """The Philox phase of a sweep is one 32-bit word: 2*sweep + colour on regular lattices,
+    64*sweep + colour on graphs.  A graph state refuses sweep numbers from 2^26 on (otherwise
+    sweeps s and s + 2^26 would draw the same numbers), through mq_metropolis, mq_emcs and
+    mq_run_graph; a regular state still runs there and refuses from 2^31 on."""
+    from marqov_amd import gpu
+    rp, ci = _ring(8)
+    g = gpu.State(ctx, "ising", [gpu.CsrLattice(ctx, rp, ci)] * 2)
+    g.set_beta(0.4)
+    g.init(seed=1)
+    lim = 1 << 26
+    g.metropolis(1, seed=1, sweep0=lim - 1)  # the last sweep number that fits
+    with pytest.raises(gpu.MarqovError, match="2\\^26"):
+        g.metropolis(1, seed=1, sweep0=lim)
+    with pytest.raises(gpu.MarqovError, match="2\\^26"):
+        g.metropolis(2, seed=1, sweep0=lim - 1)
+    with pytest.raises(gpu.MarqovError, match="2\\^26"):
+        g.emcs(0, 1, 1, lim)  # sweep0 = step * nmetro
+    with pytest.raises(gpu.MarqovError, match="2\\^26"):
+        g.run_graph(1, 1, 1, lim - 1, 0, 2, 1)
+    r = gpu.State(ctx, "ising", [gpu.RegularLattice(ctx, 2, 8)] * 2)
+    r.set_beta(0.4)
+    r.init(seed=1)
+    r.metropolis(1, seed=1, sweep0=lim)
+    r.metropolis(1, seed=1, sweep0=(1 << 31) - 1)
+    with pytest.raises(gpu.MarqovError, match="2\\^31"):
+        r.metropolis(1, seed=1, sweep0=1 << 31)
+    assert np.all(np.isfinite(g.measure())) and np.all(np.isfinite(r.measure()))

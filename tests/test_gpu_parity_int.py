@@ -353,3 +353,47 @@ def test_wolff_wave_chained_calls_bit_exact(ctx, model, dim, L):
             assert np.array_equal(st.download(r), cur[r]), (call, r)
         if call == 0:
             assert sizes.max() * 8 > N
+
+
+@pytest.mark.parametrize("model", ["ising", "potts"])
+@pytest.mark.parametrize("vset", ["0", "16", "4096"])
+@pytest.mark.parametrize("dim,L", [(2, 256), (3, 32)])
+def test_wolff_membership_modes_bit_exact(ctx, model, vset, dim, L, monkeypatch):
+    """The large-lattice Wolff kernel keeps a cluster's membership in an LDS set of packed
+    sites while the cluster cannot outgrow MARQOV_AMD_WOLFF_VSET sites (the alignment test is
+    then a plain load and the flip a byte store) and in the spin bytes after that (one global
+    atomic per claim).  With the set off (0), left early by nearly every cluster (16) and at
+    its largest (4096), three chained calls of four clusters each equal the CPU replay on
+    ordered starts with 10% defects above the bond-percolation threshold (cluster sizes from a
+    few sites to most of the lattice)."""
+    from marqov_amd import gpu
+    monkeypatch.setenv("MARQOV_AMD_WOLFF_VSET", vset)
+    monkeypatch.setenv("MARQOV_AMD_KERNELS", "1")
+    N = L ** dim
+    lat = gpu.RegularLattice(ctx, dim, L)
+    st = gpu.State(ctx, model, [lat] * 3, J=-1.0, q=3)
+    # bond probability 1 - e^-0.92 = 0.60 (2D) and 1 - e^-0.36 = 0.30 (3D), above the thresholds
+    # 0.5 and 0.25: seeds on the majority grow clusters of much of the lattice, seeds on defects
+    # stay small
+    beta = {2: 0.46, 3: 0.18}[dim] * (1 if model == "ising" else 2)
+    st.set_beta(beta)
+    cur = []
+    for r in range(3):
+        s0 = (np.ones(N, np.int8) if model == "ising" else np.zeros(N, np.int8))
+        s0[np.random.default_rng(80 + r).random(N) < 0.1] = -1 if model == "ising" else 2
+        st.upload(r, s0)
+        cur.append(s0)
+    allsz = []
+    for call in range(3):
+        sizes = st.wolff(4, seed=23, cluster0=4 * call)
+        allsz.append(sizes)
+        for r in range(3):
+            if model == "ising":
+                cur[r], esz = ob.wolff_ising(dim, L, cur[r], beta, -1.0, 4, 23, 4 * call, r)
+            else:
+                cur[r], esz = ob.wolff_potts(dim, L, cur[r], 3, beta, -1.0, 4, 23, 4 * call, r)
+            assert np.array_equal(sizes[r], esz), (call, r, sizes[r], esz)
+            assert np.array_equal(st.download(r), cur[r]), (call, r)
+    allsz = np.concatenate(allsz, axis=1)
+    print(f"[wolff membership] {model} {dim}D L={L} vset={vset}: sizes {np.sort(allsz.ravel()).tolist()}", flush=True)
+    assert allsz.max() > 4096 and allsz.min() < 4096
