@@ -158,6 +158,20 @@ MQ_API int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t cluster
 MQ_API int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, uint64_t step, double* mean_cluster);
 /* observables, FP64 (observables.h:32-66,173-352; Potts.h:40-65): out[r*MQ_NOBS+k] */
 MQ_API int mq_measure(mq_state* st, double* out);
+/* No counterpart in the reference: mq_measure for callers that cannot allocate
+ * (code under HIP graph capture, tight serving or sampling loops, harnesses that
+ * forbid allocation).  Writes the observables of mq_measure to DEVICE memory,
+ * d_out[r*MQ_NOBS+k] (nrep*MQ_NOBS doubles), asynchronously on the context's
+ * stream (mq_sync before reading them on the host); integer models give the
+ * same bits as mq_measure.  It allocates nothing, copies nothing to the host and
+ * does not synchronise: its partial sums live in one static device array per
+ * GPU (128 B per replica, at most MQ_NOALLOC_MAX_REP replicas, 4 MiB, held
+ * until the process ends), which each call clears and refills.  Calls that
+ * overlap on one GPU (from contexts on different streams) share that array and
+ * race; order them, e.g. by using one context per GPU for these calls.  States
+ * with more than MQ_NOALLOC_MAX_REP replicas are refused. */
+#define MQ_NOALLOC_MAX_REP 32768
+MQ_API int mq_measure_noalloc(mq_state* st, double* d_out);
 /* Fourier component of the magnetisation at the smallest wavevector along each
  * axis, MagFTComp (observables.h:104-170; the reference measures it for Ising,
  * axes 0 and 1, Ising.h:82-94, and for Phi4, axes 0-2, Phi4.h:116-140):

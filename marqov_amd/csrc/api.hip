@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <numeric>
 
 #include "mq_internal.hpp"
@@ -166,6 +167,37 @@ int ensure_wolff_scratch(mq_state* st, int nclusters)
 
 }  // namespace
 
+// ============================================================== scratch
+// Partial sums of mq_measure_noalloc: per replica 8 integer sums, then (at word
+// MQ_NOALLOC_MAX_REP*8) 8 FP64 sums, 128 B per replica, 4 MiB in all.  Every
+// device the library runs on holds its own copy until the process ends.  In host
+// code `scratch` names a host-side stand-in, not device memory: host code only
+// ever uses the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned long long scratch[(size_t)MQ_NOALLOC_MAX_REP * 16];
+
+namespace {
+constexpr int MQ_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_of_dev[MQ_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up once
+// per device.
+hipError_t scratch_address(int device, void** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= MQ_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch_of_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_of_dev[device] = p;
+    }
+    *out = g_scratch_of_dev[device];
+    return hipSuccess;
+}
+}  // namespace
+
 // ============================================================== context
 extern "C" int mq_ctx_create(int device, mq_ctx** out)
 {
@@ -188,6 +220,13 @@ extern "C" int mq_ctx_create(int device, mq_ctx** out)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->num_cus = prop.multiProcessorCount;
         if (prop.maxGridSize[1] > 0) c->max_grid_y = prop.maxGridSize[1];
+    }
+    if (scratch_address(device, &c->scratch) != hipSuccess) {
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return MQ_ERR_HIP;
     }
     *out = c;
     return MQ_OK;
@@ -922,6 +961,83 @@ extern "C" int mq_measure(mq_state* st, double* out)
         out[(size_t)r * MQ_NOBS + 1] = m;
         out[(size_t)r * MQ_NOBS + 2] = m * m;
         out[(size_t)r * MQ_NOBS + 3] = m * m * m * m;
+    }
+    return MQ_OK;
+}
+
+// Observables of replicas rep0 .. rep0+nrep-1 (one lattice of N sites) from the
+// sums of the measurement kernels, with the arithmetic and the operation order
+// of mq_measure's host loop (no contraction), so integer models give the same
+// bits.
+__global__ void k_finish_obs(const long long* __restrict__ isum, const double* __restrict__ fsum,
+                             const double* __restrict__ betad, int rep0, int nrep, double N, int model, int q,
+                             double J, double mass, double lambda, double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int r = rep0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= rep0 + nrep) return;
+    const long long* I = isum + (size_t)r * 8;
+    const double* F = fsum + (size_t)r * 8;
+    double e, m;
+    if (model == MQ_POTTS) {
+        e = J * (double)I[0] / N;
+        long long mx = 0, counted = 0;
+        const int qc = q < 7 ? q : 7;
+        for (int k = 0; k < qc; ++k) {
+            mx = I[1 + k] > mx ? I[1 + k] : mx;
+            counted += I[1 + k];
+        }
+        if (q == 8) {
+            const long long rest = (long long)N - counted;
+            mx = rest > mx ? rest : mx;
+        }
+        m = ((double)q * (double)mx - N) / (N * (q - 1));
+    } else if (model == MQ_ISING || model == MQ_EA_ISING) {
+        e = J * (double)I[0] / N;
+        m = fabs((double)I[1]) / N;
+    } else {
+        e = J * F[0] / N;
+        if (model == MQ_PHI4) {
+            const double b = betad[r];
+            e += (mass / b) * F[4] / N + (lambda / b) * F[5] / N;
+        }
+        m = sqrt(F[1] * F[1] + F[2] * F[2] + F[3] * F[3]) / N;
+    }
+    out[(size_t)r * MQ_NOBS + 0] = e;
+    out[(size_t)r * MQ_NOBS + 1] = m;
+    out[(size_t)r * MQ_NOBS + 2] = m * m;
+    out[(size_t)r * MQ_NOBS + 3] = m * m * m * m;
+}
+
+extern "C" int mq_measure_noalloc(mq_state* st, double* d_out)
+{
+    if (!st) return MQ_ERR_ARG;
+    if (!d_out) return mq_fail(st->ctx, MQ_ERR_ARG, "out is NULL");
+    if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
+    if (st->nrep > MQ_NOALLOC_MAX_REP)
+        return mq_fail(st->ctx, MQ_ERR_ARG,
+                       "mq_measure_noalloc: more replicas than its static scratch holds (MQ_NOALLOC_MAX_REP = " +
+                           std::to_string(MQ_NOALLOC_MAX_REP) + ")");
+    if (!st->ctx->scratch) return mq_fail(st->ctx, MQ_ERR_STATE, "scratch address unknown");
+    hipStream_t s = st->ctx->stream;
+    long long* isum = static_cast<long long*>(st->ctx->scratch);
+    double* fsum = reinterpret_cast<double*>(isum + (size_t)MQ_NOALLOC_MAX_REP * 8);
+    // the array keeps the previous call's sums: clear this state's part with fill kernels
+    MQ_HIP(st->ctx, zero64(isum, (int64_t)st->nrep * 8, s));
+    MQ_HIP(st->ctx, zero64(fsum, (int64_t)st->nrep * 8, s));
+    if (st->eabits) MQ_HIP(st->ctx, launch_measure_eabits_all(st, isum));
+    for (const auto& g : st->groups) {
+        if (st->eabits) break;
+        hipError_t e = st->msc ? launch_measure_msc(st, g, isum)
+                       : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g, isum, fsum)
+                                                      : launch_measure_csr(st, g, isum, fsum);
+        if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
+    }
+    for (const auto& g : st->groups) {
+        hipLaunchKernelGGL(k_finish_obs, dim3((unsigned)((g.nrep + 255) / 256)), dim3(256), 0, s, isum, fsum,
+                           st->d_betad, g.rep0, g.nrep, (double)g.lat->N, st->p.model, st->p.q, st->p.J, st->p.mass,
+                           st->p.lambda, d_out);
+        MQ_HIP(st->ctx, hipGetLastError());
     }
     return MQ_OK;
 }

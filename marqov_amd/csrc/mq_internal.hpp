@@ -185,6 +185,7 @@ struct mq_ctx {
     std::string err;
     int num_cus = 256;
     int max_grid_y = 65535;  // workgroups along y, the replica dimension of most launches
+    void* scratch = nullptr; // device address of this device's static scratch array (api.hip)
 };
 
 struct mq_lattice {
