@@ -103,8 +103,11 @@ MQ_API int mq_lattice_csr(mq_ctx* ctx, int64_t nsites, const int32_t* row_ptr, c
  * with col_idx = NULL to learn nnz first.  2 <= n <= 2^30, 0 < radius < 1/2. */
 MQ_API int mq_rgg_build(mq_ctx* ctx, int dim, int64_t n, double radius, uint64_t seed, int64_t* nnz,
                         int32_t* row_ptr, int32_t* col_idx, int64_t col_cap, double* coords);
-/* number of colour classes and their sizes (ncolors <= 64) */
-MQ_API int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes64);
+/* number of colour classes and, when sizes is non-NULL, their sizes
+ * (sizes[ncolors]: call with sizes = NULL to learn the count first).  A graph
+ * of more than 64 classes sweeps with phase stride*sweep + colour, stride the
+ * next power of two of the class count (at most 65536 classes). */
+MQ_API int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes);
 MQ_API int64_t mq_lattice_nsites(const mq_lattice* lat);
 /* colour class of every site in the reference's site order (regular lattices:
  * (x0+x1+x2)&1; graphs: greedy colouring in vertex order) */
@@ -143,8 +146,10 @@ MQ_API int mq_state_download(mq_state* st, int rep, void* host, int64_t bytes);
  * replaces EMCS's Metropolis loop (emcs.h:93-106) with Metropolis::move
  * (metropolis.h:86-126).  Sweep t uses the 32-bit Philox counter phase
  * 2*(sweep0+t) + colour on regular lattices and 64*(sweep0+t) + colour on
- * graphs, so every sweep number must stay below 2^31 (regular) or 2^26 (a
- * state with any graph lattice); calls that would pass it are refused. */
+ * graphs (stride*(sweep0+t) + colour for graphs of more than 64 colour classes,
+ * see mq_lattice_colors), so every sweep number must stay below 2^31 (regular),
+ * 2^26 (graphs of up to 64 classes) or 2^32/stride; calls that would pass it
+ * are refused. */
 MQ_API int mq_metropolis(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0);
 /* nclusters single-cluster Wolff moves of every replica; replaces EMCS's
  * cluster loop (emcs.h:83-89) with Wolff::move (wolff.h:48-125, Ising.h:122-169,
@@ -181,7 +186,10 @@ MQ_API int mq_measure_noalloc(mq_state* st, double* d_out);
 MQ_API int mq_measure_ft(mq_state* st, double* out);
 /* total energy per replica (sum over bonds, FP64), as Core::calcAction/beta (core.h:1145-1151) */
 MQ_API int mq_energy(mq_state* st, double* out);
-/* FP64 renormalisation of O(3) spins to unit length */
+/* FP64 renormalisation of Heisenberg (O(3)) spins to unit length: FP32
+ * reflections about a random direction drift |s| by rounding, where the
+ * reference's doubles keep it (Heisenberg.h:191-243).  The C++ GpuEMCS calls it
+ * every 1000 EMCS by default; Phi4 fields have free length and are refused. */
 MQ_API int mq_renormalize(mq_state* st);
 
 /* ---- parallel tempering ---------------------------------------------- */
@@ -213,12 +221,22 @@ MQ_API int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep);
  * applies), 1 = streaming kernels only, 2 = generic streaming kernel only.
  * block: threads per block of the streaming sweep kernels (64..512, power of 2);
  * xcd_remap: give each XCD a contiguous slab of rows; lds_block: threads per
- * block of the LDS multi-sweep kernel (64..512).  Results do not depend on these. */
+ * block of the LDS multi-sweep kernel (64..512).
+ * Wolff: wolff_block: threads of the workgroup growing one replica's clusters on
+ * large lattices and graphs (0 = automatic: 1024 up to half as many replicas as
+ * compute units, else 256; or 256, 512, 1024); wolff_ring: LDS frontier entries
+ * per replica of the small-lattice kernel (power of two, 64..2048, default 512);
+ * wolff_vset: clusters up to this many sites keep their membership in an LDS
+ * hash set, larger ones in global memory (0..4096, default 4096).
+ * Results do not depend on any of these. */
 typedef struct mq_tuning {
     int kernels;
     int block;
     int xcd_remap;
     int lds_block;
+    int wolff_block;
+    int wolff_ring;
+    int wolff_vset;
 } mq_tuning;
 MQ_API int mq_state_set_tuning(mq_state* st, const mq_tuning* t);
 MQ_API int mq_state_get_tuning(const mq_state* st, mq_tuning* t);

@@ -150,6 +150,7 @@ public:
         for (auto* l : lattices) ls.push_back(l->get());
         nrep_ = (int)ls.size();
         const mq_model_params p = params_of(model);
+        model_ = p.model;
         ctx.check(mq_state_create(ctx.get(), &p, nrep_, ls.data(), &h_), "mq_state_create");
     }
     ~Batch() { mq_state_destroy(h_); }
@@ -157,7 +158,11 @@ public:
     Batch& operator=(const Batch&) = delete;
     mq_state* get() const { return h_; }
     int nrep() const { return nrep_; }
+    int model() const { return model_; }
     void check(int rc, const char* what) const { ctx_.check(rc, what); }
+    // FP64 renormalisation of Heisenberg spins to unit length (FP32 reflections
+    // about a random direction drift |s| by rounding)
+    void renormalize() { ctx_.check(mq_renormalize(h_), "mq_renormalize"); }
 
     void set_beta(const std::vector<double>& beta) { ctx_.check(mq_state_set_beta(h_, beta.data()), "mq_state_set_beta"); }
     void init(uint64_t seed) { ctx_.check(mq_state_init(h_, seed), "mq_state_init"); }
@@ -224,27 +229,49 @@ private:
     Context& ctx_;
     mq_state* h_ = nullptr;
     int nrep_ = 0;
+    int model_ = 0;
 };
 
 // One elementary Monte Carlo step of every replica (emcs.h:69-109).
 // `step` numbers the EMCS so that every step draws fresh Philox counters; a
 // run resumed from a checkpoint passes the number of EMCS already done, and
 // then continues the same trajectory.
+// Heisenberg spins are renormalised to unit length in FP64 after every
+// `renorm_every`-th EMCS (by EMCS number, so a resumed run renormalises at the
+// same steps; 0 = never): Wolff reflections about a random direction are FP32
+// and drift |s| by rounding, while the reference's doubles keep it at 1.
 class GpuEMCS {
 public:
-    GpuEMCS(Batch& batch, uint64_t seed, uint64_t step0 = 0) : b_(batch), seed_(seed), step_(step0) {}
+    static constexpr int kRenormEvery = 1000;
+    GpuEMCS(Batch& batch, uint64_t seed, uint64_t step0 = 0, int renorm_every = kRenormEvery)
+        : b_(batch), seed_(seed), step_(step0), renorm_every_(renorm_every)
+    {}
+    // one step, returning each replica's mean cluster size (one device-to-host
+    // copy and a stream synchronisation, as Core reports it per EMCS)
     std::vector<double> move(int ncluster, int nmetro)
     {
         std::vector<double> mean(b_.nrep());
         b_.check(mq_emcs(b_.get(), ncluster, nmetro, seed_, step_++, mean.data()), "mq_emcs");
+        after_step();
         return mean;
+    }
+    // the same step without reading anything back (warm-up): no copy, no synchronisation
+    void advance(int ncluster, int nmetro)
+    {
+        b_.check(mq_emcs(b_.get(), ncluster, nmetro, seed_, step_++, nullptr), "mq_emcs");
+        after_step();
     }
     uint64_t steps() const { return step_; }
 
 private:
+    void after_step()
+    {
+        if (renorm_every_ > 0 && b_.model() == MQ_HEISENBERG && step_ % (uint64_t)renorm_every_ == 0) b_.renormalize();
+    }
     Batch& b_;
     uint64_t seed_;
     uint64_t step_;
+    int renorm_every_;
 };
 
 }  // namespace marqov_amd

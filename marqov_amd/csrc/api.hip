@@ -295,13 +295,13 @@ extern "C" int mq_lattice_regular(mq_ctx* ctx, int dim, int L, mq_lattice** out)
 
 extern "C" int64_t mq_lattice_nsites(const mq_lattice* lat) { return lat ? lat->N : -1; }
 
-extern "C" int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes64)
+extern "C" int mq_lattice_colors(const mq_lattice* lat, int* ncolors, int64_t* sizes)
 {
     if (!lat) return MQ_ERR_ARG;
     if (!ncolors) return mq_fail(lat->ctx, MQ_ERR_ARG, "mq_lattice_colors: ncolors is NULL");
     *ncolors = lat->ncolors;
-    if (sizes64)
-        for (int c = 0; c < lat->ncolors; ++c) sizes64[c] = lat->color_off[c + 1] - lat->color_off[c];
+    if (sizes)
+        for (int c = 0; c < lat->ncolors; ++c) sizes[c] = lat->color_off[c + 1] - lat->color_off[c];
     return MQ_OK;
 }
 
@@ -412,18 +412,6 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     st->spin_total = off;
     st->tbl = 2 * qmax_of(st) + 1;
     if (const char* kc = std::getenv("MARQOV_AMD_KERNELS")) st->kernel_choice = std::atoi(kc);
-    if (const char* wr = std::getenv("MARQOV_AMD_WOLFF_RING")) {
-        const int v = std::atoi(wr);
-        if (v >= 64 && v <= 2048 && (v & (v - 1)) == 0) st->wolff_ring = v;
-    }
-    if (const char* wv = std::getenv("MARQOV_AMD_WOLFF_VSET")) {
-        const int v = std::atoi(wv);
-        if (v >= 0 && v <= MQ_VSET / 2) st->wolff_vset = v;
-    }
-    if (const char* wb = std::getenv("MARQOV_AMD_WOLFF_BLOCK")) {
-        const int v = std::atoi(wb);
-        if (v == 256 || v == 512 || v == 1024) st->wolff_block = v;
-    }
     st->rep_sample.resize(nrep);
     std::iota(st->rep_sample.begin(), st->rep_sample.end(), 0);
     st->nsamples = nrep;
@@ -818,7 +806,7 @@ int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep
         for (const auto& g : st->groups) {
             const int nc = g.lat->ncolors;
             for (int c = 0; c < nc; ++c) {
-                const uint32_t phase = (uint32_t)(sw * (uint64_t)(g.lat->kind == LAT_REGULAR ? 2 : 64) + c);
+                const uint32_t phase = (uint32_t)(sw * (uint64_t)g.lat->color_stride + c);
                 hipError_t e = st->msc ? launch_sweep_msc(st, g, c, phase, key)
                                : (g.lat->kind == LAT_REGULAR) ? launch_sweep_regular(st, g, c, phase, key)
                                                               : launch_sweep_csr(st, g, c, phase, key);
@@ -1102,10 +1090,18 @@ extern "C" int mq_state_set_tuning(mq_state* st, const mq_tuning* t)
     auto pow2 = [](int v) { return v >= 64 && v <= 512 && (v & (v - 1)) == 0; };
     if (t->kernels < 0 || t->kernels > 2 || !pow2(t->block) || !pow2(t->lds_block) || (t->xcd_remap != 0 && t->xcd_remap != 1))
         return mq_fail(st->ctx, MQ_ERR_ARG, "tuning: kernels in 0..2, block and lds_block powers of two in 64..512");
+    const bool ring_ok = t->wolff_ring >= 64 && t->wolff_ring <= 2048 && (t->wolff_ring & (t->wolff_ring - 1)) == 0;
+    if ((t->wolff_block != 0 && t->wolff_block != 256 && t->wolff_block != 512 && t->wolff_block != 1024) || !ring_ok ||
+        t->wolff_vset < 0 || t->wolff_vset > MQ_VSET / 2)
+        return mq_fail(st->ctx, MQ_ERR_ARG,
+                       "tuning: wolff_block 0, 256, 512 or 1024; wolff_ring a power of two in 64..2048; wolff_vset in 0..4096");
     st->kernel_choice = t->kernels;
     st->tune_block = t->block;
     st->tune_xcd = t->xcd_remap;
     st->lds_block = t->lds_block;
+    st->wolff_block = t->wolff_block;
+    st->wolff_ring = t->wolff_ring;
+    st->wolff_vset = t->wolff_vset;
     return MQ_OK;
 }
 
@@ -1117,5 +1113,8 @@ extern "C" int mq_state_get_tuning(const mq_state* st, mq_tuning* t)
     t->block = st->tune_block;
     t->xcd_remap = st->tune_xcd;
     t->lds_block = st->lds_block;
+    t->wolff_block = st->wolff_block;
+    t->wolff_ring = st->wolff_ring;
+    t->wolff_vset = st->wolff_vset;
     return MQ_OK;
 }

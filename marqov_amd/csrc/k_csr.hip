@@ -747,27 +747,34 @@ extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, co
             }
         (void)fill;
     }
-    // greedy colouring in vertex order
+    // greedy colouring in vertex order: the smallest colour no earlier neighbour has
+    // (at most degree + 1 colours)
     std::vector<int32_t> colour(N, -1);
+    std::vector<int64_t> seen_by;  // seen_by[c] == u: colour c is taken by a neighbour of u
     int ncol = 0;
     for (int64_t u = 0; u < N; ++u) {
-        uint64_t used = 0;
+        const int deg = row_ptr[u + 1] - row_ptr[u];
+        if ((int)seen_by.size() < deg + 1) seen_by.resize(deg + 1, -1);
         for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) {
             const int c = colour[col_idx[k]];
-            if (c >= 0) used |= 1ull << c;
+            if (c >= 0 && c <= deg) seen_by[c] = u;
         }
         int c = 0;
-        while (c < 64 && (used >> c) & 1ull) ++c;
-        if (c >= MQ_MAX_COLORS) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "greedy colouring needs more than 64 colours");
+        while (seen_by[c] == u) ++c;
         colour[u] = c;
         ncol = std::max(ncol, c + 1);
     }
+    uint32_t stride = 64;
+    while (stride < (uint32_t)ncol) stride <<= 1;
+    if (stride > MQ_MAX_COLOR_STRIDE)
+        return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "greedy colouring needs more than 65536 colours");
     auto* l = new mq_lattice();
     l->ctx = ctx;
     l->kind = LAT_CSR;
     l->N = N;
     l->nnz = nnz;
     l->ncolors = ncol;
+    l->color_stride = stride;
     l->color_off.assign(ncol + 1, 0);
     for (int64_t u = 0; u < N; ++u) l->color_off[colour[u] + 1]++;
     for (int c = 0; c < ncol; ++c) l->color_off[c + 1] += l->color_off[c];

@@ -306,7 +306,9 @@ extern "C" int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep)
 extern "C" int mq_renormalize(mq_state* st)
 {
     if (!st) return MQ_ERR_ARG;
-    if (st->site_bytes != 16) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "renormalisation applies to O(3) spins");
+    if (st->p.model != MQ_HEISENBERG)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED,
+                       "renormalisation applies to O(3) spins (Heisenberg; Phi4 fields have free length)");
     const int64_t n = st->spin_total / 16;
     hipLaunchKernelGGL(k_renorm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st->ctx->stream,
                        reinterpret_cast<float4*>(st->d_spins), n);
@@ -321,8 +323,13 @@ extern "C" int mq_run_graph(mq_state* st, int nsweeps, int ntemp, uint64_t seed,
     if (st->beta.empty()) return mq_fail(st->ctx, MQ_ERR_STATE, "beta not set");
     if (st->p.model == MQ_EA_ISING && !st->d_bonds) return mq_fail(st->ctx, MQ_ERR_STATE, "EA couplings not set");
     if (nsweeps < 0 || steps <= 0 || repeats <= 0) return mq_fail(st->ctx, MQ_ERR_ARG, "bad graph sizes");
-    const bool regular = st->lat[0]->kind == LAT_REGULAR;
-    const uint64_t per_sweep = regular ? 2 : 64;
+    // the replay advances every sweep kernel's phase by one device offset
+    const uint64_t per_sweep = max_color_stride(st);
+    for (const auto* l : st->lat)
+        if (l->color_stride != per_sweep)
+            return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED,
+                           "graph replay needs one colour stride for all lattices of the state (graphs of up to 64 "
+                           "colour classes, or of one class count)");
     const uint64_t total = sweep0 + (uint64_t)nsweeps * steps * repeats;
     if (sweep0 >= sweep_limit(st) || total > sweep_limit(st)) return mq_fail(st->ctx, MQ_ERR_ARG, sweep_limit_msg(st));
     if (ntemp > 1) {

@@ -24,7 +24,10 @@ enum : uint32_t {
 
 enum { LAT_REGULAR = 0, LAT_CSR = 1 };
 
-#define MQ_MAX_COLORS 64
+// Colour classes of a graph lattice: the Philox phase of a sweep is
+// stride*sweep + colour, stride = 64 up to 64 classes, else the next power of two
+// (at most MQ_MAX_COLOR_STRIDE, which leaves 2^16 sweep numbers).
+#define MQ_MAX_COLOR_STRIDE (1 << 16)
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  Key bumps are uniform per launch, so
@@ -195,6 +198,7 @@ struct mq_lattice {
     int64_t N = 0;
     // CSR graphs: vertices renumbered so that each colour class is contiguous
     int ncolors = 2;
+    uint32_t color_stride = 2;       // phase = color_stride*sweep + colour (2 on regular lattices)
     std::vector<int64_t> color_off;  // ncolors+1 (positions in the new numbering)
     std::vector<int32_t> pos_of;     // original vertex -> new position
     std::vector<int32_t> vert_of;    // new position -> original vertex
@@ -342,18 +346,22 @@ inline int wolff_block(const mq_state* st, int nwg)
 }
 
 // Sweep numbers a state accepts: the Philox phase of a sweep is one 32-bit
-// counter word, 2*sweep + colour on regular lattices and 64*sweep + colour on
-// graphs (up to 64 colour classes), so sweeps must stay below 2^31 or 2^26.
-inline uint64_t sweep_limit(const mq_state* st)
+// counter word, color_stride*sweep + colour (2 on regular lattices, 64 or more
+// on graphs), so sweeps must stay below 2^32 / stride: 2^31 on regular lattices,
+// 2^26 on graphs of up to 64 colour classes.
+inline uint32_t max_color_stride(const mq_state* st)
 {
-    for (const auto* l : st->lat)
-        if (l->kind != LAT_REGULAR) return 1ull << 26;
-    return 1ull << 31;
+    uint32_t m = 2;
+    for (const auto* l : st->lat) m = l->color_stride > m ? l->color_stride : m;
+    return m;
 }
-inline const char* sweep_limit_msg(const mq_state* st)
+inline uint64_t sweep_limit(const mq_state* st) { return (1ull << 32) / max_color_stride(st); }
+inline std::string sweep_limit_msg(const mq_state* st)
 {
-    return sweep_limit(st) == (1ull << 26) ? "sweep counter exceeds 2^26 (graph lattices: phase = 64*sweep + colour)"
-                                           : "sweep counter exceeds 2^31 (phase = 2*sweep + colour)";
+    const uint32_t s = max_color_stride(st);
+    int lg = 0;
+    while ((1ull << lg) < sweep_limit(st)) ++lg;
+    return "sweep counter exceeds 2^" + std::to_string(lg) + " (phase = " + std::to_string(s) + "*sweep + colour)";
 }
 
 hipError_t launch_energy_all(mq_state* st);

@@ -14,8 +14,8 @@
 //   lattice/util/io.h:57-105): first line the number of coordinates per
 //   vertex, second line skipped, then one line per vertex with that many
 //   coordinates and the vertex's neighbours, tab-separated; neighbour indices
-//   are shifted by idxoffs.  Reading stops at the end of the file or at an
-//   empty line.
+//   are shifted by idxoffs.  Every line up to the end of the file is a vertex;
+//   an empty line is a vertex without neighbours.
 #pragma once
 
 #include <algorithm>
@@ -82,10 +82,11 @@ inline Csr read_csv_graph(const std::string& path, int idxoffs = 0)
         throw std::runtime_error("graph file " + path + ": first line must be the number of coordinates");
     }
     std::getline(in, row);  // second header line
+    // every further line is a vertex, an empty one a vertex without coordinates or
+    // neighbours (import_geometry, lattice/util/io.h:99-123)
     std::vector<std::vector<int32_t>> nb;
     while (std::getline(in, row)) {
         if (!row.empty() && row.back() == '\r') row.pop_back();
-        if (row.empty()) break;
         std::istringstream ss(row);
         std::string w;
         int k = 0;

@@ -94,7 +94,8 @@ class RegularLattice:
 
     def colors(self):
         n = ctypes.c_int()
-        sizes = np.zeros(64, np.int64)
+        self.ctx.check(self.ctx.lib.mq_lattice_colors(self.h, ctypes.byref(n), None), "mq_lattice_colors")
+        sizes = np.zeros(max(n.value, 1), np.int64)
         self.ctx.check(self.ctx.lib.mq_lattice_colors(self.h, ctypes.byref(n), _ptr(sizes)), "mq_lattice_colors")
         return sizes[: n.value].copy()
 
@@ -235,14 +236,20 @@ class State:
         self.ctx.check(self.ctx.lib.mq_pt_get_temps(self.h, _ptr(t)), "mq_pt_get_temps")
         return t
 
-    def set_tuning(self, kernels=0, block=256, xcd_remap=1, lds_block=256):
-        t = L.Tuning(kernels, block, xcd_remap, lds_block)
+    def set_tuning(self, **kw):
+        """Launch shapes (mq_tuning); fields not given keep their current values."""
+        cur = self.get_tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise TypeError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(kw)
+        t = L.Tuning(*[int(cur[f]) for f, _ in L.Tuning._fields_])
         self.ctx.check(self.ctx.lib.mq_state_set_tuning(self.h, ctypes.byref(t)), "mq_state_set_tuning")
 
     def get_tuning(self):
         t = L.Tuning()
         self.ctx.check(self.ctx.lib.mq_state_get_tuning(self.h, ctypes.byref(t)), "mq_state_get_tuning")
-        return {"kernels": t.kernels, "block": t.block, "xcd_remap": t.xcd_remap, "lds_block": t.lds_block}
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_}
 
     def run_graph(self, nsweeps, ntemp, seed, sweep0, step0, steps, repeats):
         self.ctx.check(self.ctx.lib.mq_run_graph(self.h, nsweeps, ntemp, seed, sweep0, step0, steps, repeats),

@@ -92,12 +92,14 @@ def read_csv_graph(path, idxoffs=0):
         ncoords = int(f.readline())
         f.readline()
         coords, nbrs = [], []
+        # every further line is a vertex; an empty line is one without coordinates or
+        # neighbours, as import_geometry reads it (lattice/util/io.h:99-123): its
+        # coordinates are NaN here
         for line in f:
-            line = line.rstrip("\n")
-            if not line:
-                break
-            w = line.split("\t")
-            coords.append([float(x) for x in w[:ncoords]])
+            line = line.rstrip("\n").rstrip("\r")
+            w = [x for x in line.split("\t") if x != ""]
+            c = [float(x) for x in w[:ncoords]]
+            coords.append(c + [float("nan")] * (ncoords - len(c)))
             nbrs.append([int(x) - idxoffs for x in w[ncoords:]])
     row_ptr = np.zeros(len(nbrs) + 1, np.int64)
     row_ptr[1:] = np.cumsum([len(x) for x in nbrs])

@@ -587,7 +587,9 @@ EXPORT void orc_philox4x32_10_export(const uint32_t* ctr, const uint32_t* key, u
 /* ------------------------------------------------------------ CSR graphs */
 /* Colour-class sweeps on a general graph (original vertex numbering); the
  * colour of every vertex is an input (the library's greedy colouring).
- * Ising/EA: counter (vertex, 64*sweep + colour, replica >> 3, METRO), 16-bit
+ * The sweep's phase is stride*sweep + colour, with stride 64 for up to 64
+ * colour classes and the next power of two of the class count above that.
+ * Ising/EA: counter (vertex, phase, replica >> 3, METRO), 16-bit
  * half (replica & 7) against the threshold's high half, TIE stream on equality;
  * Potts: counter (vertex, phase, replica, METRO), proposes (word0*Q)>>32 and
  * accepts iff word 1 < T. */
@@ -595,9 +597,11 @@ EXPORT void orc_csr_sweeps(int64_t N, const int32_t* rp, const int32_t* ci, cons
                            int8_t* s, const int8_t* bonds, const uint32_t* thi, const uint32_t* tlo, int qmax,
                            int potts, int Q, int nsweeps, uint64_t seed, uint32_t sweep0, uint32_t rep)
 {
+    uint32_t stride = 64u;
+    while (stride < (uint32_t)ncol) stride <<= 1;
     for (int t = 0; t < nsweeps; ++t)
         for (int c = 0; c < ncol; ++c) {
-            uint32_t phase = 64u * (sweep0 + (uint32_t)t) + (uint32_t)c;
+            uint32_t phase = stride * (sweep0 + (uint32_t)t) + (uint32_t)c;
             for (int64_t v = 0; v < N; ++v) {
                 if (colour[v] != c) continue;
                 uint32_t w[4];

@@ -347,3 +347,89 @@ def test_gpu_ini_L2_vs_exact(tmp_path):
             se = g.std(ddof=1) / np.sqrt(len(g))
             print(f"L=2 beta={beta} {what}: {g.mean():.5f} +- {se:.5f} exact {want:.5f}", flush=True)
             assert abs(g.mean() - want) < 4 * se + 1e-9, (beta, what, g.mean(), se, want)
+
+
+def test_csv_empty_lines_are_isolated_vertices(tmp_path):
+    """import_geometry (lattice/util/io.h:99-123) reads every line after the two header lines
+    as a vertex, an empty line as one with no coordinates and no neighbours; a file ending
+    in an empty line therefore ends with an isolated vertex.  Both readers (C++ front-end,
+    Python) read it so, also with CRLF line ends."""
+    from marqov_amd import graphs
+    lines = ["2", "coordinates and neighbours",
+             "0.1\t0.2\t1\t3", "0.3\t0.4\t0", "", "0.5\t0.6\t0", ""]
+    for eol in ("\n", "\r\n"):
+        path = str(tmp_path / f"iso{len(eol)}.csv")
+        with open(path, "w", newline="") as f:
+            f.write(eol.join(lines) + eol)
+        want_rp = np.array([0, 2, 3, 3, 4, 4], np.int32)
+        want_ci = np.array([1, 3, 0, 0], np.int32)
+        rp, ci, xy = graphs.read_csv_graph(path)
+        assert np.array_equal(rp, want_rp) and np.array_equal(ci, want_ci), (rp, ci)
+        assert xy.shape == (5, 2) and np.isnan(xy[2]).all() and xy[3, 1] == 0.6
+        out = subprocess.run([LCHK, "csv", path], capture_output=True, text=True, check=True).stdout.splitlines()
+        rp2, ci2 = _read_csr(out, 1)
+        assert np.array_equal(rp2, want_rp) and np.array_equal(ci2, want_ci), (rp2, ci2)
+
+
+LCHK_SAN = os.path.join(ROOT, "tests", "cpp", "lattice_check_san")
+MQG_SAN = os.path.join(ROOT, "tests", "cpp", "marqov_gpu_san")
+SAN_ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:exitcode=97",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1:exitcode=97")
+
+
+def _sanitizer_clean(r):
+    bad = ("AddressSanitizer", "runtime error:", "LeakSanitizer")
+    return r.returncode != 97 and not any(b in r.stderr for b in bad)
+
+
+def test_host_parsers_under_address_and_ub_sanitizers(tmp_path):
+    """The .ini registry reader and planner (registry.hpp, marqov_gpu.cpp) and the CSV graph
+    reader (lattices.hpp), built with AddressSanitizer and UndefinedBehaviorSanitizer, on every
+    configuration of the test data and on malformed inputs: each run either succeeds or ends
+    with the parser's own error message, and no sanitizer reports anything."""
+    if not (os.path.exists(LCHK_SAN) and os.path.exists(MQG_SAN)):
+        pytest.skip("sanitizer builds absent (make -C tests/cpp)")
+    from marqov_amd import graphs
+    runs = []
+    for conf in ("config", "config_noend", "config_resume_once", "config_resume_twice"):
+        d = os.path.join(ROOT, "tests", "data", conf)
+        for f in sorted(os.listdir(d)):
+            runs.append([MQG_SAN, f[:-4], "--config", d, "--dry-run"])
+    bad_ini = {
+        "empty": "",
+        "no_end": "[Ising]\nL = 8, 12\n",
+        "no_value": "[Ising]\nL =\nbeta =\n[END]\n",
+        "junk_numbers": "[Ising]\nL = 8x, -4, 99999999999999999999\nbeta = nan, 1e400, ,;\nrep = -3\n[MC]\nwarmupsteps=1e99\n[END]\n",
+        "long_line": "[Ising]\nL = " + ", ".join(["16"] * 5000) + "\n[END]\n",
+        "crlf": "[Ising]\r\nL = 8\r\nbeta = 0.4\r\n[MC]\r\nnmetro = 1\r\n[END]\r\n",
+        "unbalanced": "[[Ising\n= 3\n]]\n[END\n",
+    }
+    for name, text in bad_ini.items():
+        d = tmp_path / f"ini_{name}"
+        d.mkdir()
+        (d / "Ising.ini").write_text(text)
+        runs.append([MQG_SAN, "Ising", "--config", str(d), "--dry-run"])
+    rp, ci, pts = graphs.rgg2d(200, graphs.rgg_radius(200), 4)
+    good = str(tmp_path / "good.csv")
+    graphs.write_csv_graph(good, rp, ci, pts)
+    runs.append([LCHK_SAN, "csv", good])
+    bad_csv = {
+        "empty": "",
+        "header_only": "0\n",
+        "no_number": "x\nheader\n1\t2\n",
+        "bad_index": "0\nh\n1\tq\n0\n",
+        "out_of_range": "0\nh\n5\n0\n",
+        "huge_index": "0\nh\n99999999999999999999\n",
+        "negative": "0\nh\n-1\n",
+        "tabs": "1\nh\n\t\t\t\n0.5\t\t0\n",
+        "isolated": "0\nh\n1\n0\n\n\n",
+    }
+    for name, text in bad_csv.items():
+        pth = str(tmp_path / f"{name}.csv")
+        with open(pth, "w") as f:
+            f.write(text)
+        runs.append([LCHK_SAN, "csv", pth])
+        runs.append([LCHK_SAN, "csv", pth, "1"])
+    for cmd in runs:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, env=SAN_ENV)
+        assert _sanitizer_clean(r), (cmd, r.returncode, r.stderr[-2000:])

@@ -6,8 +6,9 @@
   on the small-lattice and the workgroup kernels, and the observables.
 * Graphs with isolated vertices (degree 0), a hub of degree 127 (the largest the
   8-replicas-per-lane kernels take) and of degree 128 (the per-replica kernels then
-  run), a single edge, and the complete graph K_64 (64 colour classes of one vertex,
-  the most a sweep's counter phase holds); K_65 is refused.
+  run), a single edge, and the complete graphs K_64, K_65 and K_130 (as many colour
+  classes of one vertex; above 64 classes the sweep's counter phase uses a wider
+  stride).
 * Potts with q = 8, the largest Potts<Q> the reference instantiates, on lattices
   and graphs (sweeps, Wolff, observables).
 * Zero counts are no-ops, negative counts are refused, and one mq_emcs equals
@@ -141,23 +142,38 @@ def test_odd_graph_sweeps_and_wolff_bit_exact(ctx, hub_deg, nrep):
         assert sizes.min() >= 1
 
 
-def test_complete_graph_64_colours_and_65_refused(ctx):
+@pytest.mark.parametrize("n", [64, 65, 130])
+def test_complete_graphs_many_colour_classes(ctx, n):
+    """The complete graph K_n needs n colour classes of one vertex each.  Up to 64 classes
+    a sweep's phase is 64*sweep + colour; K_65 and K_130 (130 classes, degree 129: the
+    per-replica kernels run) use the stride 128 and 256, and sweeps, Wolff clusters and the
+    sweep-number limit 2^32/stride follow it, equal to the CPU replay."""
     from marqov_amd import gpu
-    n = 64
     rp, ci = _csr(n, [(a, b) for a in range(n) for b in range(a + 1, n)])
     lat = gpu.CsrLattice(ctx, rp, ci)
-    assert len(lat.colors()) == 64
+    assert len(lat.colors()) == n
     col = _colours(ctx, lat)
-    st = gpu.State(ctx, "ising", [lat] * 2, J=-1.0)
-    st.set_beta([0.01, 0.05])
+    R = 16
+    betas = np.linspace(0.005, 0.05, R)
+    st = gpu.State(ctx, "ising", [lat] * R, J=-1.0)
+    st.set_beta(betas)
     st.init(seed=1)
     st.metropolis(3, seed=2, sweep0=5)
-    for r in range(2):
-        exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(n, 1, r), [0.01, 0.05][r], -1.0, 3, 2, 5, r)
+    sizes = st.wolff(3, seed=4, cluster0=0)
+    st.metropolis(1, seed=2, sweep0=8)
+    for r in range(R):
+        exp = ob.csr_sweeps(rp, ci, col, ob.init_ising(n, 1, r), betas[r], -1.0, 3, 2, 5, r)
+        exp, esz = ob.csr_wolff(rp, ci, exp, betas[r], -1.0, 3, 4, 0, r)
+        exp = ob.csr_sweeps(rp, ci, col, exp, betas[r], -1.0, 1, 2, 8, r)
+        assert np.array_equal(sizes[r], esz), r
         assert np.array_equal(st.download(r), exp), r
-    rp65, ci65 = _csr(65, [(a, b) for a in range(65) for b in range(a + 1, 65)])
-    with pytest.raises(gpu.MarqovError, match="more than 64 colours"):
-        gpu.CsrLattice(ctx, rp65, ci65)
+    stride = 64
+    while stride < n:
+        stride *= 2
+    lim = (1 << 32) // stride
+    st.metropolis(1, seed=2, sweep0=lim - 1)
+    with pytest.raises(gpu.MarqovError, match="sweep counter exceeds"):
+        st.metropolis(1, seed=2, sweep0=lim)
 
 
 def test_zero_and_negative_counts(ctx):

@@ -76,11 +76,11 @@ def test_csr_ea_bonds_bit_exact(ctx):
 @pytest.mark.parametrize("model,beta", [("ising", 0.9), ("potts", 1.6)])
 def test_csr_wolff_bit_exact(ctx, model, beta, block, monkeypatch):
     from marqov_amd import gpu
-    monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
     rp, ci = _graph(2000, 7)
     N = len(rp) - 1
     lat = _lat(ctx, rp, ci)
     st = gpu.State(ctx, model, [lat, lat], J=-1.0, q=3)
+    st.set_tuning(wolff_block=int(block))
     st.set_beta(beta)
     st.init(seed=2)
     sizes = st.wolff(6, seed=13, cluster0=2)
@@ -165,11 +165,11 @@ def test_csr_wolff_frontier_spill_bit_exact(ctx, model, block, monkeypatch):
     spill to the global queue (with 256- and 1024-thread workgroups); the clusters must
     still equal the CPU replay."""
     from marqov_amd import gpu
-    monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
     n = 24000
     rp, ci = _random_graph(n, 16, 5)
     lat = _lat(ctx, rp, ci)
     st = gpu.State(ctx, model, [lat, lat], J=-1.0, q=3)
+    st.set_tuning(wolff_block=int(block))
     beta = 1.0 if model == "ising" else 2.0
     st.set_beta(beta)
     starts = []
@@ -184,3 +184,35 @@ def test_csr_wolff_frontier_spill_bit_exact(ctx, model, block, monkeypatch):
         assert esz.max() > n // 2
         assert np.array_equal(sizes[r], esz), (sizes[r], esz)
         assert np.array_equal(st.download(r), exp)
+
+
+@pytest.mark.parametrize("model,R", [("ising", 40), ("potts", 64)])
+def test_csr_octet_batches_sweeps_wolff_sweeps_bit_exact(ctx, model, R):
+    """Graph states of 5 and 8 replica octets (the 8-replicas-per-lane kernels) on a graph of
+    at least three colour classes: sweeps, Wolff clusters, sweeps again and the measurement
+    equal the CPU replay for every replica."""
+    from marqov_amd import gpu
+    rp, ci = _graph(2000, 13)
+    N = len(rp) - 1
+    lat = _lat(ctx, rp, ci)
+    col = _colours(ctx, lat)
+    assert col.max() + 1 >= 3
+    st = gpu.State(ctx, model, [lat] * R, J=-1.0, q=3)
+    betas = np.linspace(0.4, 1.4, R)
+    st.set_beta(betas)
+    st.init(seed=24)
+    st.metropolis(3, seed=25, sweep0=0)
+    sizes = st.wolff(2, seed=26, cluster0=0)
+    st.metropolis(2, seed=25, sweep0=3)
+    obs = st.measure()
+    potts = model == "potts"
+    for r in range(R):
+        s = ob.init_potts(N, 3, 24, r) if potts else ob.init_ising(N, 24, r)
+        s = ob.csr_sweeps(rp, ci, col, s, betas[r], -1.0, 3, 25, 0, r, potts=potts)
+        s, esz = ob.csr_wolff(rp, ci, s, betas[r], -1.0, 2, 26, 0, r, potts=potts)
+        s = ob.csr_sweeps(rp, ci, col, s, betas[r], -1.0, 2, 25, 3, r, potts=potts)
+        assert np.array_equal(sizes[r], esz), r
+        got = st.download(r)
+        assert np.array_equal(got, s), r
+        if not potts:
+            assert obs[r, 1] == pytest.approx(abs(int(got.astype(np.int64).sum())) / N, abs=1e-12), r

@@ -301,11 +301,11 @@ def test_wolff_frontier_spill_bit_exact(ctx, model, kernels, ring, dim, L, block
     to the global queue must leave the clusters equal to the CPU replay."""
     from marqov_amd import gpu
     monkeypatch.setenv("MARQOV_AMD_KERNELS", kernels)
-    monkeypatch.setenv("MARQOV_AMD_WOLFF_RING", ring)
-    monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
     N = L ** dim
     lat = gpu.RegularLattice(ctx, dim, L)
     st = gpu.State(ctx, model, [lat] * 2, J=-1.0, q=3)
+    st.set_tuning(wolff_block=int(block))
+    st.set_tuning(wolff_ring=int(ring))
     beta = 1.0 if model == "ising" else 2.0  # bond probability 1 - e^-2 for both
     st.set_beta(beta)
     starts = []
@@ -360,18 +360,18 @@ def test_wolff_wave_chained_calls_bit_exact(ctx, model, dim, L):
 @pytest.mark.parametrize("dim,L", [(2, 256), (3, 32)])
 def test_wolff_membership_modes_bit_exact(ctx, model, vset, dim, L, monkeypatch):
     """The large-lattice Wolff kernel keeps a cluster's membership in an LDS set of packed
-    sites while the cluster cannot outgrow MARQOV_AMD_WOLFF_VSET sites (the alignment test is
+    sites while the cluster cannot outgrow the tuning's wolff_vset sites (the alignment test is
     then a plain load and the flip a byte store) and in the spin bytes after that (one global
     atomic per claim).  With the set off (0), left early by nearly every cluster (16) and at
     its largest (4096), three chained calls of four clusters each equal the CPU replay on
     ordered starts with 10% defects above the bond-percolation threshold (cluster sizes from a
     few sites to most of the lattice)."""
     from marqov_amd import gpu
-    monkeypatch.setenv("MARQOV_AMD_WOLFF_VSET", vset)
     monkeypatch.setenv("MARQOV_AMD_KERNELS", "1")
     N = L ** dim
     lat = gpu.RegularLattice(ctx, dim, L)
     st = gpu.State(ctx, model, [lat] * 3, J=-1.0, q=3)
+    st.set_tuning(wolff_vset=int(vset))
     # bond probability 1 - e^-0.92 = 0.60 (2D) and 1 - e^-0.36 = 0.30 (3D), above the thresholds
     # 0.5 and 0.25: seeds on the majority grow clusters of much of the lattice, seeds on defects
     # stay small

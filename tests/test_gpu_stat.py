@@ -288,8 +288,8 @@ def test_vec_wolff_block_invariant(ctx, model, embedding, lattice, monkeypatch):
         lat = gpu.CsrLattice(ctx, rp, ci)
     out = {}
     for block in ("256", "1024"):
-        monkeypatch.setenv("MARQOV_AMD_WOLFF_BLOCK", block)
         st = gpu.State(ctx, model, [lat] * 3, J=-1.0, proposal="small", delta=0.6, embedding=embedding)
+        st.set_tuning(wolff_block=int(block))
         st.set_beta(0.9 if lattice == "regular" else 1.2)
         st.init(seed=21)
         sizes = []
@@ -309,7 +309,7 @@ def test_vec_wolff_block_invariant(ctx, model, embedding, lattice, monkeypatch):
                                                      ("phi4", "cartesian", "graph")])
 def test_vec_wolff_visited_set_invariant(ctx, model, embedding, lattice, monkeypatch):
     """O(3)/Phi4 clusters keep their visited set in LDS while they cannot outgrow
-    MARQOV_AMD_WOLFF_VSET sites and move it to the global bitmap when a level could
+    the tuning's wolff_vset sites and move it to the global bitmap when a level could
     pass that limit.  Membership does not depend on where it is recorded, so the spins
     and cluster sizes are bit-identical with the set off (0), with a tiny limit (16: nearly
     every cluster moves mid-growth) and with the default (2048)."""
@@ -323,8 +323,8 @@ def test_vec_wolff_visited_set_invariant(ctx, model, embedding, lattice, monkeyp
         lat = gpu.CsrLattice(ctx, rp, ci)
     out = {}
     for vset in ("0", "16", "2048"):
-        monkeypatch.setenv("MARQOV_AMD_WOLFF_VSET", vset)
         st = gpu.State(ctx, model, [lat] * 3, J=-1.0, proposal="small", delta=0.6, embedding=embedding)
+        st.set_tuning(wolff_vset=int(vset))
         st.set_beta(0.9 if lattice == "regular" else 1.2)
         st.init(seed=31)
         sizes = []
@@ -340,3 +340,29 @@ def test_vec_wolff_visited_set_invariant(ctx, model, embedding, lattice, monkeyp
         assert np.array_equal(out[vset][0], s0), vset
         for r in range(3):
             assert np.array_equal(out[vset][1][r], out["0"][1][r]), (vset, r)
+
+
+def test_heisenberg_renormalisation_cadence_bounds_drift(ctx):
+    """FP32 reflections about a random direction drift |s| by rounding.  3D L = 8 Heisenberg,
+    16 replicas, 10^5 EMCS of 4 random-direction Wolff clusters each: without renormalisation
+    the largest ||s| - 1| grows well past FP32 rounding; with the FP64 renormalisation every
+    1000 EMCS (GpuEMCS's default cadence) it stays at the drift of at most 1000 EMCS."""
+    from marqov_amd import gpu
+    lat = gpu.RegularLattice(ctx, 3, 8)
+    drift = {}
+    for renorm in (False, True):
+        st = gpu.State(ctx, "heisenberg", [lat] * 16, J=-1.0, embedding="random")
+        st.set_beta(0.693)
+        st.init(seed=12)
+        worst = 0.0
+        for call in range(100):  # 1000 EMCS x 4 clusters per call
+            st.wolff(4000, seed=13, cluster0=4000 * call, sizes=False)
+            if call == 99:  # 1000 EMCS after the last renormalisation
+                worst = max(float(np.abs(np.linalg.norm(st.download(r).astype(np.float64), axis=1) - 1).max())
+                            for r in range(16))
+            if renorm:
+                st.renormalize()
+        drift[renorm] = worst
+    print(f"[renormalisation] max ||s|-1| after 10^5 EMCS: without {drift[False]:.2e}, "
+          f"with every 1000 EMCS {drift[True]:.2e}", flush=True)
+    assert drift[True] < 1e-5 and drift[True] < drift[False] / 3, drift
