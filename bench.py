@@ -34,6 +34,27 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# Algorithmic bytes per Wolff cluster site: (z + 1) neighbour tests of s_r bytes + the
+# flip write + a 4 B push and a 4 B pop, plus 4 index bytes per neighbour on a graph:
+# 2D Ising int8 13 B, 3D Ising int8 15 B, 3D Heisenberg with the Cartesian component
+# (4 B of FP32) 36 B, the RGG of mean degree 6 39 B.
+WOLFF_BYTES = {"2d_int8": 13.0, "3d_int8": 15.0, "3d_cartesian": 36.0, "rgg_int8": 39.0}
+
+# thermalised states handed from the GPU legs to the CPU baseline (the same workload)
+CPU_WOLFF_INPUTS = {}
+
+
+def wolff_frac(sites_per_ns, kind):
+    return round(sites_per_ns * WOLFF_BYTES[kind] / HBM_PEAK_GBS, 6)
+
+
+def tune_wolff(ctx, st, nclusters):
+    """The project's Wolff autotuner on this state (same clusters for every candidate, state restored)."""
+    from marqov_amd import autotune
+    shape, _ = autotune.tune_wolff(ctx, st, nclusters=nclusters, force=True,
+                                   cache_path=os.path.join("/tmp", f"marqov_amd_tuning_{os.getuid()}.json"))
+    return shape
+
 # HBM bytes per headline launch (3D L=256, 32 replicas) by launch shape (block, XCD remap), from
 # rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (profiles/k_sweep_ising3_row16_3d_L256_R32.md)
 HEADLINE_TRAFFIC = {(64, 0): 872.6e6, (64, 1): 807.6e6, (128, 0): 839.1e6, (128, 1): 807.6e6,
@@ -80,17 +101,21 @@ def timed_wolff(ctx, st, nclusters, seed, cluster0):
     return ms, box["sizes"]
 
 
-def wolff_rate(ctx, st, rounds, nsw, ncl, seed, ncalls, sweep0, cluster0=0):
+def wolff_rate(ctx, st, rounds, nsw, ncl, seed, ncalls, sweep0, cluster0=0, tune_ncl=0):
     """Wolff throughput on a thermalised state: `rounds` x (nsw sweeps + ncl clusters)
-    untimed, then `ncalls` timed calls of ncl clusters per replica.  A call lasts as long
-    as its slowest replica's clusters, so this is the batch's real rate.  Returns
-    (sites/ns, mean cluster size of the timed clusters)."""
+    untimed, then (with tune_ncl > 0) the Wolff autotuner on tune_ncl clusters, then
+    `ncalls` timed calls of ncl clusters per replica.  A call lasts as long as its slowest
+    replica's clusters, so this is the batch's real rate.  Returns (sites/ns, mean cluster
+    size of the timed clusters, the Wolff launch shape used)."""
     c, sw = cluster0, sweep0
     for _ in range(rounds):
         st.metropolis(nsw, seed, sweep0=sw)
         sw += nsw
         st.wolff(ncl, seed, cluster0=c, sizes=False)
         c += ncl
+    shape = tune_wolff(ctx, st, tune_ncl) if tune_ncl > 0 else None
+    tn = st.get_tuning()
+    used = {k: tn[k] for k in ("wolff_block", "wolff_ring", "wolff_vset")}
     ms_tot, sites, n = 0.0, 0, 0
     for _ in range(ncalls):
         ms, sizes = timed_wolff(ctx, st, ncl, seed, c)
@@ -98,7 +123,7 @@ def wolff_rate(ctx, st, rounds, nsw, ncl, seed, ncalls, sweep0, cluster0=0):
         ms_tot += ms
         sites += int(sizes.sum())
         n += sizes.size
-    return sites / (ms_tot * 1e6), sites / n
+    return sites / (ms_tot * 1e6), sites / n, {"tuned": shape is not None, **used}
 
 
 def leg_c2_msc(gpu, ctx, L=256, R=32, steps=100):
@@ -141,7 +166,9 @@ def leg_c1(gpu, ctx):
         ms_w, sizes = timed_wolff(ctx, st, 10 * nstep, 1, 2 * 10 ** 6)
         mean_cl = float(sizes.mean())
         ms_m, _ = timed(ctx, lambda: st.metropolis(nstep, 1, sweep0=10 ** 5))
-        res[name] = {"replicas": nrep, "wolff_sites_per_ns": round(float(sizes.sum()) / (ms_w * 1e6), 4),
+        w = float(sizes.sum()) / (ms_w * 1e6)
+        res[name] = {"replicas": nrep, "wolff_sites_per_ns": round(w, 4),
+                     "wolff_roofline_frac": wolff_frac(w, "2d_int8"),
                      "attempts_per_ns": round(nrep * 4096 * nstep / (ms_m * 1e6), 3),
                      "mean_cluster": round(mean_cl, 1),
                      "emcs_per_s": round(nrep * nstep / ((ms_w + ms_m) * 1e-3), 1)}
@@ -176,7 +203,12 @@ def leg_c3(gpu, ctx):
                                  force=True, sweeps=3)
         st.metropolis(5, 3, 0)
         ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
-        w_rate, w_mean = wolff_rate(ctx, st, 30, 10, 10, 3, 10, sweep0=5 + nstep)
+        # (the Wolff autotuner would save and restore the 8 GiB batch per candidate: the
+        # automatic shape is used here and reported)
+        w_rate, w_mean, w_shape = wolff_rate(ctx, st, 30, 10, 10, 3, 10, sweep0=5 + nstep)
+        if name == "batched":
+            CPU_WOLFF_INPUTS["heis_3d_L128"] = {"model": "heisenberg", "beta": beta, "dim": 3, "L": L,
+                                               "spins": st.download(0)}
         # acceptance of one sweep of replica 0 on the thermalised state
         before = st.download(0)
         st.metropolis(1, 3, sweep0=10 ** 6)
@@ -184,6 +216,7 @@ def leg_c3(gpu, ctx):
         res[name] = {"replicas": nrep, "attempts_per_ns": round(nrep * N * nstep / (ms * 1e6), 3),
                      "delta": C3_DELTA, "acceptance": round(acceptance, 3), "launch_shape": shape,
                      "wolff_sites_per_ns": round(w_rate, 4),
+                     "wolff_roofline_frac": wolff_frac(w_rate, "3d_cartesian"), "wolff_launch_shape": w_shape,
                      "mean_cluster": round(w_mean, 1),
                      "roofline_frac": round(48.0 * nrep * N * nstep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         if name == "batched":
@@ -274,13 +307,33 @@ def leg_c5(gpu, ctx):
             ms, _ = timed(ctx, lambda: st.metropolis(nstep, 5, sweep0=2))
             res[f"{model}_{name}_attempts_per_ns"] = round(nrep * n * nstep / (ms * 1e6), 3)
             if name == "batched":
-                w_rate, w_mean = wolff_rate(ctx, st, 20, 4, 4, 5, 8, sweep0=2 + nstep)
+                # thermalise with rounds of (4 sweeps + 4 clusters) until the mean energy moves
+                # by less than 2e-4 per site over 100 rounds (at most 4000 rounds): the state
+                # coarsens for hundreds of rounds from the random start
+                sw, cl, rounds, e_prev = 2 + nstep, 0, 0, None
+                while rounds < 4000:
+                    for _ in range(100):
+                        st.metropolis(4, 5, sweep0=sw)
+                        st.wolff(4, 5, cluster0=cl, sizes=False)
+                        sw, cl = sw + 4, cl + 4
+                    rounds += 100
+                    e = float(st.measure()[:, 0].mean())
+                    if e_prev is not None and abs(e - e_prev) < 2e-4:
+                        break
+                    e_prev = e
+                w_rate, w_mean, w_shape = wolff_rate(ctx, st, 0, 4, 4, 5, 8, sweep0=sw, cluster0=cl, tune_ncl=4)
                 res[f"{model}_wolff_sites_per_ns"] = round(w_rate, 4)
+                res[f"{model}_wolff_roofline_frac"] = wolff_frac(w_rate, "rgg_int8")
                 res[f"{model}_wolff_mean_cluster"] = round(w_mean, 1)
+                res[f"{model}_wolff_thermalisation_rounds"] = rounds
+                res[f"{model}_wolff_launch_shape"] = w_shape
+                CPU_WOLFF_INPUTS[f"rgg_{model}"] = {"model": model, "beta": beta, "rp": rp, "ci": ci,
+                                                    "spins": st.download(0)}
             del st
         res[f"{model}_batched_over_plain"] = round(res[f"{model}_batched_attempts_per_ns"] /
                                                    res[f"{model}_plain_attempts_per_ns"], 2)
-    res["wolff"] = "64 replicas after 20 x (4 sweeps + 4 clusters); 8 timed calls of 4 clusters per replica"
+    res["wolff"] = ("64 replicas thermalised by rounds of (4 sweeps + 4 clusters) until E/N moves by < 2e-4 "
+                    "over 100 rounds; Wolff autotuner; 8 timed calls of 4 clusters per replica")
     return res
 
 
@@ -383,7 +436,8 @@ def main():
     wolff_b = None
     if not a.quick:
         log("headline state: Wolff clusters")
-        wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps)
+        wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps, tune_ncl=4)
+        CPU_WOLFF_INPUTS["3d_L256"] = {"model": "ising", "beta": beta, "dim": 3, "L": L, "spins": st.download(0)}
     del st
 
     line = {
@@ -424,12 +478,14 @@ def main():
         ms_p, _ = timed(ctx, lambda: sp.metropolis(k, seed=2, sweep0=4))
         line["plain"] = {"attempts_per_ns": round(k * N / (ms_p * 1e6), 3), "replicas": 1, "steps": k}
         line["batched_over_plain"] = round(value / line["plain"]["attempts_per_ns"], 3)
-        w_rate, w_mean = wolff_rate(ctx, sp, 20, 4, 10, 2, 8, sweep0=4 + k)
+        w_rate, w_mean, _ = wolff_rate(ctx, sp, 20, 4, 10, 2, 8, sweep0=4 + k)
         line["plain"]["wolff_sites_per_ns"] = round(w_rate, 4)
         line["plain"]["wolff_mean_cluster"] = round(w_mean, 1)
         line["wolff"] = {"sites_per_ns": round(wolff_b[0], 4), "mean_cluster": round(wolff_b[1], 1),
                          "replicas": R, "plain_sites_per_ns": round(w_rate, 4),
                          "batched_over_plain": round(wolff_b[0] / w_rate, 2),
+                         "roofline_frac": wolff_frac(wolff_b[0], "3d_int8"), "bytes_per_site": WOLFF_BYTES["3d_int8"],
+                         "launch_shape": wolff_b[2],
                          "schedule": "after 20 x (4 sweeps + 10 clusters); 8 timed calls of 10 clusters per replica"}
         del sp
         for name, fn in (("c2_msc", leg_c2_msc), ("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3),
@@ -498,6 +554,19 @@ def main():
                                   f"one simulation per thread, {hz['seconds']:.1f} s timed"}
             except (OSError, ValueError, subprocess.TimeoutExpired):
                 pass
+        # Wolff on the thermalised GPU states of the C2, C3 and C5 legs (the same workload):
+        # the restatement's stack Wolff, one copy of the state per thread, a fixed time each
+        models = {"ising": ob.ISING, "potts": ob.POTTS, "heisenberg": ob.HEIS}
+        for key, inp in CPU_WOLFF_INPUTS.items():
+            sp = inp["spins"]
+            spins = sp.astype(np.float64) if sp.ndim == 2 else sp.astype(np.int32)
+            r = ob.bench_wolff(models[inp["model"]], inp["beta"], spins, 2.0, nth, dim=inp.get("dim", 0),
+                               L=inp.get("L", 0), rp=inp.get("rp"), ci=inp.get("ci"))
+            line["cpu_baseline"][f"wolff_{key}"] = {
+                "value": round(float(r[0]), 5), "unit": "sites/ns", "cores": nth, "kind": "port",
+                "per_core": round(float(r[1]), 5), "mean_cluster": round(float(r[4]), 1),
+                "sample": f"{nth} threads x stack Wolff clusters ({int(r[3])} in all) for {r[2]:.1f} s on copies of "
+                          f"replica 0 of the GPU's thermalised {key} state, beta {inp['beta']}"}
         cpu_model = ""
         try:
             with open("/proc/cpuinfo") as f:

@@ -525,3 +525,67 @@ EXPORT int ref_bench_regular(int model, int dim, int L, double J, int Q, double 
     out[2] = wall;
     return 0;
 }
+
+/*
+ * CPU baseline of the Wolff update on a given state (the restatement's stack
+ * Wolff, Ising.h:122-169, Heisenberg.h:191-243, Potts.h:262-322, wolff.h:48-125):
+ * nthreads threads, each on its own copy of the state with its own RNGCache
+ * (seed + thread), grow single clusters at uniformly drawn seed sites until
+ * `seconds` of wall time have passed, as CXX11Scheduler runs one simulation per
+ * worker thread.  Regular lattice when rp is NULL (dim, L), else the CSR graph
+ * (N, rp, ci).  Integer models start from is0[N], O(3)/Phi4 from ds0[3N].
+ * out[0] = aggregate cluster sites/ns, out[1] = mean per-thread sites/ns,
+ * out[2] = wall seconds, out[3] = clusters grown, out[4] = mean cluster size.
+ */
+EXPORT int ref_bench_wolff(int model, int dim, int L, int64_t N, const int* rp, const int* ci, double J, int Q,
+                           double beta, const int* is0, const double* ds0, double seconds, int nthreads,
+                           uint64_t seed, double* out)
+{
+    const bool vec = (model == HEIS || model == PHI4);
+    Params p{model, J, Q, 4.5, 1.0};
+    std::vector<double> secs(nthreads, 0), sites(nthreads, 0), ncl(nthreads, 0);
+    std::atomic<int> ready{0};
+    auto work = [&](int t, auto& lat) {
+        std::vector<int> is;
+        std::vector<double> ds;
+        if (vec) ds.assign(ds0, ds0 + 3 * lat.N);
+        else is.assign(is0, is0 + lat.N);
+        using Lat = std::decay_t<decltype(lat)>;
+        Sim<Lat> sim(lat, p, nullptr, is.data(), ds.data(), seed + uint64_t(t));
+        ready++;
+        while (ready.load() < nthreads) std::this_thread::yield();
+        const auto a = std::chrono::steady_clock::now();
+        double s = 0, n = 0, el = 0;
+        do {
+            s += double(sim.wolff(beta, int64_t(sim.rng.integer(uint64_t(lat.N)))));
+            n += 1;
+            el = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        } while (el < seconds);
+        secs[t] = el;
+        sites[t] = s;
+        ncl[t] = n;
+    };
+    std::vector<std::thread> th;
+    if (rp) {
+        Csr lat{N, rp, ci};
+        for (int t = 0; t < nthreads; ++t) th.emplace_back([&, t]() { work(t, lat); });
+        for (auto& x : th) x.join();
+    } else {
+        Regular lat(dim, L);
+        for (int t = 0; t < nthreads; ++t) th.emplace_back([&, t]() { work(t, lat); });
+        for (auto& x : th) x.join();
+    }
+    double wall = *std::max_element(secs.begin(), secs.end());
+    double total = 0, per = 0, clusters = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        total += sites[t];
+        per += sites[t] / (secs[t] * 1e9);
+        clusters += ncl[t];
+    }
+    out[0] = total / (wall * 1e9);
+    out[1] = per / nthreads;
+    out[2] = wall;
+    out[3] = clusters;
+    out[4] = total / clusters;
+    return 0;
+}

@@ -49,6 +49,7 @@ _SIG = {
     "ref_set_plant": (None, [_I]),
     "ref_rngcache_draws": (None, [_U64, _I, _U64, _I64, _P]),
     "ref_bench_regular": (_I, [_I, _I, _I, _D, _I, _D, _I, _I, _I, _U64, _I, _P]),
+    "ref_bench_wolff": (_I, [_I, _I, _I, _I64, _P, _P, _D, _I, _D, _P, _P, _D, _I, _U64, _P]),
 }
 
 _lib = None
@@ -260,6 +261,23 @@ def ref_chains(model, dim, L, beta, nseeds, seed0=100, threads=8, **kw):
     with ThreadPoolExecutor(max_workers=max(1, min(threads, nseeds))) as ex:
         res = list(ex.map(lambda s: ref_run(model, dim, L, beta, seed=seed0 + s, **kw)[1], range(nseeds)))
     return np.array(res)
+
+
+def bench_wolff(model, beta, spins, seconds, nthreads, dim=0, L=0, rp=None, ci=None, J=-1.0, Q=3, seed=1):
+    """Timed CPU Wolff on copies of `spins` (int spins or (N, 3) vectors) on a regular lattice
+    (dim, L) or a CSR graph (rp, ci): (aggregate sites/ns, per-thread sites/ns, wall s,
+    clusters, mean cluster size)."""
+    vec = model in (HEIS, PHI4)
+    N = len(rp) - 1 if rp is not None else L ** dim
+    is0 = None if vec else np.ascontiguousarray(spins, np.int32).reshape(-1)
+    ds0 = np.ascontiguousarray(spins, np.float64).reshape(-1) if vec else None
+    out = np.zeros(5)
+    rpa = None if rp is None else np.ascontiguousarray(rp, np.int32)
+    cia = None if ci is None else np.ascontiguousarray(ci, np.int32)
+    lib().ref_bench_wolff(model, dim, L, N, None if rpa is None else p(rpa), None if cia is None else p(cia), J, Q,
+                          beta, None if is0 is None else p(is0), None if ds0 is None else p(ds0), seconds, nthreads,
+                          seed, p(out))
+    return tuple(out)
 
 
 def csr_sweeps(rp, ci, colour, s, beta, J, nsweeps, seed, sweep0, rep, potts=False, Q=3, bonds=None):
