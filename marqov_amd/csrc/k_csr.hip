@@ -527,6 +527,11 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
     __shared__ int32_t s_deg[CAP];
     __shared__ float s_proj[VEC ? CAP : 1];
     __shared__ uint32_t s_vset[VEC ? MQ_VSET : 1];
+    // the level chunk being worked: entry rows, degree scan, projections; wavefront sums
+    __shared__ int32_t s_ckb[1024];
+    __shared__ int32_t s_cpre[1024];
+    __shared__ float s_cproj[VEC ? 1024 : 1];
+    __shared__ int s_wsum[16];
     const int r = blockIdx.x;
     const int grep = rep0 + r;
     // element (position p) of this replica: sp8[p * ps] / sp4[p * ps]
@@ -589,19 +594,23 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                 glob = true;
                 __syncthreads();
             }
-            // Each wavefront takes 64 frontier entries, scans their degrees, and then
-            // runs one lane per edge: a lane finds its entry by a binary search over
-            // the scan, loads the far end, draws the edge's Philox uniform and
-            // claims.  A narrow level costs one Philox evaluation per lane instead
-            // of one per neighbour in a row; new entries are placed with one LDS
-            // atomic per wavefront.
+            // The level's edges are spread over the whole workgroup: chunks of up to
+            // blockDim frontier entries are copied to LDS with a block-wide scan of their
+            // degrees, and lane t of the workgroup takes the chunk's edges t, t + blockDim,
+            // ...: it finds its entry by a binary search over the scan, loads the far end,
+            // draws the edge's Philox uniform and claims.  A level of W entries of mean
+            // degree d then costs about W*d/blockDim rounds of two dependent round trips
+            // (neighbour index, claim) instead of W*d/64 rounds in one wavefront.  New
+            // entries are placed with one LDS atomic per wavefront.
             const int lane = (int)(threadIdx.x & 63u);
+            const int wv = (int)(threadIdx.x >> 6);
             const int nwv = (int)(blockDim.x >> 6);
-            for (int wb = head + (int)(threadIdx.x >> 6) * 64; wb < tail; wb += nwv * 64) {
-                const int idx = wb + lane;
+            for (int c0 = head; c0 < tail; c0 += (int)blockDim.x) {
+                const int n = min((int)blockDim.x, tail - c0);
+                const int idx = c0 + (int)threadIdx.x;
                 int kb = 0, deg = 0;
                 float pc = 0.f;
-                if (idx < tail) {
+                if ((int)threadIdx.x < n) {
                     if (idx - hprev < CAP) {
                         const int sl = idx & (CAP - 1);
                         kb = s_kb[sl];
@@ -614,29 +623,41 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                         if (VEC) pc = cproj<KIND>(sp4[(int64_t)cur * ps], axis, rd);
                     }
                 }
+                // block-wide exclusive scan of the degrees
                 int inc = deg;
 #pragma unroll
                 for (int o = 1; o < 64; o <<= 1) {
                     const int v = __shfl_up(inc, o);
                     if (lane >= o) inc += v;
                 }
-                const int total = __shfl(inc, 63);
-                const int pre = inc - deg;
-                for (int t0 = 0; t0 < total; t0 += 64) {  // wavefront-uniform
-                    const int t = t0 + lane;
-                    // owner: the last lane whose scan start is <= t (its degree is then > t - start)
-                    int e = 0;
-#pragma unroll
-                    for (int stp = 32; stp > 0; stp >>= 1) {
-                        const int pe = __shfl(pre, e + stp);
-                        if (pe <= t) e += stp;
-                    }
-                    const int k = __shfl(kb, e) + (t - __shfl(pre, e));
-                    const float epc = VEC ? __shfl(pc, e) : 0.f;
+                if (lane == 63) s_wsum[wv] = inc;
+                __syncthreads();
+                int wpre = 0;
+                for (int w = 0; w < wv; ++w) wpre += s_wsum[w];
+                const int pre = wpre + inc - deg;
+                if ((int)threadIdx.x < n) {
+                    s_cpre[threadIdx.x] = pre;
+                    s_ckb[threadIdx.x] = kb;
+                    if (VEC) s_cproj[threadIdx.x] = pc;
+                }
+                int total = 0;
+                for (int w = 0; w < nwv; ++w) total += s_wsum[w];
+                __syncthreads();
+                for (int t0 = 0; t0 < total; t0 += (int)blockDim.x) {  // workgroup-uniform
+                    const int t = t0 + (int)threadIdx.x;
                     bool ok = false;
                     int32_t j = 0, jk0 = 0, jk1 = 0;
                     float pnew = 0.f;
                     if (t < total) {
+                        // owner: the last entry whose scan start is <= t (its degree is then > t - start)
+                        int lo = 0, hi = n - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_cpre[mid] <= t) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        const int k = s_ckb[lo] + (t - s_cpre[lo]);
+                        const float epc = VEC ? s_cproj[lo] : 0.f;
                         j = ci[k];
                         const uint32_t eid = (uint32_t)edge[k];
                         // j's row, kept with its ring entry if j joins: in flight with the
@@ -688,6 +709,7 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
                         if (VEC || pos - head >= CAP) q[pos] = j;  // VEC: every member, for the bitmap
                     }
                 }
+                __syncthreads();  // the chunk's LDS copy is read until here
             }
             __syncthreads();
             hprev = head;
