@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="headline with the default launch shape (no autotuner)")
     ap.add_argument("--shape", default="", help="headline launch shape BLOCK,XCD_REMAP instead of the autotuner's")
     ap.add_argument("--legs", default="", help="comma-separated subset of the extra legs to run")
+    ap.add_argument("--legs-only", action="store_true",
+                    help="with --legs: skip the headline state's Wolff and plain comparisons (for profiling legs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for HBM traffic)")
     ap.add_argument("--locate-c5-beta", action="store_true",
@@ -269,6 +271,9 @@ def leg_c4(gpu, ctx):
         acc = st.pt_swap(nt, 0, 4, 10 ** 6)
         res[f"{name}_attempts_per_ns"] = round(sites * nsw * steps * 4 / (ms * 1e6), 3)
         res[f"{name}_pt_accept_even_pairs"] = round(float(acc.sum()) / (len(acc) * 16), 3)
+        # the same sweeps launched eagerly (no graph, no energy or swap): the sweep kernels alone
+        ms_e, _ = timed(ctx, lambda: st.metropolis(nsw * steps, 4, sweep0=10 ** 6))
+        res[f"{name}_eager_sweeps_attempts_per_ns"] = round(sites * nsw * steps / (ms_e * 1e6), 3)
         del st
     res["graph"] = f"{steps} x ({nsw} sweeps + energy + PT swap) per replay, 4 replays"
     os.environ["MARQOV_AMD_KERNELS"] = "1"
@@ -434,7 +439,7 @@ def main():
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     obs = st.measure()
     wolff_b = None
-    if not a.quick:
+    if not a.quick and not a.legs_only:
         log("headline state: Wolff clusters")
         wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps, tune_ncl=4)
         CPU_WOLFF_INPUTS["3d_L256"] = {"model": "ising", "beta": beta, "dim": 3, "L": L, "spins": st.download(0)}
@@ -468,7 +473,7 @@ def main():
         "observables": {"e_mean": float(obs[:, 0].mean()), "m_abs_mean": float(obs[:, 1].mean())},
     }
 
-    if not a.quick:
+    if not a.quick and not a.legs_only:
         log("plain 3D Ising")
         sp = gpu.State(ctx, "ising", [lat], J=-1.0)
         sp.set_beta(beta)
@@ -488,6 +493,7 @@ def main():
                          "launch_shape": wolff_b[2],
                          "schedule": "after 20 x (4 sweeps + 10 clusters); 8 timed calls of 10 clusters per replica"}
         del sp
+    if not a.quick:
         for name, fn in (("c2_msc", leg_c2_msc), ("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3),
                          ("c4_ea_batch", leg_c4), ("c5_rgg", leg_c5)):
             if a.legs and name not in a.legs.split(","):

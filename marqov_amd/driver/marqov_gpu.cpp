@@ -291,7 +291,10 @@ void print_usage()
 }
 
 // MagFTComp columns of the reference's observable tuples: Ising axes 0 and 1
-// (Ising.h:82-94), Phi4 every axis (Phi4.h:116-140); regular lattices only
+// (Ising.h:82-94), Phi4 every axis of the lattice (Phi4.h:116-140); regular
+// lattices only.  The reference's Phi4 tuple always holds three (axes 0, 1, 2);
+// on a 2D lattice its axis-2 component reads crds(i)[2] past the end of a
+// two-coordinate site (undefined behaviour), so 2D Phi4 writes two columns here.
 int ft_columns(const Plan& pl, const Spec& s)
 {
     if (pl.lattice != "regular") return 0;
