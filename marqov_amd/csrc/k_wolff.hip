@@ -89,6 +89,8 @@ __device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
 // overwritten before it is read.  The reader of that level knows `head` as its
 // own `hprev`.
 constexpr int WCAP = 4096;
+// bond tasks a lane takes at once on a wide level
+constexpr int WKT = 4;
 
 // LDS bytes of one replica's spins in the small-lattice kernel: a byte per
 // site, or a bit per site when packed (16-byte multiples)
@@ -222,6 +224,125 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                 glob = true;
                 __syncthreads();
             }
+            if (ntask > (int)blockDim.x) {
+                // A wide level: each lane takes up to WKT bond tasks t, t + blockDim, ... at
+                // once, issues all their loads, draws their uniforms and then issues all
+                // their claims before using any result, so the level waits on one memory
+                // round trip per WKT tasks rather than one per task.  Claims of one lane
+                // still act in order, and a site claimed twice is taken once.
+                for (int t0 = (int)threadIdx.x; t0 < ntask; t0 += WKT * (int)blockDim.x) {
+                    uint32_t jp[WKT], jn[WKT], jan[WKT], u[WKT], old[WKT];
+                    float pc[WKT];
+                    int sv[WKT];
+                    float4 vj[WKT];
+                    bool act[WKT], ok[WKT], try_[WKT];
+#pragma unroll
+                    for (int k = 0; k < WKT; ++k) {
+                        const int t = t0 + k * (int)blockDim.x;
+                        act[k] = t < ntask;
+                        ok[k] = try_[k] = false;
+                        jp[k] = jn[k] = jan[k] = 0u;
+                        u[k] = 0xFFFFFFFFu;
+                        old[k] = 0u;
+                        pc[k] = 0.f;
+                        sv[k] = 0;
+                        vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (act[k]) {
+                            const int e = t / NS;
+                            const int sl = t - e * NS;
+                            const int idx = head + e;
+                            const bool in_ring = idx - hprev < WCAP;
+                            const uint32_t cur = in_ring ? s_ring[idx & (WCAP - 1)] : (uint32_t)q[idx];
+                            const uint32_t x0 = cur & CM, x1 = (cur >> SB) & CM, x2 = SitePack<DIM>::z(cur);
+                            const int d = sl >> 1;
+                            const uint32_t xd = d == 0 ? x0 : (d == 1 ? x1 : x2);
+                            uint32_t nx;
+                            if ((sl & 1) == 0) nx = (xd == uL - 1) ? 0u : xd + 1u;
+                            else nx = (xd == 0) ? uL - 1u : xd - 1u;
+                            jp[k] = (cur & ~(CM << (SB * d))) | (nx << (SB * d));
+                            const uint32_t curn = x0 + uL * x1 + LL * x2;
+                            jn[k] = curn + (nx - xd) * (d == 0 ? 1u : (d == 1 ? uL : LL));  // mod 2^32
+                            jan[k] = (((x0 + x1 + x2) & 1u) ? 0u : half) + (jn[k] >> 1);
+                            if (!VEC && !glob) sv[k] = sp8[jan[k]];  // in flight during the Philox rounds
+                            if (VEC) {
+                                vj[k] = sp4[jan[k]];
+                                const uint32_t cura = (((x0 + x1 + x2) & 1u) ? half : 0u) + (curn >> 1);
+                                pc[k] = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[cura], axis, rd);
+                            }
+                            const uint64_t b = (uint64_t)((sl & 1) ? jn[k] : curn) * DIM + (uint64_t)d;
+                            const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                            u[k] = u4_get(bw, (int)(b & 3));
+                        }
+                    }
+                    if (KIND == WK_ISING && glob) {
+                        // all claims in flight together (the claim tests alignment)
+                        const bool up = __builtin_amdgcn_readfirstlane(val) > 0;
+#pragma unroll
+                        for (int k = 0; k < WKT; ++k) {
+                            try_[k] = act[k] && u[k] < tw;
+                            if (try_[k]) {
+                                uint32_t sh;
+                                uint32_t* w = spin_word(sp8 + jan[k], sh);
+                                const uint32_t m = 0xFEu << sh;
+                                old[k] = up ? atomicOr(w, m) : atomicAnd(w, ~m);
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < WKT; ++k)
+                            ok[k] = try_[k] && (int8_t)(old[k] >> ((jan[k] & 3u) * 8u)) == (int8_t)val;
+                    } else if (KIND == WK_POTTS && glob) {
+#pragma unroll
+                        for (int k = 0; k < WKT; ++k) {
+                            if (act[k] && u[k] < tw) {
+                                uint32_t sh;
+                                ok[k] = claim_potts(sp8 + jan[k], peek_word(sp8 + jan[k], sh), val, newq);
+                            }
+                        }
+                    } else if (!VEC) {
+#pragma unroll
+                        for (int k = 0; k < WKT; ++k) {
+                            if (act[k] && sv[k] == val && u[k] < tw) {
+                                ok[k] = vset_insert(s_vset, jp[k]);
+                                if (ok[k]) sp8[jan[k]] = (int8_t)newq;
+                            }
+                        }
+                    } else {
+                        float cpl[WKT];
+#pragma unroll
+                        for (int k = 0; k < WKT; ++k) {
+                            cpl[k] = J * pc[k] * proj<KIND>(vj[k], axis, rd);
+                            try_[k] = act[k] && cpl[k] > 0.f && u01f(u[k]) < -expm1f(-2.0f * beta * cpl[k]);
+                            if (try_[k] && glob) old[k] = atomicOr(&vis[jn[k] >> 5], 1u << (jn[k] & 31));
+                        }
+#pragma unroll
+                        for (int k = 0; k < WKT; ++k) {
+                            if (!try_[k]) continue;
+                            ok[k] = glob ? !(old[k] & (1u << (jn[k] & 31))) : vset_insert(s_vset, jp[k]);
+                        }
+                    }
+                    uint32_t nok = 0;
+#pragma unroll
+                    for (int k = 0; k < WKT; ++k) nok += ok[k] ? 1u : 0u;
+                    int pos = wave_reserve(&s_tail, nok);
+#pragma unroll
+                    for (int k = 0; k < WKT; ++k) {
+                        if (!ok[k]) continue;
+                        float pnew = 0.f;
+                        if (VEC) {
+                            // the claim succeeded, so nobody changed vj since it was read
+                            const float4 v = reflect<KIND>(vj[k], axis, rd);
+                            sp4[jan[k]] = v;
+                            pnew = proj<KIND>(v, axis, rd);
+                        }
+                        if (pos - head < WCAP) {
+                            s_ring[pos & (WCAP - 1)] = jp[k];
+                            if (VEC) s_proj[pos & (WCAP - 1)] = pnew;
+                        }
+                        if (VEC || pos - head >= WCAP) q[pos] = (int32_t)jp[k];  // VEC: every member, for the bitmap
+                        ++pos;
+                    }
+                }
+            } else
             for (int t = (int)threadIdx.x; t < ntask; t += (int)blockDim.x) {
                 const int e = t / NS;
                 const int sl = t - e * NS;
