@@ -298,9 +298,11 @@ def leg_c5(gpu, ctx):
     t0 = time.perf_counter()
     rp, ci = ctx.rgg(2, n, graphs.rgg_radius(n), 5)  # built on the device (mq_rgg_build)
     t_build = time.perf_counter() - t0
-    lat = gpu.CsrLattice(ctx, rp, ci)
+    t0 = time.perf_counter()
+    lat = gpu.CsrLattice(ctx, rp, ci)  # host greedy colouring, renumbering, upload
+    t_lat = time.perf_counter() - t0
     res = {"N": n, "edges": int(rp[-1] // 2), "colours": int(len(lat.colors())),
-           "graph_build_ms": round(1e3 * t_build, 1)}
+           "graph_build_ms": round(1e3 * t_build, 1), "colouring_and_upload_ms": round(1e3 * t_lat, 1)}
     # pseudo-critical points of these graphs: Binder-cumulant crossings of N = 16,384 and 65,536
     # (`bench.py --locate-c5-beta`): Ising 0.494, Potts-3 1.088
     for model, beta in (("ising", 0.494), ("potts", 1.088)):
