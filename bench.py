@@ -341,6 +341,20 @@ def leg_c5(gpu, ctx):
                                                    res[f"{model}_plain_attempts_per_ns"], 2)
     res["wolff"] = ("64 replicas thermalised by rounds of (4 sweeps + 4 clusters) until E/N moves by < 2e-4 "
                     "over 100 rounds; Wolff autotuner; 8 timed calls of 4 clusters per replica")
+    # the Ising sweep on 1-bit storage (32 replicas per uint32 word): the same chain as int8 storage,
+    # (d+2)*4/32 B of spins + 4(d+1)/64 B of CSR indices per attempt instead of (d+2) + 4(d+1)/64
+    st = gpu.State(ctx, "ising", [lat] * 64, J=-1.0, spin_bits=1)
+    st.set_beta(0.494)
+    st.init(seed=5)
+    st.metropolis(2, 5, 0)
+    ms, _ = timed(ctx, lambda: st.metropolis(10, 5, sweep0=2))
+    rate = 64 * n * 10 / (ms * 1e6)
+    dbar = res["edges"] * 2 / n
+    bpa = (dbar + 2) * 4 / 32 + 4 * (dbar + 1) / 64
+    res["ising_bits_batched_attempts_per_ns"] = round(rate, 3)
+    res["ising_bits_bytes_per_attempt"] = round(bpa, 3)
+    res["ising_bits_over_int8"] = round(rate / res["ising_batched_attempts_per_ns"], 2)
+    del st
     return res
 
 

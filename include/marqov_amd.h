@@ -70,8 +70,12 @@ typedef struct mq_model_params {
     int proposal;   /* MQ_PROPOSE_* (Heisenberg) */
     double delta;   /* small-angle step (Heisenberg) */
     int embedding;  /* MQ_EMBED_* (Heisenberg Wolff) */
-    int spin_bits;  /* Ising storage: 0 or 8 = int8 per spin, 1 = multi-spin coded
-                       (1 bit per spin; 3D, L a power of two >= 64; no Wolff) */
+    int spin_bits;  /* storage: 0 or 8 = int8 per spin; 1 = one bit per spin (no Wolff):
+                       Ising on 3D lattices with L a power of two >= 64 packs 32 sites
+                       per word; EA (regular lattices) and Ising on graphs (degrees
+                       <= 127) pack 32 replicas per word, in runs of multiples of 32
+                       replicas per lattice.  Ising on graphs then makes the same moves
+                       as int8 storage, bit for bit. */
 } mq_model_params;
 
 /* ---- context ---------------------------------------------------------- */

@@ -118,6 +118,8 @@ int check_state(mq_state* st)
 int ensure_bond_storage(mq_state* st)
 {
     if (st->d_bonds) return MQ_OK;
+    if (st->csrbits)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "1-bit Ising on graphs has no couplings (use int8 storage for +-J)");
     st->sample_bond_off.assign(st->nsamples, -1);
     int64_t off = 0;
     for (int r = 0; r < st->nrep; ++r) {
@@ -334,7 +336,8 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
             return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "all replicas of a state must use lattices of one kind and dim");
     }
     const bool eabits = params->spin_bits == 1 && model == MQ_EA_ISING;
-    const bool msc = params->spin_bits == 1 && !eabits;
+    const bool csrbits = params->spin_bits == 1 && model == MQ_ISING && kind0 == LAT_CSR;
+    const bool msc = params->spin_bits == 1 && !eabits && !csrbits;
     if (params->spin_bits != 0 && params->spin_bits != 1 && params->spin_bits != 8)
         return mq_fail(ctx, MQ_ERR_ARG, "spin_bits must be 0, 1 or 8");
     if (eabits) {
@@ -350,6 +353,19 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
         }
         if (params->J == 0.0) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "bit-sliced EA needs J != 0");
     }
+    if (csrbits) {
+        for (int r = 0; r < nrep;) {
+            int e = r + 1;
+            while (e < nrep && lattices[e] == lattices[r]) ++e;
+            if ((e - r) % 32 != 0)
+                return mq_fail(ctx, MQ_ERR_ARG, "1-bit Ising on graphs: each run of replicas on one graph must be a multiple of 32");
+            if (!csrbits_supported(lattices[r]))
+                return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "1-bit Ising on graphs needs vertex degrees <= 127");
+            if (lattices[r]->N * (int64_t)((e - r) / 8) >= (1ll << 31))
+                return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "1-bit Ising on graphs: vertices x octets of replicas must stay below 2^31");
+            r = e;
+        }
+    }
     if (msc) {
         if (model != MQ_ISING) return mq_fail(ctx, MQ_ERR_UNSUPPORTED, "multi-spin coding is for the Ising model");
         for (int r = 0; r < nrep; ++r)
@@ -363,6 +379,7 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     st->nrep = nrep;
     st->msc = msc;
     st->eabits = eabits;
+    st->csrbits = csrbits;
     st->lat.assign(lattices, lattices + nrep);
     st->site_bytes = is_vec(model) ? 16 : 1;
     auto rep_bytes = [&](const mq_lattice* l) {
@@ -387,6 +404,11 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
             g.stride = (g.lat->N * 4 + 255) / 256 * 256;
             g.pos_stride = 4;
             off += g.stride * (g.nrep / 32);
+        } else if (csrbits) {
+            // one uint32 word per vertex and word of 32 replicas, words of a vertex adjacent
+            g.stride = 4;
+            g.pos_stride = 4 * (g.nrep / 32);
+            off += (g.lat->N * g.pos_stride + 255) / 256 * 256;
         } else if (g.lat->kind == LAT_CSR) {
             g.stride = st->site_bytes;
             g.pos_stride = (int64_t)st->site_bytes * g.nrep;
@@ -397,12 +419,12 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
             off += g.stride * g.nrep;
         }
         for (int k = r; k < e; ++k) {
-            st->rep_off[k] = g.off + (eabits ? (int64_t)((k - r) / 32) : (int64_t)(k - r)) * g.stride;
+            st->rep_off[k] = g.off + ((eabits || csrbits) ? (int64_t)((k - r) / 32) : (int64_t)(k - r)) * g.stride;
             st->rep_pos_stride[k] = g.pos_stride;
         }
-        if ((eabits ? g.nrep / 32 : g.nrep) > ctx->max_grid_y) {  // replicas run along the grid's y dimension
+        if (((eabits || csrbits) ? g.nrep / 32 : g.nrep) > ctx->max_grid_y) {  // replicas run along the grid's y dimension
             const std::string msg = "at most " + std::to_string(ctx->max_grid_y) +
-                                    (eabits ? " words of 32 replicas" : " replicas") + " per lattice in one state";
+                                    ((eabits || csrbits) ? " words of 32 replicas" : " replicas") + " per lattice in one state";
             delete st;
             return mq_fail(ctx, MQ_ERR_UNSUPPORTED, msg);
         }
@@ -638,6 +660,7 @@ extern "C" int mq_state_init(mq_state* st, uint64_t seed)
     for (const auto& g : st->groups) {
         if (st->eabits) MQ_HIP(st->ctx, launch_init_eabits(st, g, key_of(seed)));
         else if (st->msc) MQ_HIP(st->ctx, launch_init_msc(st, g, key_of(seed)));
+        else if (st->csrbits) MQ_HIP(st->ctx, launch_init_csrbits(st, g, key_of(seed)));
         else if (g.lat->kind == LAT_REGULAR) MQ_HIP(st->ctx, launch_init_regular(st, g, key_of(seed)));
         else MQ_HIP(st->ctx, launch_init_csr(st, g, key_of(seed)));
     }
@@ -645,6 +668,13 @@ extern "C" int mq_state_init(mq_state* st, uint64_t seed)
     return MQ_OK;
 }
 
+
+static const Group* group_of(const mq_state* st, int rep)
+{
+    for (const auto& g : st->groups)
+        if (rep >= g.rep0 && rep < g.rep0 + g.nrep) return &g;
+    return nullptr;
+}
 
 // Copy one replica's sites between a dense host array (elem bytes per site, in
 // device position order) and the device: one contiguous copy when the replica
@@ -660,9 +690,7 @@ static int copy_replica(mq_state* st, int rep, void* host, size_t elem, bool to_
         else MQ_HIP(st->ctx, hipMemcpy(host, dev, (size_t)N * elem, hipMemcpyDeviceToHost));
         return MQ_OK;
     }
-    const Group* g = nullptr;
-    for (const auto& gg : st->groups)
-        if (rep >= gg.rep0 && rep < gg.rep0 + gg.nrep) g = &gg;
+    const Group* g = group_of(st, rep);
     const size_t bytes = (size_t)(N * ps);
     std::vector<char> img(bytes);
     char* gbase = st->d_spins + g->off;
@@ -710,6 +738,23 @@ extern "C" int mq_state_upload(mq_state* st, int rep, const void* host, int64_t 
             w = (h[i] < 0) ? (w | bit) : (w & ~bit);
         }
         MQ_HIP(st->ctx, hipMemcpy(st->d_spins + st->rep_off[rep], dev.data(), (size_t)N * 4, hipMemcpyHostToDevice));
+    } else if (st->csrbits) {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected one int8 per site");
+        const int8_t* h = static_cast<const int8_t*>(host);
+        for (int64_t i = 0; i < N; ++i)
+            if (h[i] != 1 && h[i] != -1) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: spin value out of range");
+        // read-modify-write of the group's words: bit (rep - rep0) % 32 of word group (rep - rep0) / 32
+        const Group* g = group_of(st, rep);
+        const int W = g->nrep / 32, w = (rep - g->rep0) / 32;
+        const uint32_t bit = 1u << ((rep - g->rep0) % 32);
+        std::vector<uint32_t> dev((size_t)(N * W));
+        MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + g->off, dev.size() * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; ++i) {
+            uint32_t& x = dev[(size_t)(dev_pos(l, i) * W + w)];
+            x = (h[i] < 0) ? (x | bit) : (x & ~bit);
+        }
+        MQ_HIP(st->ctx, hipMemcpy(st->d_spins + g->off, dev.data(), dev.size() * 4, hipMemcpyHostToDevice));
     } else if (st->msc) {
         if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "upload: expected one int8 per site");
         const int8_t* h = static_cast<const int8_t*>(host);
@@ -763,6 +808,15 @@ extern "C" int mq_state_download(mq_state* st, int rep, void* host, int64_t byte
         int8_t* h = static_cast<int8_t*>(host);
         const int b = rep % 32;
         for (int64_t i = 0; i < N; ++i) h[i] = ((dev[(size_t)dev_pos(l, i)] >> b) & 1u) ? (int8_t)-1 : (int8_t)1;
+    } else if (st->csrbits) {
+        if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
+        const Group* g = group_of(st, rep);
+        const int W = g->nrep / 32, w = (rep - g->rep0) / 32, b = (rep - g->rep0) % 32;
+        std::vector<uint32_t> dev((size_t)(N * W));
+        MQ_HIP(st->ctx, hipMemcpy(dev.data(), st->d_spins + g->off, dev.size() * 4, hipMemcpyDeviceToHost));
+        int8_t* h = static_cast<int8_t*>(host);
+        for (int64_t i = 0; i < N; ++i)
+            h[i] = ((dev[(size_t)(dev_pos(l, i) * W + w)] >> b) & 1u) ? (int8_t)-1 : (int8_t)1;
     } else if (st->msc) {
         if (bytes != N) return mq_fail(st->ctx, MQ_ERR_ARG, "download: expected one int8 per site");
         std::vector<uint32_t> dev((size_t)(N / 32));
@@ -789,7 +843,7 @@ int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep
     const Key2 key = key_of(seed);
     if (nsweeps > 0 && (sweep0 >= sweep_limit(st) || (uint64_t)nsweeps > sweep_limit(st) - sweep0))
         return mq_fail(st->ctx, MQ_ERR_ARG, sweep_limit_msg(st));
-    if (nsweeps > 0 && st->kernel_choice == 0 && !st->msc && !st->eabits) {
+    if (nsweeps > 0 && st->kernel_choice == 0 && !st->msc && !st->eabits && !st->csrbits) {
         hipError_t e = launch_sweeps_lds(st, nsweeps, (uint32_t)sweep0, key);
         if (e == hipSuccess) return MQ_OK;
         if (e != hipErrorNotSupported) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("lds sweep: ") + hipGetErrorString(e));
@@ -807,7 +861,8 @@ int mq_metropolis_async(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep
             const int nc = g.lat->ncolors;
             for (int c = 0; c < nc; ++c) {
                 const uint32_t phase = (uint32_t)(sw * (uint64_t)g.lat->color_stride + c);
-                hipError_t e = st->msc ? launch_sweep_msc(st, g, c, phase, key)
+                hipError_t e = st->msc       ? launch_sweep_msc(st, g, c, phase, key)
+                               : st->csrbits ? launch_sweep_csrbits(st, g, c, phase, key)
                                : (g.lat->kind == LAT_REGULAR) ? launch_sweep_regular(st, g, c, phase, key)
                                                               : launch_sweep_csr(st, g, c, phase, key);
                 if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
@@ -834,7 +889,7 @@ extern "C" int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t clu
     if (nclusters == 0) return MQ_OK;
     if (st->p.model == MQ_EA_ISING)
         return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff is not defined for the +-J spin glass (frustrated bonds)");
-    if (st->msc) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff runs on int8 storage (spin_bits = 8)");
+    if (st->msc || st->csrbits) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff runs on int8 storage (spin_bits = 8)");
     if (cluster0 + (uint64_t)nclusters > (1ull << 32)) return mq_fail(st->ctx, MQ_ERR_ARG, "cluster counter exceeds 2^32");
     for (const auto* l : st->lat)
         if (l->kind == LAT_REGULAR && l->dim == 3 && l->L > 1024)
@@ -877,6 +932,19 @@ extern "C" int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, ui
     return MQ_OK;
 }
 
+hipError_t launch_measure_all(mq_state* st, long long* isum, double* fsum)
+{
+    if (st->eabits) return launch_measure_eabits_all(st, isum);
+    for (const auto& g : st->groups) {
+        hipError_t e = st->msc       ? launch_measure_msc(st, g, isum)
+                       : st->csrbits ? launch_measure_csrbits(st, g, isum)
+                       : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g, isum, fsum)
+                                                      : launch_measure_csr(st, g, isum, fsum);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // Runs the measurement kernels and brings isum, fsum and the current betas to
 // the pinned host buffer h_pack ([nrep][17] words) with one copy.
 static int measure_sums(mq_state* st)
@@ -889,14 +957,8 @@ static int measure_sums(mq_state* st)
     }
     MQ_HIP(st->ctx, zero64(st->d_isum, (int64_t)st->nrep * 8, s));
     MQ_HIP(st->ctx, zero64(st->d_fsum, (int64_t)st->nrep * 8, s));
-    if (st->eabits) MQ_HIP(st->ctx, launch_measure_eabits_all(st, st->d_isum));
-    for (const auto& g : st->groups) {
-        if (st->eabits) break;
-        hipError_t e = st->msc ? launch_measure_msc(st, g, st->d_isum)
-                       : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g, st->d_isum, st->d_fsum)
-                                                      : launch_measure_csr(st, g, st->d_isum, st->d_fsum);
-        if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
-    }
+    hipError_t me = launch_measure_all(st, st->d_isum, st->d_fsum);
+    if (me != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(me));
     const int64_t n = (int64_t)st->nrep * 17;
     hipLaunchKernelGGL(k_pack_obs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        reinterpret_cast<const unsigned long long*>(st->d_isum), st->d_fsum, st->d_betad, st->nrep,
@@ -1013,14 +1075,8 @@ extern "C" int mq_measure_noalloc(mq_state* st, double* d_out)
     // the array keeps the previous call's sums: clear this state's part with fill kernels
     MQ_HIP(st->ctx, zero64(isum, (int64_t)st->nrep * 8, s));
     MQ_HIP(st->ctx, zero64(fsum, (int64_t)st->nrep * 8, s));
-    if (st->eabits) MQ_HIP(st->ctx, launch_measure_eabits_all(st, isum));
-    for (const auto& g : st->groups) {
-        if (st->eabits) break;
-        hipError_t e = st->msc ? launch_measure_msc(st, g, isum)
-                       : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g, isum, fsum)
-                                                      : launch_measure_csr(st, g, isum, fsum);
-        if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(e));
-    }
+    hipError_t me = launch_measure_all(st, isum, fsum);
+    if (me != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("measure launch: ") + hipGetErrorString(me));
     for (const auto& g : st->groups) {
         hipLaunchKernelGGL(k_finish_obs, dim3((unsigned)((g.nrep + 255) / 256)), dim3(256), 0, s, isum, fsum,
                            st->d_betad, g.rep0, g.nrep, (double)g.lat->N, st->p.model, st->p.q, st->p.J, st->p.mass,
@@ -1037,7 +1093,7 @@ extern "C" int mq_measure_ft(mq_state* st, double* out)
     if (st->p.model == MQ_POTTS)
         return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED,
                        "MagFTComp: Ising, EA, Heisenberg and Phi4 spins only (the reference measures none for Potts)");
-    if (st->msc || st->eabits)
+    if (st->msc || st->eabits || st->csrbits)
         return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "MagFTComp: int8 or float4 storage only (spin_bits = 8)");
     for (const auto* l : st->lat)
         if (l->kind != LAT_REGULAR)

@@ -60,20 +60,6 @@ __global__ void __launch_bounds__(256) k_csr_init(char* __restrict__ base, int64
     }
 }
 
-// Neighbours are visited in chunks of NBCH: the chunk's column indices are
-// loaded together, then its spins, so a vertex costs two dependent memory
-// round trips per chunk instead of two per neighbour.
-constexpr int NBCH = 8;
-
-// Slots past the end of the row repeat the row's last slot, so every load of
-// a chunk is unconditional (the compiler then issues them back to back);
-// callers mask those slots out of the sums.
-__device__ __forceinline__ void chunk_cols(const int32_t* __restrict__ ci, int k0, int b, int (&nb)[NBCH])
-{
-#pragma unroll
-    for (int t = 0; t < NBCH; ++t) nb[t] = ci[min(k0 + t, b - 1)];
-}
-
 __device__ __forceinline__ void gather_i8(const int8_t* s, int64_t ps, const int32_t* __restrict__ ci, int k0, int b,
                                           int (&v)[NBCH])
 {

@@ -231,49 +231,6 @@ __device__ __forceinline__ void acc_add3(uint32_t (&A)[NP], uint32_t b0, uint32_
     }
 }
 
-// Bit-sliced add of Y into X over NP planes (values that overflow NP planes are
-// not produced by the callers).
-template <int NP>
-__device__ __forceinline__ void bs_add(uint32_t (&X)[NP], const uint32_t (&Y)[NP])
-{
-    uint32_t carry = 0u;
-#pragma unroll
-    for (int p = 0; p < NP; ++p) {
-        const uint32_t x = X[p], y = Y[p];
-        X[p] = x ^ y ^ carry;
-        carry = (x & y) | (x & carry) | (y & carry);
-    }
-}
-
-// Reduce-scatter of bit-sliced per-bit counts across the 64 lanes of a
-// wavefront: after five exchange steps (partners lane ^ 16, 8, 4, 2, 1, each
-// lane keeping the half of the 32 bit positions its own lane bit selects) lane
-// l holds the sum over its 32-lane half of bit position l & 31; the two halves
-// are added as integers.  Returns that count.
-template <int NP>
-__device__ __forceinline__ int bs_reduce_scatter(uint32_t (&A)[NP])
-{
-    const int lane = (int)__lane_id();
-#pragma unroll
-    for (int m = 16; m >= 1; m >>= 1) {
-        // bit positions j with (j & m) != 0
-        const uint32_t hi = m == 16 ? 0xFFFF0000u : m == 8 ? 0xFF00FF00u : m == 4 ? 0xF0F0F0F0u : m == 2 ? 0xCCCCCCCCu : 0xAAAAAAAAu;
-        const uint32_t keep = (lane & m) ? hi : ~hi;
-        uint32_t Y[NP];
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            Y[p] = (uint32_t)__shfl_xor((int)A[p], m, 64) & keep;
-            A[p] &= keep;
-        }
-        bs_add<NP>(A, Y);
-    }
-    const int j = lane & 31;
-    int c = 0;
-#pragma unroll
-    for (int p = 0; p < NP; ++p) c += (int)((A[p] >> j) & 1u) << p;
-    return c + __shfl_xor(c, 32, 64);
-}
-
 // Per replica: isum[r*8+0] += sum over bonds b s_i s_j, isum[r*8+1] += sum s.
 // One launch over every lattice size; a wavefront takes up to 64*MSPL colour-0
 // sites of one word of 32 replicas (table entries ngroups.. of the group

@@ -158,16 +158,7 @@ hipError_t energy_async(mq_state* st)
     hipError_t e;
     if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     if ((e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
-    if (st->eabits) {
-        if ((e = launch_measure_eabits_all(st, st->d_isum)) != hipSuccess) return e;
-    }
-    for (const auto& g : st->groups) {
-        if (st->eabits) break;
-        e = st->msc ? launch_measure_msc(st, g, st->d_isum)
-            : (g.lat->kind == LAT_REGULAR) ? launch_measure_regular(st, g, st->d_isum, st->d_fsum)
-                                           : launch_measure_csr(st, g, st->d_isum, st->d_fsum);
-        if (e != hipSuccess) return e;
-    }
+    if ((e = launch_measure_all(st, st->d_isum, st->d_fsum)) != hipSuccess) return e;
     const int vec = (st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4);
     hipLaunchKernelGGL(k_energy, dim3((unsigned)((st->nrep + 255) / 256)), dim3(256), 0, s, st->d_isum, st->d_fsum,
                        st->nrep, vec, st->p.model == MQ_POTTS, st->p.J, st->d_energy);

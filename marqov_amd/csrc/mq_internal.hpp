@@ -89,6 +89,65 @@ __device__ __forceinline__ uint32_t fdiv_u24(uint32_t n, uint32_t d, float inv, 
 __device__ __forceinline__ float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }
 
 // ---------------------------------------------------------------------------
+// Bit-sliced counters: plane p of a count holds bit p of 32 per-bit counts.
+// Bit-sliced add of Y into X over NP planes (values that overflow NP planes are
+// not produced by the callers).
+template <int NP>
+__device__ __forceinline__ void bs_add(uint32_t (&X)[NP], const uint32_t (&Y)[NP])
+{
+    uint32_t carry = 0u;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const uint32_t x = X[p], y = Y[p];
+        X[p] = x ^ y ^ carry;
+        carry = (x & y) | (x & carry) | (y & carry);
+    }
+}
+
+// Reduce-scatter of bit-sliced per-bit counts across the 64 lanes of a
+// wavefront: after five exchange steps (partners lane ^ 16, 8, 4, 2, 1, each
+// lane keeping the half of the 32 bit positions its own lane bit selects) lane
+// l holds the sum over its 32-lane half of bit position l & 31; the two halves
+// are added as integers.  Returns that count.
+template <int NP>
+__device__ __forceinline__ int bs_reduce_scatter(uint32_t (&A)[NP])
+{
+    const int lane = (int)__lane_id();
+#pragma unroll
+    for (int m = 16; m >= 1; m >>= 1) {
+        // bit positions j with (j & m) != 0
+        const uint32_t hi = m == 16 ? 0xFFFF0000u : m == 8 ? 0xFF00FF00u : m == 4 ? 0xF0F0F0F0u : m == 2 ? 0xCCCCCCCCu : 0xAAAAAAAAu;
+        const uint32_t keep = (lane & m) ? hi : ~hi;
+        uint32_t Y[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            Y[p] = (uint32_t)__shfl_xor((int)A[p], m, 64) & keep;
+            A[p] &= keep;
+        }
+        bs_add<NP>(A, Y);
+    }
+    const int j = lane & 31;
+    int c = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) c += (int)((A[p] >> j) & 1u) << p;
+    return c + __shfl_xor(c, 32, 64);
+}
+
+// ---------------------------------------------------------------------------
+// Graph rows are visited in chunks of NBCH neighbours: the chunk's column
+// indices are loaded together, then its spins, so a vertex costs two dependent
+// memory round trips per chunk instead of two per neighbour.  Slots past the
+// end of the row repeat the row's last slot, so every load of a chunk is
+// unconditional (the compiler then issues them back to back); callers mask
+// those slots out of their sums.
+constexpr int NBCH = 8;
+__device__ __forceinline__ void chunk_cols(const int32_t* __restrict__ ci, int k0, int b, int (&nb)[NBCH])
+{
+#pragma unroll
+    for (int t = 0; t < NBCH; ++t) nb[t] = ci[min(k0 + t, b - 1)];
+}
+
+// ---------------------------------------------------------------------------
 // Wolff cluster membership on int8 spins.  A cluster claims a site by changing
 // its spin byte with one atomic on the containing 32-bit word (spin buffers are
 // 256-byte aligned, so the word never leaves the buffer); the returned old
@@ -281,6 +340,7 @@ struct mq_state {
     size_t lds_need[2] = {0, 0};        // LDS bytes per block without / with couplings
     bool msc = false;                   // 1-bit multi-spin-coded Ising storage
     bool eabits = false;                // EA: 32 replicas of one sample per uint32 word
+    bool csrbits = false;               // Ising on graphs: 32 replicas of one graph per uint32 word
     void* d_wtab = nullptr;             // per word group: per-bit threshold words
     void* d_bits_groups = nullptr;      // lattice-group table of the fused bit-sliced sweep
     int bits_ngroups = 0, bits_blocks = 0;
@@ -331,6 +391,13 @@ hipError_t launch_init_csr(mq_state* st, const Group& g, Key2 key);
 hipError_t launch_sweep_csr(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
 hipError_t launch_measure_csr(mq_state* st, const Group& g, long long* isum, double* fsum);
 hipError_t launch_wolff_csr(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
+bool csrbits_supported(const mq_lattice* lat);
+hipError_t launch_init_csrbits(mq_state* st, const Group& g, Key2 key);
+hipError_t launch_sweep_csrbits(mq_state* st, const Group& g, int colour, uint32_t phase, Key2 key);
+hipError_t launch_measure_csrbits(mq_state* st, const Group& g, long long* isum);
+// measurement sums of every replica into isum / fsum ([nrep][8] each, zeroed by
+// the caller), one launch per lattice group or one fused launch for bit-sliced EA
+hipError_t launch_measure_all(mq_state* st, long long* isum, double* fsum);
 
 // Threads of the workgroup that grows one replica's clusters in global memory.
 // A wide frontier level is worked through blockDim entries at a time, each
