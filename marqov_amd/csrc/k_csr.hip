@@ -734,26 +734,43 @@ extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, co
             if (j < 0 || j >= N || j == i) return mq_fail(ctx, MQ_ERR_ARG, "bad neighbour index or self loop");
         }
     }
-    // symmetry and canonical slots: slot of (u -> v) mirrored to (v -> u)
+    // symmetry and canonical slots: slot of (u -> v) mirrored to (v -> u), in O(nnz)
+    // with local sorts: the incoming slots of every vertex are listed by a counting
+    // transpose (in increasing source), each row's own slots are sorted by column, and
+    // the two lists of a vertex must then name the same neighbours in the same order
     std::vector<int32_t> mirror(nnz, -1);
     {
-        std::vector<int32_t> fill(row_ptr, row_ptr + N);
-        std::vector<std::pair<int64_t, int32_t>> keyed;
-        keyed.reserve(nnz);
-        for (int64_t u = 0; u < N; ++u)
-            for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) keyed.push_back({u * N + col_idx[k], k});
-        std::vector<std::pair<int64_t, int32_t>> sorted = keyed;
-        std::sort(sorted.begin(), sorted.end());
-        for (size_t t = 1; t < sorted.size(); ++t)
-            if (sorted[t].first == sorted[t - 1].first) return mq_fail(ctx, MQ_ERR_ARG, "duplicate edge");
-        for (int64_t u = 0; u < N; ++u)
-            for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) {
-                const int64_t rev = (int64_t)col_idx[k] * N + u;
-                auto it = std::lower_bound(sorted.begin(), sorted.end(), std::make_pair(rev, (int32_t)-1));
-                if (it == sorted.end() || it->first != rev) return mq_fail(ctx, MQ_ERR_ARG, "adjacency is not symmetric");
-                mirror[k] = it->second;
+        std::vector<int32_t> by_col(nnz);  // row u's slots sorted by column
+        for (int64_t u = 0; u < N; ++u) {
+            const int a = row_ptr[u], b = row_ptr[u + 1];
+            for (int k = a; k < b; ++k) by_col[k] = k;
+            std::sort(by_col.begin() + a, by_col.begin() + b,
+                      [&](int32_t x, int32_t y) { return col_idx[x] < col_idx[y]; });
+            for (int k = a + 1; k < b; ++k)
+                if (col_idx[by_col[k]] == col_idx[by_col[k - 1]]) return mq_fail(ctx, MQ_ERR_ARG, "duplicate edge");
+        }
+        std::vector<int64_t> tp(N + 1, 0);  // incoming slots per vertex
+        for (int64_t k = 0; k < nnz; ++k) tp[col_idx[k] + 1]++;
+        for (int64_t v = 0; v < N; ++v) tp[v + 1] += tp[v];
+        std::vector<int32_t> tin(nnz), src(nnz);
+        {
+            std::vector<int64_t> fill(tp.begin(), tp.end() - 1);
+            for (int64_t u = 0; u < N; ++u)
+                for (int k = row_ptr[u]; k < row_ptr[u + 1]; ++k) {
+                    tin[fill[col_idx[k]]++] = k;  // sources in increasing order
+                    src[k] = (int32_t)u;
+                }
+        }
+        for (int64_t v = 0; v < N; ++v) {
+            const int a = row_ptr[v], deg = row_ptr[v + 1] - a;
+            if (tp[v + 1] - tp[v] != deg) return mq_fail(ctx, MQ_ERR_ARG, "adjacency is not symmetric");
+            for (int t = 0; t < deg; ++t) {
+                const int32_t ko = by_col[a + t];       // v -> u
+                const int32_t ki = tin[tp[v] + t];      // u' -> v
+                if (col_idx[ko] != src[ki]) return mq_fail(ctx, MQ_ERR_ARG, "adjacency is not symmetric");
+                mirror[ko] = ki;
             }
-        (void)fill;
+        }
     }
     // greedy colouring in vertex order: the smallest colour no earlier neighbour has
     // (at most degree + 1 colours)
