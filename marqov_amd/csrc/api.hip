@@ -315,6 +315,7 @@ extern "C" void mq_lattice_destroy(mq_lattice* lat)
     dfree(lat->d_edge);
     dfree(lat->d_pos_of);
     dfree(lat->d_vert_of);
+    if (lat->d_ell) (void)hipFree(lat->d_ell);
     delete lat;
 }
 

@@ -267,6 +267,7 @@ struct mq_lattice {
     int32_t* d_edge = nullptr;       // canonical (original) slot id of each slot
     int32_t* d_pos_of = nullptr;     // original vertex -> position
     int32_t* d_vert_of = nullptr;    // position -> original vertex
+    void* d_ell = nullptr;           // padded row table of the int8 graph Wolff (built on first use)
     std::vector<int32_t> colour;     // colour of each original vertex
     std::vector<int32_t> h_slot_of_orig;  // original CSR slot -> device slot
     std::vector<int32_t> h_mirror;        // original slot (u->v) -> original slot (v->u)
