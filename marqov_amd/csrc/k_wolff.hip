@@ -91,6 +91,10 @@ __device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
 constexpr int WCAP = 4096;
 // bond tasks a lane takes at once on a wide level
 constexpr int WKT = 4;
+// int8 levels of more than SITE_MIN * blockDim bond tasks with claims in global
+// memory run one lane per site (thermalised 3D Ising L=256 x 32, the same clusters:
+// 0.495 sites/ns with bond tasks only, 0.506 / 0.523 / 0.522 with SITE_MIN 1 / 2 / 4)
+constexpr int SITE_MIN = 2;
 
 // LDS bytes of one replica's spins in the small-lattice kernel: a byte per
 // site, or a bit per site when packed (16-byte multiples)
@@ -224,6 +228,96 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                 glob = true;
                 __syncthreads();
             }
+            if ((KIND == WK_ISING || KIND == WK_POTTS) && glob && ntask > SITE_MIN * (int)blockDim.x) {
+                // A wide level with claims in global memory: one lane per frontier site.  The
+                // site's coordinates are unpacked once, its DIM forward bonds (owner*DIM + d)
+                // lie in at most two consecutive Philox blocks, and its 2*DIM claims are in
+                // flight together: 5 Philox evaluations per 3D site instead of 6, and the
+                // per-task addressing once per site.
+                const int nsite = tail - head;
+                for (int e = (int)threadIdx.x; e < nsite; e += (int)blockDim.x) {
+                    const int idx = head + e;
+                    const uint32_t cur = (idx - hprev < WCAP) ? s_ring[idx & (WCAP - 1)] : (uint32_t)q[idx];
+                    const uint32_t x0 = cur & CM, x1 = (cur >> SB) & CM, x2 = SitePack<DIM>::z(cur);
+                    const uint32_t curn = x0 + uL * x1 + LL * x2;
+                    const uint32_t nh = ((x0 + x1 + x2) & 1u) ? 0u : half;  // the neighbours' colour half
+                    uint32_t jp[NS], jn[NS], jan[NS], u[NS], seen[NS], prev[NS];
+                    bool ok[NS];
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const int d = sl >> 1;
+                        const uint32_t xd = d == 0 ? x0 : (d == 1 ? x1 : x2);
+                        uint32_t nx;
+                        if ((sl & 1) == 0) nx = (xd == uL - 1) ? 0u : xd + 1u;
+                        else nx = (xd == 0) ? uL - 1u : xd - 1u;
+                        jp[sl] = (cur & ~(CM << (SB * d))) | (nx << (SB * d));
+                        jn[sl] = curn + (nx - xd) * (d == 0 ? 1u : (d == 1 ? uL : LL));  // mod 2^32
+                        jan[sl] = nh + (jn[sl] >> 1);
+                        seen[sl] = 0u;
+                        if (KIND == WK_POTTS) {
+                            uint32_t sh;
+                            seen[sl] = peek_word(sp8 + jan[sl], sh);  // in flight during the Philox rounds
+                        }
+                    }
+                    // forward bonds: blocks (curn*DIM) >> 2 and the next one
+                    const uint64_t bf = (uint64_t)curn * DIM;
+                    const uint32_t blk0 = (uint32_t)(bf >> 2);
+                    const uint4 wf0 = philox4x32_10(blk0, ncl, (uint32_t)grep, ST_WBOND, key);
+                    const uint4 wf1 = (DIM == 3) ? philox4x32_10(blk0 + 1u, ncl, (uint32_t)grep, ST_WBOND, key) : wf0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        const uint64_t b = bf + (uint64_t)d;
+                        u[2 * d] = ((uint32_t)(b >> 2) == blk0) ? u4_get(wf0, (int)(b & 3)) : u4_get(wf1, (int)(b & 3));
+                        // backward bond: owned by the neighbour in direction -d
+                        const uint64_t bb = (uint64_t)jn[2 * d + 1] * DIM + (uint64_t)d;
+                        const uint4 wb = philox4x32_10((uint32_t)(bb >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                        u[2 * d + 1] = u4_get(wb, (int)(bb & 3));
+                    }
+                    if (KIND == WK_ISING) {
+                        const bool up = __builtin_amdgcn_readfirstlane(val) > 0;
+#pragma unroll
+                        for (int sl = 0; sl < NS; ++sl) {
+                            prev[sl] = 0u;
+                            if (u[sl] < tw) {  // the claim itself tests alignment
+                                uint32_t sh;
+                                uint32_t* w = spin_word(sp8 + jan[sl], sh);
+                                const uint32_t m = 0xFEu << sh;
+                                prev[sl] = up ? atomicOr(w, m) : atomicAnd(w, ~m);
+                            }
+                        }
+#pragma unroll
+                        for (int sl = 0; sl < NS; ++sl)
+                            ok[sl] = u[sl] < tw && (int8_t)(prev[sl] >> ((jan[sl] & 3u) * 8u)) == (int8_t)val;
+                    } else {
+                        bool tr[NS];
+#pragma unroll
+                        for (int sl = 0; sl < NS; ++sl) {
+                            const uint32_t sh = (jan[sl] & 3u) * 8u;
+                            tr[sl] = (int8_t)(seen[sl] >> sh) == (int8_t)val && u[sl] < tw;
+                            prev[sl] = 0u;
+                            if (tr[sl]) {
+                                uint32_t sh2;
+                                uint32_t* w = spin_word(sp8 + jan[sl], sh2);
+                                prev[sl] = atomicCAS(w, seen[sl], (seen[sl] & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
+                            }
+                        }
+#pragma unroll
+                        for (int sl = 0; sl < NS; ++sl)  // a word changed under the first swap: retry from what it held
+                            ok[sl] = tr[sl] && (prev[sl] == seen[sl] || claim_potts(sp8 + jan[sl], prev[sl], val, newq));
+                    }
+                    uint32_t nok = 0;
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) nok += ok[sl] ? 1u : 0u;
+                    int pos = wave_reserve(&s_tail, nok);
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        if (!ok[sl]) continue;
+                        if (pos - head < WCAP) s_ring[pos & (WCAP - 1)] = jp[sl];
+                        else q[pos] = (int32_t)jp[sl];
+                        ++pos;
+                    }
+                }
+            } else
             if (ntask > (int)blockDim.x) {
                 // A wide level: each lane takes up to WKT bond tasks t, t + blockDim, ... at
                 // once, issues all their loads, draws their uniforms and then issues all
