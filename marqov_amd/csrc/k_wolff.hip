@@ -317,6 +317,76 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
                         ++pos;
                     }
                 }
+            } else if (VEC && glob && ntask > SITE_MIN * (int)blockDim.x) {
+                // The same for O(3)/Phi4 with the visited set in the global bitmap: the site's
+                // projection is read once, its 2*DIM neighbour spins are loaded together, and
+                // its bitmap claims are in flight together.
+                const int nsite = tail - head;
+                for (int e = (int)threadIdx.x; e < nsite; e += (int)blockDim.x) {
+                    const int idx = head + e;
+                    const bool in_ring = idx - hprev < WCAP;
+                    const uint32_t cur = in_ring ? s_ring[idx & (WCAP - 1)] : (uint32_t)q[idx];
+                    const uint32_t x0 = cur & CM, x1 = (cur >> SB) & CM, x2 = SitePack<DIM>::z(cur);
+                    const uint32_t curn = x0 + uL * x1 + LL * x2;
+                    const uint32_t par = (x0 + x1 + x2) & 1u;
+                    const uint32_t nh = par ? 0u : half;  // the neighbours' colour half
+                    const float pc = in_ring ? s_proj[idx & (WCAP - 1)]
+                                             : proj<KIND>(sp4[(par ? half : 0u) + (curn >> 1)], axis, rd);
+                    uint32_t jp[NS], jn[NS], jan[NS], u[NS], old[NS];
+                    float4 vj[NS];
+                    bool ok[NS], tr[NS];
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const int d = sl >> 1;
+                        const uint32_t xd = d == 0 ? x0 : (d == 1 ? x1 : x2);
+                        uint32_t nx;
+                        if ((sl & 1) == 0) nx = (xd == uL - 1) ? 0u : xd + 1u;
+                        else nx = (xd == 0) ? uL - 1u : xd - 1u;
+                        jp[sl] = (cur & ~(CM << (SB * d))) | (nx << (SB * d));
+                        jn[sl] = curn + (nx - xd) * (d == 0 ? 1u : (d == 1 ? uL : LL));  // mod 2^32
+                        jan[sl] = nh + (jn[sl] >> 1);
+                        vj[sl] = sp4[jan[sl]];  // in flight during the Philox rounds
+                    }
+                    const uint64_t bf = (uint64_t)curn * DIM;
+                    const uint32_t blk0 = (uint32_t)(bf >> 2);
+                    const uint4 wf0 = philox4x32_10(blk0, ncl, (uint32_t)grep, ST_WBOND, key);
+                    const uint4 wf1 = (DIM == 3) ? philox4x32_10(blk0 + 1u, ncl, (uint32_t)grep, ST_WBOND, key) : wf0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        const uint64_t b = bf + (uint64_t)d;
+                        u[2 * d] = ((uint32_t)(b >> 2) == blk0) ? u4_get(wf0, (int)(b & 3)) : u4_get(wf1, (int)(b & 3));
+                        const uint64_t bb = (uint64_t)jn[2 * d + 1] * DIM + (uint64_t)d;
+                        const uint4 wb = philox4x32_10((uint32_t)(bb >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                        u[2 * d + 1] = u4_get(wb, (int)(bb & 3));
+                    }
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const float cpl = J * pc * proj<KIND>(vj[sl], axis, rd);
+                        tr[sl] = cpl > 0.f && u01f(u[sl]) < -expm1f(-2.0f * beta * cpl);
+                        old[sl] = 0u;
+                        if (tr[sl]) old[sl] = atomicOr(&vis[jn[sl] >> 5], 1u << (jn[sl] & 31));
+                    }
+                    uint32_t nok = 0;
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        ok[sl] = tr[sl] && !(old[sl] & (1u << (jn[sl] & 31)));
+                        nok += ok[sl] ? 1u : 0u;
+                    }
+                    int pos = wave_reserve(&s_tail, nok);
+#pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        if (!ok[sl]) continue;
+                        // the claim succeeded, so nobody changed vj since it was read
+                        const float4 v = reflect<KIND>(vj[sl], axis, rd);
+                        sp4[jan[sl]] = v;
+                        if (pos - head < WCAP) {
+                            s_ring[pos & (WCAP - 1)] = jp[sl];
+                            s_proj[pos & (WCAP - 1)] = proj<KIND>(v, axis, rd);
+                        }
+                        q[pos] = (int32_t)jp[sl];  // every member, for the bitmap
+                        ++pos;
+                    }
+                }
             } else
             if (ntask > (int)blockDim.x) {
                 // A wide level: each lane takes up to WKT bond tasks t, t + blockDim, ... at
