@@ -165,6 +165,16 @@ MQ_API int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t cluster
  * Equal to mq_wolff(ncluster, cluster0 = step*ncluster) followed by
  * mq_metropolis(nmetro, sweep0 = step*nmetro); negative counts are refused. */
 MQ_API int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, uint64_t step, double* mean_cluster);
+/* nsteps EMCS of every replica in one persistent launch; replaces the EMCS
+ * loops of Core::wrmploop / gameloop (core.h:1034-1050, 1001-1020) between
+ * measurements.  Equal, bit for bit, to nsteps calls of
+ * mq_emcs(ncluster, nmetro, seed, step0 + k), but each replica runs its steps
+ * without waiting for the others, so a batch is not held to its slowest
+ * replica at every step.  Regular lattices, int8 Ising or Potts without
+ * couplings.  mean_cluster (host, may be NULL): [nsteps][nrep], NaN when
+ * ncluster = 0. */
+MQ_API int mq_emcs_run(mq_state* st, int nsteps, int ncluster, int nmetro, uint64_t seed, uint64_t step0,
+                       double* mean_cluster);
 /* observables, FP64 (observables.h:32-66,173-352; Potts.h:40-65): out[r*MQ_NOBS+k] */
 MQ_API int mq_measure(mq_state* st, double* out);
 /* No counterpart in the reference: mq_measure for callers that cannot allocate

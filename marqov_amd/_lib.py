@@ -70,6 +70,7 @@ SIGNATURES = {
     "mq_metropolis": (_I, [_P, _I, _U64, _U64]),
     "mq_wolff": (_I, [_P, _I, _U64, _U64, _P]),
     "mq_emcs": (_I, [_P, _I, _I, _U64, _U64, _P]),
+    "mq_emcs_run": (_I, [_P, _I, _I, _I, _U64, _U64, _P]),
     "mq_measure": (_I, [_P, _P]),
     "mq_measure_noalloc": (_I, [_P, _P]),
     "mq_measure_ft": (_I, [_P, _P]),

@@ -933,6 +933,54 @@ extern "C" int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, ui
     return MQ_OK;
 }
 
+extern "C" int mq_emcs_run(mq_state* st, int nsteps, int ncluster, int nmetro, uint64_t seed, uint64_t step0,
+                           double* mean_cluster)
+{
+    int rc = check_state(st);
+    if (rc) return rc;
+    if (nsteps < 0 || ncluster < 0 || nmetro < 0)
+        return mq_fail(st->ctx, MQ_ERR_ARG, "nsteps, ncluster and nmetro must be >= 0");
+    if (nsteps == 0) return MQ_OK;
+    if (st->p.model != MQ_ISING && st->p.model != MQ_POTTS)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "mq_emcs_run: Ising and Potts models");
+    if (st->msc || st->eabits || st->csrbits)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "mq_emcs_run: int8 storage (spin_bits = 8)");
+    if (st->d_bonds) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "mq_emcs_run: no couplings");
+    for (const auto* l : st->lat) {
+        if (l->kind != LAT_REGULAR) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "mq_emcs_run: regular lattices");
+        if (l->dim == 3 && l->L > 1024)
+            return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff on 3D lattices needs L <= 1024 (10-bit site coordinates)");
+    }
+    const uint64_t last = step0 + (uint64_t)nsteps;  // one past the last step
+    if (ncluster > 0 && (step0 >= (1ull << 32) || last * (uint64_t)ncluster > (1ull << 32)))
+        return mq_fail(st->ctx, MQ_ERR_ARG, "cluster counter exceeds 2^32");
+    if (nmetro > 0 && (step0 >= sweep_limit(st) || last * (uint64_t)nmetro > sweep_limit(st)))
+        return mq_fail(st->ctx, MQ_ERR_ARG, sweep_limit_msg(st));
+    const int64_t ncl_all = (int64_t)ncluster * nsteps;
+    if (ncl_all > (1 << 30)) return mq_fail(st->ctx, MQ_ERR_ARG, "nsteps * ncluster too large");
+    rc = ensure_wolff_scratch(st, (int)std::max<int64_t>(ncl_all, 1));
+    if (rc) return rc;
+    for (const auto& g : st->groups) {
+        hipError_t e = launch_emcs_regular(st, g, nsteps, ncluster, nmetro, step0, key_of(seed));
+        if (e != hipSuccess) return mq_fail(st->ctx, MQ_ERR_HIP, std::string("emcs launch: ") + hipGetErrorString(e));
+    }
+    if (mean_cluster) {
+        std::vector<int64_t> sizes((size_t)std::max<int64_t>(ncl_all, 1) * st->nrep);
+        if (ncluster > 0)
+            MQ_HIP(st->ctx, hipMemcpyAsync(sizes.data(), st->d_csize, sizeof(int64_t) * (size_t)(ncl_all * st->nrep),
+                                           hipMemcpyDeviceToHost, st->ctx->stream));
+        MQ_HIP(st->ctx, hipStreamSynchronize(st->ctx->stream));
+        for (int k = 0; k < nsteps; ++k)
+            for (int r = 0; r < st->nrep; ++r) {
+                double a = 0;
+                for (int c = 0; c < ncluster; ++c)
+                    a += (double)sizes[((size_t)k * st->nrep + r) * ncluster + c];
+                mean_cluster[(size_t)k * st->nrep + r] = a / ncluster;  // NaN for ncluster = 0, as emcs.h:108
+            }
+    }
+    return MQ_OK;
+}
+
 hipError_t launch_measure_all(mq_state* st, long long* isum, double* fsum)
 {
     if (st->eabits) return launch_measure_eabits_all(st, isum);

@@ -120,9 +120,11 @@ struct SitePack {
     }
 };
 
+// Grows nclusters clusters of replica r (group-local index) with the calling
+// workgroup; the body of k_wolff_regular and of the persistent EMCS kernel.
 template <int DIM, int KIND>
-__global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base, int64_t stride, int L, int nclusters,
-                                                       uint32_t cl0, int rep0, int Q,
+__device__ __forceinline__ void wolff_regular_clusters(const int r, char* __restrict__ base, int64_t stride, int L,
+                                                       int nclusters, uint32_t cl0, int rep0, int Q,
                                                        const uint32_t* __restrict__ wthr,
                                                        const float* __restrict__ betaf, float J,
                                                        int32_t* __restrict__ queue, uint32_t* __restrict__ visit,
@@ -137,7 +139,6 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
     __shared__ uint32_t s_ring[WCAP];
     __shared__ float s_proj[VEC ? WCAP : 1];
     __shared__ uint32_t s_vset[MQ_VSET];
-    const int r = blockIdx.x;
     const int grep = rep0 + r;
     const uint32_t uL = (uint32_t)L;
     const uint32_t LL = uL * uL;
@@ -597,6 +598,147 @@ __global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base,
     }
 }
 
+template <int DIM, int KIND>
+__global__ void __launch_bounds__(1024) k_wolff_regular(char* __restrict__ base, int64_t stride, int L, int nclusters,
+                                                       uint32_t cl0, int rep0, int Q,
+                                                       const uint32_t* __restrict__ wthr,
+                                                       const float* __restrict__ betaf, float J,
+                                                       int32_t* __restrict__ queue, uint32_t* __restrict__ visit,
+                                                       const int64_t* __restrict__ rep_site_off,
+                                                       int64_t* __restrict__ csize, int vlim, Key2 key)
+{
+    wolff_regular_clusters<DIM, KIND>((int)blockIdx.x, base, stride, L, nclusters, cl0, rep0, Q, wthr, betaf, J, queue,
+                                      visit, rep_site_off, csize, vlim, key);
+}
+
+// Workgroup sweeps of one replica for the persistent EMCS kernel: the decisions
+// of the Ising and Potts sweep kernels (the same Philox counters and exact
+// acceptance), Ising one group of 8 half-sites per thread and Philox block,
+// Potts one half-site per thread and block.
+template <int DIM>
+__device__ __forceinline__ void wg_sweep_ising(int8_t* sp, const RegGeom& G, int colour, uint32_t phase, uint32_t grep,
+                                               const uint32_t* s_thr, Key2 key)
+{
+    int8_t* own = sp + colour * G.half;
+    const int8_t* oth = sp + (1 - colour) * G.half;
+    const int64_t ngr = (G.half + 7) >> 3;
+    constexpr int QMAX = 2 * DIM;
+    for (int64_t g = threadIdx.x; g < ngr; g += blockDim.x) {
+        const int64_t h0 = g << 3;
+        const uint4 w = philox4x32_10((uint32_t)g, phase, grep, ST_METRO, key);
+        uint4 tie = make_uint4(0, 0, 0, 0);
+        bool have_tie = false;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int64_t h = h0 + m;
+            if (h >= G.half) break;
+            int64_t nb[2 * DIM];
+            reg_nbrs<DIM>(G, colour, h, nb);
+            int hs = 0;
+#pragma unroll
+            for (int sl = 0; sl < 2 * DIM; ++sl) hs += oth[nb[sl]];
+            const int s = own[h];
+            const int e = s * hs + QMAX;
+            const uint32_t hi16 = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+            const uint32_t thi = s_thr[2 * e];
+            bool acc = hi16 < thi;
+            if (hi16 == thi) {
+                if (!have_tie) {
+                    tie = philox4x32_10((uint32_t)g, phase, grep, ST_TIE, key);
+                    have_tie = true;
+                }
+                acc = ((u4_get(tie, m >> 1) >> (16 * (m & 1))) & 0xFFFFu) < s_thr[2 * e + 1];
+            }
+            if (acc) own[h] = (int8_t)(-s);
+        }
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void wg_sweep_potts(int8_t* sp, const RegGeom& G, int Q, int colour, uint32_t phase,
+                                               uint32_t grep, const uint32_t* s_thr, Key2 key)
+{
+    int8_t* own = sp + colour * G.half;
+    const int8_t* oth = sp + (1 - colour) * G.half;
+    for (int64_t h = threadIdx.x; h < G.half; h += blockDim.x) {
+        int64_t nb[2 * DIM];
+        reg_nbrs<DIM>(G, colour, h, nb);
+        const uint4 w = philox4x32_10((uint32_t)h, phase, grep, ST_METRO, key);
+        const int snew = (int)(((uint64_t)w.x * (uint64_t)Q) >> 32);
+        const int sold = own[h];
+        int cn = 0, co = 0;
+#pragma unroll
+        for (int sl = 0; sl < 2 * DIM; ++sl) {
+            const int v = oth[nb[sl]];
+            cn += (v == snew);
+            co += (v == sold);
+        }
+        const int e = cn - co + 2 * DIM;
+        const uint64_t T = ((uint64_t)s_thr[2 * e] << 16) | s_thr[2 * e + 1];
+        if ((uint64_t)w.y < T) own[h] = (int8_t)snew;
+    }
+}
+
+// Persistent EMCS: nsteps x (ncluster Wolff clusters, then nmetro sweeps) of one
+// replica per workgroup, with no synchronisation between replicas, so a replica
+// whose clusters are small goes on with its next steps while another still grows
+// a large one.  Step k has the counters of mq_emcs(step0 + k): clusters
+// (step0 + k) * ncluster + c, sweeps (step0 + k) * nmetro + t; the chain equals
+// nsteps calls of mq_emcs bit for bit.  Replicas change only their own spins,
+// and a workgroup's own writes are visible to its own later reads after its
+// barriers.
+template <int DIM, int KIND>
+__global__ void __launch_bounds__(1024) k_emcs_regular(char* __restrict__ base, int64_t stride, int L, int nsteps,
+                                                      int ncluster, int nmetro, uint64_t step0, int rep0, int nrep_all,
+                                                      int Q, const uint32_t* __restrict__ wthr,
+                                                      const uint32_t* __restrict__ thr, int tbl,
+                                                      int32_t* __restrict__ queue, uint32_t* __restrict__ visit,
+                                                      const int64_t* __restrict__ rep_site_off,
+                                                      int64_t* __restrict__ csize, int vlim, int lgL16, Key2 key)
+{
+    __shared__ uint32_t s_thr[2 * 16];
+    __shared__ uint32_t s_thi[13];
+    __shared__ uint32_t s_tlo[13];
+    const int r = (int)blockIdx.x;
+    const uint32_t grep = (uint32_t)(rep0 + r);
+    if ((int)threadIdx.x < 2 * tbl) s_thr[threadIdx.x] = thr[(int64_t)grep * 2 * tbl + threadIdx.x];
+    if (lgL16 && threadIdx.x < 13) {
+        const uint32_t hi = thr[grep * 26u + 2u * threadIdx.x];
+        const uint32_t lo = thr[grep * 26u + 2u * threadIdx.x + 1u];
+        s_thi[threadIdx.x] = hi > 0xFFFFu ? 0xFFFFu : hi;
+        s_tlo[threadIdx.x] = hi > 0xFFFFu ? 0x10000u : lo;
+    }
+    const RegGeom G = reg_geom(DIM, L);
+    int8_t* sp = reinterpret_cast<int8_t*>(base + r * stride);
+    __syncthreads();
+    for (int k = 0; k < nsteps; ++k) {
+        const uint64_t step = step0 + (uint64_t)k;
+        if (ncluster > 0) {
+            wolff_regular_clusters<DIM, KIND>(r, base, stride, L, ncluster, (uint32_t)(step * (uint64_t)ncluster), rep0,
+                                              Q, wthr, nullptr, 0.f, queue, visit, rep_site_off,
+                                              csize ? csize + (int64_t)k * nrep_all * ncluster : nullptr, vlim, key);
+            __syncthreads();
+        }
+        for (int t = 0; t < nmetro; ++t) {
+            const uint64_t sw = step * (uint64_t)nmetro + (uint64_t)t;
+            for (int colour = 0; colour < 2; ++colour) {
+                const uint32_t phase = (uint32_t)(2 * sw + (uint64_t)colour);
+                if (KIND == WK_ISING && DIM == 3 && lgL16) {
+                    // 3D Ising, L a power of two >= 32: the headline sweep's 16-site chunks
+                    const uint32_t nch = (uint32_t)(G.half >> 4);
+                    for (uint32_t ch = threadIdx.x; ch < nch; ch += blockDim.x)
+                        row16_chunk(sp, ch, lgL16, colour, phase, grep, s_thi, s_tlo, key);
+                } else if (KIND == WK_ISING) {
+                    wg_sweep_ising<DIM>(sp, G, colour, phase, grep, s_thr, key);
+                } else {
+                    wg_sweep_potts<DIM>(sp, G, Q, colour, phase, grep, s_thr, key);
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
 
 // One wavefront per replica for small int8 lattices (N <= 16384): the
 // replica's spins and a 16-bit frontier ring (spilling to the global queue)
@@ -795,6 +937,31 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
 }
 
 }  // namespace
+
+hipError_t launch_emcs_regular(mq_state* st, const Group& g, int nsteps, int ncluster, int nmetro, uint64_t step0,
+                               Key2 key)
+{
+    const int L = g.lat->L, dim = g.lat->dim;
+    const dim3 grid(g.nrep), block(wolff_block(st, g.nrep));
+    hipStream_t s = st->ctx->stream;
+    char* base = st->d_spins + g.off;
+    int lgL16 = 0;  // the row-vectorised sweep applies: 3D Ising, L a power of two in [32, 1024]
+    if (st->p.model == MQ_ISING && dim == 3 && (L & (L - 1)) == 0 && L >= 32 && L <= 1024 && st->tbl == 13)
+        while ((1 << lgL16) < L) ++lgL16;
+#define MQ_E(D, K)                                                                                                 \
+    hipLaunchKernelGGL((k_emcs_regular<D, K>), grid, block, 0, s, base, g.stride, L, nsteps, ncluster, nmetro,       \
+                       step0, g.rep0, st->nrep, st->p.q, st->d_wthr, st->d_thr, st->tbl, st->d_queue, st->d_visit,  \
+                       st->d_rep_site_off, st->d_csize, st->wolff_vset, lgL16, key)
+    if (st->p.model == MQ_ISING) {
+        if (dim == 3) MQ_E(3, WK_ISING);
+        else MQ_E(2, WK_ISING);
+    } else {
+        if (dim == 3) MQ_E(3, WK_POTTS);
+        else MQ_E(2, WK_POTTS);
+    }
+#undef MQ_E
+    return hipGetLastError();
+}
 
 hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key)
 {

@@ -89,6 +89,173 @@ __device__ __forceinline__ uint32_t fdiv_u24(uint32_t n, uint32_t d, float inv, 
 __device__ __forceinline__ float u01f(uint32_t w) { return (float)(w >> 8) * 0x1.0p-24f; }
 
 // ---------------------------------------------------------------------------
+// Regular-lattice geometry in the colour-split layout: colour c of site i is
+// (x0+x1+x2)&1, half index h = i >> 1; rows of H = L/2 half-sites.
+struct RegGeom {
+    int L, H;
+    int64_t plane, half;
+};
+
+__device__ __forceinline__ RegGeom reg_geom(int dim, int L)
+{
+    RegGeom g;
+    g.L = L;
+    g.H = L >> 1;
+    g.plane = (int64_t)g.H * L;
+    g.half = (dim == 3) ? g.plane * L : g.plane;
+    return g;
+}
+
+// neighbour half-indices (other colour) of colour-c site h, slot order
+// +x0,-x0,+x1,-x1,+x2,-x2 (regular_hypercubic.h:31-66)
+template <int DIM>
+__device__ __forceinline__ void reg_nbrs(const RegGeom& G, int colour, int64_t h, int64_t nb[2 * DIM])
+{
+    int64_t row;
+    int k, x1, x2 = 0;
+    if (G.half < (1 << 24)) {  // 32-bit indices, reciprocal division
+        uint32_t kk, xx1;
+        const uint32_t r32 = fdiv_u24((uint32_t)h, (uint32_t)G.H, 1.0f / (float)G.H, kk);
+        if (DIM == 3) x2 = (int)fdiv_u24(r32, (uint32_t)G.L, 1.0f / (float)G.L, xx1);
+        else xx1 = r32;
+        row = r32;
+        k = (int)kk;
+        x1 = (int)xx1;
+    } else {
+        row = h / G.H;
+        k = (int)(h - row * G.H);
+        x1 = (int)(row % G.L);
+        if (DIM == 3) x2 = (int)(row / G.L);
+    }
+    const int p = (colour + x1 + x2) & 1;
+    const int64_t rb = h - k;
+    nb[0] = rb + (p ? ((k + 1 == G.H) ? 0 : k + 1) : k);
+    nb[1] = rb + (p ? k : ((k == 0) ? G.H - 1 : k - 1));
+    nb[2] = (x1 == G.L - 1) ? h - (int64_t)(G.L - 1) * G.H : h + G.H;
+    nb[3] = (x1 == 0) ? h + (int64_t)(G.L - 1) * G.H : h - G.H;
+    if (DIM == 3) {
+        nb[4] = (x2 == G.L - 1) ? h - (int64_t)(G.L - 1) * G.plane : h + G.plane;
+        nb[5] = (x2 == 0) ? h + (int64_t)(G.L - 1) * G.plane : h - G.plane;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One chunk of the row-vectorised 3D Ising sweep (L a power of two, 32 <= L <= 1024,
+// no couplings): 16 consecutive half-sites of one row of colour `colour` of the
+// replica at rbase.  A chunk's own 16 bytes, five 16-byte chunks of the other
+// colour (same row, y+-1, z+-1) and one dword of the adjacent chunk for the
+// x-shifted neighbour; spins are +-1 bytes (bit 1 set iff -1), so the doubled
+// count of down neighbours is a byte-wise sum of (word & 0x02020202) and the
+// table index e = q + 6 is own_down ? 2n : 12 - 2n per byte.  Random numbers
+// and acceptance are those of the generic Ising sweep (group g = h >> 3):
+// 16-bit uniforms against the threshold's high half, the tie stream on equality.
+// s_thi / s_tlo: the replica's 13 threshold halves ("always" as 0xFFFF / 0x10000).
+__device__ __forceinline__ void row16_chunk(int8_t* __restrict__ rbase, uint32_t chunk, int lgL, int colour,
+                                            uint32_t phase, uint32_t grep, const uint32_t* s_thi,
+                                            const uint32_t* s_tlo, Key2 key)
+{
+    const uint32_t L = 1u << lgL;
+    const uint32_t H = L >> 1;
+    const uint32_t lgcpr = (uint32_t)lgL - 5u;        // chunks of 16 per row: H/16
+    const uint32_t lgplane = 2u * (uint32_t)lgL - 1u;  // H*L
+    const uint32_t plane = 1u << lgplane;
+    const uint32_t half = plane << lgL;
+    const uint32_t row = chunk >> lgcpr;
+    const uint32_t c = chunk & ((1u << lgcpr) - 1u);
+    const uint32_t y = row & (L - 1u);
+    const uint32_t z = row >> lgL;
+    const uint32_t p = (colour + y + z) & 1u;
+    const uint32_t rb = row * H;
+    const uint32_t h0 = rb + (c << 4);
+    int8_t* own = rbase + (int64_t)colour * half;
+    const int8_t* oth = rbase + (int64_t)(1 - colour) * half;
+
+    const uint4 S = *reinterpret_cast<const uint4*>(own + h0);
+    const uint4 O = *reinterpret_cast<const uint4*>(oth + h0);
+    const uint32_t cmax = (1u << lgcpr) - 1u;
+    const uint32_t xo = p ? rb + ((c == cmax) ? 0u : ((c + 1u) << 4)) : rb + ((c == 0u) ? H : (c << 4)) - 4u;
+    const uint32_t X = *reinterpret_cast<const uint32_t*>(oth + xo);
+    const uint32_t hym = (y == 0u) ? h0 + (L - 1u) * H : h0 - H;
+    const uint32_t hyp = (y == L - 1u) ? h0 - (L - 1u) * H : h0 + H;
+    const uint32_t hzm = (z == 0u) ? h0 + (L - 1u) * plane : h0 - plane;
+    const uint32_t hzp = (z == L - 1u) ? h0 - (L - 1u) * plane : h0 + plane;
+    const uint4 Ym = *reinterpret_cast<const uint4*>(oth + hym);
+    const uint4 Yp = *reinterpret_cast<const uint4*>(oth + hyp);
+    const uint4 Zm = *reinterpret_cast<const uint4*>(oth + hzm);
+    const uint4 Zp = *reinterpret_cast<const uint4*>(oth + hzp);
+
+    // x-shifted neighbour words
+    uint32_t xs0, xs1, xs2, xs3;
+    if (p) {
+        xs0 = __builtin_amdgcn_alignbyte(O.y, O.x, 1);
+        xs1 = __builtin_amdgcn_alignbyte(O.z, O.y, 1);
+        xs2 = __builtin_amdgcn_alignbyte(O.w, O.z, 1);
+        xs3 = __builtin_amdgcn_alignbyte(X, O.w, 1);
+    } else {
+        xs0 = __builtin_amdgcn_alignbyte(O.x, X, 3);
+        xs1 = __builtin_amdgcn_alignbyte(O.y, O.x, 3);
+        xs2 = __builtin_amdgcn_alignbyte(O.z, O.y, 3);
+        xs3 = __builtin_amdgcn_alignbyte(O.w, O.z, 3);
+    }
+    const uint32_t M2 = 0x02020202u;
+    // per byte: 4*e with e = q + 6 (LDS byte address of the threshold entry)
+    auto eword = [&](uint32_t s, uint32_t o, uint32_t xs, uint32_t ym, uint32_t yp, uint32_t zm, uint32_t zp) {
+        const uint32_t n2 = (o & M2) + (xs & M2) + (ym & M2) + (yp & M2) + (zm & M2) + (zp & M2);
+        const uint32_t mask = ((s & M2) >> 1) * 0xFFu;
+        return ((n2 & mask) | ((0x0C0C0C0Cu - n2) & ~mask)) << 2;
+    };
+    uint32_t E[4];
+    E[0] = eword(S.x, O.x, xs0, Ym.x, Yp.x, Zm.x, Zp.x);
+    E[1] = eword(S.y, O.y, xs1, Ym.y, Yp.y, Zm.y, Zp.y);
+    E[2] = eword(S.z, O.z, xs2, Ym.z, Yp.z, Zm.z, Zp.z);
+    E[3] = eword(S.w, O.w, xs3, Ym.w, Yp.w, Zm.w, Zp.w);
+
+    const uint32_t g0 = h0 >> 3;
+    const char* thi_b = reinterpret_cast<const char*>(s_thi);
+    uint32_t acc = 0u;  // bit 15-site: site accepted
+    bool any_tie = false;
+#pragma unroll
+    for (int half8 = 0; half8 < 2; ++half8) {
+        const uint4 w = philox4x32_10(g0 + half8, phase, grep, ST_METRO, key);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int site = half8 * 8 + m;
+            const uint32_t ea = (E[site >> 2] >> (8 * (site & 3))) & 0xFFu;
+            const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+            const uint32_t t = *reinterpret_cast<const uint32_t*>(thi_b + ea);
+            acc = acc + acc + (u < t ? 1u : 0u);
+            any_tie |= (u == t);
+        }
+    }
+    if (__builtin_expect(any_tie, 0)) {
+        // an equal high half: redo this thread's 16 decisions with the low halves
+        acc = 0u;
+#pragma unroll
+        for (int half8 = 0; half8 < 2; ++half8) {
+            const uint4 w = philox4x32_10(g0 + half8, phase, grep, ST_METRO, key);
+            const uint4 tw = philox4x32_10(g0 + half8, phase, grep, ST_TIE, key);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int site = half8 * 8 + m;
+                const uint32_t ea = (E[site >> 2] >> (8 * (site & 3))) & 0xFFu;
+                const uint32_t u = (u4_get(w, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+                const uint32_t lo = (u4_get(tw, m >> 1) >> (16 * (m & 1))) & 0xFFFFu;
+                const uint32_t t = *reinterpret_cast<const uint32_t*>(thi_b + ea);
+                const uint32_t tl = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(s_tlo) + ea);
+                const bool a = (u < t) || (u == t && lo < tl);
+                acc = acc + acc + (a ? 1u : 0u);
+            }
+        }
+    }
+    // acc holds site 0 in bit 15 ... site 15 in bit 0: expand 4 bits -> 4 bytes
+    const uint32_t rev = __builtin_bitreverse32(acc) >> 16;  // site k in bit k
+    auto expand = [](uint32_t nib) { return ((nib * 0x00204081u) & 0x01010101u) * 0xFEu; };
+    const uint4 Snew = make_uint4(S.x ^ expand(rev & 15u), S.y ^ expand((rev >> 4) & 15u),
+                                  S.z ^ expand((rev >> 8) & 15u), S.w ^ expand((rev >> 12) & 15u));
+    *reinterpret_cast<uint4*>(own + h0) = Snew;
+}
+
+// ---------------------------------------------------------------------------
 // Bit-sliced counters: plane p of a count holds bit p of 32 per-bit counts.
 // Bit-sliced add of Y into X over NP planes (values that overflow NP planes are
 // not produced by the callers).
@@ -373,6 +540,8 @@ hipError_t launch_sweep_regular(mq_state* st, const Group& g, int colour, uint32
 hipError_t launch_measure_regular(mq_state* st, const Group& g, long long* isum, double* fsum);
 hipError_t launch_measure_ft_regular(mq_state* st, const Group& g);
 hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uint32_t cl0, Key2 key);
+hipError_t launch_emcs_regular(mq_state* st, const Group& g, int nsteps, int ncluster, int nmetro, uint64_t step0,
+                               Key2 key);
 hipError_t launch_bonds_regular(mq_state* st, const Group& g, int sample, double p_plus, Key2 key);
 hipError_t launch_sweeps_lds(mq_state* st, int nsweeps, uint32_t sweep0, Key2 key);
 hipError_t prepare_lds(mq_state* st);

@@ -199,6 +199,14 @@ class State:
         self.ctx.check(self.ctx.lib.mq_emcs(self.h, ncluster, nmetro, seed, step, _ptr(mc)), "mq_emcs")
         return mc
 
+    def emcs_run(self, nsteps, ncluster, nmetro, seed, step0, means=True):
+        """nsteps EMCS per replica in one persistent launch (mq_emcs_run); returns the
+        (nsteps, nrep) mean cluster sizes or None."""
+        mc = np.zeros((nsteps, self.nrep), np.float64) if means else None
+        self.ctx.check(self.ctx.lib.mq_emcs_run(self.h, nsteps, ncluster, nmetro, seed, step0,
+                                                None if mc is None else _ptr(mc)), "mq_emcs_run")
+        return mc
+
     def measure(self):
         out = np.zeros((self.nrep, L.MQ_NOBS), np.float64)
         self.ctx.check(self.ctx.lib.mq_measure(self.h, _ptr(out)), "mq_measure")
