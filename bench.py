@@ -10,6 +10,8 @@ every replica; value = spin-flip attempts/ns over the timed steps.
 Beside it on the same JSON line (skip with --quick):
   plain        the same sweep, one replica per launch
   wolff        Wolff clusters on the thermalised 32-replica state vs one replica
+  emcs         20 EMCS (10 clusters + 1 sweep) of that state: separate launches for the
+               batch vs one persistent launch with each replica at its own pace
   c1_2d_L64    2D Ising L=64 EMCS schedule (10 Wolff + 1 sweep per EMCS), batched vs plain
   c3_heis_L128 Heisenberg FP32 sweeps + Wolff, batched vs plain
   c4_ea_batch  512 +-J samples x 32 temperatures, L in {8,12,16,24}, HIP-graph
@@ -126,6 +128,34 @@ def wolff_rate(ctx, st, rounds, nsw, ncl, seed, ncalls, sweep0, cluster0=0, tune
         sites += int(sizes.sum())
         n += sizes.size
     return sites / (ms_tot * 1e6), sites / n, {"tuned": shape is not None, **used}
+
+
+def emcs_compare(ctx, st, nsteps, ncluster, nmetro, seed, step0):
+    """nsteps EMCS (ncluster Wolff clusters + nmetro sweeps) of every replica, timed twice
+    from the same state: as separate mq_wolff / mq_metropolis launches for the whole batch,
+    and as one persistent mq_emcs_run launch in which each replica runs at its own pace.
+    Both give the same states (replicas 0 and R-1 compared)."""
+    saved = [st.download(r) for r in range(st.nrep)]
+
+    def separate():
+        for k in range(nsteps):
+            st.wolff(ncluster, seed, cluster0=(step0 + k) * ncluster, sizes=False)
+            st.metropolis(nmetro, seed, sweep0=(step0 + k) * nmetro)
+
+    ms_sep, _ = timed(ctx, separate)
+    ends = [st.download(r) for r in (0, st.nrep - 1)]
+    for r, s in enumerate(saved):
+        st.upload(r, s)
+    box = {}
+    ms_per, _ = timed(ctx, lambda: box.setdefault("mc", st.emcs_run(nsteps, ncluster, nmetro, seed, step0)))
+    same = all(np.array_equal(a, st.download(r)) for a, r in zip(ends, (0, st.nrep - 1)))
+    sites = float(box["mc"].sum()) * ncluster
+    return {"emcs": f"{nsteps} x ({ncluster} Wolff clusters + {nmetro} sweep) per replica", "replicas": st.nrep,
+            "separate_emcs_per_s": round(nsteps * 1e3 / ms_sep, 2),
+            "persistent_emcs_per_s": round(nsteps * 1e3 / ms_per, 2),
+            "persistent_over_separate": round(ms_sep / ms_per, 2),
+            "persistent_wolff_sites_per_ns_incl_sweeps": round(sites / (ms_per * 1e6), 4),
+            "mean_cluster": round(float(box["mc"].mean()), 1), "states_equal": bool(same)}
 
 
 def leg_c2_msc(gpu, ctx, L=256, R=32, steps=100):
@@ -454,11 +484,13 @@ def main():
     bytes_per_launch = 3.0 * R * (N // 2)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
     obs = st.measure()
-    wolff_b = None
+    wolff_b = emcs_b = None
     if not a.quick and not a.legs_only:
         log("headline state: Wolff clusters")
         wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps, tune_ncl=4)
         CPU_WOLFF_INPUTS["3d_L256"] = {"model": "ising", "beta": beta, "dim": 3, "L": L, "spins": st.download(0)}
+        log("headline state: EMCS, separate launches and persistent")
+        emcs_b = emcs_compare(ctx, st, 20, 10, 1, seed=4, step0=1 << 20)
     del st
 
     line = {
@@ -508,6 +540,7 @@ def main():
                          "roofline_frac": wolff_frac(wolff_b[0], "3d_int8"), "bytes_per_site": WOLFF_BYTES["3d_int8"],
                          "launch_shape": wolff_b[2],
                          "schedule": "after 20 x (4 sweeps + 10 clusters); 8 timed calls of 10 clusters per replica"}
+        line["emcs"] = emcs_b
         del sp
     if not a.quick:
         for name, fn in (("c2_msc", leg_c2_msc), ("c1_2d_L64", leg_c1), ("c3_heis_L128", leg_c3),

@@ -170,8 +170,9 @@ MQ_API int mq_emcs(mq_state* st, int ncluster, int nmetro, uint64_t seed, uint64
  * measurements.  Equal, bit for bit, to nsteps calls of
  * mq_emcs(ncluster, nmetro, seed, step0 + k), but each replica runs its steps
  * without waiting for the others, so a batch is not held to its slowest
- * replica at every step.  Regular lattices, int8 Ising or Potts without
- * couplings.  mean_cluster (host, may be NULL): [nsteps][nrep], NaN when
+ * replica at every step.  Regular lattices without couplings: Ising and Potts
+ * (int8 storage), Heisenberg and Phi4.  No renormalisation happens inside a
+ * run.  mean_cluster (host, may be NULL): [nsteps][nrep], NaN when
  * ncluster = 0. */
 MQ_API int mq_emcs_run(mq_state* st, int nsteps, int ncluster, int nmetro, uint64_t seed, uint64_t step0,
                        double* mean_cluster);

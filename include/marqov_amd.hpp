@@ -10,6 +10,7 @@
 // their parameters and sign convention (J = -1 is ferromagnetic).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <memory>
@@ -260,6 +261,25 @@ public:
     {
         b_.check(mq_emcs(b_.get(), ncluster, nmetro, seed_, step_++, nullptr), "mq_emcs");
         after_step();
+    }
+    // nsteps such steps: persistent launches (mq_emcs_run, each replica at its own
+    // pace) where they apply, split at the renormalisation cadence; else step by step
+    void advance(int ncluster, int nmetro, int nsteps)
+    {
+        while (nsteps > 0) {
+            int n = nsteps;
+            const bool renorm = renorm_every_ > 0 && b_.model() == MQ_HEISENBERG;
+            if (renorm) n = std::min<uint64_t>((uint64_t)n, (uint64_t)renorm_every_ - step_ % (uint64_t)renorm_every_);
+            const int rc = mq_emcs_run(b_.get(), n, ncluster, nmetro, seed_, step_, nullptr);
+            if (rc == MQ_ERR_UNSUPPORTED) {
+                for (int k = 0; k < nsteps; ++k) advance(ncluster, nmetro);
+                return;
+            }
+            b_.check(rc, "mq_emcs_run");
+            step_ += (uint64_t)n;
+            nsteps -= n;
+            if (renorm && step_ % (uint64_t)renorm_every_ == 0) b_.renormalize();
+        }
     }
     uint64_t steps() const { return step_; }
 

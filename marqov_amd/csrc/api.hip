@@ -941,8 +941,8 @@ extern "C" int mq_emcs_run(mq_state* st, int nsteps, int ncluster, int nmetro, u
     if (nsteps < 0 || ncluster < 0 || nmetro < 0)
         return mq_fail(st->ctx, MQ_ERR_ARG, "nsteps, ncluster and nmetro must be >= 0");
     if (nsteps == 0) return MQ_OK;
-    if (st->p.model != MQ_ISING && st->p.model != MQ_POTTS)
-        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "mq_emcs_run: Ising and Potts models");
+    if (st->p.model == MQ_EA_ISING)
+        return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "Wolff is not defined for the +-J spin glass (frustrated bonds)");
     if (st->msc || st->eabits || st->csrbits)
         return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "mq_emcs_run: int8 storage (spin_bits = 8)");
     if (st->d_bonds) return mq_fail(st->ctx, MQ_ERR_UNSUPPORTED, "mq_emcs_run: no couplings");

@@ -51,16 +51,6 @@ __global__ void __launch_bounds__(256) k_init_int8(int8_t* __restrict__ base, in
     }
 }
 
-// fresh uniform direction in the reference's component order (randomdir.h:45-65)
-__device__ __forceinline__ float3 rnddir_f(uint32_t w0, uint32_t w1)
-{
-    const float a0 = 2.0f * u01f(w0);
-    const float a1 = 2.0f * u01f(w1);
-    const float rho = sqrtf(fmaxf(a0 * (2.0f - a0), 0.0f));
-    float sn, cs;
-    sincospif(a1, &sn, &cs);
-    return make_float3(1.0f - a0, rho * sn, rho * cs);
-}
 
 template <int DIM>
 __global__ void __launch_bounds__(256) k_init_f4(float4* __restrict__ base, int64_t stride4, int L, int rep0, Key2 key)
@@ -376,46 +366,9 @@ __global__ void __launch_bounds__(512) k_sweep_o3(float4* __restrict__ base, int
     if (h >= G.half) return;
     float4* own = base + r * stride4 + colour * G.half;
     const float4* oth = base + r * stride4 + (1 - colour) * G.half;
-    int64_t nb[2 * DIM];
-    reg_nbrs<DIM>(G, colour, h, nb);
-    float nx = 0.f, ny = 0.f, nz = 0.f;
-#pragma unroll
-    for (int sl = 0; sl < 2 * DIM; ++sl) {
-        const float4 v = oth[nb[sl]];
-        float b = 1.0f;
-        if (BONDS) {
-            const int8_t* bs = bonds + rep_bond_off[grep];
-            const int d = sl >> 1;
-            b = (sl & 1) ? (float)bs[(int64_t)(1 - colour) * DIM * G.half + (int64_t)d * G.half + nb[sl]]
-                         : (float)bs[(int64_t)colour * DIM * G.half + (int64_t)d * G.half + h];
-        }
-        nx += b * v.x;
-        ny += b * v.y;
-        nz += b * v.z;
-    }
-    const float4 so = own[h];
-    const uint4 w = philox4x32_10((uint32_t)h, phase, (uint32_t)grep, ST_METRO, key);
-    const float3 rd = rnddir_f(w.x, w.y);
-    float3 sn;
-    if (model == MQ_PHI4) {
-        sn = make_float3(so.x + 0.5f * rd.x, so.y + 0.5f * rd.y, so.z + 0.5f * rd.z);
-    } else if (proposal == MQ_PROPOSE_SMALL_ANGLE) {
-        float3 t = make_float3(so.x + delta * rd.x, so.y + delta * rd.y, so.z + delta * rd.z);
-        const float inv = rsqrtf(t.x * t.x + t.y * t.y + t.z * t.z);
-        sn = make_float3(t.x * inv, t.y * inv, t.z * inv);
-    } else {
-        sn = rd;
-    }
-    const float beta = betaf[grep];
-    float dE = J * ((sn.x - so.x) * nx + (sn.y - so.y) * ny + (sn.z - so.z) * nz);
-    if (model == MQ_PHI4) {
-        const float n2 = sn.x * sn.x + sn.y * sn.y + sn.z * sn.z;
-        const float o2 = so.x * so.x + so.y * so.y + so.z * so.z;
-        dE += (mass / beta) * (n2 - o2) + (lambda / beta) * ((n2 - 1.0f) * (n2 - 1.0f) - (o2 - 1.0f) * (o2 - 1.0f));
-    }
-    bool acc = dE <= 0.0f;
-    if (!acc) acc = u01f(w.z) < __expf(-beta * dE);
-    if (acc) own[h] = make_float4(sn.x, sn.y, sn.z, 0.0f);
+    const int8_t* bs = BONDS ? bonds + rep_bond_off[grep] : nullptr;
+    o3_site_update<DIM, BONDS>(own, oth, bs, G, colour, h, phase, (uint32_t)grep, betaf[grep], model, proposal, J, delta,
+                               lambda, mass, key);
 }
 
 // --------------------------------------------------------------- measure

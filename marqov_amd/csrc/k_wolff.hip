@@ -680,7 +680,7 @@ __device__ __forceinline__ void wg_sweep_potts(int8_t* sp, const RegGeom& G, int
 }
 
 // Persistent EMCS: nsteps x (ncluster Wolff clusters, then nmetro sweeps) of one
-// replica per workgroup, with no synchronisation between replicas, so a replica
+// replica per workgroup (int8 Ising / Potts, FP32 Heisenberg / Phi4), with no synchronisation between replicas, so a replica
 // whose clusters are small goes on with its next steps while another still grows
 // a large one.  Step k has the counters of mq_emcs(step0 + k): clusters
 // (step0 + k) * ncluster + c, sweeps (step0 + k) * nmetro + t; the chain equals
@@ -692,6 +692,8 @@ __global__ void __launch_bounds__(1024) k_emcs_regular(char* __restrict__ base, 
                                                       int ncluster, int nmetro, uint64_t step0, int rep0, int nrep_all,
                                                       int Q, const uint32_t* __restrict__ wthr,
                                                       const uint32_t* __restrict__ thr, int tbl,
+                                                      const float* __restrict__ betaf, float J, int model, int proposal,
+                                                      float delta, float lambda, float mass,
                                                       int32_t* __restrict__ queue, uint32_t* __restrict__ visit,
                                                       const int64_t* __restrict__ rep_site_off,
                                                       int64_t* __restrict__ csize, int vlim, int lgL16, Key2 key)
@@ -715,7 +717,7 @@ __global__ void __launch_bounds__(1024) k_emcs_regular(char* __restrict__ base, 
         const uint64_t step = step0 + (uint64_t)k;
         if (ncluster > 0) {
             wolff_regular_clusters<DIM, KIND>(r, base, stride, L, ncluster, (uint32_t)(step * (uint64_t)ncluster), rep0,
-                                              Q, wthr, nullptr, 0.f, queue, visit, rep_site_off,
+                                              Q, wthr, betaf, J, queue, visit, rep_site_off,
                                               csize ? csize + (int64_t)k * nrep_all * ncluster : nullptr, vlim, key);
             __syncthreads();
         }
@@ -723,7 +725,15 @@ __global__ void __launch_bounds__(1024) k_emcs_regular(char* __restrict__ base, 
             const uint64_t sw = step * (uint64_t)nmetro + (uint64_t)t;
             for (int colour = 0; colour < 2; ++colour) {
                 const uint32_t phase = (uint32_t)(2 * sw + (uint64_t)colour);
-                if (KIND == WK_ISING && DIM == 3 && lgL16) {
+                if (KIND == WK_CART || KIND == WK_RDIR) {
+                    float4* sp4 = reinterpret_cast<float4*>(base + r * stride);
+                    float4* own = sp4 + colour * G.half;
+                    const float4* oth = sp4 + (1 - colour) * G.half;
+                    const float beta = betaf[grep];
+                    for (int64_t h = threadIdx.x; h < G.half; h += blockDim.x)
+                        o3_site_update<DIM, false>(own, oth, nullptr, G, colour, h, phase, grep, beta, model, proposal,
+                                                   J, delta, lambda, mass, key);
+                } else if (KIND == WK_ISING && DIM == 3 && lgL16) {
                     // 3D Ising, L a power of two >= 32: the headline sweep's 16-site chunks
                     const uint32_t nch = (uint32_t)(G.half >> 4);
                     for (uint32_t ch = threadIdx.x; ch < nch; ch += blockDim.x)
@@ -950,14 +960,26 @@ hipError_t launch_emcs_regular(mq_state* st, const Group& g, int nsteps, int ncl
         while ((1 << lgL16) < L) ++lgL16;
 #define MQ_E(D, K)                                                                                                 \
     hipLaunchKernelGGL((k_emcs_regular<D, K>), grid, block, 0, s, base, g.stride, L, nsteps, ncluster, nmetro,       \
-                       step0, g.rep0, st->nrep, st->p.q, st->d_wthr, st->d_thr, st->tbl, st->d_queue, st->d_visit,  \
-                       st->d_rep_site_off, st->d_csize, st->wolff_vset, lgL16, key)
-    if (st->p.model == MQ_ISING) {
-        if (dim == 3) MQ_E(3, WK_ISING);
-        else MQ_E(2, WK_ISING);
+                       step0, g.rep0, st->nrep, st->p.q, st->d_wthr, st->d_thr, st->tbl, st->d_betaf, J, model,     \
+                       st->p.proposal, (float)st->p.delta, (float)st->p.lambda, (float)st->p.mass, st->d_queue,     \
+                       st->d_visit, st->d_rep_site_off, st->d_csize, st->wolff_vset, lgL16, key)
+    const int model = st->p.model;
+    const float J = (float)st->p.J;
+    int kind;
+    if (model == MQ_ISING) kind = WK_ISING;
+    else if (model == MQ_POTTS) kind = WK_POTTS;
+    else if (model == MQ_HEISENBERG && st->p.embedding == MQ_EMBED_CARTESIAN) kind = WK_CART;
+    else kind = WK_RDIR;
+    if (dim == 3) {
+        if (kind == WK_ISING) MQ_E(3, WK_ISING);
+        else if (kind == WK_POTTS) MQ_E(3, WK_POTTS);
+        else if (kind == WK_CART) MQ_E(3, WK_CART);
+        else MQ_E(3, WK_RDIR);
     } else {
-        if (dim == 3) MQ_E(3, WK_POTTS);
-        else MQ_E(2, WK_POTTS);
+        if (kind == WK_ISING) MQ_E(2, WK_ISING);
+        else if (kind == WK_POTTS) MQ_E(2, WK_POTTS);
+        else if (kind == WK_CART) MQ_E(2, WK_CART);
+        else MQ_E(2, WK_RDIR);
     }
 #undef MQ_E
     return hipGetLastError();

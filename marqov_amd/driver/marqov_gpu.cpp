@@ -573,7 +573,7 @@ int main(int argc, char** argv)
                 if (!resume) {  // the warm-up runs only for a fresh start (wrmploop, core.h:1034-1050)
                     std::printf("|");
                     for (int k = 0; k < 10; ++k) {
-                        for (int i = 0; i < pl.nwarm / 10; ++i) emcs.advance(pl.ncluster, pl.nmetro);
+                        emcs.advance(pl.ncluster, pl.nmetro, pl.nwarm / 10);
                         std::printf(".");
                         std::fflush(stdout);
                     }

@@ -6,20 +6,27 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-BETA = {("ising", 2): 0.4406868, ("ising", 3): 0.2216546, ("potts", 2): 1.0050525, ("potts", 3): 0.55}
+BETA = {("ising", 2): 0.4406868, ("ising", 3): 0.2216546, ("potts", 2): 1.0050525, ("potts", 3): 0.55,
+        ("heisenberg", 3): 0.693, ("heisenberg", 2): 1.0, ("phi4", 3): 0.69, ("cart", 3): 0.693}
 
 
 @pytest.mark.parametrize("model,dim,L,nrep,nc,nm", [
     ("ising", 2, 64, 8, 10, 1), ("ising", 3, 32, 8, 5, 2), ("ising", 3, 64, 4, 10, 1),
     ("potts", 2, 32, 8, 4, 1), ("potts", 3, 16, 8, 3, 2),
-    ("ising", 2, 48, 4, 0, 3), ("ising", 3, 16, 4, 6, 0), ("ising", 3, 256, 2, 2, 1)])
+    ("ising", 2, 48, 4, 0, 3), ("ising", 3, 16, 4, 6, 0), ("ising", 3, 256, 2, 2, 1),
+    ("heisenberg", 3, 16, 8, 5, 1), ("cart", 3, 24, 4, 3, 2), ("heisenberg", 2, 32, 4, 4, 1), ("phi4", 3, 12, 8, 4, 1)])
 def test_emcs_run_equals_emcs_calls(ctx, model, dim, L, nrep, nc, nm):
     from marqov_amd import gpu
     lat = gpu.RegularLattice(ctx, dim, L)
     betas = BETA[(model, dim)] * np.linspace(0.9, 1.1, nrep)
+    kw = dict(J=-1.0, q=3)
+    if model == "cart":  # Heisenberg, small-angle proposal, Cartesian embedding
+        model, kw = "heisenberg", dict(J=-1.0, proposal="small", delta=0.8, embedding="cartesian")
+    elif model in ("heisenberg", "phi4"):
+        kw = dict(J=-1.0, embedding="random")
 
     def fresh():
-        st = gpu.State(ctx, model, [lat] * nrep, J=-1.0, q=3)
+        st = gpu.State(ctx, model, [lat] * nrep, **kw)
         st.set_beta(betas)
         st.init(seed=6)
         return st
@@ -57,10 +64,11 @@ def test_emcs_run_refusals(ctx):
     from marqov_amd import gpu, graphs
     from marqov_amd.gpu import MarqovError
     lat = gpu.RegularLattice(ctx, 2, 16)
-    heis = gpu.State(ctx, "heisenberg", [lat] * 2, J=-1.0)
-    heis.set_beta(0.5)
-    with pytest.raises(MarqovError, match="Ising and Potts"):
-        heis.emcs_run(1, 1, 1, seed=1, step0=0)
+    ea = gpu.State(ctx, "ea", [lat] * 2, J=-1.0)
+    ea.set_beta(0.5)
+    ea.random_bonds()
+    with pytest.raises(MarqovError, match="spin glass"):
+        ea.emcs_run(1, 1, 1, seed=1, step0=0)
     g = gpu.CsrLattice(ctx, *graphs.regular_csr(2, 8))
     st = gpu.State(ctx, "ising", [g] * 2, J=-1.0)
     st.set_beta(0.4)
