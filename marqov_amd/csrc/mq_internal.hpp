@@ -634,15 +634,17 @@ hipError_t launch_measure_all(mq_state* st, long long* isum, double* fsum);
 
 // Threads of the workgroup that grows one replica's clusters in global memory.
 // A wide frontier level is worked through blockDim entries at a time, each
-// waiting on its claims' round trips, so with few replicas the widest block
-// keeps the most claims in flight.  From half as many replicas as CUs on,
-// 256-thread blocks are faster (measured on 3D Heisenberg L=128: 1024 threads
-// lead at 64 and 128 replicas, 0.16 / 0.27 sites/ns against 0.13 / 0.24, and
-// fall to 0.20 against 0.43 at 256 replicas on 256 CUs).
+// waiting on its claims' round trips, so the widest block keeps the most claims
+// in flight while there is at most one replica per CU; beyond that, two
+// 512-thread blocks per CU do better.  Measured on the same clusters (3D,
+// sites/ns for 256 / 512 / 1024 threads): Heisenberg L=128 x 256 replicas
+// 0.57 / 0.74 / 0.86, L=64 x 512 1.09 / 1.33 / 1.27, L=64 x 1024
+// 1.73 / 2.12 / 1.96; Ising L=128 x 256 1.72 / 2.47 / 2.77, L=64 x 1024
+// 6.18 / 7.01 / 6.39.
 inline int wolff_block(const mq_state* st, int nwg)
 {
     if (st->wolff_block > 0) return st->wolff_block;
-    return 2 * nwg <= st->ctx->num_cus ? 1024 : 256;
+    return nwg <= st->ctx->num_cus ? 1024 : 512;
 }
 
 // Sweep numbers a state accepts: the Philox phase of a sweep is one 32-bit
