@@ -9,11 +9,13 @@ every replica; value = spin-flip attempts/ns over the timed steps.
 
 Beside it on the same JSON line (skip with --quick):
   plain        the same sweep, one replica per launch
-  wolff        Wolff clusters on the thermalised 32-replica state vs one replica
+  wolff        Wolff clusters on the thermalised 32-replica state vs one replica, and the
+               next 100 clusters per replica in one persistent launch (no per-call lockstep)
   emcs         20 EMCS (10 clusters + 1 sweep) of that state: separate launches for the
                batch vs one persistent launch with each replica at its own pace
   c1_2d_L64    2D Ising L=64 EMCS schedule (10 Wolff + 1 sweep per EMCS), batched vs plain
-  c3_heis_L128 Heisenberg FP32 sweeps + Wolff, batched vs plain
+  c3_heis_L128 Heisenberg FP32 sweeps + Wolff (separate calls and one persistent launch),
+               batched vs plain
   c4_ea_batch  512 +-J samples x 32 temperatures, L in {8,12,16,24}, HIP-graph
                captured {sweep, PT swap}, batched vs plain
   c5_rgg       2D random geometric graph N=2^20, Ising and Potts-3, 64 replicas vs 1
@@ -110,7 +112,7 @@ def wolff_rate(ctx, st, rounds, nsw, ncl, seed, ncalls, sweep0, cluster0=0, tune
     untimed, then (with tune_ncl > 0) the Wolff autotuner on tune_ncl clusters, then
     `ncalls` timed calls of ncl clusters per replica.  A call lasts as long as its slowest
     replica's clusters, so this is the batch's real rate.  Returns (sites/ns, mean cluster
-    size of the timed clusters, the Wolff launch shape used)."""
+    size of the timed clusters, the Wolff launch shape used, the next cluster counter)."""
     c, sw = cluster0, sweep0
     for _ in range(rounds):
         st.metropolis(nsw, seed, sweep0=sw)
@@ -127,7 +129,19 @@ def wolff_rate(ctx, st, rounds, nsw, ncl, seed, ncalls, sweep0, cluster0=0, tune
         ms_tot += ms
         sites += int(sizes.sum())
         n += sizes.size
-    return sites / (ms_tot * 1e6), sites / n, {"tuned": shape is not None, **used}
+    return sites / (ms_tot * 1e6), sites / n, {"tuned": shape is not None, **used}, c
+
+
+def persistent_wolff_rate(ctx, st, nsteps, ncl, seed, cluster0):
+    """The same clusters as nsteps mq_wolff calls of ncl clusters (cluster0 a multiple of ncl),
+    grown in one persistent mq_emcs_run launch with no sweeps: each replica runs its
+    nsteps * ncl clusters without waiting for the other replicas at every call.  Returns
+    (sites/ns, mean cluster size, max over replicas / mean of the sites grown)."""
+    assert cluster0 % ncl == 0
+    box = {}
+    ms, _ = timed(ctx, lambda: box.setdefault("mc", st.emcs_run(nsteps, ncl, 0, seed, cluster0 // ncl)))
+    per_rep = box["mc"].sum(axis=0) * ncl
+    return float(per_rep.sum()) / (ms * 1e6), float(box["mc"].mean()), float(per_rep.max() / per_rep.mean())
 
 
 def emcs_compare(ctx, st, nsteps, ncluster, nmetro, seed, step0):
@@ -237,7 +251,8 @@ def leg_c3(gpu, ctx):
         ms, _ = timed(ctx, lambda: st.metropolis(nstep, 3, sweep0=5))
         # (the Wolff autotuner would save and restore the 8 GiB batch per candidate: the
         # automatic shape is used here and reported)
-        w_rate, w_mean, w_shape = wolff_rate(ctx, st, 30, 10, 10, 3, 10, sweep0=5 + nstep)
+        w_rate, w_mean, w_shape, w_next = wolff_rate(ctx, st, 30, 10, 10, 3, 10, sweep0=5 + nstep)
+        w_pers = persistent_wolff_rate(ctx, st, 10, 10, 3, w_next) if name == "batched" else None
         if name == "batched":
             CPU_WOLFF_INPUTS["heis_3d_L128"] = {"model": "heisenberg", "beta": beta, "dim": 3, "L": L,
                                                "spins": st.download(0)}
@@ -251,6 +266,12 @@ def leg_c3(gpu, ctx):
                      "wolff_roofline_frac": wolff_frac(w_rate, "3d_cartesian"), "wolff_launch_shape": w_shape,
                      "mean_cluster": round(w_mean, 1),
                      "roofline_frac": round(48.0 * nrep * N * nstep / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if w_pers is not None:
+            res[name]["wolff_persistent"] = {
+                "sites_per_ns": round(w_pers[0], 4), "mean_cluster": round(w_pers[1], 1),
+                "roofline_frac": wolff_frac(w_pers[0], "3d_cartesian"),
+                "max_over_mean_replica_sites": round(w_pers[2], 2),
+                "schedule": "the next 100 clusters per replica (10 x 10) in one mq_emcs_run launch without sweeps"}
         if name == "batched":
             obs = st.measure()
             res[name]["e"] = float(obs[:, 0].mean())
@@ -358,7 +379,7 @@ def leg_c5(gpu, ctx):
                     if e_prev is not None and abs(e - e_prev) < 2e-4:
                         break
                     e_prev = e
-                w_rate, w_mean, w_shape = wolff_rate(ctx, st, 0, 4, 4, 5, 8, sweep0=sw, cluster0=cl, tune_ncl=4)
+                w_rate, w_mean, w_shape, _ = wolff_rate(ctx, st, 0, 4, 4, 5, 8, sweep0=sw, cluster0=cl, tune_ncl=4)
                 res[f"{model}_wolff_sites_per_ns"] = round(w_rate, 4)
                 res[f"{model}_wolff_roofline_frac"] = wolff_frac(w_rate, "rgg_int8")
                 res[f"{model}_wolff_mean_cluster"] = round(w_mean, 1)
@@ -488,6 +509,8 @@ def main():
     if not a.quick and not a.legs_only:
         log("headline state: Wolff clusters")
         wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps, tune_ncl=4)
+        log("headline state: Wolff clusters, one persistent launch")
+        wolff_p = persistent_wolff_rate(ctx, st, 10, 10, 2, wolff_b[3])
         CPU_WOLFF_INPUTS["3d_L256"] = {"model": "ising", "beta": beta, "dim": 3, "L": L, "spins": st.download(0)}
         log("headline state: EMCS, separate launches and persistent")
         emcs_b = emcs_compare(ctx, st, 20, 10, 1, seed=4, step0=1 << 20)
@@ -531,7 +554,7 @@ def main():
         ms_p, _ = timed(ctx, lambda: sp.metropolis(k, seed=2, sweep0=4))
         line["plain"] = {"attempts_per_ns": round(k * N / (ms_p * 1e6), 3), "replicas": 1, "steps": k}
         line["batched_over_plain"] = round(value / line["plain"]["attempts_per_ns"], 3)
-        w_rate, w_mean, _ = wolff_rate(ctx, sp, 20, 4, 10, 2, 8, sweep0=4 + k)
+        w_rate, w_mean, _, _ = wolff_rate(ctx, sp, 20, 4, 10, 2, 8, sweep0=4 + k)
         line["plain"]["wolff_sites_per_ns"] = round(w_rate, 4)
         line["plain"]["wolff_mean_cluster"] = round(w_mean, 1)
         line["wolff"] = {"sites_per_ns": round(wolff_b[0], 4), "mean_cluster": round(wolff_b[1], 1),
@@ -539,6 +562,11 @@ def main():
                          "batched_over_plain": round(wolff_b[0] / w_rate, 2),
                          "roofline_frac": wolff_frac(wolff_b[0], "3d_int8"), "bytes_per_site": WOLFF_BYTES["3d_int8"],
                          "launch_shape": wolff_b[2],
+                         "persistent": {"sites_per_ns": round(wolff_p[0], 4), "mean_cluster": round(wolff_p[1], 1),
+                                        "roofline_frac": wolff_frac(wolff_p[0], "3d_int8"),
+                                        "max_over_mean_replica_sites": round(wolff_p[2], 2),
+                                        "schedule": "the next 100 clusters per replica (10 x 10) in one "
+                                                    "mq_emcs_run launch without sweeps"},
                          "schedule": "after 20 x (4 sweeps + 10 clusters); 8 timed calls of 10 clusters per replica"}
         line["emcs"] = emcs_b
         del sp

@@ -19,8 +19,10 @@
 //     (vertex, phase, replica >> 2, PROPOSE) and accepts with the Ising rule's
 //     16-bit halves; O(3)/Phi4 use counter (vertex, phase, replica, METRO),
 //     words 0,1 for the direction and word 2 for acceptance.
-//   Wolff bonds: the edge {u, v} is keyed by the original CSR slot of v in row u
-//     with u < v (its canonical slot), counter (slot >> 2, cluster, replica, WBOND).
+//   Wolff tests: the test from u to its neighbour v is keyed by the original CSR
+//     slot of v in row u, counter (slot >> 2, cluster, replica, WBOND), word
+//     slot & 3: every test has its own uniform, as the reference draws a fresh
+//     random number per test.
 #include <algorithm>
 #include <numeric>
 
@@ -492,8 +494,8 @@ constexpr int CWCAP = 4096;
 // each frontier entry carries the flipped spin's projection.  A claimed site's
 // row start and degree are loaded together with the claim and kept with its
 // ring entry, so a level waits on two dependent global round trips (neighbour
-// index, claim) instead of three.  Bond randoms are keyed by the canonical edge
-// id.
+// index, claim) instead of three.  Test randoms are keyed by the tested edge's
+// original slot in the tested-from vertex's row.
 template <int KIND>
 __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int64_t rs, int64_t ps, int64_t N,
                                                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
@@ -687,7 +689,7 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
 
 // Ising / Potts Wolff on a CSR graph with inline neighbours.  As k_csr_wolff,
 // plus: each frontier entry of a narrow level carries its first EW
-// (neighbour, canonical slot) pairs in LDS, written by the lane that claimed the
+// (neighbour, original slot) pairs in LDS, written by the lane that claimed the
 // entry, which loaded them from the lattice's padded row table (ell[pos*EW + t],
 // one 128-byte line per vertex) in the same round trip as the claim.  A level of
 // at most ECAP entries, all of degree <= EW, whose entries were written by a
@@ -702,7 +704,7 @@ constexpr int EW = 16;
 constexpr int ECAP = 128;
 constexpr int ESPEC = 128;
 
-// Bond test of graph site j over the edge with canonical slot eid: the bond's
+// Test of graph site j over the edge with original slot eid: the test's
 // Philox uniform against the add threshold; for Potts also whether j still
 // holds the cluster's old value (its word is returned in seen for the claim).
 template <int KIND>
@@ -931,7 +933,7 @@ __global__ void __launch_bounds__(1024) k_csr_wolff8(int8_t* __restrict__ base, 
 
 #undef MQ_BOND
 
-// Padded row table of a graph: ell[p*EW + t] = (position, canonical slot) of the
+// Padded row table of a graph: ell[p*EW + t] = (position, original slot) of the
 // t-th neighbour of position p, (-1, -1) past its degree.
 __global__ void __launch_bounds__(256) k_build_ell(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                                    const int32_t* __restrict__ edge, int64_t N, int2* __restrict__ ell)
@@ -965,7 +967,7 @@ extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, co
             if (j < 0 || j >= N || j == i) return mq_fail(ctx, MQ_ERR_ARG, "bad neighbour index or self loop");
         }
     }
-    // symmetry and canonical slots: slot of (u -> v) mirrored to (v -> u), in O(nnz)
+    // symmetry and mirrored slots: slot of (u -> v) mirrored to (v -> u), in O(nnz)
     // with local sorts: the incoming slots of every vertex are listed by a counting
     // transpose (in increasing source), each row's own slots are sorted by column, and
     // the two lists of a vertex must then name the same neighbours in the same order
@@ -1084,7 +1086,7 @@ extern "C" int mq_lattice_csr(mq_ctx* ctx, int64_t N, const int32_t* row_ptr, co
             const int k = row_ptr[u] + t;
             const int64_t v = col_idx[k];
             ci[rp[p] + t] = l->pos_of[v];
-            edge[rp[p] + t] = (u < v) ? k : mirror[k];
+            edge[rp[p] + t] = k;  // the test from u to v keys its uniform by its own slot
             slot_of_orig[k] = rp[p] + t;
         }
     }

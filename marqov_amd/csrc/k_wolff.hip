@@ -3,10 +3,11 @@
 //
 // The cluster is grown in natural site indices (x0 fastest); spins are read
 // and written through the colour-split layout.  A site joins the cluster when
-// the bond to an in-cluster neighbour is active; activation uses one Philox
-// uniform per bond keyed by (bond = owner*dim + d, cluster number, replica),
-// so the cluster is the connected component of active bonds and does not
-// depend on visit order.  Ising and Potts sites join with one atomic on their
+// the test from an in-cluster neighbour passes; every test has its own Philox
+// uniform keyed by (site * 2 * dim + neighbour slot, cluster number, replica),
+// as the reference draws a fresh random number per test, so the cluster is the
+// set of sites reachable from the seed along passing tests and does not depend
+// on visit order.  Ising and Potts sites join with one atomic on their
 // spin byte.  O(3)/Phi4 membership is an LDS hash set (an LDS compare-and-swap
 // per claim, no global round trip) while the cluster cannot outgrow vlim
 // sites, then an atomicOr on a per-replica bitmap, cleared again from the
@@ -80,6 +81,44 @@ __device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
         v.z -= 2.f * d * rd.z;
     }
     return v;
+}
+
+// Uniform of the Wolff test from site i (natural index) to its neighbour over
+// slot sl (2d: +e_d, 2d+1: -e_d): word e & 3 of bond-stream block e >> 2 with
+// e = i * 2 * DIM + sl (for 3D written as t = 3i + d, block t >> 1, so that it
+// stays in 32 bits for N <= 2^30).
+template <int DIM>
+__device__ __forceinline__ uint32_t test_uniform(uint32_t i, int sl, uint32_t ncl, uint32_t grep, Key2 key)
+{
+    if (DIM == 2) return u4_get(philox4x32_10(i, ncl, grep, ST_WBOND, key), sl);
+    const uint32_t t = 3u * i + (uint32_t)(sl >> 1);
+    return u4_get(philox4x32_10(t >> 1, ncl, grep, ST_WBOND, key), (int)(((t & 1u) << 1) | (uint32_t)(sl & 1)));
+}
+
+// All 2 * DIM test uniforms of site i: one Philox block in 2D, two in 3D
+// (e = 6i ... 6i + 5 spans blocks (3i) >> 1 and the next one).
+template <int DIM>
+__device__ __forceinline__ void site_test_uniforms(uint32_t i, uint32_t ncl, uint32_t grep, Key2 key,
+                                                   uint32_t (&u)[2 * DIM])
+{
+    if constexpr (DIM == 2) {
+        const uint4 w = philox4x32_10(i, ncl, grep, ST_WBOND, key);
+        u[0] = w.x;
+        u[1] = w.y;
+        u[2] = w.z;
+        u[3] = w.w;
+    } else {
+        const uint32_t b0 = (3u * i) >> 1;
+        const uint4 w0 = philox4x32_10(b0, ncl, grep, ST_WBOND, key);
+        const uint4 w1 = philox4x32_10(b0 + 1u, ncl, grep, ST_WBOND, key);
+        const bool odd = (i & 1u) != 0u;  // 6i = 2 mod 4: slots 0, 1 are words 2, 3 of the first block
+        u[0] = odd ? w0.z : w0.x;
+        u[1] = odd ? w0.w : w0.y;
+        u[2] = odd ? w1.x : w0.z;
+        u[3] = odd ? w1.y : w0.w;
+        u[4] = odd ? w1.z : w1.x;
+        u[5] = odd ? w1.w : w1.y;
+    }
 }
 
 // Frontier ring in LDS: entry pos written during the level whose first entry
@@ -260,20 +299,7 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
                             seen[sl] = peek_word(sp8 + jan[sl], sh);  // in flight during the Philox rounds
                         }
                     }
-                    // forward bonds: blocks (curn*DIM) >> 2 and the next one
-                    const uint64_t bf = (uint64_t)curn * DIM;
-                    const uint32_t blk0 = (uint32_t)(bf >> 2);
-                    const uint4 wf0 = philox4x32_10(blk0, ncl, (uint32_t)grep, ST_WBOND, key);
-                    const uint4 wf1 = (DIM == 3) ? philox4x32_10(blk0 + 1u, ncl, (uint32_t)grep, ST_WBOND, key) : wf0;
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        const uint64_t b = bf + (uint64_t)d;
-                        u[2 * d] = ((uint32_t)(b >> 2) == blk0) ? u4_get(wf0, (int)(b & 3)) : u4_get(wf1, (int)(b & 3));
-                        // backward bond: owned by the neighbour in direction -d
-                        const uint64_t bb = (uint64_t)jn[2 * d + 1] * DIM + (uint64_t)d;
-                        const uint4 wb = philox4x32_10((uint32_t)(bb >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        u[2 * d + 1] = u4_get(wb, (int)(bb & 3));
-                    }
+                    site_test_uniforms<DIM>(curn, ncl, (uint32_t)grep, key, u);
                     if (KIND == WK_ISING) {
                         const bool up = __builtin_amdgcn_readfirstlane(val) > 0;
 #pragma unroll
@@ -348,18 +374,7 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
                         jan[sl] = nh + (jn[sl] >> 1);
                         vj[sl] = sp4[jan[sl]];  // in flight during the Philox rounds
                     }
-                    const uint64_t bf = (uint64_t)curn * DIM;
-                    const uint32_t blk0 = (uint32_t)(bf >> 2);
-                    const uint4 wf0 = philox4x32_10(blk0, ncl, (uint32_t)grep, ST_WBOND, key);
-                    const uint4 wf1 = (DIM == 3) ? philox4x32_10(blk0 + 1u, ncl, (uint32_t)grep, ST_WBOND, key) : wf0;
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        const uint64_t b = bf + (uint64_t)d;
-                        u[2 * d] = ((uint32_t)(b >> 2) == blk0) ? u4_get(wf0, (int)(b & 3)) : u4_get(wf1, (int)(b & 3));
-                        const uint64_t bb = (uint64_t)jn[2 * d + 1] * DIM + (uint64_t)d;
-                        const uint4 wb = philox4x32_10((uint32_t)(bb >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        u[2 * d + 1] = u4_get(wb, (int)(bb & 3));
-                    }
+                    site_test_uniforms<DIM>(curn, ncl, (uint32_t)grep, key, u);
 #pragma unroll
                     for (int sl = 0; sl < NS; ++sl) {
                         const float cpl = J * pc * proj<KIND>(vj[sl], axis, rd);
@@ -434,9 +449,7 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
                                 const uint32_t cura = (((x0 + x1 + x2) & 1u) ? half : 0u) + (curn >> 1);
                                 pc[k] = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[cura], axis, rd);
                             }
-                            const uint64_t b = (uint64_t)((sl & 1) ? jn[k] : curn) * DIM + (uint64_t)d;
-                            const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                            u[k] = u4_get(bw, (int)(b & 3));
+                            u[k] = test_uniform<DIM>(curn, sl, ncl, (uint32_t)grep, key);
                         }
                     }
                     if (KIND == WK_ISING && glob) {
@@ -525,26 +538,22 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
                 const uint32_t jn = curn + (nx - xd) * (d == 0 ? 1u : (d == 1 ? uL : LL));  // mod 2^32
                 // the neighbours of a site all have the other colour
                 const uint32_t jan = (((x0 + x1 + x2) & 1u) ? 0u : half) + (jn >> 1);
-                const uint64_t b = (uint64_t)((sl & 1) ? jn : curn) * DIM + (uint64_t)d;
                 bool ok = false;
                 float pnew = 0.f;
                 if (KIND == WK_ISING || KIND == WK_POTTS) {
                     if (!glob) {
                         const int sv = sp8[jan];  // in flight during the Philox rounds
-                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        if (sv == val && u4_get(bw, (int)(b & 3)) < tw) {
+                        if (sv == val && test_uniform<DIM>(curn, sl, ncl, (uint32_t)grep, key) < tw) {
                             ok = vset_insert(s_vset, jp);
                             if (ok) sp8[jan] = (int8_t)newq;
                         }
                     } else if (KIND == WK_ISING) {
                         // the claim itself tests alignment
-                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        if (u4_get(bw, (int)(b & 3)) < tw) ok = claim_ising(sp8 + jan, val);
+                        if (test_uniform<DIM>(curn, sl, ncl, (uint32_t)grep, key) < tw) ok = claim_ising(sp8 + jan, val);
                     } else {
                         uint32_t sh;
                         const uint32_t seen = peek_word(sp8 + jan, sh);  // in flight during the Philox rounds
-                        const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
-                        if ((int8_t)(seen >> sh) == (int8_t)val && u4_get(bw, (int)(b & 3)) < tw) {
+                        if ((int8_t)(seen >> sh) == (int8_t)val && test_uniform<DIM>(curn, sl, ncl, (uint32_t)grep, key) < tw) {
                             uint32_t* w = spin_word(sp8 + jan, sh);
                             const uint32_t prev = atomicCAS(w, seen, (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
                             // the word changed under the first swap: retry from what it held
@@ -555,9 +564,9 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
                     const uint32_t cura = (((x0 + x1 + x2) & 1u) ? half : 0u) + (curn >> 1);
                     const float pc = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[cura], axis, rd);
                     const float4 vj = sp4[jan];  // in flight during the Philox rounds
-                    const uint4 bw = philox4x32_10((uint32_t)(b >> 2), ncl, (uint32_t)grep, ST_WBOND, key);
+                    const uint32_t ub = test_uniform<DIM>(curn, sl, ncl, (uint32_t)grep, key);
                     const float cpl = J * pc * proj<KIND>(vj, axis, rd);
-                    if (cpl > 0.f && u01f(u4_get(bw, (int)(b & 3))) < -expm1f(-2.0f * beta * cpl)) {
+                    if (cpl > 0.f && u01f(ub) < -expm1f(-2.0f * beta * cpl)) {
                         if (glob) {
                             const uint32_t ob = atomicOr(&vis[jn >> 5], 1u << (jn & 31));
                             ok = !(ob & (1u << (jn & 31)));
@@ -788,7 +797,7 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
     unsigned char* mine = smem + (size_t)wave * rep_lds;
     uint32_t* SW = reinterpret_cast<uint32_t*>(mine);  // PACK: bit h of word h>>5 = site address h is down
     int8_t* S = reinterpret_cast<int8_t*>(mine);
-    uint32_t* cb_b = reinterpret_cast<uint32_t*>(mine + wave_spin_bytes(N, PACK));  // bond id
+    uint32_t* cb_b = reinterpret_cast<uint32_t*>(mine + wave_spin_bytes(N, PACK));  // test word e
     uint16_t* cb_j = reinterpret_cast<uint16_t*>(cb_b + NCB);                 // natural index
     uint16_t* cb_a = cb_j + NCB;                                              // colour-split address
     uint16_t* ring = cb_a + NCB;
@@ -872,7 +881,7 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
                     const uint64_t m = __ballot(cand);
                     if (cand) {
                         const int p = ncand + __popcll(m & lt);
-                        cb_b[p] = ((sl & 1) ? jn[sl] : cur) * (uint32_t)DIM + (uint32_t)(sl >> 1);
+                        cb_b[p] = cur * (uint32_t)NS + (uint32_t)sl;  // the test's word e (N <= 16384)
                         cb_j[p] = (uint16_t)jn[sl];
                         cb_a[p] = (uint16_t)jav[sl];
                     }

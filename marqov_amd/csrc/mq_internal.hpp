@@ -494,7 +494,7 @@ struct mq_lattice {
     std::vector<int32_t> h_rp, h_ci; // original numbering (host copy)
     int32_t* d_rp = nullptr;         // new numbering
     int32_t* d_ci = nullptr;
-    int32_t* d_edge = nullptr;       // canonical (original) slot id of each slot
+    int32_t* d_edge = nullptr;       // original slot id of each slot (keys the Wolff test uniforms)
     int32_t* d_pos_of = nullptr;     // original vertex -> position
     int32_t* d_vert_of = nullptr;    // position -> original vertex
     void* d_ell = nullptr;           // padded row table of the int8 graph Wolff (built on first use)

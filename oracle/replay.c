@@ -16,10 +16,13 @@
  *    dE = J * (s_new - s_old) * sum_j b_ij s_j.  Site order differs from the
  *    reference's random-site order (emcs.h:93-106) by design: all sites of one
  *    colour, then all of the other.
- *  - orc_wolff_ising: single-cluster Wolff for Ising (Ising.h:122-169); a site
- *    joins through an active bond with p = -expm1(2*beta*J) (Ising.h:139).
- *    Bond activations come from per-bond counters, so the cluster is the
- *    connected component of active bonds and does not depend on visit order.
+ *  - orc_wolff_ising: single-cluster Wolff for Ising (Ising.h:122-169); an
+ *    aligned neighbour not yet in the cluster joins with p = -expm1(2*beta*J)
+ *    (Ising.h:139).  As in the reference, every such test draws its own
+ *    uniform (rng.real() per candidate, Ising.h:150-158); here the uniform is
+ *    keyed by the test (site, neighbour slot), so the cluster is the set of
+ *    sites reachable from the seed along passing tests and does not depend on
+ *    visit order.
  *  - orc_potts_*: Potts Metropolis (Potts.h:87-107,165-178) and Wolff
  *    (Potts.h:189-322) on the same schedule.
  *  - orc_energy_* / orc_mag_*: observables (hamiltonian/util/observables.h:32-66,
@@ -34,7 +37,9 @@
  *   STREAM_METRO : c0 = (site>>1)>>3, c1 = 2*sweep+colour, 16-bit half (site>>1)&7
  *   STREAM_TIE   : same counter as METRO, low 16 bits of the 32-bit uniform
  *   STREAM_WSEED : c0 = 0, c1 = cluster number
- *   STREAM_WBOND : c0 = bond>>2 with bond = site*dim+d, c1 = cluster number, word bond&3
+ *   STREAM_WBOND : c0 = e>>2 with e = site*2*dim+slot (the test from site to its
+ *                  neighbour over slot), c1 = cluster number, word e&3; on a graph
+ *                  e is the CSR slot of the test's edge (row of the tested-from site)
  */
 #include <math.h>
 #include <stdint.h>
@@ -238,18 +243,30 @@ EXPORT void orc_csr_o3_colour(int64_t N, const int32_t* rp, const int32_t* ci, c
     }
 }
 
+/* Uniform of the Wolff test from site i to its neighbour over slot (2d: +e_d,
+ * 2d+1: -e_d): word e & 3 of counter (e >> 2, ncl, rep, STREAM_WBOND) with
+ * e = i * 2 * dim + slot.  A site's 2*dim tests use two Philox blocks in 3D and
+ * one in 2D. */
+static uint32_t test_uniform(int dim, int64_t i, int slot, uint32_t ncl, uint32_t rep, uint64_t seed)
+{
+    const uint64_t e = (uint64_t)i * (uint64_t)(2 * dim) + (uint64_t)slot;
+    uint32_t w[4];
+    orc_philox_u64key((uint32_t)(e >> 2), ncl, rep, ST_WBOND, seed, w);
+    return w[e & 3];
+}
+
 /* One O(3)/Phi4 Wolff cluster (Heisenberg.h:191-243 with a Cartesian axis;
  * Phi4.h:182-244 / the random-direction embedding: s -= 2 (s.r) r) restated in
  * FP64 on the GPU's random numbers: seed and embedding from counter
  * (0, ncl, rep, STREAM_WSEED) (seed = (w0 N) >> 32; axis = (w1 3) >> 32 or the
- * direction of words w1, w2 as in orc_o3_halfsweep); bond b = owner*dim + d
- * (owner = the site on the + side's lower end, as in orc_wolff_ising) is active
- * from in-cluster a to b's other end j iff cpl = J p_a' p_j > 0 and
- * u_b < 1 - exp(-2 beta cpl), u_b = (word >> 8) 2^-24 of counter
- * (b >> 2, ncl, rep, STREAM_WBOND), p_a' the projection of a after its flip and
- * p_j that of j before the cluster.  The cluster is the component of active
- * bonds; *min_margin receives min |p - u| over every (member, neighbour) test,
- * so a cluster grown in FP32 in any order can differ only if it is tiny.
+ * direction of words w1, w2 as in orc_o3_halfsweep); the test from in-cluster
+ * a to its neighbour j over slot passes iff cpl = J p_a' p_j > 0 and
+ * u < 1 - exp(-2 beta cpl), u = (word >> 8) 2^-24 of the word test_uniform
+ * names (e = a*2*dim + slot), p_a' the projection of a after its flip and p_j
+ * that of j before the cluster.  The cluster is the set reachable from the seed
+ * along passing tests; *min_margin receives min |p - u| over every
+ * (member, neighbour) test, so a cluster grown in FP32 in any order can differ
+ * only if it is tiny.
  * Returns the cluster size; s_out = s_in with the cluster flipped. */
 EXPORT int64_t orc_o3_wolff(int dim, int L, const double* s_in, double* s_out, int random_dir, double J, double beta,
                             uint32_t ncl, uint32_t rep, uint64_t seed, double* min_margin)
@@ -300,11 +317,7 @@ EXPORT int64_t orc_o3_wolff(int dim, int L, const double* s_in, double* s_out, i
         for (int d = 0; d < dim; ++d)
             for (int sg = 1; sg >= -1; sg -= 2) {
                 const int64_t j = nbr(dim, L, a, d, sg);
-                const int64_t owner = sg > 0 ? a : j;
-                const uint64_t b = (uint64_t)owner * (uint64_t)dim + (uint64_t)d;
-                uint32_t w[4];
-                orc_philox_u64key((uint32_t)(b >> 2), ncl, rep, ST_WBOND, seed, w);
-                const double u = (double)(w[b & 3] >> 8) * 0x1.0p-24;
+                const double u = (double)(test_uniform(dim, a, 2 * d + (sg < 0), ncl, rep, seed) >> 8) * 0x1.0p-24;
                 const double cpl = J * pa * proj0[j];
                 const double p = cpl > 0.0 ? -expm1(-2.0 * beta * cpl) : 0.0;
                 const double mg = fabs(p - u);
@@ -458,15 +471,6 @@ static int ensure_work(int64_t N)
     return g_stack && g_mark;
 }
 
-static uint32_t bond_uniform(int dim, int L, int64_t i, int slot, uint32_t ncl, uint32_t rep, uint64_t seed)
-{
-    int d = slot >> 1;
-    int64_t owner = (slot & 1) ? nbr(dim, L, i, d, -1) : i;
-    int64_t b = owner * dim + d;
-    uint32_t w[4];
-    orc_philox_u64key((uint32_t)(b >> 2), ncl, rep, ST_WBOND, seed, w);
-    return w[b & 3];
-}
 
 /* Returns the cluster sizes in sizes[nclusters]. */
 EXPORT int orc_wolff_ising(int dim, int L, int8_t* s, uint32_t tw, int nclusters,
@@ -489,7 +493,7 @@ EXPORT int orc_wolff_ising(int dim, int L, int8_t* s, uint32_t tw, int nclusters
             for (int slot = 0; slot < 2 * dim; ++slot) {
                 int64_t j = orc_neighbour(dim, L, cur, slot);
                 if (g_mark[j] || s[j] != val) continue;
-                if (bond_uniform(dim, L, cur, slot, ncl, rep, seed) < tw) {
+                if (test_uniform(dim, cur, slot, ncl, rep, seed) < tw) {
                     g_mark[j] = 1;
                     g_stack[top++] = (int32_t)j;
                     ++size;
@@ -529,7 +533,7 @@ EXPORT int orc_wolff_potts(int dim, int L, int8_t* s, int Q, uint32_t tw, int nc
             for (int slot = 0; slot < 2 * dim; ++slot) {
                 int64_t j = orc_neighbour(dim, L, cur, slot);
                 if (g_mark[j] || s[j] != oldq) continue;
-                if (bond_uniform(dim, L, cur, slot, ncl, rep, seed) < tw) {
+                if (test_uniform(dim, cur, slot, ncl, rep, seed) < tw) {
                     g_mark[j] = 1;
                     g_stack[top++] = (int32_t)j;
                     ++size;
@@ -649,16 +653,9 @@ EXPORT void orc_csr_sweeps(int64_t N, const int32_t* rp, const int32_t* ci, cons
         }
 }
 
-static int64_t csr_mirror(const int32_t* rp, const int32_t* ci, int64_t u, int64_t k)
-{
-    int64_t v = ci[k];
-    for (int64_t t = rp[v]; t < rp[v + 1]; ++t)
-        if (ci[t] == u) return t;
-    return -1;
-}
-
-/* Wolff on a graph: edge {u,v} keyed by its canonical slot (slot of v in row
- * u for u < v).  potts = 0: Ising; 1: Potts with Q states. */
+/* Wolff on a graph: the test from u to v over CSR slot k (v = ci[k]) uses word
+ * k & 3 of counter (k >> 2, ncl, rep, STREAM_WBOND).  potts = 0: Ising; 1:
+ * Potts with Q states. */
 EXPORT int orc_csr_wolff(int64_t N, const int32_t* rp, const int32_t* ci, int8_t* s, int potts, int Q, uint32_t tw,
                          int nclusters, uint64_t seed, uint32_t cl0, uint32_t rep, int64_t* sizes)
 {
@@ -678,10 +675,9 @@ EXPORT int orc_csr_wolff(int64_t N, const int32_t* rp, const int32_t* ci, int8_t
             for (int64_t k = rp[u]; k < rp[u + 1]; ++k) {
                 int64_t v = ci[k];
                 if (g_mark[v] || s[v] != val) continue;
-                int64_t id = (u < v) ? k : csr_mirror(rp, ci, u, k);
                 uint32_t b[4];
-                orc_philox_u64key((uint32_t)(id >> 2), ncl, rep, ST_WBOND, seed, b);
-                if (b[id & 3] < tw) {
+                orc_philox_u64key((uint32_t)(k >> 2), ncl, rep, ST_WBOND, seed, b);
+                if (b[k & 3] < tw) {
                     g_mark[v] = 1;
                     g_stack[top++] = (int32_t)v;
                     ++size;
@@ -696,8 +692,8 @@ EXPORT int orc_csr_wolff(int64_t N, const int32_t* rp, const int32_t* ci, int8_t
 }
 
 /* One O(3)/Phi4 Wolff cluster on a CSR graph, FP64, as orc_o3_wolff: seed
- * vertex (w0 N) >> 32, bond uniforms keyed by the canonical edge id (the slot of
- * the edge in its lower endpoint's row), as orc_csr_wolff. */
+ * vertex (w0 N) >> 32, test uniforms keyed by the CSR slot of the tested edge in
+ * the tested-from vertex's row, as orc_csr_wolff. */
 EXPORT int64_t orc_csr_o3_wolff(int64_t N, const int32_t* rp, const int32_t* ci, const double* s_in, double* s_out,
                                 int random_dir, double J, double beta, uint32_t ncl, uint32_t rep, uint64_t seed,
                                 double* min_margin)
@@ -744,10 +740,9 @@ EXPORT int64_t orc_csr_o3_wolff(int64_t N, const int32_t* rp, const int32_t* ci,
         const double pa = random_dir ? va[0] * rd[0] + va[1] * rd[1] + va[2] * rd[2] : va[axis];
         for (int64_t k = rp[a]; k < rp[a + 1]; ++k) {
             const int64_t j = ci[k];
-            const int64_t id = (a < j) ? k : csr_mirror(rp, ci, a, k);
             uint32_t w[4];
-            orc_philox_u64key((uint32_t)(id >> 2), ncl, rep, ST_WBOND, seed, w);
-            const double u = (double)(w[id & 3] >> 8) * 0x1.0p-24;
+            orc_philox_u64key((uint32_t)(k >> 2), ncl, rep, ST_WBOND, seed, w);
+            const double u = (double)(w[k & 3] >> 8) * 0x1.0p-24;
             const double cpl = J * pa * proj0[j];
             const double p = cpl > 0.0 ? -expm1(-2.0 * beta * cpl) : 0.0;
             const double mg = fabs(p - u);

@@ -73,9 +73,10 @@ def test_o3_sweep_decisions_match_fp64_restatement(ctx, model, proposal, dim, L,
     ("phi4", "random", 3, 8, 0.69), ("phi4", "random", 2, 16, 1.0)])
 def test_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, dim, L, beta):
     """Each O(3)/Phi4 cluster, grown by the GPU from a downloaded FP32 state, equals the
-    FP64 restatement's component of active bonds (same seed site, embedding and bond
-    uniforms): equal size, the same sites flipped, flipped spins to FP32 rounding.  A
-    cluster is compared when no bond test of it lies within 1e-5 of its threshold."""
+    FP64 restatement's cluster, the sites reachable from the seed along passing tests (same
+    seed site, embedding and test uniforms): equal size, the same sites flipped, flipped
+    spins to FP32 rounding.  A cluster is compared when no test of it lies within 1e-5 of
+    its threshold."""
     from marqov_amd import gpu
     lat = gpu.RegularLattice(ctx, dim, L)
     nrep, seed = 2, 13
@@ -86,7 +87,10 @@ def test_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, dim, L,
     st.metropolis(5, seed=seed, sweep0=0)
     compared = skipped = 0
     sizes_seen = []
-    for k in range(8):
+    # at least 8 clusters, more (up to 32) until one of more than 10 sites has been compared
+    for k in range(32):
+        if k >= 8 and compared >= 12 and max(sizes_seen) > 10:
+            break
         pre = [st.download(r).astype(np.float64) for r in range(nrep)]
         sizes = st.wolff(1, seed=seed, cluster0=k)
         post = [st.download(r).astype(np.float64) for r in range(nrep)]
@@ -103,15 +107,15 @@ def test_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, dim, L,
             moved_cpu = np.any(cpu != pre[r], axis=1)
             assert np.array_equal(moved_gpu, moved_cpu), (k, r)
             assert np.abs(post[r] - cpu).max() < 2e-5, (k, r)
-    assert compared >= 12 and skipped <= 4, (compared, skipped)
+    assert compared >= 12 and skipped <= max(4, compared // 4), (compared, skipped)
     assert max(sizes_seen) > 10, sizes_seen
 
 
 @pytest.mark.parametrize("model,embedding,beta", [("heisenberg", "cartesian", 1.2), ("heisenberg", "random", 1.2),
                                                   ("phi4", "random", 1.0)])
 def test_csr_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, beta):
-    """The graph kernel's O(3)/Phi4 clusters (bond uniforms keyed by the canonical edge
-    id) against the FP64 restatement on a 2D random geometric graph of mean degree 8."""
+    """The graph kernel's O(3)/Phi4 clusters (test uniforms keyed by the tested edge's
+    CSR slot) against the FP64 restatement on a 2D random geometric graph of mean degree 8."""
     from marqov_amd import graphs, gpu
     n = 3000
     rp, ci, _ = graphs.rgg2d(n, graphs.rgg_radius(n, 8.0), 17)
@@ -124,7 +128,10 @@ def test_csr_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, bet
     st.metropolis(5, seed=seed, sweep0=0)
     compared = skipped = 0
     sizes_seen = []
-    for k in range(8):
+    # at least 8 clusters, more (up to 32) until one of more than 10 sites has been compared
+    for k in range(32):
+        if k >= 8 and compared >= 12 and max(sizes_seen) > 10:
+            break
         pre = [st.download(r).astype(np.float64) for r in range(nrep)]
         sizes = st.wolff(1, seed=seed, cluster0=k)
         post = [st.download(r).astype(np.float64) for r in range(nrep)]
@@ -139,7 +146,7 @@ def test_csr_o3_wolff_clusters_match_fp64_restatement(ctx, model, embedding, bet
             assert int(sizes[r, 0]) == size, (k, r)
             assert np.array_equal(np.any(post[r] != pre[r], axis=1), np.any(cpu != pre[r], axis=1)), (k, r)
             assert np.abs(post[r] - cpu).max() < 2e-5, (k, r)
-    assert compared >= 12 and skipped <= 4, (compared, skipped)
+    assert compared >= 12 and skipped <= max(4, compared // 4), (compared, skipped)
     assert max(sizes_seen) > 10, sizes_seen
 
 

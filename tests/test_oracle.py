@@ -336,7 +336,7 @@ def _graph_chain(rp, ci, col, beta, J, nsteps, seed, rep, potts, bonds=None, wol
                                                      (False, 0.8, 3, 0, True), (True, 1.0, 6, 0, False),
                                                      (True, 0.8, 7, 1, False)])
 def test_graph_replay_matches_enumeration(potts, beta, rep, wolff, ea):
-    """Colour-class sweeps (and Wolff clusters keyed by canonical edges) on an
+    """Colour-class sweeps (and Wolff clusters keyed by the tested edges' slots) on an
     irregular 10-vertex graph sample the exact Boltzmann distribution: <E/N> and
     <m^2> (Ising) or <m> (Potts) within 4 sigma of exhaustive enumeration.  The
     replica numbers exercise every word / half-word of the shared Philox blocks."""
