@@ -130,10 +130,13 @@ __device__ __forceinline__ void site_test_uniforms(uint32_t i, uint32_t ncl, uin
 constexpr int WCAP = 4096;
 // bond tasks a lane takes at once on a wide level
 constexpr int WKT = 4;
-// int8 levels of more than SITE_MIN * blockDim bond tasks with claims in global
-// memory run one lane per site (thermalised 3D Ising L=256 x 32, the same clusters:
-// 0.495 sites/ns with bond tasks only, 0.506 / 0.523 / 0.522 with SITE_MIN 1 / 2 / 4)
-constexpr int SITE_MIN = 2;
+// Levels of more than SITE_MIN * blockDim tests with claims in global memory run
+// one lane per site (two Philox blocks for a 3D site's six tests); narrower levels,
+// and levels whose members are in the LDS set, one lane per test.  Same clusters,
+// sites/ns with one lane per test only / SITE_MIN 1 / 2 / 4 / 6: thermalised 3D Ising
+// L=256 x 32 0.583 / 0.537 / 0.589 / 0.604 / 0.579; Heisenberg L=128 x 128 (Cartesian)
+// 0.498 / 0.449 / 0.484 / 0.500; one lane per site on every level: 0.454 and 0.297.
+constexpr int SITE_MIN = 4;
 
 // LDS bytes of one replica's spins in the small-lattice kernel: a byte per
 // site, or a bit per site when packed (16-byte multiples)
@@ -270,10 +273,9 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
             }
             if ((KIND == WK_ISING || KIND == WK_POTTS) && glob && ntask > SITE_MIN * (int)blockDim.x) {
                 // A wide level with claims in global memory: one lane per frontier site.  The
-                // site's coordinates are unpacked once, its DIM forward bonds (owner*DIM + d)
-                // lie in at most two consecutive Philox blocks, and its 2*DIM claims are in
-                // flight together: 5 Philox evaluations per 3D site instead of 6, and the
-                // per-task addressing once per site.
+                // site's coordinates are unpacked once, its 2*DIM test uniforms come from two
+                // Philox blocks (one in 2D) instead of one block per test, and its 2*DIM
+                // claims are in flight together.
                 const int nsite = tail - head;
                 for (int e = (int)threadIdx.x; e < nsite; e += (int)blockDim.x) {
                     const int idx = head + e;
