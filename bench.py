@@ -379,7 +379,8 @@ def leg_c5(gpu, ctx):
                     if e_prev is not None and abs(e - e_prev) < 2e-4:
                         break
                     e_prev = e
-                w_rate, w_mean, w_shape, _ = wolff_rate(ctx, st, 0, 4, 4, 5, 8, sweep0=sw, cluster0=cl, tune_ncl=4)
+                w_rate, w_mean, w_shape, w_next = wolff_rate(ctx, st, 0, 4, 4, 5, 8, sweep0=sw, cluster0=cl,
+                                                             tune_ncl=4)
                 res[f"{model}_wolff_sites_per_ns"] = round(w_rate, 4)
                 res[f"{model}_wolff_roofline_frac"] = wolff_frac(w_rate, "rgg_int8")
                 res[f"{model}_wolff_mean_cluster"] = round(w_mean, 1)
@@ -387,11 +388,17 @@ def leg_c5(gpu, ctx):
                 res[f"{model}_wolff_launch_shape"] = w_shape
                 CPU_WOLFF_INPUTS[f"rgg_{model}"] = {"model": model, "beta": beta, "rp": rp, "ci": ci,
                                                     "spins": st.download(0)}
+                # the next 32 clusters per replica in one launch: no lockstep every 4 clusters
+                ms1, sz1 = timed_wolff(ctx, st, 32, 5, w_next)
+                res[f"{model}_wolff_one_launch_sites_per_ns"] = round(float(sz1.sum()) / (ms1 * 1e6), 4)
+                res[f"{model}_wolff_one_launch_max_over_mean_replica_sites"] = round(
+                    float(sz1.sum(axis=1).max() / sz1.sum(axis=1).mean()), 2)
             del st
         res[f"{model}_batched_over_plain"] = round(res[f"{model}_batched_attempts_per_ns"] /
                                                    res[f"{model}_plain_attempts_per_ns"], 2)
     res["wolff"] = ("64 replicas thermalised by rounds of (4 sweeps + 4 clusters) until E/N moves by < 2e-4 "
-                    "over 100 rounds; Wolff autotuner; 8 timed calls of 4 clusters per replica")
+                    "over 100 rounds; Wolff autotuner; 8 timed calls of 4 clusters per replica, then one "
+                    "call of the next 32 clusters per replica (one_launch)")
     # the Ising sweep on 1-bit storage (32 replicas per uint32 word): the same chain as int8 storage,
     # (d+2)*4/32 B of spins + 4(d+1)/64 B of CSR indices per attempt instead of (d+2) + 4(d+1)/64
     st = gpu.State(ctx, "ising", [lat] * 64, J=-1.0, spin_bits=1)
@@ -508,7 +515,7 @@ def main():
     wolff_b = emcs_b = None
     if not a.quick and not a.legs_only:
         log("headline state: Wolff clusters")
-        wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps, tune_ncl=4)
+        wolff_b = wolff_rate(ctx, st, 20, 4, 10, 2, 8, sweep0=a.warmup + a.steps, tune_ncl=10)
         log("headline state: Wolff clusters, one persistent launch")
         wolff_p = persistent_wolff_rate(ctx, st, 10, 10, 2, wolff_b[3])
         CPU_WOLFF_INPUTS["3d_L256"] = {"model": "ising", "beta": beta, "dim": 3, "L": L, "spins": st.download(0)}
