@@ -764,16 +764,17 @@ __global__ void __launch_bounds__(1024) k_emcs_regular(char* __restrict__ base, 
 // One wavefront per replica for small int8 lattices (N <= 16384): the
 // replica's spins and a 16-bit frontier ring (spilling to the global queue)
 // live in the wave's slice of LDS for all nclusters clusters and are written
-// back once.  A level is processed 64 frontier entries at a time in two
-// passes: (A) lanes take entries and list the neighbours that still hold the
-// cluster's old value (candidate bonds) in LDS; (B) lanes take candidates,
-// draw the bond's Philox uniform and claim the site.  Pass B runs one bond
-// per lane, so a narrow front costs one Philox evaluation per lane instead of
-// 2*DIM.  New entries are placed by wave-wide prefix counts; nothing
-// synchronises the workgroup.  Same bond counters and membership rule as
-// k_wolff_regular, so the clusters are identical.  (Evaluating every bond of
-// the lattice up front instead of pass B was measured at +5% on the 2D L=64
-// critical batch: the level bookkeeping, not the Philox draws, sets the pace.)
+// back once.  A level is processed 64 frontier entries at a time, one per
+// lane.  2D: the lane reads its four neighbours' spins, draws the entry's four
+// test uniforms (one Philox block) and claims every neighbour that still holds
+// the cluster's old value and whose test passes, the claims in flight
+// together.  3D: (A) lanes list the candidate tests (neighbours that still
+// hold the old value) in LDS; (B) lanes take candidates, one per lane, draw the
+// test's uniform and claim.  Single-pass 3D (two Philox blocks per entry) and
+// two-pass 2D measured 6% and 4% slower (3D L=16 x 4,096, 2D L=64 x 16,384, the
+// same clusters).  New entries are placed by wave-wide prefix counts; nothing
+// synchronises the workgroup.  Same test counters and membership rule as
+// k_wolff_regular, so the clusters are identical.
 // PACK (Ising, N a multiple of 32): the replica's spins sit in LDS as one bit
 // per site (set = down), 1/8 of the bytes, so a CU holds up to twice as many
 // replicas' waves and a batch larger than the CUs' share of the byte layout
@@ -785,7 +786,6 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
                                                     const int64_t* __restrict__ rep_site_off,
                                                     int64_t* __restrict__ csize, Key2 key, int cap, int rep_lds)
 {
-    constexpr int NCB = 64 * 2 * DIM;  // candidate list entries per pass A
     extern __shared__ __align__(16) unsigned char smem[];
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     const int r = (int)blockIdx.x * (int)(blockDim.x >> 6) + wave;
@@ -799,9 +799,11 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
     unsigned char* mine = smem + (size_t)wave * rep_lds;
     uint32_t* SW = reinterpret_cast<uint32_t*>(mine);  // PACK: bit h of word h>>5 = site address h is down
     int8_t* S = reinterpret_cast<int8_t*>(mine);
-    uint32_t* cb_b = reinterpret_cast<uint32_t*>(mine + wave_spin_bytes(N, PACK));  // test word e
-    uint16_t* cb_j = reinterpret_cast<uint16_t*>(cb_b + NCB);                 // natural index
-    uint16_t* cb_a = cb_j + NCB;                                              // colour-split address
+    // 3D only: the candidate list of the two-pass level (test word, natural index, address)
+    constexpr int NCB = DIM == 3 ? 64 * 2 * DIM : 0;
+    uint32_t* cb_b = reinterpret_cast<uint32_t*>(mine + wave_spin_bytes(N, PACK));
+    uint16_t* cb_j = reinterpret_cast<uint16_t*>(cb_b + NCB);
+    uint16_t* cb_a = cb_j + NCB;
     uint16_t* ring = cb_a + NCB;
     const uint32_t cmask = (uint32_t)cap - 1u;
     uint32_t* gw = reinterpret_cast<uint32_t*>(base + r * stride);  // N is a multiple of 4 (L even)
@@ -842,81 +844,68 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
         int head = 0, hprev = 0, tail = 1;
         while (head < tail) {
             int next = tail;
-            for (int chunk = head; chunk < tail; chunk += 64) {
-                // pass A: candidate bonds of up to 64 frontier entries
-                const int idx = chunk + lane;
-                const bool act = idx < tail;
-                // ring slots are always valid addresses: read unconditionally, fix up spills
-                uint32_t cur = (uint32_t)ring[(uint32_t)idx & cmask];
-                if (act && idx - hprev >= cap) cur = (uint32_t)q[idx];
-                if (!act) cur = 0u;
-                uint32_t x[3] = {0u, 0u, 0u};
-                {
-                    uint32_t rem;
-                    const uint32_t r1 = fdiv_u24(cur, uL, invL, rem);
-                    x[0] = rem;
-                    if (DIM == 3) x[2] = fdiv_u24(r1, uL, invL, x[1]);
-                    else x[1] = r1;
-                }
-                const uint32_t nbase = ((x[0] + x[1] + x[2]) & 1u) ? 0u : half;
-                int ncand = 0;
-                // neighbours of every lane (inactive lanes use site 0): the 2*DIM byte
-                // reads are issued together, unconditionally, before any is tested
-                constexpr int NS = 2 * DIM;
-                uint32_t jn[NS], jav[NS];
-                int sv[NS];
-#pragma unroll
-                for (int sl = 0; sl < NS; ++sl) {
-                    const int d = sl >> 1;
-                    const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
-                    uint32_t j;
-                    if ((sl & 1) == 0) j = (x[d] == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
-                    else j = (x[d] == 0) ? cur + (uL - 1) * pw : cur - pw;
-                    jn[sl] = j;
-                    jav[sl] = nbase + (j >> 1);
-                }
-#pragma unroll
-                for (int sl = 0; sl < NS; ++sl) sv[sl] = PACK ? (int)((SW[jav[sl] >> 5] >> (jav[sl] & 31u)) & 1u) : (int)S[jav[sl]];
-#pragma unroll
-                for (int sl = 0; sl < NS; ++sl) {
-                    const bool cand = act && sv[sl] == (PACK ? vbit : val);
-                    const uint64_t m = __ballot(cand);
-                    if (cand) {
-                        const int p = ncand + __popcll(m & lt);
-                        cb_b[p] = cur * (uint32_t)NS + (uint32_t)sl;  // the test's word e (N <= 16384)
-                        cb_j[p] = (uint16_t)jn[sl];
-                        cb_a[p] = (uint16_t)jav[sl];
+            constexpr int NS = 2 * DIM;
+            if constexpr (DIM == 2) {
+                // one pass: a 2D entry's four test uniforms are one Philox block
+                for (int chunk = head; chunk < tail; chunk += 64) {
+                    // up to 64 frontier entries, one per lane
+                    const int idx = chunk + lane;
+                    const bool act = idx < tail;
+                    // ring slots are always valid addresses: read unconditionally, fix up spills
+                    uint32_t cur = (uint32_t)ring[(uint32_t)idx & cmask];
+                    if (act && idx - hprev >= cap) cur = (uint32_t)q[idx];
+                    if (!act) cur = 0u;
+                    uint32_t x[3] = {0u, 0u, 0u};
+                    {
+                        uint32_t rem;
+                        const uint32_t r1 = fdiv_u24(cur, uL, invL, rem);
+                        x[0] = rem;
+                        if (DIM == 3) x[2] = fdiv_u24(r1, uL, invL, x[1]);
+                        else x[1] = r1;
                     }
-                    ncand += __popcll(m);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-                // pass B: one candidate bond per lane
-                for (int cc = 0; cc < ncand; cc += 64) {
-                    const int k = cc + lane;
-                    bool ok = false;
-                    uint32_t j = 0;
-                    if (k < ncand) {
-                        const uint32_t b = cb_b[k];
-                        j = cb_j[k];
-                        const uint32_t ja = cb_a[k];
-                        const uint4 bw = philox4x32_10(b >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
-                        if (u4_get(bw, (int)(b & 3u)) < tw) {
+                    const uint32_t nbase = ((x[0] + x[1] + x[2]) & 1u) ? 0u : half;
+                    // neighbours of every lane (inactive lanes use site 0): the 2*DIM spin reads
+                    // are issued together, unconditionally, before any is tested
+                    uint32_t jn[NS], jav[NS], u[NS];
+                    int sv[NS];
+                    bool ok[NS];
+    #pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const int d = sl >> 1;
+                        const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
+                        uint32_t j;
+                        if ((sl & 1) == 0) j = (x[d] == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
+                        else j = (x[d] == 0) ? cur + (uL - 1) * pw : cur - pw;
+                        jn[sl] = j;
+                        jav[sl] = nbase + (j >> 1);
+                    }
+    #pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) sv[sl] = PACK ? (int)((SW[jav[sl] >> 5] >> (jav[sl] & 31u)) & 1u) : (int)S[jav[sl]];
+                    // the entry's 2*DIM test uniforms: one Philox block in 2D, two in 3D
+                    site_test_uniforms<DIM>(cur, ncl, (uint32_t)grep, key, u);
+                    // a passing test to a neighbour that still holds the old value claims it (the
+                    // claim re-checks: another lane may have taken the site since it was read)
+    #pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        ok[sl] = false;
+                        if (act && sv[sl] == (PACK ? vbit : val) && u[sl] < tw) {
+                            const uint32_t ja = jav[sl];
                             const uint32_t sh = (ja & 3u) * 8u;
                             if (PACK) {
                                 const uint32_t mb = 1u << (ja & 31u);
                                 const uint32_t old = vbit ? atomicAnd(&SW[ja >> 5], ~mb) : atomicOr(&SW[ja >> 5], mb);
-                                ok = ((old & mb) != 0u) == (vbit != 0);
+                                ok[sl] = ((old & mb) != 0u) == (vbit != 0);
                             } else if (KIND == WK_ISING) {
                                 const uint32_t mm = 0xFEu << sh;
                                 const uint32_t old = val > 0 ? atomicOr(&SW[ja >> 2], mm) : atomicAnd(&SW[ja >> 2], ~mm);
-                                ok = (int8_t)(old >> sh) == (int8_t)val;
+                                ok[sl] = (int8_t)(old >> sh) == (int8_t)val;
                             } else {
                                 uint32_t seen = SW[ja >> 2];
                                 while ((int8_t)(seen >> sh) == (int8_t)val) {
                                     const uint32_t nw = (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh);
                                     const uint32_t prev = atomicCAS(&SW[ja >> 2], seen, nw);
                                     if (prev == seen) {
-                                        ok = true;
+                                        ok[sl] = true;
                                         break;
                                     }
                                     seen = prev;
@@ -924,15 +913,115 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
                             }
                         }
                     }
-                    const uint64_t m = __ballot(ok);
-                    if (ok) {
-                        const int pos = next + __popcll(m & lt);
-                        if (pos - head < cap) ring[(uint32_t)pos & cmask] = (uint16_t)j;
-                        else q[pos] = (int32_t)j;
+                    // new entries placed by wave-wide prefix counts, slot by slot
+    #pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const uint64_t m = __ballot(ok[sl]);
+                        if (ok[sl]) {
+                            const int pos = next + __popcll(m & lt);
+                            if (pos - head < cap) ring[(uint32_t)pos & cmask] = (uint16_t)jn[sl];
+                            else q[pos] = (int32_t)jn[sl];
+                        }
+                        next += __popcll(m);
                     }
-                    next += __popcll(m);
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            } else {
+                // two passes: (A) lanes take entries and list the neighbours that still
+                // hold the old value (candidate tests) in LDS; (B) lanes take candidates,
+                // draw the test's Philox uniform and claim, one test per lane (a 3D entry's
+                // six uniforms would cost two blocks per lane, more than the candidates need)
+                for (int chunk = head; chunk < tail; chunk += 64) {
+                    // pass A: candidate bonds of up to 64 frontier entries
+                    const int idx = chunk + lane;
+                    const bool act = idx < tail;
+                    // ring slots are always valid addresses: read unconditionally, fix up spills
+                    uint32_t cur = (uint32_t)ring[(uint32_t)idx & cmask];
+                    if (act && idx - hprev >= cap) cur = (uint32_t)q[idx];
+                    if (!act) cur = 0u;
+                    uint32_t x[3] = {0u, 0u, 0u};
+                    {
+                        uint32_t rem;
+                        const uint32_t r1 = fdiv_u24(cur, uL, invL, rem);
+                        x[0] = rem;
+                        if (DIM == 3) x[2] = fdiv_u24(r1, uL, invL, x[1]);
+                        else x[1] = r1;
+                    }
+                    const uint32_t nbase = ((x[0] + x[1] + x[2]) & 1u) ? 0u : half;
+                    int ncand = 0;
+                    // neighbours of every lane (inactive lanes use site 0): the 2*DIM byte
+                    // reads are issued together, unconditionally, before any is tested
+                    uint32_t jn[NS], jav[NS];
+                    int sv[NS];
+    #pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const int d = sl >> 1;
+                        const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
+                        uint32_t j;
+                        if ((sl & 1) == 0) j = (x[d] == uL - 1) ? cur - (uL - 1) * pw : cur + pw;
+                        else j = (x[d] == 0) ? cur + (uL - 1) * pw : cur - pw;
+                        jn[sl] = j;
+                        jav[sl] = nbase + (j >> 1);
+                    }
+    #pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) sv[sl] = PACK ? (int)((SW[jav[sl] >> 5] >> (jav[sl] & 31u)) & 1u) : (int)S[jav[sl]];
+    #pragma unroll
+                    for (int sl = 0; sl < NS; ++sl) {
+                        const bool cand = act && sv[sl] == (PACK ? vbit : val);
+                        const uint64_t m = __ballot(cand);
+                        if (cand) {
+                            const int p = ncand + __popcll(m & lt);
+                            cb_b[p] = cur * (uint32_t)NS + (uint32_t)sl;  // the test's word e (N <= 16384)
+                            cb_j[p] = (uint16_t)jn[sl];
+                            cb_a[p] = (uint16_t)jav[sl];
+                        }
+                        ncand += __popcll(m);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                    // pass B: one candidate bond per lane
+                    for (int cc = 0; cc < ncand; cc += 64) {
+                        const int k = cc + lane;
+                        bool ok = false;
+                        uint32_t j = 0;
+                        if (k < ncand) {
+                            const uint32_t b = cb_b[k];
+                            j = cb_j[k];
+                            const uint32_t ja = cb_a[k];
+                            const uint4 bw = philox4x32_10(b >> 2, ncl, (uint32_t)grep, ST_WBOND, key);
+                            if (u4_get(bw, (int)(b & 3u)) < tw) {
+                                const uint32_t sh = (ja & 3u) * 8u;
+                                if (PACK) {
+                                    const uint32_t mb = 1u << (ja & 31u);
+                                    const uint32_t old = vbit ? atomicAnd(&SW[ja >> 5], ~mb) : atomicOr(&SW[ja >> 5], mb);
+                                    ok = ((old & mb) != 0u) == (vbit != 0);
+                                } else if (KIND == WK_ISING) {
+                                    const uint32_t mm = 0xFEu << sh;
+                                    const uint32_t old = val > 0 ? atomicOr(&SW[ja >> 2], mm) : atomicAnd(&SW[ja >> 2], ~mm);
+                                    ok = (int8_t)(old >> sh) == (int8_t)val;
+                                } else {
+                                    uint32_t seen = SW[ja >> 2];
+                                    while ((int8_t)(seen >> sh) == (int8_t)val) {
+                                        const uint32_t nw = (seen & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh);
+                                        const uint32_t prev = atomicCAS(&SW[ja >> 2], seen, nw);
+                                        if (prev == seen) {
+                                            ok = true;
+                                            break;
+                                        }
+                                        seen = prev;
+                                    }
+                                }
+                            }
+                        }
+                        const uint64_t m = __ballot(ok);
+                        if (ok) {
+                            const int pos = next + __popcll(m & lt);
+                            if (pos - head < cap) ring[(uint32_t)pos & cmask] = (uint16_t)j;
+                            else q[pos] = (int32_t)j;
+                        }
+                        next += __popcll(m);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                }
             }
             hprev = head;
             head = tail;
@@ -1017,8 +1106,8 @@ hipError_t launch_wolff_regular(mq_state* st, const Group& g, int nclusters, uin
         // wolff_ring entries with spill to the global queue
         int cap = 64;
         while (cap < N && cap < st->wolff_ring) cap <<= 1;
-        const int ncb = 64 * 2 * dim;  // candidate list: uint32 bond + two uint16 per entry
         const bool pack = kind == WK_ISING && N % 32 == 0;
+        const int ncb = dim == 3 ? 64 * 2 * dim : 0;  // 3D candidate list: uint32 test word + two uint16
         const int rep_lds = (int)(wave_spin_bytes((uint32_t)N, pack) + 8 * ncb + (2 * cap + 15) / 16 * 16);
         const int wpb = std::max(1, std::min(4, 65536 / rep_lds));
         const dim3 wgrid((unsigned)((g.nrep + wpb - 1) / wpb)), wblock((unsigned)(64 * wpb));
