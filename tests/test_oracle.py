@@ -186,6 +186,58 @@ def test_planted_sign_error_is_caught():
     assert abs(me - ex["e"]) > 10 * se
 
 
+def _py_neighbour(dim, L, i, sl):
+    """Neighbour over slot sl (2d: +e_d, 2d+1: -e_d) of site i = sum_j x_j L^j, periodic."""
+    d = sl >> 1
+    pw = L ** d
+    x = (i // pw) % L
+    return i + ((1 if x + 1 < L else 1 - L) if sl % 2 == 0 else (-1 if x > 0 else L - 1)) * pw
+
+
+@pytest.mark.parametrize("dim,L,beta", [(2, 8, 0.4), (3, 4, 0.25)])
+def test_wolff_cluster_is_reachable_set_of_passing_tests(dim, L, beta):
+    """The oracle's Wolff cluster (grown depth-first from a stack) is the set of sites reachable
+    from the seed along passing tests, found here breadth-first from nothing but raw Philox words:
+    the test from site i to its neighbour over slot k uses word e & 3 of counter
+    (e >> 2, cluster, replica, 4), e = i*2*dim + k, and passes when the word is below the add
+    threshold and the neighbour holds the seed's value (one uniform per test, as the reference
+    draws rng.real() per candidate, Ising.h:146-158).  Keying each test by its undirected bond
+    instead (a plausible mistake) changes the clusters."""
+    N = L ** dim
+    seed, rep = 77, 3
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    tw = ob.lib().orc_wolff_threshold(-np.expm1(2.0 * beta * -1.0))
+
+    def word(e, ncl):
+        return ob.philox(((e >> 2) & 0xFFFFFFFF, ncl, rep, 4), key)[e & 3]
+
+    def reach(s, s0, uniform):
+        val, seen, front = s[s0], {s0}, [s0]
+        while front:
+            nxt = []
+            for i in front:
+                for sl in range(2 * dim):
+                    j = _py_neighbour(dim, L, i, sl)
+                    if j not in seen and s[j] == val and uniform(i, sl, j) < tw:
+                        seen.add(j)
+                        nxt.append(j)
+            front = nxt
+        return seen
+
+    s = ob.init_ising(N, 5, rep)
+    differs, sizes_seen = 0, []
+    for ncl in range(8):
+        s0 = (ob.philox((0, ncl, rep, 3), key)[0] * N) >> 32
+        per_test = reach(s, s0, lambda i, sl, j: word(i * 2 * dim + sl, ncl))
+        out, sizes = ob.wolff_ising(dim, L, s, beta, -1.0, 1, seed, ncl, rep)
+        assert set(np.nonzero(out != s)[0].tolist()) == per_test and sizes[0] == len(per_test), ncl
+        per_bond = reach(s, s0, lambda i, sl, j: word((i if sl % 2 == 0 else j) * dim + (sl >> 1), ncl))
+        differs += per_bond != per_test
+        sizes_seen.append(len(per_test))
+        s = out
+    assert max(sizes_seen) > 4 and differs >= 1, (sizes_seen, differs)
+
+
 def test_replay_potts_matches_enumeration():
     dim, L, Q, beta = 3, 2, 3, 0.7
     N = L ** dim
