@@ -696,7 +696,7 @@ __global__ void __launch_bounds__(1024) k_csr_wolff(char* __restrict__ base, int
 // level of at most ESPEC entries, then needs no degree scan, no search and no
 // neighbour-index load: lane t takes pair t % EW of entry t / EW and waits on
 // one memory round trip (the claim).  Other levels run the block-wide scan of
-// k_csr_wolff.  Same bonds, same clusters.  On the 2^20-vertex RGG (64 thermalised
+// k_csr_wolff.  Same tests, same clusters.  On the 2^20-vertex RGG (64 thermalised
 // replicas, the same clusters timed) this lifts Ising 0.324 -> 0.360 and Potts-3
 // 0.230 -> 0.268 sites/ns; inline buffers of 64 or 256 entries, or loading the
 // pairs in levels of up to 256 entries, measured 3-5% below 128 / 128.
@@ -732,7 +732,7 @@ __device__ __forceinline__ bool claim8(int8_t* sj, int val, int newq, uint32_t s
     return (prev == seen) || claim_potts(sj, prev, val, newq);
 }
 
-// One bond task of the int8 graph Wolff: test, then (row, inline pairs, claim)
+// One test of the int8 graph Wolff: the uniform, then (row, inline pairs, claim)
 // issued together so that they share one memory round trip.
 // inline pairs are moved as clang vector types, which stay in registers (the HIP
 // vector structs do not when copied through an array)

@@ -84,7 +84,7 @@ __device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
 }
 
 // Uniform of the Wolff test from site i (natural index) to its neighbour over
-// slot sl (2d: +e_d, 2d+1: -e_d): word e & 3 of bond-stream block e >> 2 with
+// slot sl (2d: +e_d, 2d+1: -e_d): word e & 3 of block e >> 2 of the test stream, with
 // e = i * 2 * DIM + sl (for 3D written as t = 3i + d, block t >> 1, so that it
 // stays in 32 bits for N <= 2^30).
 template <int DIM>
@@ -128,7 +128,7 @@ __device__ __forceinline__ void site_test_uniforms(uint32_t i, uint32_t ncl, uin
 // overwritten before it is read.  The reader of that level knows `head` as its
 // own `hprev`.
 constexpr int WCAP = 4096;
-// bond tasks a lane takes at once on a wide level
+// tests a lane takes at once on a wide level
 constexpr int WKT = 4;
 // Levels of more than SITE_MIN * blockDim tests with claims in global memory run
 // one lane per site (two Philox blocks for a 3D site's six tests); narrower levels,
@@ -145,7 +145,7 @@ __host__ __device__ inline uint32_t wave_spin_bytes(uint32_t N, bool pack) { ret
 // Sites are handled as packed coordinates: 10 bits per axis in 3D (L <= 1024),
 // 16 bits in 2D.  Frontier entries and LDS set keys are packed sites, so a
 // neighbour, its natural index and its colour-split address come from adds and
-// multiplies, with no integer division per bond.
+// multiplies, with no integer division per test.
 template <int DIM>
 struct SitePack {
     static constexpr int SB = DIM == 3 ? 10 : 16;
@@ -205,7 +205,7 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
         const uint32_t seed_a = ((sx0 + sx1 + sx2) & 1u) * half + (seed >> 1);
         // Membership: while the cluster cannot outgrow vlim sites, an LDS hash set of
         // packed sites (one LDS compare-and-swap per claim, and for int8 spins the
-        // alignment test is a plain load issued before the bond's Philox draw); after
+        // alignment test is a plain load issued before the test's Philox draw); after
         // that, the spin byte itself (Ising, Potts: "still holds the old value" is "not
         // in the cluster", claimed with one global atomic) or the global bitmap
         // (O(3)/Phi4).  The choice is block-uniform.
@@ -253,7 +253,7 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
             const int tail = s_tail;
             __syncthreads();
             if (head >= tail) break;
-            // One lane per (frontier site, bond slot): a level costs one Philox
+            // One lane per (frontier site, neighbour slot): a level costs one Philox
             // evaluation and one memory round trip per lane; new entries are placed
             // with one LDS atomic per wavefront.
             const int ntask = (tail - head) * NS;
@@ -407,7 +407,7 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
                 }
             } else
             if (ntask > (int)blockDim.x) {
-                // A wide level: each lane takes up to WKT bond tasks t, t + blockDim, ... at
+                // A wide level: each lane takes up to WKT tests t, t + blockDim, ... at
                 // once, issues all their loads, draws their uniforms and then issues all
                 // their claims before using any result, so the level waits on one memory
                 // round trip per WKT tasks rather than one per task.  Claims of one lane
@@ -932,7 +932,7 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
                 // draw the test's Philox uniform and claim, one test per lane (a 3D entry's
                 // six uniforms would cost two blocks per lane, more than the candidates need)
                 for (int chunk = head; chunk < tail; chunk += 64) {
-                    // pass A: candidate bonds of up to 64 frontier entries
+                    // pass A: candidate tests of up to 64 frontier entries
                     const int idx = chunk + lane;
                     const bool act = idx < tail;
                     // ring slots are always valid addresses: read unconditionally, fix up spills
@@ -978,7 +978,7 @@ __global__ void __launch_bounds__(256) k_wolff_wave(char* __restrict__ base, int
                         ncand += __popcll(m);
                     }
                     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-                    // pass B: one candidate bond per lane
+                    // pass B: one candidate test per lane
                     for (int cc = 0; cc < ncand; cc += 64) {
                         const int k = cc + lane;
                         bool ok = false;
