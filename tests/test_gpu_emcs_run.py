@@ -14,7 +14,8 @@ BETA = {("ising", 2): 0.4406868, ("ising", 3): 0.2216546, ("potts", 2): 1.005052
     ("ising", 2, 64, 8, 10, 1), ("ising", 3, 32, 8, 5, 2), ("ising", 3, 64, 4, 10, 1),
     ("potts", 2, 32, 8, 4, 1), ("potts", 3, 16, 8, 3, 2),
     ("ising", 2, 48, 4, 0, 3), ("ising", 3, 16, 4, 6, 0), ("ising", 3, 256, 2, 2, 1),
-    ("heisenberg", 3, 16, 8, 5, 1), ("cart", 3, 24, 4, 3, 2), ("heisenberg", 2, 32, 4, 4, 1), ("phi4", 3, 12, 8, 4, 1)])
+    ("heisenberg", 3, 16, 8, 5, 1), ("cart", 3, 24, 4, 3, 2), ("heisenberg", 2, 32, 4, 4, 1), ("phi4", 3, 12, 8, 4, 1),
+    ("cart", 3, 32, 4, 5, 0)])
 def test_emcs_run_equals_emcs_calls(ctx, model, dim, L, nrep, nc, nm):
     from marqov_amd import gpu
     lat = gpu.RegularLattice(ctx, dim, L)
