@@ -84,9 +84,9 @@ __device__ __forceinline__ float4 reflect(float4 v, int axis, const float3& rd)
 }
 
 // Uniform of the Wolff test from site i (natural index) to its neighbour over
-// slot sl (2d: +e_d, 2d+1: -e_d): word e & 3 of block e >> 2 of the test stream, with
-// e = i * 2 * DIM + sl (for 3D written as t = 3i + d, block t >> 1, so that it
-// stays in 32 bits for N <= 2^30).
+// slot sl (2d: +e_d, 2d+1: -e_d): word e & 3 of block e >> 2 of the test stream,
+// with e = i * 2 * DIM + sl (for 3D written as t = 3i + d, block t >> 1, so that
+// it stays in 32 bits for N <= 2^30).  The two tests along one axis share a block.
 template <int DIM>
 __device__ __forceinline__ uint32_t test_uniform(uint32_t i, int sl, uint32_t ncl, uint32_t grep, Key2 key)
 {
@@ -128,8 +128,8 @@ __device__ __forceinline__ void site_test_uniforms(uint32_t i, uint32_t ncl, uin
 // overwritten before it is read.  The reader of that level knows `head` as its
 // own `hprev`.
 constexpr int WCAP = 4096;
-// tests a lane takes at once on a wide level
-constexpr int WKT = 4;
+// pair tasks (the two tests of a site along one axis) a lane takes at once
+constexpr int WKP = 2;
 // Levels of more than SITE_MIN * blockDim tests with claims in global memory run
 // one lane per site (two Philox blocks for a 3D site's six tests); narrower levels,
 // and levels whose members are in the LDS set, one lane per test.  Same clusters,
@@ -405,120 +405,157 @@ __device__ __forceinline__ void wolff_regular_clusters(const int r, char* __rest
                         ++pos;
                     }
                 }
-            } else
-            if (ntask > (int)blockDim.x) {
-                // A wide level: each lane takes up to WKT tests t, t + blockDim, ... at
-                // once, issues all their loads, draws their uniforms and then issues all
-                // their claims before using any result, so the level waits on one memory
-                // round trip per WKT tasks rather than one per task.  Claims of one lane
-                // still act in order, and a site claimed twice is taken once.
-                for (int t0 = (int)threadIdx.x; t0 < ntask; t0 += WKT * (int)blockDim.x) {
-                    uint32_t jp[WKT], jn[WKT], jan[WKT], u[WKT], old[WKT];
-                    float pc[WKT];
-                    int sv[WKT];
-                    float4 vj[WKT];
-                    bool act[WKT], ok[WKT], try_[WKT];
+            } else if (ntask > (int)blockDim.x) {
+                // A wide level in pair tasks: a task is the two tests of one frontier site
+                // along one axis (slots 2d and 2d + 1), whose uniforms are two words of one
+                // Philox block.  A lane takes up to WKP tasks p, p + blockDim, ... at once,
+                // issues all their loads, draws their uniforms and then issues all their
+                // claims before using any result, so the level waits on one memory round
+                // trip per lane.  Claims of one lane still act in order, and a site claimed
+                // twice is taken once.
+                const int npair = (tail - head) * DIM;
+                constexpr int NT = 2 * WKP;  // tests per lane
+                for (int p0 = (int)threadIdx.x; p0 < npair; p0 += WKP * (int)blockDim.x) {
+                    uint32_t jp[NT], jn[NT], jan[NT], u[NT], old[NT], seen[NT];
+                    float pc[WKP];
+                    int sv[NT];
+                    float4 vj[NT];
+                    bool act[NT], ok[NT], try_[NT];
 #pragma unroll
-                    for (int k = 0; k < WKT; ++k) {
-                        const int t = t0 + k * (int)blockDim.x;
-                        act[k] = t < ntask;
-                        ok[k] = try_[k] = false;
-                        jp[k] = jn[k] = jan[k] = 0u;
-                        u[k] = 0xFFFFFFFFu;
-                        old[k] = 0u;
+                    for (int k = 0; k < WKP; ++k) {
+                        const int t = p0 + k * (int)blockDim.x;
                         pc[k] = 0.f;
-                        sv[k] = 0;
-                        vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        if (act[k]) {
-                            const int e = t / NS;
-                            const int sl = t - e * NS;
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int m = 2 * k + h;
+                            act[m] = t < npair;
+                            ok[m] = try_[m] = false;
+                            jp[m] = jn[m] = jan[m] = 0u;
+                            u[m] = 0xFFFFFFFFu;
+                            old[m] = 0u;
+                            seen[m] = 0u;
+                            sv[m] = 0;
+                            vj[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        }
+                        if (t < npair) {
+                            const int e = t / DIM;
+                            const int d = t - e * DIM;
                             const int idx = head + e;
                             const bool in_ring = idx - hprev < WCAP;
                             const uint32_t cur = in_ring ? s_ring[idx & (WCAP - 1)] : (uint32_t)q[idx];
                             const uint32_t x0 = cur & CM, x1 = (cur >> SB) & CM, x2 = SitePack<DIM>::z(cur);
-                            const int d = sl >> 1;
                             const uint32_t xd = d == 0 ? x0 : (d == 1 ? x1 : x2);
-                            uint32_t nx;
-                            if ((sl & 1) == 0) nx = (xd == uL - 1) ? 0u : xd + 1u;
-                            else nx = (xd == 0) ? uL - 1u : xd - 1u;
-                            jp[k] = (cur & ~(CM << (SB * d))) | (nx << (SB * d));
                             const uint32_t curn = x0 + uL * x1 + LL * x2;
-                            jn[k] = curn + (nx - xd) * (d == 0 ? 1u : (d == 1 ? uL : LL));  // mod 2^32
-                            jan[k] = (((x0 + x1 + x2) & 1u) ? 0u : half) + (jn[k] >> 1);
-                            if (!VEC && !glob) sv[k] = sp8[jan[k]];  // in flight during the Philox rounds
+                            const uint32_t pw = d == 0 ? 1u : (d == 1 ? uL : LL);
+                            const uint32_t nh = ((x0 + x1 + x2) & 1u) ? 0u : half;  // the neighbours' colour half
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                const int m = 2 * k + h;
+                                uint32_t nx;
+                                if (h == 0) nx = (xd == uL - 1) ? 0u : xd + 1u;
+                                else nx = (xd == 0) ? uL - 1u : xd - 1u;
+                                jp[m] = (cur & ~(CM << (SB * d))) | (nx << (SB * d));
+                                jn[m] = curn + (nx - xd) * pw;  // mod 2^32
+                                jan[m] = nh + (jn[m] >> 1);
+                                // in flight during the Philox rounds: the spin byte while members
+                                // are in the LDS set, the word a Potts claim swaps in global memory
+                                if (!VEC && !glob) sv[m] = sp8[jan[m]];
+                                if (KIND == WK_POTTS && glob) {
+                                    uint32_t sh;
+                                    seen[m] = peek_word(sp8 + jan[m], sh);
+                                }
+                                if (VEC) vj[m] = sp4[jan[m]];
+                            }
                             if (VEC) {
-                                vj[k] = sp4[jan[k]];
                                 const uint32_t cura = (((x0 + x1 + x2) & 1u) ? half : 0u) + (curn >> 1);
                                 pc[k] = in_ring ? s_proj[idx & (WCAP - 1)] : proj<KIND>(sp4[cura], axis, rd);
                             }
-                            u[k] = test_uniform<DIM>(curn, sl, ncl, (uint32_t)grep, key);
+                            // words e = curn*2*DIM + 2d and e + 1 of the test stream: one block
+                            uint32_t blk, w0;
+                            if (DIM == 2) {
+                                blk = curn;
+                                w0 = 2u * (uint32_t)d;
+                            } else {
+                                const uint32_t t3 = 3u * curn + (uint32_t)d;
+                                blk = t3 >> 1;
+                                w0 = (t3 & 1u) << 1;
+                            }
+                            const uint4 bw = philox4x32_10(blk, ncl, (uint32_t)grep, ST_WBOND, key);
+                            u[2 * k] = w0 == 0u ? bw.x : bw.z;
+                            u[2 * k + 1] = w0 == 0u ? bw.y : bw.w;
                         }
                     }
                     if (KIND == WK_ISING && glob) {
                         // all claims in flight together (the claim tests alignment)
                         const bool up = __builtin_amdgcn_readfirstlane(val) > 0;
 #pragma unroll
-                        for (int k = 0; k < WKT; ++k) {
-                            try_[k] = act[k] && u[k] < tw;
-                            if (try_[k]) {
+                        for (int m = 0; m < NT; ++m) {
+                            try_[m] = act[m] && u[m] < tw;
+                            if (try_[m]) {
                                 uint32_t sh;
-                                uint32_t* w = spin_word(sp8 + jan[k], sh);
-                                const uint32_t m = 0xFEu << sh;
-                                old[k] = up ? atomicOr(w, m) : atomicAnd(w, ~m);
+                                uint32_t* w = spin_word(sp8 + jan[m], sh);
+                                const uint32_t mk = 0xFEu << sh;
+                                old[m] = up ? atomicOr(w, mk) : atomicAnd(w, ~mk);
                             }
                         }
 #pragma unroll
-                        for (int k = 0; k < WKT; ++k)
-                            ok[k] = try_[k] && (int8_t)(old[k] >> ((jan[k] & 3u) * 8u)) == (int8_t)val;
+                        for (int m = 0; m < NT; ++m)
+                            ok[m] = try_[m] && (int8_t)(old[m] >> ((jan[m] & 3u) * 8u)) == (int8_t)val;
                     } else if (KIND == WK_POTTS && glob) {
 #pragma unroll
-                        for (int k = 0; k < WKT; ++k) {
-                            if (act[k] && u[k] < tw) {
-                                uint32_t sh;
-                                ok[k] = claim_potts(sp8 + jan[k], peek_word(sp8 + jan[k], sh), val, newq);
+                        for (int m = 0; m < NT; ++m) {
+                            if (act[m] && u[m] < tw) {
+                                const uint32_t sh = (jan[m] & 3u) * 8u;
+                                if ((int8_t)(seen[m] >> sh) == (int8_t)val) {
+                                    uint32_t sh2;
+                                    uint32_t* w = spin_word(sp8 + jan[m], sh2);
+                                    const uint32_t prev = atomicCAS(w, seen[m], (seen[m] & ~(0xFFu << sh)) | ((uint32_t)(uint8_t)newq << sh));
+                                    // the word changed under the first swap: retry from what it held
+                                    ok[m] = (prev == seen[m]) || claim_potts(sp8 + jan[m], prev, val, newq);
+                                }
                             }
                         }
                     } else if (!VEC) {
 #pragma unroll
-                        for (int k = 0; k < WKT; ++k) {
-                            if (act[k] && sv[k] == val && u[k] < tw) {
-                                ok[k] = vset_insert(s_vset, jp[k]);
-                                if (ok[k]) sp8[jan[k]] = (int8_t)newq;
+                        for (int m = 0; m < NT; ++m) {
+                            if (act[m] && sv[m] == val && u[m] < tw) {
+                                ok[m] = vset_insert(s_vset, jp[m]);
+                                if (ok[m]) sp8[jan[m]] = (int8_t)newq;
                             }
                         }
                     } else {
-                        float cpl[WKT];
+                        float cpl[NT];
 #pragma unroll
-                        for (int k = 0; k < WKT; ++k) {
-                            cpl[k] = J * pc[k] * proj<KIND>(vj[k], axis, rd);
-                            try_[k] = act[k] && cpl[k] > 0.f && u01f(u[k]) < -expm1f(-2.0f * beta * cpl[k]);
-                            if (try_[k] && glob) old[k] = atomicOr(&vis[jn[k] >> 5], 1u << (jn[k] & 31));
+                        for (int m = 0; m < NT; ++m) {
+                            cpl[m] = J * pc[m >> 1] * proj<KIND>(vj[m], axis, rd);
+                            try_[m] = act[m] && cpl[m] > 0.f && u01f(u[m]) < -expm1f(-2.0f * beta * cpl[m]);
+                            if (try_[m] && glob) old[m] = atomicOr(&vis[jn[m] >> 5], 1u << (jn[m] & 31));
                         }
 #pragma unroll
-                        for (int k = 0; k < WKT; ++k) {
-                            if (!try_[k]) continue;
-                            ok[k] = glob ? !(old[k] & (1u << (jn[k] & 31))) : vset_insert(s_vset, jp[k]);
+                        for (int m = 0; m < NT; ++m) {
+                            if (!try_[m]) continue;
+                            ok[m] = glob ? !(old[m] & (1u << (jn[m] & 31))) : vset_insert(s_vset, jp[m]);
                         }
                     }
                     uint32_t nok = 0;
 #pragma unroll
-                    for (int k = 0; k < WKT; ++k) nok += ok[k] ? 1u : 0u;
+                    for (int m = 0; m < NT; ++m) nok += ok[m] ? 1u : 0u;
                     int pos = wave_reserve(&s_tail, nok);
 #pragma unroll
-                    for (int k = 0; k < WKT; ++k) {
-                        if (!ok[k]) continue;
+                    for (int m = 0; m < NT; ++m) {
+                        if (!ok[m]) continue;
                         float pnew = 0.f;
                         if (VEC) {
                             // the claim succeeded, so nobody changed vj since it was read
-                            const float4 v = reflect<KIND>(vj[k], axis, rd);
-                            sp4[jan[k]] = v;
+                            const float4 v = reflect<KIND>(vj[m], axis, rd);
+                            sp4[jan[m]] = v;
                             pnew = proj<KIND>(v, axis, rd);
                         }
                         if (pos - head < WCAP) {
-                            s_ring[pos & (WCAP - 1)] = jp[k];
+                            s_ring[pos & (WCAP - 1)] = jp[m];
                             if (VEC) s_proj[pos & (WCAP - 1)] = pnew;
                         }
-                        if (VEC || pos - head >= WCAP) q[pos] = (int32_t)jp[k];  // VEC: every member, for the bitmap
+                        if (VEC || pos - head >= WCAP) q[pos] = (int32_t)jp[m];  // VEC: every member, for the bitmap
                         ++pos;
                     }
                 }
