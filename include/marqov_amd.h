@@ -157,8 +157,13 @@ MQ_API int mq_state_download(mq_state* st, int rep, void* host, int64_t bytes);
 MQ_API int mq_metropolis(mq_state* st, int nsweeps, uint64_t seed, uint64_t sweep0);
 /* nclusters single-cluster Wolff moves of every replica; replaces EMCS's
  * cluster loop (emcs.h:83-89) with Wolff::move (wolff.h:48-125, Ising.h:122-169,
- * Heisenberg.h:191-243, Potts.h:262-322).  sizes (host, may be NULL) receives
- * nrep*nclusters cluster sizes, replica-major. */
+ * Heisenberg.h:191-243, Potts.h:262-322).  Cluster c is cluster number
+ * cluster0 + c (refused from 2^32 on): its seed site and embedding come from
+ * Philox counter (0, number, replica, 3), and every test from a member to a
+ * neighbour has its own uniform, as the reference draws rng.real() per test, so
+ * the cluster (the sites reachable from the seed along passing tests) depends
+ * on the state and the number only, not on launch shapes.  sizes (host, may be
+ * NULL) receives nrep*nclusters cluster sizes, replica-major. */
 MQ_API int mq_wolff(mq_state* st, int nclusters, uint64_t seed, uint64_t cluster0, int64_t* sizes);
 /* one EMCS (emcs.h:69-109): ncluster Wolff moves then nmetro sweeps; writes
  * the per-replica mean cluster size (NaN when ncluster = 0, as emcs.h:108).
