@@ -35,23 +35,31 @@
 // (BimodalPDF, src/lattice/util/prob_density.h:25-35), one disorder sample per
 // repid shared by all temperatures.
 //
-// Checkpoints: after its game loop every GPU batch writes
-// <out>/<L>/checkpoint_g<group>.bin (the spins of every simulation in the
-// reference's site order, the EMCS count, seeds and the plan).  When the file is
-// present the next run of the same configuration resumes, as MARQOV resumes from
-// an existing HDF5 dump (core.h:543-607, 650-700): it loads the spins, skips the
-// warm-up (wrmploop runs only at step 0, core.h:1034-1050) and appends another
-// game loop as the next block.  Philox counters are keyed by the EMCS number, so a
-// resumed trajectory is exactly the one an uninterrupted run would have made.
-// --fresh ignores and overwrites existing checkpoints.
+// Run files: every simulation has the HDF5 file <out>/<L>/<name>.h5 in MARQOV's
+// layout (runfile.hpp; core.h:650-917): each run of the configuration adds a step
+// /step<N> with the configuration, the observable time series of its game loop
+// and the final state (spins and the generator's words).  When every simulation of
+// a GPU batch has such a file, the next run continues from their last step, as
+// MARQOV does (core.h:543-607, 650-700): it loads the spins, skips the warm-up
+// (wrmploop runs only at step 0, core.h:1034-1050) and records its game loop as
+// the next step.  Philox counters are keyed by the EMCS number, which the state
+// stores, so a continued trajectory is exactly the one an uninterrupted run would
+// have made.  --fresh starts again and overwrites the files.
 //
-// Output (HDF5 is not part of this path): one text file per simulation,
-// <out>/<L>/beta<beta>_<repid>.txt (the reference's outpath and defaultfilter
-// name, util/filters.h:28-37, with .txt for .h5), holding a commented header
-// and one row per measurement: step, E/N, |m|, m^2, m^4, mean cluster size of
-// that EMCS, and for Ising / Phi4 on regular lattices the MagFTComp columns of
-// their reference observable tuples (mq_measure_ft).  <out>/<L>/summary.txt
-// holds per-simulation means.
+// Observables in the run files are the reference tuples' (Ising.h:82-94: m, e,
+// magft0, magft1 and the constant vector test series "dummy"; Heisenberg.h:88-90
+// and Potts.h:143-144: m; Phi4.h:133-140: m and one magft<d> per lattice axis),
+// MagFTComp on regular lattices only (graphs carry no coordinates here).  The
+// Edwards-Anderson tuple (EdwardsAndersonIsing.h:313-317: running averages of
+// two-site correlations over the whole run) is not computed on this path; its
+// files hold e and m.
+//
+// Text output: one file per simulation, <out>/<L>/<name>.txt (the reference's
+// outpath and defaultfilter name, util/filters.h:28-37), holding a commented
+// header and one row per measurement: step, E/N, |m|, m^2, m^4, mean cluster
+// size of that EMCS, and for Ising / Phi4 on regular lattices the MagFTComp
+// columns (mq_measure_ft); a continued run appends its rows as the next block.
+// <out>/<L>/summary.txt holds per-simulation means.
 #include <sys/stat.h>
 
 #include <cerrno>
@@ -69,6 +77,11 @@
 #include "lattices.hpp"
 #include "marqov_amd.hpp"
 #include "registry.hpp"
+#include "runfile.hpp"
+
+#ifndef MQ_DRIVER_VERSION
+#define MQ_DRIVER_VERSION "unversioned"
+#endif
 
 using namespace marqov_amd;
 
@@ -212,75 +225,6 @@ struct Plan {
     std::string outpath;
 };
 
-// ---- checkpoints
-constexpr uint64_t CKPT_MAGIC = 0x3130544B50434D51ull;  // "QMCPKT01"
-
-struct CkptHeader {
-    uint64_t magic;
-    int32_t model, q, dim, L, nrep, per_site, block, lattice;
-    uint64_t base_seed, seed, step;
-    double J, lambda, mass;
-};
-
-bool file_exists(const std::string& p)
-{
-    struct stat sb;
-    return stat(p.c_str(), &sb) == 0;
-}
-
-int lattice_code(const std::string& l)
-{
-    return l == "regular" ? 0 : (l == "csr-hypercube" ? 1 : (l == "rgg" ? 2 : 3));
-}
-
-void write_ckpt(const std::string& path, const CkptHeader& h, const std::vector<Sim>& sims,
-                const std::vector<std::vector<char>>& spins)
-{
-    const std::string tmp = path + ".tmp";
-    FILE* fp = std::fopen(tmp.c_str(), "wb");
-    if (!fp) throw std::runtime_error("cannot write " + tmp);
-    bool ok = std::fwrite(&h, sizeof(h), 1, fp) == 1;
-    for (const Sim& x : sims) {
-        ok = ok && std::fwrite(&x.p.beta, sizeof(double), 1, fp) == 1;
-        ok = ok && std::fwrite(&x.repid, sizeof(int), 1, fp) == 1;
-    }
-    for (const auto& v : spins) ok = ok && std::fwrite(v.data(), 1, v.size(), fp) == v.size();
-    ok = (std::fclose(fp) == 0) && ok;
-    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) throw std::runtime_error("cannot write " + path);
-}
-
-// Reads a checkpoint and checks that it belongs to this batch.
-CkptHeader read_ckpt(const std::string& path, const CkptHeader& want, const std::vector<Sim>& sims, std::size_t bytes,
-                     std::vector<std::vector<char>>& spins)
-{
-    FILE* fp = std::fopen(path.c_str(), "rb");
-    if (!fp) throw std::runtime_error("cannot read " + path);
-    CkptHeader h{};
-    bool ok = std::fread(&h, sizeof(h), 1, fp) == 1 && h.magic == CKPT_MAGIC;
-    auto mismatch = [&](const std::string& what) {
-        std::fclose(fp);
-        throw std::runtime_error(path + " does not belong to this configuration (" + what +
-                                 "); remove it or run with --fresh");
-    };
-    if (!ok) mismatch("not a checkpoint");
-    if (h.model != want.model || h.q != want.q || h.dim != want.dim || h.L != want.L || h.nrep != want.nrep ||
-        h.per_site != want.per_site || h.lattice != want.lattice || h.J != want.J || h.lambda != want.lambda ||
-        h.mass != want.mass)
-        mismatch("model, lattice or simulations differ");
-    if (h.base_seed != want.base_seed) mismatch("written with --seed " + std::to_string(h.base_seed));
-    for (const Sim& x : sims) {
-        double b;
-        int rep;
-        if (std::fread(&b, sizeof(double), 1, fp) != 1 || std::fread(&rep, sizeof(int), 1, fp) != 1) mismatch("short file");
-        if (b != x.p.beta || rep != x.repid) mismatch("temperatures or replicas differ");
-    }
-    spins.assign(sims.size(), std::vector<char>(bytes));
-    for (auto& v : spins)
-        if (std::fread(v.data(), 1, bytes, fp) != bytes) mismatch("short file");
-    std::fclose(fp);
-    return h;
-}
-
 void print_usage()
 {
     std::fprintf(stderr,
@@ -337,6 +281,75 @@ void write_sim(const Plan& pl, const Spec& s, const Sim& sim, uint64_t seed, con
         write_row(fp, (unsigned long long)(pl.nwarm + t), &obs[((size_t)t * nrep + r) * MQ_NOBS], cl[(size_t)t * nrep + r],
                   &ft[((size_t)t * nrep + r) * 3], nft);
     std::fclose(fp);
+}
+
+// ---- run files (runfile.hpp)
+constexpr const char* RNG_NAME = "Philox4x32-10";
+
+// the reference Hamiltonians' `name` members
+std::string model_name(const Spec& s)
+{
+    switch (s.model) {
+    case MQ_ISING: return "Ising";
+    case MQ_HEISENBERG: return "Heisenberg";
+    case MQ_PHI4: return "Phi4";
+    case MQ_POTTS: return "Potts";
+    default: return "EdwardsAndersonIsing";
+    }
+}
+
+runfile::Meta run_meta(const Plan& pl, const Spec& s, const Sim& sim, uint64_t base_seed, int nwarm_run)
+{
+    runfile::Meta m;
+    m.beta = sim.p.beta;
+    m.id = sim.id;
+    m.repid = sim.repid;
+    m.seed = base_seed;
+    m.nsteps = nwarm_run + pl.nmeas;  // the EMCS this run makes
+    m.warmupsteps = nwarm_run;
+    m.gameloopsteps = pl.nmeas;
+    m.ncluster = pl.ncluster;
+    m.nmetro = pl.nmetro;
+    // the reference's lattice writers: RegularHypercubic for hypercubes (whichever
+    // kernels run them), GraphFromCSV for --graph files, "unknown" otherwise
+    m.lattice = (pl.lattice == "regular" || pl.lattice == "csr-hypercube") ? "RegularHypercubic"
+                : (pl.lattice == "csv" ? "GraphFromCSV" : "unknown");
+    m.dim = pl.dim;
+    m.npoints = pl.nsites;
+    m.model = model_name(s);
+    if (s.model == MQ_PHI4)  // Phi4::paramname (Phi4.h:152-163)
+        m.params = {{"beta", sim.p.beta}, {"lambda", sim.p.lambda}, {"mass", sim.p.mass}};
+    else
+        m.params = {{"param0", sim.p.J}};
+    m.version = MQ_DRIVER_VERSION;
+    return m;
+}
+
+// the observable series of simulation r of a batch over the game loop
+std::vector<runfile::Series> run_series(const Plan& pl, const Spec& s, const std::vector<double>& obs,
+                                        const std::vector<double>& ft, int nrep, int r)
+{
+    auto col = [&](int k) {
+        std::vector<double> v(pl.nmeas);
+        for (int t = 0; t < pl.nmeas; ++t) v[t] = obs[((size_t)t * nrep + r) * MQ_NOBS + k];
+        return v;
+    };
+    std::vector<runfile::Series> out;
+    out.push_back({"m", "magnetization", 1, col(1)});
+    if (s.model == MQ_ISING || s.model == MQ_EA_ISING) out.push_back({"e", "", 1, col(0)});
+    const int nft = ft_columns(pl, s);
+    for (int d = 0; d < nft; ++d) {
+        std::vector<double> v(pl.nmeas);
+        for (int t = 0; t < pl.nmeas; ++t) v[t] = ft[((size_t)t * nrep + r) * 3 + d];
+        out.push_back({"magft" + std::to_string(d), "Fourier Component of Magnetization", 1, v});
+    }
+    if (s.model == MQ_ISING) {  // IsingGenericVectorValuedObs (Ising.h:35-48): 42 + 0.1 k, k < 5
+        runfile::Series d{"dummy", "testing vector-valued observables ...", 5, {}};
+        for (int t = 0; t < pl.nmeas; ++t)
+            for (int k = 0; k < 5; ++k) d.values.push_back(42 + 0.1 * k);
+        out.push_back(d);
+    }
+    return out;
 }
 
 }  // namespace
@@ -527,19 +540,66 @@ int main(int argc, char** argv)
                 const int per_site = (s.model == MQ_HEISENBERG || s.model == MQ_PHI4) ? 3 : 1;
                 const std::size_t nsites = reglat ? reglat->size() : gl[0]->size();
                 const std::size_t bytes = nsites * (per_site == 3 ? 3 * sizeof(float) : 1);
-                const int32_t lsize = pl.lattice == "csv" ? (int32_t)pl.nsites : pl.L;
-                CkptHeader ck{CKPT_MAGIC, s.model, s.q, pl.dim, lsize, nr, per_site, 0, lattice_code(pl.lattice),
-                              o.seed, seed, 0, sims[0].p.J, sims[0].p.lambda, sims[0].p.mass};
-                const std::string ckpath = pl.outpath + "checkpoint_g" + std::to_string(g) + ".bin";
-                const bool resume = !o.fresh && file_exists(ckpath);
+                std::vector<std::string> names(nr), h5paths(nr);
+                int have = 0;
+                for (int r = 0; r < nr; ++r) {
+                    names[r] = sim_name(sims[r].p.beta, sims[r].repid, (int)g, (int)pl.groups.size());
+                    h5paths[r] = pl.outpath + names[r] + ".h5";
+                    if (!o.fresh && h5store::is_hdf5(h5paths[r])) ++have;
+                }
+                if (have > 0 && have < nr)
+                    throw std::runtime_error("only " + std::to_string(have) + " of the " + std::to_string(nr) +
+                                             " simulations of group " + std::to_string(g) + " in " + pl.outpath +
+                                             " have run files; remove them or run with --fresh");
+                const bool resume = have == nr;
                 uint64_t step0 = 0;
                 int block = 0;
-                if (resume) {
-                    std::vector<std::vector<char>> sp;
-                    const CkptHeader h = read_ckpt(ckpath, ck, sims, bytes, sp);
-                    step0 = h.step;
-                    block = h.block + 1;
-                    for (int r = 0; r < nr; ++r) b->upload(r, sp[r]);
+                if (resume) {  // continue from the last step of every simulation's run file
+                    for (int r = 0; r < nr; ++r) {
+                        const std::string& path = h5paths[r];
+                        auto mismatch = [&](const std::string& what) {
+                            throw std::runtime_error(path + " does not belong to this configuration (" + what +
+                                                     "); remove it or run with --fresh");
+                        };
+                        const runfile::Last last = runfile::read_last(path, nsites, per_site, per_site == 3);
+                        const runfile::Meta want = run_meta(pl, s, sims[r], o.seed, 0);
+                        if (last.state.rng != RNG_NAME || last.state.rngstate.size() != 3)
+                            mismatch("state written by another generator");
+                        if (last.state.rngstate[0] != o.seed)
+                            mismatch("written with --seed " + std::to_string(last.state.rngstate[0]));
+                        if (last.state.rngstate[1] != seed) mismatch("batch key differs");
+                        if (last.meta.model != want.model || last.meta.lattice != want.lattice ||
+                            last.meta.npoints != want.npoints)
+                            mismatch("model or lattice differ");
+                        if (last.meta.beta != sims[r].p.beta || last.meta.repid != sims[r].repid)
+                            mismatch("temperature or replica differ");
+                        for (const auto& pr : want.params)
+                            if (runfile::read_param(path, last.step, pr.first) != pr.second)
+                                mismatch("Hamiltonian parameter " + pr.first + " differs");
+                        if (r == 0) {
+                            block = last.step + 1;
+                            step0 = last.state.rngstate[2];
+                        } else if (last.step + 1 != block || last.state.rngstate[2] != step0) {
+                            mismatch("its last step differs from the other simulations' of the batch");
+                        }
+                        std::vector<char> sp(bytes);
+                        if (per_site == 3) {
+                            float* f = reinterpret_cast<float*>(sp.data());
+                            for (std::size_t k = 0; k < nsites * 3; ++k) {
+                                const double v = last.state.reals[k];
+                                if (!std::isfinite(v)) mismatch("spins are not finite");
+                                f[k] = (float)v;
+                            }
+                        } else {
+                            const int lo = s.model == MQ_POTTS ? 0 : -1, hi = s.model == MQ_POTTS ? s.q - 1 : 1;
+                            for (std::size_t k = 0; k < nsites; ++k) {
+                                const int32_t v = last.state.ints[k];
+                                if (v < lo || v > hi || (s.model != MQ_POTTS && v == 0)) mismatch("spin values out of range");
+                                sp[k] = (char)(int8_t)v;
+                            }
+                        }
+                        b->upload(r, sp);
+                    }
                 } else {
                     b->init(seed);
                 }
@@ -598,19 +658,27 @@ int main(int argc, char** argv)
                 const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 std::printf("| group %d: %d simulations, %llu EMCS each, %.2f s\n", (int)g, nr,
                             (unsigned long long)(emcs.steps() - step0), secs);
-                {
-                    std::vector<std::vector<char>> sp(nr);
-                    for (int r = 0; r < nr; ++r) {
-                        sp[r].resize(bytes);
-                        ctx.check(mq_state_download(b->get(), r, sp[r].data(), (int64_t)bytes), "mq_state_download");
+                for (int r = 0; r < nr; ++r) {  // the step of each simulation's run file
+                    std::vector<char> sp(bytes);
+                    ctx.check(mq_state_download(b->get(), r, sp.data(), (int64_t)bytes), "mq_state_download");
+                    runfile::State st;
+                    st.rng = RNG_NAME;
+                    st.rngstate = {o.seed, seed, emcs.steps()};
+                    st.per_site = per_site;
+                    st.vector = per_site == 3;
+                    if (st.vector) {
+                        const float* f = reinterpret_cast<const float*>(sp.data());
+                        st.reals.assign(f, f + nsites * 3);
+                    } else {
+                        st.ints.resize(nsites);
+                        for (std::size_t k = 0; k < nsites; ++k) st.ints[k] = (int8_t)sp[k];
                     }
-                    ck.step = emcs.steps();
-                    ck.block = block;
-                    write_ckpt(ckpath, ck, sims, sp);
+                    runfile::write_step(h5paths[r], block, run_meta(pl, s, sims[r], o.seed, resume ? 0 : pl.nwarm),
+                                        run_series(pl, s, obs, ft, nr, r), st);
                 }
                 for (int r = 0; r < nr; ++r) {
                     const Sim& x = sims[r];
-                    const std::string name = sim_name(x.p.beta, x.repid, (int)g, (int)pl.groups.size());
+                    const std::string& name = names[r];
                     write_sim(pl, s, x, seed, obs, cl, ft, nr, r, pl.outpath + name + ".txt", block, meas0);
                     double se = 0, se2 = 0, sm = 0, sm2 = 0, sm4 = 0, sc = 0;
                     for (int t = 0; t < pl.nmeas; ++t) {

@@ -8,6 +8,8 @@ import subprocess
 import numpy as np
 import pytest
 
+import h5tools
+
 import exact_path  # noqa: F401
 import exact
 
@@ -155,6 +157,33 @@ def _series(path):
     return hdr, data
 
 
+# observables of the reference tuples in the run files (Ising.h:82-94, Heisenberg.h:88-90,
+# Potts.h:143-144, Phi4.h:133-140); the EA files hold e and m
+RUNFILE_OBS = {"Ising": ["m", "e"], "IsingRGG": ["m", "e"], "Heisenberg": ["m"], "Phi4": ["m"], "Potts": ["m"],
+               "EdwardsAnderson-Ising": ["m", "e"]}
+
+
+def _check_runfile(h5, data, ham, nft, steps=1):
+    """The HDF5 run file of a simulation (read with h5ls / h5dump): steps /step0 ..
+    /step<steps-1>, and the observable series of the last step equal, bit for bit, the
+    text file's columns for that step's rows (data); returns the last step's spins."""
+    if not h5tools.available():
+        pytest.skip("HDF5 tools absent")
+    t = h5tools.tree(h5)
+    assert sorted(k for k in t if k.count("/") == 1) == [f"/step{s}" for s in range(steps)]
+    st = f"/step{steps - 1}"
+    names = RUNFILE_OBS[ham] + [f"magft{d}" for d in range(nft)] + (["dummy"] if ham in ("Ising", "IsingRGG") else [])
+    assert sorted(k.rsplit("/", 1)[1] for k in t if k.startswith(st + "/observables/")) == sorted(names)
+    cols = {"e": 1, "m": 2}
+    for d in range(nft):
+        cols[f"magft{d}"] = 6 + d
+    for n in names:
+        if n in cols:
+            assert np.array_equal(h5tools.values(h5, f"{st}/observables/{n}"), data[:, cols[n]]), (h5, n)
+    assert h5tools.string(h5, f"{st}/state/RNG") == "Philox4x32-10"
+    return h5tools.values(h5, f"{st}/state/hamiltonianstatespace")
+
+
 def _hypercube_graph(dim, L):
     out = subprocess.run([LCHK, "cube", str(dim), str(L)], capture_output=True, text=True, check=True).stdout
     rp, ci = _read_csr(out.splitlines(), 1)
@@ -193,6 +222,9 @@ def test_gpu_ising_ini_vs_exact(tmp_path):
                 hdr, d = _series(tmp_path / str(L) / f"beta{beta:.6f}_{rep}.txt")
                 # the 4x4 torus runs on the checkerboard kernels and carries the MagFTComp columns
                 assert d.shape == (4000, 8 if L == 4 else 6), d.shape
+                if rep < 2:
+                    spins = _check_runfile(tmp_path / str(L) / f"beta{beta:.6f}_{rep}.h5", d, "Ising", 2 if L == 4 else 0)
+                    assert spins.shape == (L * L,) and set(np.unique(spins)) <= {-1.0, 1.0}
                 assert np.allclose(d[:, 0], np.arange(200, 4200))
                 assert np.all(d[:, 5] >= 1.0)
                 e_rep.append(d[:, 1].mean())
@@ -230,6 +262,8 @@ def test_gpu_ini_models_run(tmp_path, ham, dirs):
                 # MagFTComp columns of the reference's Ising / Phi4 observable tuples
                 nft = {"Phi4": x["dim"]}.get(ham, 0) if x["lattice"] == "regular" else 0
                 assert d.shape == (x["measure"], 6 + nft)
+                spins = _check_runfile(tmp_path / dname / f"{s['name']}.h5", d, ham, nft)
+                assert spins.size == x["nsites"] * (3 if ham in ("Heisenberg", "Phi4") else 1)
                 assert np.all(np.isfinite(d[:, 1:5]))
                 assert np.all((d[:, 6:] >= 0) & (d[:, 6:] <= 1))
                 assert hdr[-1].split()[7:] == [f"magft{k}" for k in range(nft)]
@@ -259,10 +293,12 @@ def _rows(path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("ham,Ls", [("Ising", ["8", "5"]), ("Heisenberg", ["6"])])
 def test_gpu_ini_resume_is_exact(tmp_path, ham, Ls):
-    """Checkpoint/resume: a run of 2 x 200 measured EMCS, stopped after the first 200
-    and resumed from the checkpoint (no second warm-up, next block appended), writes
-    exactly the rows of one uninterrupted run of 400 measured EMCS: the Philox
-    counters are keyed by the EMCS number, so the resumed trajectory is the same."""
+    """Resume from the HDF5 run files: a run of 2 x 200 measured EMCS, stopped after the
+    first 200 and continued from the last step of every simulation's run file (no second
+    warm-up, the game loop recorded as /step1 and appended to the text file as the next
+    block), writes exactly the rows of one uninterrupted run of 400 measured EMCS, and
+    ends in the same spins: the Philox counters are keyed by the EMCS number, which the
+    run file's state stores (core.h:543-607)."""
     once, twice = tmp_path / "once", tmp_path / "twice"
     for d, out in (("config_resume_once", once), ("config_resume_twice", twice)):
         conf = os.path.join(ROOT, "tests", "data", d)
@@ -282,6 +318,21 @@ def test_gpu_ini_resume_is_exact(tmp_path, ham, Ls):
                 assert len(a) == 400 and not ablocks
                 assert len(bblocks) == 1 and "resumed after" in bblocks[0]
                 assert a == b, (L, s["name"])
+                # run files: one step for the single run, two for the continued one; the
+                # continued run's series and final spins are the single run's
+                da = np.loadtxt(once / L / f"{s['name']}.txt", ndmin=2)
+                db = np.loadtxt(twice / L / f"{s['name']}.txt", ndmin=2)
+                nft = da.shape[1] - 6
+                sa = _check_runfile(once / L / f"{s['name']}.h5", da, ham, nft, steps=1)
+                _check_runfile(twice / L / f"{s['name']}.h5", db[200:], ham, nft, steps=2)
+                h0 = h5tools.values(twice / L / f"{s['name']}.h5", "/step0/observables/m")
+                assert np.array_equal(h0, db[:200, 2])
+                sb = h5tools.values(twice / L / f"{s['name']}.h5", "/step1/state/hamiltonianstatespace")
+                assert np.array_equal(sa, sb), (L, s["name"])
+                wa = h5tools.values(once / L / f"{s['name']}.h5", "/step0/state/rngstate", integer=True)
+                wb = h5tools.values(twice / L / f"{s['name']}.h5", "/step1/state/rngstate", integer=True)
+                assert wa == wb and wa[0] == 11
+                assert h5tools.values(twice / L / f"{s['name']}.h5", "/step1/config/marqovconfig/warmupsteps")[0] == 0
                 nfiles += 1
     assert nfiles > 0
     # a different seed does not silently continue somebody else's chain
