@@ -319,14 +319,22 @@ def leg_c4(gpu, ctx):
         st.init(seed=4)
         st.run_graph(nsw, nt, 4, 0, 0, steps, 1)
         ms, _ = timed(ctx, lambda: st.run_graph(nsw, nt, 4, nsw * steps, steps, steps, 4))
+        # the same steps captured in a HIP graph and replayed (mq_tuning.graph = 1)
+        st.set_tuning(graph=1)
+        st.run_graph(nsw, nt, 4, 2 * nsw * steps * 5, 5 * steps, steps, 1)
+        ms_g, _ = timed(ctx, lambda: st.run_graph(nsw, nt, 4, 2 * nsw * steps * 6, 6 * steps, steps, 4))
+        st.set_tuning(graph=0)
         acc = st.pt_swap(nt, 0, 4, 10 ** 6)
         res[f"{name}_attempts_per_ns"] = round(sites * nsw * steps * 4 / (ms * 1e6), 3)
+        res[f"{name}_hip_graph_attempts_per_ns"] = round(sites * nsw * steps * 4 / (ms_g * 1e6), 3)
         res[f"{name}_pt_accept_even_pairs"] = round(float(acc.sum()) / (len(acc) * 16), 3)
         # the same sweeps launched eagerly (no graph, no energy or swap): the sweep kernels alone
         ms_e, _ = timed(ctx, lambda: st.metropolis(nsw * steps, 4, sweep0=10 ** 6))
         res[f"{name}_eager_sweeps_attempts_per_ns"] = round(sites * nsw * steps / (ms_e * 1e6), 3)
         del st
-    res["graph"] = f"{steps} x ({nsw} sweeps + energy + PT swap) per replay, 4 replays"
+    res["graph"] = (f"mq_run_graph: {steps} x ({nsw} sweeps + energy + PT swap) per repetition, 4 repetitions, "
+                    "launched on the stream (the default); hip_graph: the same steps captured in one HIP graph "
+                    "and replayed")
     os.environ["MARQOV_AMD_KERNELS"] = "1"
     sq = gpu.State(ctx, "ea", [lats[24]], J=-1.0)
     del os.environ["MARQOV_AMD_KERNELS"]

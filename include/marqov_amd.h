@@ -248,6 +248,8 @@ MQ_API int mq_pt_get_temps(mq_state* st, int32_t* temp_of_rep);
  * per replica of the small-lattice kernel (power of two, 64..2048, default 512);
  * wolff_vset: clusters up to this many sites keep their membership in an LDS
  * hash set, larger ones in global memory (0..4096, default 4096).
+ * graph: how mq_run_graph issues its steps: 0 = automatic (the stream),
+ * 1 = captured into one HIP graph and replayed, 2 = launched on the stream.
  * Results do not depend on any of these. */
 typedef struct mq_tuning {
     int kernels;
@@ -257,14 +259,20 @@ typedef struct mq_tuning {
     int wolff_block;
     int wolff_ring;
     int wolff_vset;
+    int graph;
 } mq_tuning;
 MQ_API int mq_state_set_tuning(mq_state* st, const mq_tuning* t);
 MQ_API int mq_state_get_tuning(const mq_state* st, mq_tuning* t);
 
 /* ---- graphs ------------------------------------------------------------ */
-/* Capture `steps` repetitions of {nsweeps Metropolis sweeps; optional PT
- * swap with ntemp temperatures} into one HIP graph and replay it `repeats`
- * times; sweep and step counters advance per repetition. */
+/* `steps` repetitions of {nsweeps Metropolis sweeps; optional PT swap with
+ * ntemp temperatures}, run `repeats` times with no host synchronisation in
+ * between; sweep and step counters advance per repetition from device memory.
+ * mq_tuning.graph = 1 captures the repetitions into one HIP graph and replays
+ * it; the default launches the same kernels, in the same order and with the
+ * same arguments, on the state's stream (on MI355X with ROCm 7 a replayed
+ * kernel node costs about 2.7 us more than a stream launch).  Both give the
+ * same states bit for bit. */
 MQ_API int mq_run_graph(mq_state* st, int nsweeps, int ntemp, uint64_t seed, uint64_t sweep0, uint64_t step0,
                  int steps, int repeats);
 

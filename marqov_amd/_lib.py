@@ -19,7 +19,7 @@ MQ_NOBS = 4
 class Tuning(ctypes.Structure):
     _fields_ = [("kernels", ctypes.c_int), ("block", ctypes.c_int), ("xcd_remap", ctypes.c_int),
                 ("lds_block", ctypes.c_int), ("wolff_block", ctypes.c_int), ("wolff_ring", ctypes.c_int),
-                ("wolff_vset", ctypes.c_int)]
+                ("wolff_vset", ctypes.c_int), ("graph", ctypes.c_int)]
 
 
 class ModelParams(ctypes.Structure):

@@ -1200,6 +1200,7 @@ extern "C" int mq_state_set_tuning(mq_state* st, const mq_tuning* t)
         t->wolff_vset < 0 || t->wolff_vset > MQ_VSET / 2)
         return mq_fail(st->ctx, MQ_ERR_ARG,
                        "tuning: wolff_block 0, 256, 512 or 1024; wolff_ring a power of two in 64..2048; wolff_vset in 0..4096");
+    if (t->graph < 0 || t->graph > 2) return mq_fail(st->ctx, MQ_ERR_ARG, "tuning: graph in 0..2");
     st->kernel_choice = t->kernels;
     st->tune_block = t->block;
     st->tune_xcd = t->xcd_remap;
@@ -1207,6 +1208,7 @@ extern "C" int mq_state_set_tuning(mq_state* st, const mq_tuning* t)
     st->wolff_block = t->wolff_block;
     st->wolff_ring = t->wolff_ring;
     st->wolff_vset = t->wolff_vset;
+    st->step_graph = t->graph;
     return MQ_OK;
 }
 
@@ -1221,5 +1223,6 @@ extern "C" int mq_state_get_tuning(const mq_state* st, mq_tuning* t)
     t->wolff_block = st->wolff_block;
     t->wolff_ring = st->wolff_ring;
     t->wolff_vset = st->wolff_vset;
+    t->graph = st->step_graph;
     return MQ_OK;
 }

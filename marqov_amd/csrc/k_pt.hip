@@ -48,10 +48,14 @@ __device__ __forceinline__ double mh_ratio(double en)
     return 0.0;
 }
 
+// table words exchanged per round trip (2*tbl = 26 for 3D lattices)
+constexpr int PT_CHUNK = 32;
+
 // One offered exchange between temperature slots ka < kb of group g: the
 // replicas a, b holding them swap temperatures with probability
 // min(1, calcMHratio((beta_ka - beta_kb)(E_a - E_b))); the uniform is the
 // 53-bit Philox draw keyed by (ka, step, g).
+
 __device__ __forceinline__ void pt_offer(int g, int ntemp, int ka, int kb, uint32_t step, Key2 key,
                                          int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
                                          const double* __restrict__ energy, const double* __restrict__ ladder,
@@ -65,24 +69,39 @@ __device__ __forceinline__ void pt_offer(int g, int ntemp, int ka, int kb, uint3
     const uint4 w = philox4x32_10((uint32_t)ka, step, (uint32_t)g, ST_PT, key);
     const double u = (double)((((uint64_t)w.x << 32) | w.y) >> 11) * 0x1.0p-53;
     if (!(u < fmin(1.0, ratio))) return;
+    // Exchange the two replicas' temperature data.  Every value is loaded before any
+    // is stored, PT_CHUNK table words at a time: the rows share one array, so loads
+    // placed after stores would wait for them, one memory round trip per word.
+    const uint32_t wa = wthr[a], wb = wthr[b];
+    const float fa = betaf[a], fb = betaf[b];
+    const double da = betad[a], db = betad[b];
+    uint32_t* ra = thr + (int64_t)a * 2 * tbl;
+    uint32_t* rb = thr + (int64_t)b * 2 * tbl;
+    for (int e0 = 0; e0 < 2 * tbl; e0 += PT_CHUNK) {
+        uint32_t xa[PT_CHUNK], xb[PT_CHUNK];
+#pragma unroll
+        for (int j = 0; j < PT_CHUNK; ++j) {
+            xa[j] = e0 + j < 2 * tbl ? ra[e0 + j] : 0u;
+            xb[j] = e0 + j < 2 * tbl ? rb[e0 + j] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < PT_CHUNK; ++j) {
+            if (e0 + j < 2 * tbl) {
+                ra[e0 + j] = xb[j];
+                rb[e0 + j] = xa[j];
+            }
+        }
+    }
     rep_of_temp[sa] = b;
     rep_of_temp[sb] = a;
     temp_of_rep[a] = kb;
     temp_of_rep[b] = ka;
-    for (int e = 0; e < 2 * tbl; ++e) {
-        const uint32_t x = thr[(int64_t)a * 2 * tbl + e];
-        thr[(int64_t)a * 2 * tbl + e] = thr[(int64_t)b * 2 * tbl + e];
-        thr[(int64_t)b * 2 * tbl + e] = x;
-    }
-    const uint32_t ww = wthr[a];
-    wthr[a] = wthr[b];
-    wthr[b] = ww;
-    const float bf = betaf[a];
-    betaf[a] = betaf[b];
-    betaf[b] = bf;
-    const double bd = betad[a];
-    betad[a] = betad[b];
-    betad[b] = bd;
+    wthr[a] = wb;
+    wthr[b] = wa;
+    betaf[a] = fb;
+    betaf[b] = fa;
+    betad[a] = db;
+    betad[b] = da;
     atomicAdd(acc + g, 1);
 }
 
@@ -156,10 +175,11 @@ hipError_t energy_async(mq_state* st)
 {
     hipStream_t s = st->ctx->stream;
     hipError_t e;
-    if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
-    if ((e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
-    if ((e = launch_measure_all(st, st->d_isum, st->d_fsum)) != hipSuccess) return e;
     const int vec = (st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4);
+    if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
+    // the FP32 sums exist for O(3)/Phi4 spins only; integer models neither write nor read them
+    if (vec && (e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
+    if ((e = launch_measure_all(st, st->d_isum, st->d_fsum)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_energy, dim3((unsigned)((st->nrep + 255) / 256)), dim3(256), 0, s, st->d_isum, st->d_fsum,
                        st->nrep, vec, st->p.model == MQ_POTTS, st->p.J, st->d_energy);
     return hipGetLastError();
@@ -333,25 +353,41 @@ extern "C" int mq_run_graph(mq_state* st, int nsweeps, int ntemp, uint64_t seed,
     if (ntemp > 1) MQ_HIP(st->ctx, zero_fill(st->d_pt_acc, (st->nrep + 1) / 2, s));
     MQ_HIP(st->ctx, hipStreamSynchronize(s));
     const Key2 key{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    // One repetition of the steps: every launch reads its sweep and step offsets
+    // from d_dyn, which the last launch advances, so issuing it again (or replaying
+    // its capture) continues the counters.
+    hipError_t e = hipSuccess;
+    auto issue = [&]() -> int {
+        st->cur_dyn = st->d_dyn;
+        int r = MQ_OK;
+        for (int t = 0; t < steps && r == MQ_OK && e == hipSuccess; ++t) {
+            r = mq_metropolis_async(st, nsweeps, seed, sweep0 + (uint64_t)t * nsweeps);
+            if (r == MQ_OK && ntemp > 1) {
+                e = energy_async(st);
+                if (e == hipSuccess) e = pt_swap_async(st, ntemp, -1, (uint32_t)(step0 + t), key);
+            }
+        }
+        if (r == MQ_OK && e == hipSuccess) {
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, st->d_dyn, (uint32_t)(per_sweep * nsweeps * steps),
+                               (uint32_t)steps);
+            e = hipGetLastError();
+        }
+        st->cur_dyn = nullptr;
+        return r;
+    };
+    if (st->step_graph != 1) {  // on the stream
+        int rc = MQ_OK;
+        for (int i = 0; i < repeats && rc == MQ_OK && e == hipSuccess; ++i) rc = issue();
+        if (rc != MQ_OK) return rc;
+        hipError_t es = hipStreamSynchronize(s);
+        if (e != hipSuccess || es != hipSuccess)
+            return mq_fail(st->ctx, MQ_ERR_HIP, std::string("step launch: ") + hipGetErrorString(e != hipSuccess ? e : es));
+        return MQ_OK;
+    }
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     MQ_HIP(st->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    st->cur_dyn = st->d_dyn;
-    int rc = MQ_OK;
-    hipError_t e = hipSuccess;
-    for (int t = 0; t < steps && rc == MQ_OK && e == hipSuccess; ++t) {
-        rc = mq_metropolis_async(st, nsweeps, seed, sweep0 + (uint64_t)t * nsweeps);
-        if (rc == MQ_OK && ntemp > 1) {
-            e = energy_async(st);
-            if (e == hipSuccess) e = pt_swap_async(st, ntemp, -1, (uint32_t)(step0 + t), key);
-        }
-    }
-    if (rc == MQ_OK && e == hipSuccess) {
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, st->d_dyn, (uint32_t)(per_sweep * nsweeps * steps),
-                           (uint32_t)steps);
-        e = hipGetLastError();
-    }
-    st->cur_dyn = nullptr;
+    int rc = issue();
     hipError_t ec = hipStreamEndCapture(s, &graph);
     if (rc != MQ_OK) {
         if (graph) (void)hipGraphDestroy(graph);

@@ -584,6 +584,7 @@ struct mq_state {
     int wolff_ring = 512;               // LDS frontier ring entries of the small-lattice Wolff kernel (power of two, 64..2048)
     int wolff_block = 0;                // threads per replica of the global-memory Wolff kernels (0 = automatic)
     int wolff_vset = MQ_VSET / 2;       // Wolff on large lattices: clusters up to this size keep their membership in LDS (0..MQ_VSET/2)
+    int step_graph = 0;                 // mq_run_graph: 0 automatic (stream), 1 HIP graph, 2 stream
 };
 
 // --------------------------------------------------------------------------

@@ -72,11 +72,14 @@ def test_eabits_measure(ctx, Ls, words):
             assert obs[r, 1] == pytest.approx(abs(s.sum()) / N, abs=1e-12), r
 
 
-def test_eabits_pt_and_graph(ctx):
+@pytest.mark.parametrize("mode", [1, 2], ids=["hip-graph", "stream"])
+def test_eabits_pt_and_graph(ctx, mode):
     """PT swaps move temperatures between bits (the per-bit threshold words are
-    rebuilt), and a graph of {sweeps, PT} equals the eager sequence."""
+    rebuilt), and mq_run_graph's {sweeps, PT} steps equal the eager sequence, captured
+    in a HIP graph or launched on the stream."""
     L = 4
     g, reps = _state(ctx, [L], words_per_L=2)
+    g.set_tuning(graph=mode)
     e, _ = _state(ctx, [L], words_per_L=2)
     nsw, steps = 2, 3
     g.run_graph(nsw, NT, 21, 0, 0, steps, 2)

@@ -95,6 +95,8 @@ def test_calls_in_the_wrong_state_refused(ctx):
         st.renormalize()
     with pytest.raises(gpu.MarqovError, match="tuning"):
         st.set_tuning(block=100)
+    with pytest.raises(gpu.MarqovError, match="graph in 0..2"):
+        st.set_tuning(graph=3)
     ea = gpu.State(ctx, "ea", [lat] * 2)
     ea.set_beta(0.5)
     with pytest.raises(gpu.MarqovError, match="EA couplings not set"):

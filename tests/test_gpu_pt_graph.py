@@ -145,9 +145,12 @@ def test_pt_swaps_bit_exact(ctx):
     assert np.array_equal(b, BETAS[rp.tor])
 
 
-def test_graph_replays_match_eager(ctx):
-    """Each graph replay (with eager work and synchronisation in between) equals the eager run."""
+@pytest.mark.parametrize("mode", [1, 2], ids=["hip-graph", "stream"])
+def test_graph_replays_match_eager(ctx, mode):
+    """Each mq_run_graph call (with eager work and synchronisation in between) equals the
+    eager run, with the steps captured in a HIP graph and replayed or launched on the stream."""
     g = _ea_state(ctx)
+    g.set_tuning(graph=mode)
     e = _ea_state(ctx)
     nsw, steps = 2, 3
     done = 0
@@ -165,11 +168,13 @@ def test_graph_replays_match_eager(ctx):
         g.measure()
 
 
-def test_graph_sweeps_only_regular_ising(ctx):
+@pytest.mark.parametrize("mode", [1, 2], ids=["hip-graph", "stream"])
+def test_graph_sweeps_only_regular_ising(ctx, mode):
     from marqov_amd import gpu
     lat = gpu.RegularLattice(ctx, 3, 32)
     a = gpu.State(ctx, "ising", [lat, lat], J=-1.0)
     b = gpu.State(ctx, "ising", [lat, lat], J=-1.0)
+    a.set_tuning(graph=mode)
     for st in (a, b):
         st.set_beta([0.22, 0.3])
         st.init(seed=3)
