@@ -160,7 +160,7 @@ def _series(path):
 # observables of the reference tuples in the run files (Ising.h:82-94, Heisenberg.h:88-90,
 # Potts.h:143-144, Phi4.h:133-140); the EA files hold e and m
 RUNFILE_OBS = {"Ising": ["m", "e"], "IsingRGG": ["m", "e"], "Heisenberg": ["m"], "Phi4": ["m"], "Potts": ["m"],
-               "EdwardsAnderson-Ising": ["m", "e"]}
+               "Potts3": ["m"], "EdwardsAnderson-Ising": ["m", "e"]}
 
 
 def _check_runfile(h5, data, ham, nft, steps=1):
@@ -291,7 +291,8 @@ def _rows(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ham,Ls", [("Ising", ["8", "5"]), ("Heisenberg", ["6"])])
+@pytest.mark.parametrize("ham,Ls", [("Ising", ["8", "5"]), ("Heisenberg", ["6"]), ("Phi4", ["6"]), ("Potts3", ["4"]),
+                                    ("EdwardsAnderson-Ising", ["4"])])
 def test_gpu_ini_resume_is_exact(tmp_path, ham, Ls):
     """Resume from the HDF5 run files: a run of 2 x 200 measured EMCS, stopped after the
     first 200 and continued from the last step of every simulation's run file (no second
