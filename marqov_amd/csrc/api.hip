@@ -456,7 +456,6 @@ extern "C" int mq_state_create(mq_ctx* ctx, const mq_model_params* params, int n
     if ((e = dalloc(&st->d_isum, (size_t)nrep * 8)) != hipSuccess) return fail(e);
     if ((e = dalloc(&st->d_fsum, (size_t)nrep * 8)) != hipSuccess) return fail(e);
     if ((e = dalloc(&st->d_rep_bond_off, (size_t)nrep)) != hipSuccess) return fail(e);
-    if ((e = dalloc(&st->d_energy, (size_t)nrep)) != hipSuccess) return fail(e);
     if ((e = hipMemsetAsync(st->d_spins, 0, (size_t)off, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipMemsetAsync(st->d_rep_bond_off, 0, sizeof(int64_t) * nrep, ctx->stream)) != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
@@ -489,7 +488,6 @@ extern "C" void mq_state_destroy(mq_state* st)
     dfree(st->d_rep_of_temp);
     dfree(st->d_pt_acc);
     dfree(st->d_pt_pairs);
-    dfree(st->d_energy);
     dfree(st->d_pack);
     if (st->h_pack) (void)hipHostFree(st->h_pack);
     dfree(st->d_ladder);

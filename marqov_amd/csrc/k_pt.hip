@@ -28,14 +28,18 @@ __global__ void k_zero64(unsigned long long* p, int64_t n)
     if (i < n) p[i] = 0ull;
 }
 
-// interaction energy J*I per replica from the measurement sums
-__global__ void k_energy(const long long* __restrict__ isum, const double* __restrict__ fsum, int nrep, int vec,
-                         int potts, double J, double* __restrict__ energy)
+// The interaction energy J*I of replica r, read by the swap from the measurement
+// sums (integer sums for int8/bit storage, FP32-accumulated doubles for O(3)/Phi4).
+struct EnergySrc {
+    const long long* isum;
+    const double* fsum;
+    int vec;
+    double J;
+};
+
+__device__ __forceinline__ double replica_energy(const EnergySrc& es, int r)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrep) return;
-    energy[r] = vec ? J * fsum[(int64_t)r * 8] : J * (double)isum[(int64_t)r * 8];
-    (void)potts;
+    return es.vec ? es.J * es.fsum[(int64_t)r * 8] : es.J * (double)es.isum[(int64_t)r * 8];
 }
 
 __device__ __forceinline__ double mh_ratio(double en)
@@ -58,13 +62,13 @@ constexpr int PT_CHUNK = 32;
 
 __device__ __forceinline__ void pt_offer(int g, int ntemp, int ka, int kb, uint32_t step, Key2 key,
                                          int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
-                                         const double* __restrict__ energy, const double* __restrict__ ladder,
+                                         const EnergySrc es, const double* __restrict__ ladder,
                                          uint32_t* __restrict__ thr, int tbl, uint32_t* __restrict__ wthr,
                                          float* __restrict__ betaf, double* __restrict__ betad, int32_t* __restrict__ acc)
 {
     const int64_t sa = (int64_t)g * ntemp + ka, sb = (int64_t)g * ntemp + kb;
     const int a = rep_of_temp[sa], b = rep_of_temp[sb];
-    const double en = (ladder[sa] - ladder[sb]) * (energy[a] - energy[b]);
+    const double en = (ladder[sa] - ladder[sb]) * (replica_energy(es, a) - replica_energy(es, b));
     const double ratio = mh_ratio(en);
     const uint4 w = philox4x32_10((uint32_t)ka, step, (uint32_t)g, ST_PT, key);
     const double u = (double)((((uint64_t)w.x << 32) | w.y) >> 11) * 0x1.0p-53;
@@ -108,7 +112,7 @@ __device__ __forceinline__ void pt_offer(int g, int ntemp, int ka, int kb, uint3
 // neighbouring temperatures (k, k+1), k = parity mod 2, in every group
 __global__ void k_pt_swap(int ngroups, int ntemp, int parity, uint32_t step, const uint32_t* __restrict__ dyn, Key2 key,
                           int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
-                          const double* __restrict__ energy, const double* __restrict__ ladder,
+                          const EnergySrc es, const double* __restrict__ ladder,
                           uint32_t* __restrict__ thr, int tbl, uint32_t* __restrict__ wthr, float* __restrict__ betaf,
                           double* __restrict__ betad, int32_t* __restrict__ acc)
 {
@@ -120,14 +124,13 @@ __global__ void k_pt_swap(int ngroups, int ntemp, int parity, uint32_t step, con
     const int g = (int)(t / npairs);
     const int k = parity + 2 * (int)(t - (int64_t)g * npairs);
     if (k + 1 >= ntemp) return;
-    pt_offer(g, ntemp, k, k + 1, step, key, rep_of_temp, temp_of_rep, energy, ladder, thr, tbl, wthr, betaf, betad,
-             acc);
+    pt_offer(g, ntemp, k, k + 1, step, key, rep_of_temp, temp_of_rep, es, ladder, thr, tbl, wthr, betaf, betad, acc);
 }
 
 // a plan of npairs disjoint slot pairs (lower slot first), the same in every group
 __global__ void k_pt_swap_pairs(int ngroups, int ntemp, int npairs, const int32_t* __restrict__ pairs, uint32_t step,
                                 Key2 key, int32_t* __restrict__ rep_of_temp, int32_t* __restrict__ temp_of_rep,
-                                const double* __restrict__ energy, const double* __restrict__ ladder,
+                                const EnergySrc es, const double* __restrict__ ladder,
                                 uint32_t* __restrict__ thr, int tbl, uint32_t* __restrict__ wthr,
                                 float* __restrict__ betaf, double* __restrict__ betad, int32_t* __restrict__ acc)
 {
@@ -135,7 +138,7 @@ __global__ void k_pt_swap_pairs(int ngroups, int ntemp, int npairs, const int32_
     if (t >= (int64_t)ngroups * npairs) return;
     const int g = (int)(t / npairs);
     const int p = (int)(t - (int64_t)g * npairs);
-    pt_offer(g, ntemp, pairs[2 * p], pairs[2 * p + 1], step, key, rep_of_temp, temp_of_rep, energy, ladder, thr, tbl,
+    pt_offer(g, ntemp, pairs[2 * p], pairs[2 * p + 1], step, key, rep_of_temp, temp_of_rep, es, ladder, thr, tbl,
              wthr, betaf, betad, acc);
 }
 
@@ -170,20 +173,21 @@ hipError_t zero_fill(void* p, int64_t n64, hipStream_t s)
     return hipGetLastError();
 }
 
-// device energies (no host synchronisation: usable inside a graph)
+bool vec_model(const mq_state* st) { return st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4; }
+
+// measurement sums on the device (no host synchronisation: usable inside a graph);
+// the swap kernels read the energies from them (replica_energy)
 hipError_t energy_async(mq_state* st)
 {
     hipStream_t s = st->ctx->stream;
     hipError_t e;
-    const int vec = (st->p.model == MQ_HEISENBERG || st->p.model == MQ_PHI4);
     if ((e = zero_fill(st->d_isum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
     // the FP32 sums exist for O(3)/Phi4 spins only; integer models neither write nor read them
-    if (vec && (e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
-    if ((e = launch_measure_all(st, st->d_isum, st->d_fsum)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_energy, dim3((unsigned)((st->nrep + 255) / 256)), dim3(256), 0, s, st->d_isum, st->d_fsum,
-                       st->nrep, vec, st->p.model == MQ_POTTS, st->p.J, st->d_energy);
-    return hipGetLastError();
+    if (vec_model(st) && (e = zero_fill(st->d_fsum, (int64_t)st->nrep * 8, s)) != hipSuccess) return e;
+    return launch_measure_all(st, st->d_isum, st->d_fsum);
 }
+
+EnergySrc energy_src(const mq_state* st) { return EnergySrc{st->d_isum, st->d_fsum, vec_model(st) ? 1 : 0, st->p.J}; }
 
 int ensure_pt(mq_state* st, int ntemp)
 {
@@ -217,7 +221,7 @@ hipError_t pt_swap_async(mq_state* st, int ntemp, int parity, uint32_t step, Key
     const int ngroups = st->nrep / ntemp;
     const int64_t work = (int64_t)ngroups * ((ntemp + 1) / 2);
     hipLaunchKernelGGL(k_pt_swap, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st->ctx->stream, ngroups, ntemp,
-                       parity, step, st->cur_dyn, key, st->d_rep_of_temp, st->d_temp_of_rep, st->d_energy, st->d_ladder,
+                       parity, step, st->cur_dyn, key, st->d_rep_of_temp, st->d_temp_of_rep, energy_src(st), st->d_ladder,
                        st->d_thr, st->tbl, st->d_wthr, st->d_betaf, st->d_betad, st->d_pt_acc);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && st->eabits) e = launch_wtab_eabits(st);  // temperatures moved between bits
@@ -288,7 +292,7 @@ extern "C" int mq_pt_swap_pairs(mq_state* st, int ntemp, int npairs, const int32
         const int ngroups = st->nrep / ntemp;
         const int64_t work = (int64_t)ngroups * np;
         hipLaunchKernelGGL(k_pt_swap_pairs, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, ngroups, ntemp, np,
-                           st->d_pt_pairs, (uint32_t)step, key, st->d_rep_of_temp, st->d_temp_of_rep, st->d_energy,
+                           st->d_pt_pairs, (uint32_t)step, key, st->d_rep_of_temp, st->d_temp_of_rep, energy_src(st),
                            st->d_ladder, st->d_thr, st->tbl, st->d_wthr, st->d_betaf, st->d_betad, st->d_pt_acc);
         MQ_HIP(st->ctx, hipGetLastError());
         if (st->eabits) MQ_HIP(st->ctx, launch_wtab_eabits(st));

@@ -556,7 +556,6 @@ struct mq_state {
     int32_t* d_pt_acc = nullptr;
     int32_t* d_pt_pairs = nullptr;  // slot pairs of the last mq_pt_swap_pairs plan
     int pt_pairs_cap = 0;
-    double* d_energy = nullptr;        // [nrep]
     uint64_t* d_pack = nullptr;        // [nrep][17] measurement staging: isum[8], fsum[8] and beta as raw words
     uint64_t* h_pack = nullptr;        // pinned host copy of d_pack
     std::vector<double> beta_ladder;   // per replica index at creation (temperature k)
