@@ -262,8 +262,17 @@ def test_gpu_ini_models_run(tmp_path, ham, dirs):
                 # MagFTComp columns of the reference's Ising / Phi4 observable tuples
                 nft = {"Phi4": x["dim"]}.get(ham, 0) if x["lattice"] == "regular" else 0
                 assert d.shape == (x["measure"], 6 + nft)
-                spins = _check_runfile(tmp_path / dname / f"{s['name']}.h5", d, ham, nft)
+                h5 = tmp_path / dname / f"{s['name']}.h5"
+                spins = _check_runfile(h5, d, ham, nft)
                 assert spins.size == x["nsites"] * (3 if ham in ("Heisenberg", "Phi4") else 1)
+                # lattice writers: RegularHypercubic (name, dim, npoints); the reference's default
+                # writer for lattices without their own (name "unknown", npoints; core.h:63-68)
+                lname = "unknown" if x["lattice"] == "rgg" else "RegularHypercubic"
+                assert h5tools.string(h5, "/step0/config/lattice/name") == lname
+                assert h5tools.values(h5, "/step0/config/lattice/npoints", integer=True) == [x["nsites"]]
+                assert h5tools.string(h5, "/step0/config/hamiltonian/Model") == {
+                    "Potts": "Potts", "Phi4": "Phi4", "Heisenberg": "Heisenberg",
+                    "EdwardsAnderson-Ising": "EdwardsAndersonIsing", "IsingRGG": "Ising"}[ham]
                 assert np.all(np.isfinite(d[:, 1:5]))
                 assert np.all((d[:, 6:] >= 0) & (d[:, 6:] <= 1))
                 assert hdr[-1].split()[7:] == [f"magft{k}" for k in range(nft)]
@@ -362,6 +371,12 @@ def test_gpu_ini_csv_graph_vs_exact(tmp_path):
         for rep in range(8):
             hdr, d = _series(out / "graph" / f"beta{beta:.6f}_{rep}.txt")
             assert d.shape == (4000, 6) and "lattice=csv" in hdr[0]
+            if rep == 0:  # the run file records the graph as GraphFromCSV does (graph_from_csv.h:65-69)
+                h5 = out / "graph" / f"beta{beta:.6f}_{rep}.h5"
+                _check_runfile(h5, d, "Ising", 0)
+                assert h5tools.string(h5, "/step0/config/lattice/name") == "GraphFromCSV"
+                assert h5tools.values(h5, "/step0/config/lattice/npoints", integer=True) == [9]
+                assert "/step0/config/lattice/dim" not in h5tools.tree(h5)
             e_rep.append(d[:, 1].mean())
             m2_rep.append(d[:, 3].mean())
         ex = exact.enumerate_graph(rp, ci, beta, J=-1.0)
