@@ -235,7 +235,7 @@ private:
 
 // One elementary Monte Carlo step of every replica (emcs.h:69-109).
 // `step` numbers the EMCS so that every step draws fresh Philox counters; a
-// run resumed from a checkpoint passes the number of EMCS already done, and
+// run continued from a saved state passes the number of EMCS already done, and
 // then continues the same trajectory.
 // Heisenberg spins are renormalised to unit length in FP64 after every
 // `renorm_every`-th EMCS (by EMCS number, so a resumed run renormalises at the
