@@ -323,6 +323,10 @@ def leg_c4(gpu, ctx):
         st.set_tuning(graph=1)
         st.run_graph(nsw, nt, 4, 2 * nsw * steps * 5, 5 * steps, steps, 1)
         ms_g, _ = timed(ctx, lambda: st.run_graph(nsw, nt, 4, 2 * nsw * steps * 6, 6 * steps, steps, 4))
+        # sweeps only (no energy or swap), replayed from a HIP graph: against the eager sweeps
+        # below, the cost of a replayed kernel node over a stream launch
+        st.run_graph(nsw, 1, 4, 2 * nsw * steps * 7, 0, steps, 1)
+        ms_gs, _ = timed(ctx, lambda: st.run_graph(nsw, 1, 4, 2 * nsw * steps * 8, 0, steps, 4))
         st.set_tuning(graph=0)
         acc = st.pt_swap(nt, 0, 4, 10 ** 6)
         res[f"{name}_attempts_per_ns"] = round(sites * nsw * steps * 4 / (ms * 1e6), 3)
@@ -331,6 +335,10 @@ def leg_c4(gpu, ctx):
         # the same sweeps launched eagerly (no graph, no energy or swap): the sweep kernels alone
         ms_e, _ = timed(ctx, lambda: st.metropolis(nsw * steps, 4, sweep0=10 ** 6))
         res[f"{name}_eager_sweeps_attempts_per_ns"] = round(sites * nsw * steps / (ms_e * 1e6), 3)
+        if bits == 1:  # 2 launches per sweep, 1 advance per repetition
+            nodes = 2 * nsw * steps * 4 + 4
+            res["hip_graph_node_overhead_us"] = round((ms_gs - ms_e * 4) * 1e3 / nodes, 2)
+            res["bits_hip_graph_sweeps_only_attempts_per_ns"] = round(sites * nsw * steps * 4 / (ms_gs * 1e6), 3)
         del st
     res["graph"] = (f"mq_run_graph: {steps} x ({nsw} sweeps + energy + PT swap) per repetition, 4 repetitions, "
                     "launched on the stream (the default); hip_graph: the same steps captured in one HIP graph "
