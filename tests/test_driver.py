@@ -350,6 +350,14 @@ def test_gpu_ini_resume_is_exact(tmp_path, ham, Ls):
     r = subprocess.run([BIN, ham, "--config", conf, "--out", str(twice), "--seed", "12"], capture_output=True,
                        text=True, timeout=110)
     assert r.returncode == 1 and "--seed 11" in r.stderr
+    # --fresh starts again: the run files are rewritten with one step, the other seed
+    r = subprocess.run([BIN, ham, "--config", conf, "--out", str(twice), "--seed", "12", "--fresh"],
+                       capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr
+    s = p[0]["groups"][0][0]
+    h5 = twice / Ls[0] / f"{s['name']}.h5"
+    assert sorted(k for k in h5tools.tree(h5) if k.count("/") == 1) == ["/step0"]
+    assert h5tools.values(h5, "/step0/state/rngstate", integer=True)[0] == 12
 
 
 @pytest.mark.gpu

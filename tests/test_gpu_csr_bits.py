@@ -114,9 +114,11 @@ def test_upload_download_round_trip(ctx):
     assert all(np.array_equal(bits.download(r), before[r]) for r in range(64) if r != 37)
 
 
-def test_pt_and_graph_replay_equal_int8(ctx):
-    """Eager sweeps + energies + PT swaps, then captured graph replays: 1-bit and int8
-    storage make the same swap decisions and end in the same states."""
+@pytest.mark.parametrize("mode", [1, 2], ids=["hip-graph", "stream"])
+def test_pt_and_graph_replay_equal_int8(ctx, mode):
+    """Eager sweeps + energies + PT swaps, then mq_run_graph steps (captured in a HIP graph
+    and replayed, or launched on the stream): 1-bit and int8 storage make the same swap
+    decisions and end in the same states."""
     from marqov_amd import gpu
     lat = gpu.CsrLattice(ctx, *_graph("rgg"))
     nrep, ntemp = 64, 32
@@ -124,6 +126,7 @@ def test_pt_and_graph_replay_equal_int8(ctx):
     bits, i8 = _pair(ctx, [lat] * nrep, -1.0)
     accs = []
     for st in (bits, i8):
+        st.set_tuning(graph=mode)
         st.set_beta(np.tile(ladder, nrep // ntemp))
         st.init(seed=7)
         acc = []
