@@ -271,24 +271,34 @@ __global__ void __launch_bounds__(256) k_measure_eabits(const char* __restrict__
     for (int p = 0; p < NE; ++p) EA[p] = 0u;
 #pragma unroll
     for (int p = 0; p < ND; ++p) DA[p] = 0u;
+    // All MSPL sites' words are loaded before any is counted, so a lane waits for one
+    // memory round trip rather than one per site; sites past the end of the colour
+    // read site 0 and are not counted.
+    uint32_t n[MSPL][2 * DIM], S[MSPL], T2[MSPL];
+    bool in[MSPL];
+#pragma unroll
     for (int t = 0; t < MSPL; ++t) {
-        const uint32_t h = first + (uint32_t)t * 64u + (uint32_t)lane;  // coalesced across the wavefront
-        if (h >= half) break;
+        const uint32_t hr = first + (uint32_t)t * 64u + (uint32_t)lane;  // coalesced across the wavefront
+        in[t] = hr < half;
+        const uint32_t h = in[t] ? hr : 0u;
         uint32_t nb[2 * DIM];
         nbrs_h<DIM>(L, 0, h, nb);
-        uint32_t n[2 * DIM];
 #pragma unroll
         for (int sl = 0; sl < 2 * DIM; ++sl) {
             const uint32_t d = (uint32_t)(sl >> 1);
             const int b = (sl & 1) ? bs[(uint32_t)DIM * half + d * half + nb[sl]] : bs[d * half + h];
-            n[sl] = c1[nb[sl]] ^ (b < 0 ? 0xFFFFFFFFu : 0u);
+            n[t][sl] = c1[nb[sl]] ^ (b < 0 ? 0xFFFFFFFFu : 0u);
         }
-        const uint32_t S = c0[h];
+        S[t] = c0[h];
+        T2[t] = c1[h];
+    }
+#pragma unroll
+    for (int t = 0; t < MSPL; ++t) {
+        if (!in[t]) continue;
         uint32_t b0, b1, b2;
-        anti_count<DIM>(S, n, b0, b1, b2);
+        anti_count<DIM>(S[t], n[t], b0, b1, b2);
         acc_add3<NE - 5>(reinterpret_cast<uint32_t(&)[NE - 5]>(EA), b0, b1, b2);
-        const uint32_t T2 = c1[h];
-        acc_add3<ND - 5>(reinterpret_cast<uint32_t(&)[ND - 5]>(DA), S ^ T2, S & T2, 0u);  // S + T2 per bit
+        acc_add3<ND - 5>(reinterpret_cast<uint32_t(&)[ND - 5]>(DA), S[t] ^ T2[t], S[t] & T2[t], 0u);  // S + T2 per bit
     }
     const int anti = bs_reduce_scatter<NE>(EA);
     const int down = bs_reduce_scatter<ND>(DA);

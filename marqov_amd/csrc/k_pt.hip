@@ -81,16 +81,19 @@ __device__ __forceinline__ void pt_offer(int g, int ntemp, int ka, int kb, uint3
     const double da = betad[a], db = betad[b];
     uint32_t* ra = thr + (int64_t)a * 2 * tbl;
     uint32_t* rb = thr + (int64_t)b * 2 * tbl;
-    for (int e0 = 0; e0 < 2 * tbl; e0 += PT_CHUNK) {
+    const int ne = 2 * tbl;
+    for (int e0 = 0; e0 < ne; e0 += PT_CHUNK) {
+        // loads at a clamped (in-row) index, unconditionally, so they issue together
         uint32_t xa[PT_CHUNK], xb[PT_CHUNK];
 #pragma unroll
         for (int j = 0; j < PT_CHUNK; ++j) {
-            xa[j] = e0 + j < 2 * tbl ? ra[e0 + j] : 0u;
-            xb[j] = e0 + j < 2 * tbl ? rb[e0 + j] : 0u;
+            const int e = min(e0 + j, ne - 1);
+            xa[j] = ra[e];
+            xb[j] = rb[e];
         }
 #pragma unroll
         for (int j = 0; j < PT_CHUNK; ++j) {
-            if (e0 + j < 2 * tbl) {
+            if (e0 + j < ne) {
                 ra[e0 + j] = xb[j];
                 rb[e0 + j] = xa[j];
             }
