@@ -260,6 +260,9 @@ class State:
         return {f: getattr(t, f) for f, _ in L.Tuning._fields_}
 
     def run_graph(self, nsweeps, ntemp, seed, sweep0, step0, steps, repeats):
+        """`repeats` x `steps` steps of {nsweeps sweeps; PT swap over ntemp temperatures if
+        ntemp > 1} with no host synchronisation in between (mq_run_graph): launched on the
+        stream, or captured in a HIP graph and replayed with set_tuning(graph=1)."""
         self.ctx.check(self.ctx.lib.mq_run_graph(self.h, nsweeps, ntemp, seed, sweep0, step0, steps, repeats),
                        "mq_run_graph")
 
