@@ -149,6 +149,8 @@ inline hid_t native<int32_t>() { return H5T_NATIVE_INT32; }
 template <>
 inline hid_t native<int64_t>() { return H5T_NATIVE_INT64; }
 template <>
+inline hid_t native<uint32_t>() { return H5T_NATIVE_UINT32; }
+template <>
 inline hid_t native<uint64_t>() { return H5T_NATIVE_UINT64; }
 
 template <class T>
@@ -213,8 +215,13 @@ void series(hid_t loc, const std::string& key, const T* data, std::size_t n, std
     check(H5Pset_chunk(pl.get(), 1, cdims), "chunks of " + key);
     if (shuffle) check(H5Pset_shuffle(pl.get()), "shuffle of " + key);
     check(H5Pset_deflate(pl.get(), 9), "deflate of " + key);
-    const std::vector<T> fill(width, T(0));
-    check(H5Pset_fill_value(pl.get(), ty.get(), fill.data()), "fill value of " + key);
+    // zero fill value, except for records of 8,192 values or more, whose fill value would
+    // not fit the dataset's object header (the reference sets none there either,
+    // cachecontainer.h:406-410)
+    if (width < 8192) {
+        const std::vector<T> fill(width, T(0));
+        check(H5Pset_fill_value(pl.get(), ty.get(), fill.data()), "fill value of " + key);
+    }
     Id ds(H5Dcreate2(loc, key.c_str(), ty.get(), sp.get(), H5P_DEFAULT, pl.get(), H5P_DEFAULT), H5Dclose,
           "cannot create " + key);
     if (comment) check(H5Oset_comment(ds.get(), comment), "comment on " + key);
@@ -224,6 +231,10 @@ void series(hid_t loc, const std::string& key, const T* data, std::size_t n, std
 // Elements per 4 MiB chunk for records of `bytes` bytes (cachecontainer.h:262,
 // core.h:887), at least one.
 inline std::size_t chunk_of(std::size_t bytes) { return std::max<std::size_t>(1, (4096u * 1024u) / bytes); }
+
+// Records per chunk of a vector-valued series of `width` values of `bytes` each:
+// 4 MiB / value / width + 1, whatever the vector's length (cachecontainer.h:403).
+inline std::size_t vector_chunk(std::size_t bytes, std::size_t width) { return (4096u * 1024u) / bytes / width + 1; }
 
 // A fixed-length array (rngstate, cachecontainer.h dumprng: dims = maxdims = n, no chunks).
 inline void words(hid_t loc, const std::string& key, const std::vector<uint64_t>& v)

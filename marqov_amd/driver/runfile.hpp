@@ -143,7 +143,7 @@ inline void write_step(const std::string& path, int step, const Meta& m, const s
         for (const Series& s : obs) {
             const std::size_t n = s.values.size() / s.width;
             // scalar series: 4 MiB chunks; vector series: 4 MiB / record + 1 (cachecontainer.h:403)
-            const std::size_t chunk = s.width == 1 ? chunk_of(sizeof(double)) : chunk_of(sizeof(double) * s.width) + 1;
+            const std::size_t chunk = s.width == 1 ? chunk_of(sizeof(double)) : vector_chunk(sizeof(double), s.width);
             series<double>(og.get(), s.name, s.values.data(), n, s.width, s.width > 1, chunk, false,
                            s.desc.empty() ? nullptr : s.desc.c_str());
         }

@@ -4,6 +4,8 @@
 // with the HDF5 project's h5ls / h5dump.
 //
 //   runfile_check <file.h5> ising|heisenberg
+//   runfile_check <file.h5> series      (the time series of the reference's cache-container
+//                                        component tests, testsuite/components/1-3)
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -17,6 +19,46 @@ int main(int argc, char** argv)
         return 2;
     }
     const std::string path = argv[1], kind = argv[2];
+    if (kind == "series") {
+        // 1-cachecontainer.cpp: 2,000,000 doubles and ints; 2-cachecontainer-long-sim.cpp:
+        // 500,000 records of a 7-vector; 3-cachecontainer-long-obs.cpp: 5 records of a
+        // 600,000-vector (longer than the 4 MiB cache)
+        try {
+            using namespace h5store;
+            Id file = create_file(path);
+            {
+                Id g = group(file.get(), "obs1");
+                const std::vector<double> v(2000000, 42.0);
+                series<double>(g.get(), "testobs", v.data(), v.size(), 1, false, chunk_of(sizeof(double)), false,
+                               "a test description");
+            }
+            {
+                Id g = group(file.get(), "obs2");
+                const std::vector<int32_t> v(2000000, 42);
+                series<int32_t>(g.get(), "testint", v.data(), v.size(), 1, false, chunk_of(sizeof(int32_t)), false,
+                                "a test int description");
+            }
+            {
+                Id g = group(file.get(), "obs3");
+                std::vector<double> v;
+                for (int i = 0; i < 500000; ++i)
+                    for (int k = 1; k <= 7; ++k) v.push_back((double)k);
+                series<double>(g.get(), "name", v.data(), 500000, 7, true, vector_chunk(sizeof(double), 7), false,
+                               "desc");
+            }
+            {
+                Id g = group(file.get(), "obs4");
+                const std::vector<double> v((std::size_t)600000 * 5, 2.0);
+                series<double>(g.get(), "name", v.data(), 5, 600000, true, vector_chunk(sizeof(double), 600000),
+                               false, "desc");
+            }
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "runfile_check: %s\n", e.what());
+            return 1;
+        }
+        std::printf("{}\n");
+        return 0;
+    }
     const bool vec = kind == "heisenberg";
     const int L = 4, N = L * L, nmeas = 6;
     try {

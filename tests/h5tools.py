@@ -37,11 +37,12 @@ def header(path, name):
     return _run([H5DUMP, "-p", "-H", "-d", name, str(path)])
 
 
-def values(path, name, integer=False):
+def values(path, name, integer=False, start=None, count=None):
     """The values of a numeric dataset, flattened (array records concatenated), as
     printed by h5dump with 17 significant digits (exact for doubles); integer=True
-    keeps integers exact (Python ints, for 64-bit words)."""
-    text = _run([H5DUMP, "-m", "%.17g", "-y", "-w", "0", "-d", name, str(path)])
+    keeps integers exact (Python ints, for 64-bit words); start/count select records."""
+    sel = ["-s", str(start), "-c", str(count)] if start is not None else []
+    text = _run([H5DUMP, "-m", "%.17g", "-y", "-w", "0", "-d", name] + sel + [str(path)])
     body = text[text.index("DATA {") + len("DATA {"):]
     body = body[:body.index("}")]
     toks = _NUM.findall(body)

@@ -129,3 +129,32 @@ def test_values_and_last_step_reader(run):
     assert got["lattice"] == "RegularHypercubic" and got["npoints"] == N
     assert got["beta"] == 0.4406868 and got["repid"] == 2 and got["seed"] == 0xFEEDFACECAFE
     assert np.array_equal(np.array(got["spins"]), want)
+
+
+def test_reference_cache_container_series(tmp_path):
+    """The time series of the reference's own cache-container tests
+    (testsuite/components/1-3: 2,000,000 doubles and ints; 500,000 records of a 7-vector;
+    5 records of a 600,000-vector, longer than the 4 MiB cache) through the driver's series
+    writer: extents, chunks (4 MiB of scalars; 4 MiB / 8 / width + 1 records of vectors),
+    fill values (none for records of 8,192 values or more, cachecontainer.h:406-410),
+    descriptions and values."""
+    path = tmp_path / "series.h5"
+    r = subprocess.run([CHK, str(path), "series"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = {
+        "/obs1/testobs": ("H5T_IEEE_F64LE", 2000000, 4096 * 1024 // 8, "VALUE  0", "a test description"),
+        "/obs2/testint": ("H5T_STD_I32LE", 2000000, 4096 * 1024 // 4, "VALUE  0", "a test int description"),
+        "/obs3/name": ("H5T_ARRAY { [7] H5T_IEEE_F64LE }", 500000, 4096 * 1024 // 8 // 7 + 1,
+                       "VALUE  [ 0, 0, 0, 0, 0, 0, 0 ]", "desc"),
+        "/obs4/name": ("H5T_ARRAY { [600000] H5T_IEEE_F64LE }", 5, 4096 * 1024 // 8 // 600000 + 1,
+                       "H5D_FILL_VALUE_DEFAULT", "desc"),
+    }
+    for ds, (ty, n, chunk, fill, desc) in want.items():
+        hd = h5tools.header(path, ds)
+        assert ty in hd and f"( {n} ) / ( H5S_UNLIMITED )" in hd and f"CHUNKED ( {chunk} )" in hd, (ds, hd)
+        assert fill in hd and f'COMMENT "{desc}"' in hd and "COMPRESSION DEFLATE { LEVEL 9 }" in hd, (ds, hd)
+    v = h5tools.values
+    assert np.array_equal(v(path, "/obs1/testobs", start=1999990, count=10), np.full(10, 42.0))
+    assert v(path, "/obs2/testint", integer=True, start=0, count=4) == [42] * 4
+    assert np.array_equal(v(path, "/obs3/name", start=499999, count=1), np.arange(1, 8, dtype=float))
+    assert np.array_equal(v(path, "/obs4/name", start=4, count=1), np.full(600000, 2.0))
