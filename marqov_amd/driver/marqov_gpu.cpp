@@ -561,7 +561,12 @@ int main(int argc, char** argv)
                             throw std::runtime_error(path + " does not belong to this configuration (" + what +
                                                      "); remove it or run with --fresh");
                         };
-                        const runfile::Last last = runfile::read_last(path, nsites, per_site, per_site == 3);
+                        runfile::Last last;
+                        try {
+                            last = runfile::read_last(path, nsites, per_site, per_site == 3);
+                        } catch (const std::exception& ex) {
+                            mismatch(std::string("unreadable last step: ") + ex.what());
+                        }
                         const runfile::Meta want = run_meta(pl, s, sims[r], o.seed, 0);
                         if (last.state.rng != RNG_NAME || last.state.rngstate.size() != 3)
                             mismatch("state written by another generator");
@@ -573,9 +578,15 @@ int main(int argc, char** argv)
                             mismatch("model or lattice differ");
                         if (last.meta.beta != sims[r].p.beta || last.meta.repid != sims[r].repid)
                             mismatch("temperature or replica differ");
-                        for (const auto& pr : want.params)
-                            if (runfile::read_param(path, last.step, pr.first) != pr.second)
-                                mismatch("Hamiltonian parameter " + pr.first + " differs");
+                        for (const auto& pr : want.params) {
+                            double v = 0.0;
+                            try {
+                                v = runfile::read_param(path, last.step, pr.first);
+                            } catch (const std::exception& ex) {
+                                mismatch(std::string("unreadable Hamiltonian parameter: ") + ex.what());
+                            }
+                            if (v != pr.second) mismatch("Hamiltonian parameter " + pr.first + " differs");
+                        }
                         if (r == 0) {
                             block = last.step + 1;
                             step0 = last.state.rngstate[2];

@@ -358,6 +358,15 @@ def test_gpu_ini_resume_is_exact(tmp_path, ham, Ls):
     h5 = twice / Ls[0] / f"{s['name']}.h5"
     assert sorted(k for k in h5tools.tree(h5) if k.count("/") == 1) == ["/step0"]
     assert h5tools.values(h5, "/step0/state/rngstate", integer=True)[0] == 12
+    # a run file whose last step has no state (copied without it) is refused, naming the file
+    bad = tmp_path / "bad.h5"
+    for grp in ("config", "environment", "observables"):
+        subprocess.run([os.path.join(h5tools.H5BIN, "h5copy"), "-p", "-i", str(h5), "-o", str(bad), "-s",
+                        f"/step0/{grp}", "-d", f"/step0/{grp}"], check=True, capture_output=True, timeout=60)
+    os.replace(bad, h5)
+    r = subprocess.run([BIN, ham, "--config", conf, "--out", str(twice), "--seed", "12"], capture_output=True,
+                       text=True, timeout=110)
+    assert r.returncode == 1 and str(h5) in r.stderr and "unreadable last step" in r.stderr, r.stderr
 
 
 @pytest.mark.gpu
